@@ -1,0 +1,204 @@
+"""Benchmark: project + SH + sort keys + stable radix sort of 4,194,304 splats
+(SH degree 3, 1920x1080, 32-bit keys) on one MI355X, as Gsplats/s and as a
+fraction of HBM peak. Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+A step is one rdx_project_sort over the table already resident in HBM:
+fused projection kernel (words, keys, digit histograms), then the radix
+passes (compaction in the first), writing the visible indices back to front.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+B_IN_SH3 = 236                 # f32 planes: centre 12, log-scale 12, quat 16, opacity 4, colour + SH 192
+B_OUT = 20                     # ewa words 16 + key 4 (one eye)
+B_SORT = 60                    # 4 passes of 32-bit keys at 8-bit digits
+METRIC = "Gsplats/s project+SH+sort @4M SH3, % HBM peak; batched vs plain; bwd Gsplats/s"
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def stage_times(ctx) -> dict:
+    from radix_amd import _native as N
+
+    names = C.create_string_buffer(32 * 32)
+    ms = (C.c_double * 32)()
+    cnt = (C.c_int * 32)()
+    k = N.lib().rdx_context_stage_times(ctx.handle, names, ms, cnt, 32)
+    if k < 0:
+        N.check(k)
+    out = {}
+    for i in range(k):
+        name = names.raw[32 * i: 32 * i + 32].split(b"\0", 1)[0].decode()
+        out[name] = {"ms_total": ms[i], "launches": cnt[i], "ms_avg": ms[i] / max(cnt[i], 1)}
+    return out
+
+
+def cpu_baseline(keys_u32, n: int, reps: int = 15, warm: int = 3) -> dict:
+    """The oracle's restatement of src/core's SortKeys.compute + SortKeys.sort
+    (32 bits) on the host, one thread, median of `reps` after `warm` warm-ups
+    (scripts/bench-core.ts:261-311)."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    lib = O.lib()
+    keys = np.ascontiguousarray(keys_u32[:n], dtype=np.uint32)
+    # distances that give these keys back: the float bits of ~key, and
+    # infinity for the invalid key
+    dist = (~keys).view(np.float32).astype(np.float64)
+    dist[keys == 0xFFFFFFFF] = np.inf
+    out_keys = np.empty(n, dtype=np.uint32)
+    ordering = np.empty(n, dtype=np.uint32)
+    times = []
+    for r in range(warm + reps):
+        t0 = time.perf_counter()
+        lib.orc_keys_compute(dist.ctypes.data, n, 32, out_keys.ctypes.data)
+        lib.orc_sort(out_keys.ctypes.data, n, 32, ordering.ctypes.data)
+        dt = time.perf_counter() - t0
+        if r >= warm:
+            times.append(dt)
+    med = statistics.median(times)
+    return {"value": n / med / 1e9, "unit": "Gsplats/s", "cores": 1, "kind": "port",
+            "sample": f"oracle/sort.c restating src/core SortKeys.compute + SortKeys.sort (32-bit, 2x16-bit LSD) "
+                      f"over the {n:,} keys of one step, median of {reps} after {warm} warm-ups, 1 thread; "
+                      f"median {med * 1e3:.1f} ms"}
+
+
+def oracle_projection_rate(planes, rig, count: int) -> dict:
+    """The FP64 oracle's projection + SH + key on a sample (labelled oracle: src/core has no CPU projection)."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    sample = np.ascontiguousarray(planes[:, :count])
+    t0 = time.perf_counter()
+    O.project_f32(sample, 3, O.view_from_rig(rig), O.params(sort_bits=32))
+    dt = time.perf_counter() - t0
+    return {"value": count / dt / 1e9, "unit": "Gsplats/s", "cores": 1, "kind": "oracle",
+            "sample": f"{count:,} splats of the same table, FP64 projection + SH3 + key, 1 thread"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--count", type=int, default=4194304)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU design is described, not built)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from radix_amd import DeviceTable, Params, Projector, workloads as W
+    from radix_amd.sort import Context
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    log(f"generating plain_4m ({args.count:,} splats, SH3)")
+    spec = W.plain_4m_spec(count=args.count)
+    planes = W.generate(spec)
+    rig = W.plain_4m_camera()
+    table = DeviceTable.from_f32_planes(planes, spec.count, 3)
+    params = Params(sort_bits=32)
+    ctx = Context()
+    proj = Projector(ctx)
+    n = spec.count
+    words = [torch.empty((n, 4), dtype=torch.int32, device=dev)]
+    keys = torch.empty(n, dtype=torch.int32, device=dev)
+    idx = torch.empty(n, dtype=torch.int32, device=dev)
+    cnt = torch.zeros(1, dtype=torch.int32, device=dev)
+    args5 = torch.zeros(5, dtype=torch.int32, device=dev)
+
+    def step():
+        proj.project_sort(table, rig, params, words=words, keys=keys, indices=idx, count=cnt, args=args5)
+
+    log(f"warmup {args.warmup}")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    log(f"timing {args.steps} steps")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / args.steps
+    visible = int(cnt.cpu().item())
+
+    # Per-kernel times with HIP events on the launch stream (second timed region)
+    N_lib = __import__("radix_amd._native", fromlist=["lib"]).lib()
+    N_lib.rdx_context_profile(ctx.handle, 1)
+    stage_times(ctx)
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    stages = stage_times(ctx)
+    N_lib.rdx_context_profile(ctx.handle, 0)
+    proj_ms = stages.get("project", {}).get("ms_avg", float("nan"))
+    kernel_bytes = (B_IN_SH3 + B_OUT) * n
+    achieved = kernel_bytes / (proj_ms * 1e-3) / 1e9
+    pipeline_bytes = (B_IN_SH3 + B_OUT + B_SORT) * n
+    value = n / (ms * 1e-3) / 1e9
+    log(f"{ms:.3f} ms/step, {value:.2f} Gsplats/s, project {proj_ms:.3f} ms ({achieved:.0f} GB/s), visible {visible:,}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gsplats/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 64-cluster cloud, random orbit camera)",
+        "config": {"workload": "plain_4m: 4,194,304 splats, SH3 f32 planes (236 B/splat), 1920x1080 camera on a "
+                               "radius-8 orbit, 60 deg, 32-bit keys, cullKeys on; step = rdx_project_sort",
+                   "splats": n, "visible": visible, "sort_bits": 32},
+        "hbm_frac_pipeline": round(pipeline_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "pipeline_bytes_per_splat": B_IN_SH3 + B_OUT + B_SORT,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_project (fused projection + SH + key + digit histograms)",
+                     "bytes_per_splat": B_IN_SH3 + B_OUT},
+        "stages_ms": {k: round(v["ms_avg"], 5) for k, v in stages.items()},
+        "wall_s_timed": round(wall, 4),
+    }
+    if not args.no_cpu:
+        log("cpu baseline")
+        keys_h = keys.cpu().numpy().view(np.uint32)
+        out["cpu_baseline"] = cpu_baseline(keys_h, n)
+        out["cpu_oracle_projection"] = oracle_projection_rate(planes, rig, 262144)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

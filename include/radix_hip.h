@@ -1,0 +1,245 @@
+/*
+ * radix_hip.h — C ABI of the MI355X (gfx950) splat pipeline: sort keys,
+ * stable LSD radix sort of splat indices, fused EWA projection with SH
+ * colour, the batched (many views, many collections) form and the
+ * differentiable compositor with its adjoint.
+ *
+ * Plain pointers and sizes only. Device pointers are HIP device memory;
+ * `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ * Every entry point returns 0 or a negative RDX_E* code; the message the
+ * reference would throw (same text) is then in rdx_last_error(), per thread.
+ *
+ * Matrices are 16 doubles in column-major order, the layout of three.js
+ * Matrix4.elements, so a JavaScript binding passes `matrix.elements` as is.
+ *
+ * Citations name files of the reference (sparkjsdev/radix) whose interface
+ * an entry point replaces.
+ */
+#ifndef RADIX_HIP_H
+#define RADIX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Error codes -------------------------------------------------------- */
+#define RDX_OK 0
+#define RDX_E_INVALID (-1)   /* bad argument (bits, counts, layout)        */
+#define RDX_E_CAPACITY (-2)  /* an output or the context is too small      */
+#define RDX_E_HIP (-3)       /* a HIP runtime call failed                  */
+#define RDX_E_NOMEM (-4)     /* device allocation failed                   */
+#define RDX_E_UNSUPPORTED (-5)
+
+/* The message of the calling thread's last failed call ("" if none). */
+const char* rdx_last_error(void);
+/* Library version string and the gfx target it was built for. */
+const char* rdx_version(void);
+
+/* ---- Sort keys (src/core/sort.ts:37-58, src/gpu/sortKey.tsl.ts:17-26) ---- */
+
+/* SortKeys.invalid(bits): all ones (src/core/sort.ts:37-39). */
+uint32_t rdx_key_invalid(int bits);
+/* SortKeys.validMax(bits) (src/core/sort.ts:41-43). */
+uint32_t rdx_key_valid_max(int bits);
+/* SortKeys.compute(distance, bits) on the host, for one float32 distance
+ * (src/core/sort.ts:46-58). */
+uint32_t rdx_key_compute(float distance, int bits);
+/* SortKeys.compute over `n` device float32 distances into device keys
+ * (GpuSortKeys.compute, src/gpu/sortKey.tsl.ts:17-26, with the CPU rule
+ * that non-finite and negative bit patterns give the invalid key). */
+int rdx_compute_keys(void* stream, const float* distances_dev, uint32_t n, int bits, uint32_t* keys_dev);
+
+/* ---- Context: device scratch, reused across calls on one stream ---------- */
+
+/* A context owns the device scratch of the sorts and projections run
+ * through it (no module-level scratch: src/core/sort.ts:22-25 is re-entrant
+ * only per thread; a context is re-entrant per stream). Calls on one
+ * context must be ordered on one stream; use one context per stream. */
+typedef struct rdx_context rdx_context;
+int rdx_context_create(rdx_context** out);
+int rdx_context_destroy(rdx_context* ctx);
+/* Grow scratch now for sorts of up to `max_keys` keys (optional; calls grow it). */
+int rdx_context_reserve(rdx_context* ctx, uint32_t max_keys);
+
+/* Radix sort tuning: digit width 4, 6 or 8 bits, keys per thread and
+ * workgroup size (threads). Zeros pick the built-in default (8, 16, 256). */
+typedef struct rdx_sort_tuning {
+  int digit_bits;
+  int keys_per_thread;
+  int block_threads;
+} rdx_sort_tuning;
+int rdx_context_set_tuning(rdx_context* ctx, const rdx_sort_tuning* tuning);
+int rdx_context_get_tuning(const rdx_context* ctx, rdx_sort_tuning* out);
+
+/* Stage timing: with profiling on, each kernel a call on this context
+ * launches is bracketed by HIP events on that call's stream. No reference
+ * counterpart: it replaces GpuProfiler's timestamp phases
+ * (src/gpu/GpuProfiler.ts:28-144) for measurement. */
+int rdx_context_profile(rdx_context* ctx, int enable);
+/* Totals per stage name since the last read (waits for the recorded
+ * events): up to max_stages entries of names (32 chars each, NUL-padded),
+ * milliseconds and launch counts; returns the number of stages, or < 0. */
+int rdx_context_stage_times(rdx_context* ctx, char* names, double* ms, int* launches, int max_stages);
+
+/* ---- The CPU-sort API, backed by the GPU --------------------------------- */
+
+/* SortKeys.sort({keys, count, bits, ordering}) (src/core/sort.ts:64-82) on
+ * HOST arrays: stable ascending order of the low `bits` bits of `keys`
+ * (keys must be below 2^bits at 24 bits, as the reference requires) into
+ * `ordering[0..count)`; *visible = count minus the keys whose top part is
+ * all ones, exactly as the reference counts it (16 bits: key & 0xffff ==
+ * 0xffff; 24: key >> 11 == 0x1fff; 32: key >> 16 == 0xffff).
+ * ordering_len < count fails with the reference's message
+ * "SortKeys.sort: ordering holds <len> of <count> indices" (sort.ts:70-72).
+ * Synchronous: returns when `ordering` holds the result. Also the body of
+ * sortSplatKeys (src/worker/splatSort.ts:17-30) and the worker's sortKeys
+ * handler (src/worker/coreWorker.ts:63-65). */
+int rdx_sort_keys(rdx_context* ctx, const uint32_t* keys, uint32_t count, int bits,
+                  uint32_t* ordering, size_t ordering_len, uint32_t* visible);
+
+/* ---- The GPU-sort API (device buffers, stream-ordered) ------------------- */
+
+/* WgpuRadixSort.sort(count, {bits}) (src/wgpu/WgpuRadixSort.ts:118-128):
+ * sorts the first `count` device keys and values in place, ascending and
+ * stable, on the low `bits` bits (16, 24 or 32; others fail with
+ * "WgpuRadixSort: bits must be 16, 24 or 32 (got <bits>)", WgpuRadixSort.ts:150-152). */
+int rdx_sort_pairs(rdx_context* ctx, void* stream, uint32_t* keys_dev, uint32_t* values_dev,
+                   uint32_t count, int bits);
+
+/* WgpuRadixSort.sortIndirect(countBuffer, {bits}) (WgpuRadixSort.ts:132-141):
+ * as rdx_sort_pairs on the first min(*count_dev, capacity) pairs, the count
+ * read on the device (nothing read back). */
+int rdx_sort_pairs_indirect(rdx_context* ctx, void* stream, uint32_t* keys_dev, uint32_t* values_dev,
+                            const uint32_t* count_dev, uint32_t capacity, int bits);
+
+/* WgpuSplatSorter.sort(projected, count, bits) (src/wgpu/WgpuSplatSorter.ts:64-71):
+ * drops keys equal to SortKeys.invalid(bits) (WgpuCompact, keeping input
+ * order), sorts the rest back to front, and writes
+ *   out_indices_dev[0..visible)  the splat indices in draw order,
+ *   *out_count_dev               visible (WgpuSplatSorter.count),
+ *   out_draw_args_dev[0..5)      [6 * 1024, ceil(visible / 1024), 0, 0, 0]
+ * (WgpuSplatSorter.ts:48-58). Any of count/args may be NULL. */
+int rdx_sort_splats(rdx_context* ctx, void* stream, const uint32_t* keys_dev, uint32_t n, int bits,
+                    uint32_t* out_indices_dev, uint32_t* out_count_dev, uint32_t* out_draw_args_dev);
+
+/* ---- Projection (ewa) ---------------------------------------------------- */
+
+#define RDX_LAYOUT_F32_SOA 0  /* float planes: center, log-scale, quat, opacity, rgb, SH */
+#define RDX_LAYOUT_EXT 1      /* extSplats: extA, extB uint32x4 (src/core/splats.ts:10-38) */
+#define RDX_LAYOUT_PACKED 2   /* packedSplats: packed uint32x4 (src/core/splats.ts:271-309) */
+
+#define RDX_SH_NONE 0
+#define RDX_SH_F32 1     /* float planes 3k+c (k = coefficient past DC, c = channel) */
+#define RDX_SH_EXT 2     /* extSh channels A.. (src/core/sh.ts:114-135)          */
+#define RDX_SH_PACKED 3  /* packedSh channels A.. (src/core/sh.ts:137-174)       */
+
+/* One splat table (a collection object's data) in device memory. */
+typedef struct rdx_splat_table {
+  int layout;            /* RDX_LAYOUT_* */
+  uint32_t count;        /* splats */
+  /* F32_SOA: planes of `plane_stride` floats each */
+  const float* center;     /* 3 planes x, y, z */
+  const float* log_scale;  /* 3 planes, natural log of the scales */
+  const float* quat;       /* 4 planes x, y, z, w (any length; normalised here) */
+  const float* opacity;    /* 1 plane, 0..1 (LoD values above 1 allowed) */
+  const float* rgb;        /* 3 planes: the base colour (SH DC) */
+  size_t plane_stride;     /* elements per plane (>= count); 0 = count */
+  /* EXT: two uint32x4 arrays; PACKED: one, with its encoding */
+  const uint32_t* ext_a;
+  const uint32_t* ext_b;
+  const uint32_t* packed;
+  float packed_encoding[4];  /* rgbMin, rgbMax, lnScaleMin, lnScaleMax (splats.ts:90-97) */
+  /* SH */
+  int sh_codec;              /* RDX_SH_* */
+  int sh_degree;             /* 0..3 */
+  const float* sh;           /* F32: 3*K planes of plane_stride floats */
+  const uint32_t* sh_channels[4]; /* EXT / PACKED: uint32x4 arrays, one per texel */
+  /* Object to world (matrixWorld), column-major */
+  double object_to_world[16];
+} rdx_splat_table;
+
+/* One eye: world to eye (camera.matrixWorldInverse) and its projection. */
+typedef struct rdx_eye {
+  double view[16];
+  double projection[16];
+  int ortho;
+} rdx_eye;
+
+/* SplatCamera.setViews({head, eyes, size, ndcZMin}) (src/render/splatCamera.ts:119-138). */
+typedef struct rdx_camera {
+  double head[16];   /* world to head */
+  int eyes;          /* 1 or 2 */
+  rdx_eye eye[2];
+  double size[2];    /* render size, pixels (width, height) */
+  double ndc_z_min;  /* 0: WebGPU clip convention, -1: WebGL */
+} rdx_camera;
+
+/* SplatViewParams + SplatObjectParams (src/render/splatParams.ts:8-87) and
+ * the key uniforms (SplatProjection.setKeys, SplatProjection.ts:24-26, 157-160). */
+typedef struct rdx_params {
+  double max_std_dev;      /* sqrt(8) */
+  double min_alpha;        /* 0.5 / 255 */
+  double falloff;          /* 1 */
+  int lod_inflate;         /* 1 */
+  double blur_amount;      /* 0.3 */
+  double pre_blur_amount;  /* 0 */
+  double clip_xy;          /* 1.4 */
+  double min_pixel_radius; /* 0 */
+  double max_pixel_radius; /* 512 */
+  double min_contribution; /* 0 */
+  double focal_distance;   /* 0 */
+  double aperture_tan;     /* 0 */
+  double sort_guard;       /* 0 */
+  int max_sh_degree;       /* 4 */
+  int sort_bits;           /* 16, 24 or 32 */
+  int cull_keys;           /* 1: keys of splats culled in every eye are invalid */
+  int key_per_eye;         /* 0: one key from the head (reference); 1: per eye */
+} rdx_params;
+
+/* The reference defaults (splatParams.ts:68-72), sort_bits 24, cull_keys 1. */
+void rdx_params_default(rdx_params* out);
+
+/* Float side output of the projection for checks (NULL members skipped):
+ * planes of `stride` floats; per-eye planes at [eye * planes * stride]. */
+typedef struct rdx_float_out {
+  size_t stride;
+  float* conic;     /* per eye 3 planes: inverse 2D covariance (A, B, C), pixels^-2 */
+  float* cov2d;     /* per eye 3 planes: blurred 2D covariance (a, b, d), pixels^2 */
+  float* opacity;   /* per eye 1 plane: blur-compensated opacity */
+  float* ndc;       /* per eye 2 planes */
+  float* view_z;    /* per eye 1 plane: eye-space z (negative ahead) */
+  float* bound;     /* per eye 3 planes: scale1, scale2 (pixels), angle */
+  float* rgb;       /* 3 planes: base colour plus SH, clamped at 0 */
+  float* distance;  /* 1 plane: the key distance (per eye with key_per_eye) */
+} rdx_float_out;
+
+/* SplatProjection.write for the ewa method (SplatProjection.ts:141-153,
+ * projections.tsl.ts:95-148), run as one kernel over the table:
+ *   words_dev[e]  eye e's 4 words per splat at [4 * (at + i) ...] (the
+ *                 projected bundle's channel w<e>, SplatProjection.ts:48-54)
+ *   keys_dev      the sort key per splat at [at + i] (sortKeys.key, little-endian)
+ *                 (with key_per_eye, eye e's keys at keys_dev + e * key_stride)
+ *   hist_dev      optional: per-digit histograms of the valid keys
+ *                 (rdx_sort_hist_words() words), the front of a fused sort
+ *   side          optional float outputs (NULL: none). */
+int rdx_project_ewa(rdx_context* ctx, void* stream, const rdx_splat_table* table, const rdx_camera* camera,
+                    const rdx_params* params, uint32_t at, uint32_t* const* words_dev, uint32_t* keys_dev,
+                    size_t key_stride, const rdx_float_out* side);
+
+/* Project and sort in one stream-ordered sequence: the fused kernel writes
+ * words, keys and the digit histograms of the valid keys; then the first
+ * radix pass drops invalid keys and every pass scatters with a single-pass
+ * look-back. Output as rdx_sort_splats for the keys it made. The headline
+ * path (SplatRenderer.update's project + sort, SplatRenderer.ts:412-456). */
+int rdx_project_sort(rdx_context* ctx, void* stream, const rdx_splat_table* table, const rdx_camera* camera,
+                     const rdx_params* params, uint32_t* const* words_dev, uint32_t* keys_dev,
+                     uint32_t* out_indices_dev, uint32_t* out_count_dev, uint32_t* out_draw_args_dev,
+                     const rdx_float_out* side);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RADIX_HIP_H */

@@ -1,0 +1,445 @@
+// The C ABI of include/radix_hip.h: contexts, the sorts, the projection and
+// the fused project + sort, each a short stream-ordered sequence of the
+// kernels in sort_kernels.hpp and project_kernels.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/radix_hip.h"
+#include "api_internal.hpp"
+#include "sort_kernels.hpp"
+
+using namespace rdx;
+
+namespace {
+thread_local std::string g_error;
+}
+
+namespace rdx {
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_error = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(RDX_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+// ---- device buffers --------------------------------------------------------
+
+int Buffer::ensure(size_t need_bytes) {
+  if (need_bytes <= bytes) return RDX_OK;
+  if (ptr) {
+    hipError_t e = hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    if (e != hipSuccess) return hip_fail(e, "hipFree");
+  }
+  size_t grow = std::max(need_bytes, (size_t)4096);
+  hipError_t e = hipMalloc(&ptr, grow);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    (void)hipGetLastError();
+    return fail(RDX_E_NOMEM, "device allocation of %zu bytes failed: %s", grow, hipGetErrorString(e));
+  }
+  bytes = grow;
+  return RDX_OK;
+}
+
+void Buffer::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+// ---- stage timing ---------------------------------------------------------------
+
+hipEvent_t Context::take_event() {
+  if (!event_pool.empty()) {
+    hipEvent_t e = event_pool.back();
+    event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+Stage::Stage(Context* c, hipStream_t s, const char* name) : ctx(c), stream(s), on(c && c->profiling) {
+  if (!on) return;
+  rec.name = name;
+  rec.start = ctx->take_event();
+  rec.end = ctx->take_event();
+  if (!rec.start || !rec.end) {
+    on = false;
+    return;
+  }
+  (void)hipEventRecord(rec.start, stream);
+}
+
+Stage::~Stage() {
+  if (!on) return;
+  (void)hipEventRecord(rec.end, stream);
+  ctx->records.push_back(rec);
+}
+
+// ---- sort plans ---------------------------------------------------------------
+
+SortPlan make_plan(const rdx_sort_tuning& t, int bits) {
+  SortPlan p;
+  p.rb = t.digit_bits;
+  p.block = t.block_threads;
+  p.kpt = t.keys_per_thread;
+  p.tile = p.block * p.kpt;
+  p.passes = (bits + p.rb - 1) / p.rb;
+  p.bits = bits;
+  return p;
+}
+
+bool tuning_supported(const rdx_sort_tuning& t) {
+  const bool rb = t.digit_bits == 4 || t.digit_bits == 6 || t.digit_bits == 8;
+  const bool shape = (t.block_threads == 256 && (t.keys_per_thread == 8 || t.keys_per_thread == 12 ||
+                                                 t.keys_per_thread == 16 || t.keys_per_thread == 20)) ||
+                     (t.block_threads == 512 && (t.keys_per_thread == 8 || t.keys_per_thread == 12 ||
+                                                 t.keys_per_thread == 16));
+  return rb && shape;
+}
+
+template <int RB, int BLOCK, int KPT>
+static void launch_pass(const SortPlan& p, uint32_t grid, hipStream_t s, const PassArgs& a) {
+  hipLaunchKernelGGL((k_onesweep<RB, BLOCK, KPT>), dim3(grid), dim3(BLOCK), 0, s, a);
+}
+
+template <int RB>
+static bool launch_pass_rb(const SortPlan& p, uint32_t grid, hipStream_t s, const PassArgs& a) {
+  if (p.block == 256) {
+    switch (p.kpt) {
+      case 8: launch_pass<RB, 256, 8>(p, grid, s, a); return true;
+      case 12: launch_pass<RB, 256, 12>(p, grid, s, a); return true;
+      case 16: launch_pass<RB, 256, 16>(p, grid, s, a); return true;
+      case 20: launch_pass<RB, 256, 20>(p, grid, s, a); return true;
+    }
+  } else if (p.block == 512) {
+    switch (p.kpt) {
+      case 8: launch_pass<RB, 512, 8>(p, grid, s, a); return true;
+      case 12: launch_pass<RB, 512, 12>(p, grid, s, a); return true;
+      case 16: launch_pass<RB, 512, 16>(p, grid, s, a); return true;
+    }
+  }
+  return false;
+}
+
+static bool launch_onesweep(const SortPlan& p, uint32_t grid, hipStream_t s, const PassArgs& a) {
+  switch (p.rb) {
+    case 4: return launch_pass_rb<4>(p, grid, s, a);
+    case 6: return launch_pass_rb<6>(p, grid, s, a);
+    case 8: return launch_pass_rb<8>(p, grid, s, a);
+  }
+  return false;
+}
+
+uint32_t pass_grid(const SortPlan& p, uint32_t n_cap) {
+  return std::max<uint32_t>(1u, (uint32_t)(((uint64_t)n_cap + p.tile - 1) / p.tile));
+}
+
+// Scratch for a sort of up to n keys (and `segments` histogram blocks).
+int Context::reserve(const SortPlan& p, uint32_t n, int segments) {
+  const size_t words = std::max<size_t>(n, 1);
+  int rc;
+  for (int i = 0; i < 2; i++) {
+    if ((rc = keys[i].ensure(words * 4))) return rc;
+    if ((rc = vals[i].ensure(words * 4))) return rc;
+  }
+  if ((rc = hist.ensure((size_t)HIST_WORDS * 4 * std::max(segments, 1)))) return rc;
+  const size_t rows = pass_grid(p, n);
+  if ((rc = status.ensure(2 * rows * HIST_BINS * 4))) return rc;
+  status_rows = std::max(status_rows, rows);
+  status_rows = status.bytes / (2 * HIST_BINS * 4);
+  return RDX_OK;
+}
+
+void launch_clear(Context* ctx, hipStream_t s, uint32_t* hist, uint32_t rows) {
+  Stage st(ctx, s, "clear");
+  const uint32_t words = rows * HIST_BINS;
+  const uint32_t grid = std::min<uint32_t>(std::max<uint32_t>((words + 255) / 256, 1u), 1024u);
+  hipLaunchKernelGGL(k_clear, dim3(grid), dim3(256), 0, s, hist, (uint32_t*)ctx->status.ptr, words);
+}
+
+void launch_histogram(Context* ctx, hipStream_t s, const SortPlan& p, const uint32_t* keys, const uint32_t* n_dev, uint32_t n_cap,
+                      uint32_t* hist, int drop, uint32_t invalid) {
+  Stage st(ctx, s, "histogram");
+  const uint32_t per_block = 256 * 16;
+  const uint32_t grid = std::min<uint32_t>(std::max<uint32_t>((n_cap + per_block - 1) / per_block, 1u), 2048u);
+  hipLaunchKernelGGL(k_histogram<256>, dim3(grid), dim3(256), 0, s, keys, n_dev, n_cap, hist, p.passes, p.rb, drop,
+                     invalid, p.bits);
+}
+
+// The P digit passes after the histograms are in `hist`: pass 0 reads
+// (keys_in, vals_in or the index), the last writes (final_keys, final_vals);
+// between them the context's two scratch pairs alternate.
+int run_passes(Context* ctx, hipStream_t s, const SortPlan& p, const uint32_t* keys_in, const uint32_t* vals_in,
+               uint32_t* final_keys, uint32_t* final_vals, uint32_t n_cap, const uint32_t* n_dev, bool drop,
+               uint32_t invalid, uint32_t* hist, uint32_t* count_out, uint32_t* draw_args) {
+  const uint32_t grid = pass_grid(p, n_cap);
+  uint32_t* status = (uint32_t*)ctx->status.ptr;
+  const size_t region = ctx->status_rows * HIST_BINS;
+  const uint32_t* kin = keys_in;
+  const uint32_t* vin = vals_in;
+  for (int pass = 0; pass < p.passes; pass++) {
+    const bool last = pass == p.passes - 1;
+    PassArgs a;
+    a.keys_in = kin;
+    a.vals_in = vin;
+    a.keys_out = last ? final_keys : (uint32_t*)ctx->keys[pass & 1].ptr;
+    a.vals_out = last ? final_vals : (uint32_t*)ctx->vals[pass & 1].ptr;
+    a.n_dev = pass == 0 ? n_dev : (drop ? nullptr : n_dev);
+    a.n = n_cap;
+    a.hist = hist + pass * HIST_BINS;
+    a.ticket = hist + HIST_TICKETS + pass;
+    a.status = status + (pass & 1) * region;
+    a.status_next = last ? nullptr : status + ((pass + 1) & 1) * region;
+    a.invalid = invalid;
+    a.drop = pass == 0 && drop;
+    a.use_hist_total = pass > 0 && drop;
+    a.shift = pass * p.rb;
+    a.key_mask = key_invalid(p.bits);
+    a.count_out = last ? count_out : nullptr;
+    a.draw_args = last ? draw_args : nullptr;
+    a.error = hist + HIST_ERROR;
+    char name[32];
+    snprintf(name, sizeof name, "sort_pass%d", pass);
+    Stage st(ctx, s, name);
+    if (!launch_onesweep(p, grid, s, a)) return fail(RDX_E_INVALID, "unsupported sort tuning");
+    kin = a.keys_out;
+    vin = a.vals_out;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "radix sort launch");
+  return RDX_OK;
+}
+
+int check_bits(int bits, const char* who) {
+  if (bits != 16 && bits != 24 && bits != 32) return fail(RDX_E_INVALID, "%s: bits must be 16, 24 or 32 (got %d)", who, bits);
+  return RDX_OK;
+}
+
+}  // namespace rdx
+
+// ============================================================================
+// extern "C"
+// ============================================================================
+
+extern "C" {
+
+const char* rdx_last_error(void) { return g_error.c_str(); }
+
+const char* rdx_version(void) { return "radix_hip 0.1 gfx950"; }
+
+uint32_t rdx_key_invalid(int bits) { return key_invalid(bits); }
+
+uint32_t rdx_key_valid_max(int bits) { return (uint32_t)(0xffull << (bits - 8)) - 1u; }
+
+uint32_t rdx_key_compute(float distance, int bits) {
+  uint32_t f;
+  memcpy(&f, &distance, 4);
+  const uint32_t mask = key_invalid(bits);
+  if (!(f < 0x7f800000u)) return mask;
+  uint32_t q;
+  if (bits == 32) {
+    q = f;
+  } else if (bits == 24) {
+    q = f >> 7;
+  } else {
+    const uint32_t lo = 95u << 23, hi = (159u << 23) - 1u;
+    q = (std::min(std::max(f, lo), hi) - lo) >> 13;
+  }
+  return std::min(mask ^ q, rdx_key_valid_max(bits));
+}
+
+int rdx_compute_keys(void* stream, const float* d, uint32_t n, int bits, uint32_t* keys) {
+  if (int rc = check_bits(bits, "SortKeys.compute")) return rc;
+  if (n == 0) return RDX_OK;
+  const uint32_t grid = std::min<uint32_t>((n + 255) / 256, 4096u);
+  hipLaunchKernelGGL(k_compute_keys, dim3(grid), dim3(256), 0, (hipStream_t)stream, d, n, bits, keys);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? RDX_OK : hip_fail(e, "rdx_compute_keys");
+}
+
+int rdx_context_create(rdx_context** out) {
+  if (!out) return fail(RDX_E_INVALID, "rdx_context_create: out is NULL");
+  Context* c = new Context();
+  c->tuning.digit_bits = 8;
+  c->tuning.keys_per_thread = 16;
+  c->tuning.block_threads = 256;
+  *out = reinterpret_cast<rdx_context*>(c);
+  return RDX_OK;
+}
+
+int rdx_context_destroy(rdx_context* ctx) {
+  if (!ctx) return RDX_OK;
+  Context* c = reinterpret_cast<Context*>(ctx);
+  c->release();
+  delete c;
+  return RDX_OK;
+}
+
+int rdx_context_reserve(rdx_context* ctx, uint32_t max_keys) {
+  if (!ctx) return fail(RDX_E_INVALID, "rdx_context_reserve: no context");
+  Context* c = reinterpret_cast<Context*>(ctx);
+  return c->reserve(make_plan(c->tuning, 32), max_keys, 1);
+}
+
+int rdx_context_set_tuning(rdx_context* ctx, const rdx_sort_tuning* t) {
+  if (!ctx || !t) return fail(RDX_E_INVALID, "rdx_context_set_tuning: NULL argument");
+  Context* c = reinterpret_cast<Context*>(ctx);
+  rdx_sort_tuning v = *t;
+  if (!v.digit_bits) v.digit_bits = 8;
+  if (!v.keys_per_thread) v.keys_per_thread = 16;
+  if (!v.block_threads) v.block_threads = 256;
+  if (!tuning_supported(v))
+    return fail(RDX_E_INVALID, "unsupported sort tuning: %d-bit digits, %d keys per thread, %d threads", v.digit_bits,
+                v.keys_per_thread, v.block_threads);
+  c->tuning = v;
+  return RDX_OK;
+}
+
+int rdx_context_get_tuning(const rdx_context* ctx, rdx_sort_tuning* out) {
+  if (!ctx || !out) return fail(RDX_E_INVALID, "rdx_context_get_tuning: NULL argument");
+  *out = reinterpret_cast<const Context*>(ctx)->tuning;
+  return RDX_OK;
+}
+
+int rdx_context_profile(rdx_context* ctx, int enable) {
+  if (!ctx) return fail(RDX_E_INVALID, "rdx_context_profile: no context");
+  reinterpret_cast<Context*>(ctx)->profiling = enable != 0;
+  return RDX_OK;
+}
+
+int rdx_context_stage_times(rdx_context* ctx, char* names, double* ms, int* launches, int max_stages) {
+  if (!ctx) return fail(RDX_E_INVALID, "rdx_context_stage_times: no context");
+  Context* c = reinterpret_cast<Context*>(ctx);
+  std::vector<std::string> order;
+  std::vector<double> total;
+  std::vector<int> count;
+  for (auto& r : c->records) {
+    hipError_t e = hipEventSynchronize(r.end);
+    float t = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, r.start, r.end);
+    if (e != hipSuccess) return hip_fail(e, "rdx_context_stage_times");
+    size_t k = 0;
+    while (k < order.size() && order[k] != r.name) k++;
+    if (k == order.size()) {
+      order.push_back(r.name);
+      total.push_back(0.0);
+      count.push_back(0);
+    }
+    total[k] += t;
+    count[k] += 1;
+    c->event_pool.push_back(r.start);
+    c->event_pool.push_back(r.end);
+  }
+  c->records.clear();
+  const int n = std::min<int>((int)order.size(), max_stages);
+  for (int k = 0; k < n; k++) {
+    if (names) {
+      memset(names + 32 * k, 0, 32);
+      strncpy(names + 32 * k, order[k].c_str(), 31);
+    }
+    if (ms) ms[k] = total[k];
+    if (launches) launches[k] = count[k];
+  }
+  return n;
+}
+
+int rdx_sort_keys(rdx_context* ctx, const uint32_t* keys, uint32_t count, int bits, uint32_t* ordering,
+                  size_t ordering_len, uint32_t* visible) {
+  if (!ctx) return fail(RDX_E_INVALID, "SortKeys.sort: no context");
+  if (ordering_len < count)
+    return fail(RDX_E_CAPACITY, "SortKeys.sort: ordering holds %zu of %u indices", ordering_len, count);
+  if (int rc = check_bits(bits, "SortKeys.sort")) return rc;
+  Context* c = reinterpret_cast<Context*>(ctx);
+  if (count == 0) {
+    if (visible) *visible = 0;
+    return RDX_OK;
+  }
+  const SortPlan p = make_plan(c->tuning, bits);
+  int rc;
+  if ((rc = c->reserve(p, count, 1))) return rc;
+  if ((rc = c->io_keys.ensure((size_t)count * 4))) return rc;
+  if ((rc = c->io_vals.ensure((size_t)count * 4))) return rc;
+  hipStream_t s = nullptr;
+  uint32_t* dkeys = (uint32_t*)c->io_keys.ptr;
+  uint32_t* dord = (uint32_t*)c->io_vals.ptr;
+  uint32_t* hist = (uint32_t*)c->hist.ptr;
+  hipError_t e = hipMemcpyAsync(dkeys, keys, (size_t)count * 4, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return hip_fail(e, "SortKeys.sort upload");
+  launch_clear(c, s, hist, pass_grid(p, count));
+  launch_histogram(c, s, p, dkeys, nullptr, count, hist, 0, 0u);
+  if ((rc = run_passes(c, s, p, dkeys, nullptr, nullptr, dord, count, nullptr, false, 0u, hist, nullptr, nullptr)))
+    return rc;
+  uint32_t hidden = 0;
+  e = hipMemcpyAsync(ordering, dord, (size_t)count * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(&hidden, hist + HIST_RULE, 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "SortKeys.sort download");
+  if (visible) *visible = count - hidden;
+  return RDX_OK;
+}
+
+static int sort_pairs_impl(rdx_context* ctx, void* stream, uint32_t* keys, uint32_t* vals, const uint32_t* count_dev,
+                           uint32_t n, int bits) {
+  if (!ctx) return fail(RDX_E_INVALID, "WgpuRadixSort: no context");
+  if (int rc = check_bits(bits, "WgpuRadixSort")) return rc;
+  Context* c = reinterpret_cast<Context*>(ctx);
+  if (n == 0) return RDX_OK;
+  const SortPlan p = make_plan(c->tuning, bits);
+  if (int rc = c->reserve(p, n, 1)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  uint32_t* hist = (uint32_t*)c->hist.ptr;
+  launch_clear(c, s, hist, pass_grid(p, n));
+  launch_histogram(c, s, p, keys, count_dev, n, hist, 0, 0u);
+  return run_passes(c, s, p, keys, vals, keys, vals, n, count_dev, false, 0u, hist, nullptr, nullptr);
+}
+
+int rdx_sort_pairs(rdx_context* ctx, void* stream, uint32_t* keys, uint32_t* vals, uint32_t count, int bits) {
+  return sort_pairs_impl(ctx, stream, keys, vals, nullptr, count, bits);
+}
+
+int rdx_sort_pairs_indirect(rdx_context* ctx, void* stream, uint32_t* keys, uint32_t* vals, const uint32_t* count_dev,
+                            uint32_t capacity, int bits) {
+  if (!count_dev) return fail(RDX_E_INVALID, "WgpuRadixSort.sortIndirect: no count buffer");
+  return sort_pairs_impl(ctx, stream, keys, vals, count_dev, capacity, bits);
+}
+
+int rdx_sort_splats(rdx_context* ctx, void* stream, const uint32_t* keys, uint32_t n, int bits, uint32_t* out_indices,
+                    uint32_t* out_count, uint32_t* out_args) {
+  if (!ctx) return fail(RDX_E_INVALID, "WgpuSplatSorter: no context");
+  if (int rc = check_bits(bits, "WgpuRadixSort")) return rc;
+  Context* c = reinterpret_cast<Context*>(ctx);
+  hipStream_t s = (hipStream_t)stream;
+  if (n == 0) {
+    hipLaunchKernelGGL(k_empty_outputs, dim3(1), dim3(64), 0, s, out_count, out_args);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? RDX_OK : hip_fail(e, "WgpuSplatSorter");
+  }
+  const SortPlan p = make_plan(c->tuning, bits);
+  if (int rc = c->reserve(p, n, 1)) return rc;
+  uint32_t* hist = (uint32_t*)c->hist.ptr;
+  const uint32_t invalid = key_invalid(bits);
+  launch_clear(c, s, hist, pass_grid(p, n));
+  launch_histogram(c, s, p, keys, nullptr, n, hist, 1, invalid);
+  return run_passes(c, s, p, keys, nullptr, nullptr, out_indices, n, nullptr, true, invalid, hist, out_count, out_args);
+}
+
+}  // extern "C"
