@@ -84,6 +84,17 @@ int rdx_context_profile(rdx_context* ctx, int enable);
  * milliseconds and launch counts; returns the number of stages, or < 0. */
 int rdx_context_stage_times(rdx_context* ctx, char* names, double* ms, int* launches, int max_stages);
 
+/* ---- Device memory for bindings (Node N-API) without a GPU runtime -------
+ * The storage a WgpuBuffer gives the reference (src/wgpu/WgpuBuffer.ts:15-201:
+ * allocation, upload, read back); plain HIP device memory here. */
+int rdx_device_alloc(size_t bytes, void** out);
+int rdx_device_free(void* ptr);
+/* Copies wait for completion (the host buffer may be reused on return). */
+int rdx_device_upload(void* stream, void* dst_dev, const void* src_host, size_t bytes);
+int rdx_device_download(void* stream, void* dst_host, const void* src_dev, size_t bytes);
+int rdx_device_fill_u32(void* stream, uint32_t* dst_dev, uint32_t value, size_t count);
+int rdx_stream_synchronize(void* stream);
+
 /* ---- The CPU-sort API, backed by the GPU --------------------------------- */
 
 /* SortKeys.sort({keys, count, bits, ordering}) (src/core/sort.ts:64-82) on

@@ -129,6 +129,12 @@ _SIGNATURES = [
     ("rdx_context_reserve", C.c_int, [C.c_void_p, C.c_uint32]),
     ("rdx_context_set_tuning", C.c_int, [C.c_void_p, C.POINTER(SortTuning)]),
     ("rdx_context_get_tuning", C.c_int, [C.c_void_p, C.POINTER(SortTuning)]),
+    ("rdx_device_alloc", C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
+    ("rdx_device_free", C.c_int, [C.c_void_p]),
+    ("rdx_device_upload", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("rdx_device_download", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("rdx_device_fill_u32", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_size_t]),
+    ("rdx_stream_synchronize", C.c_int, [C.c_void_p]),
     ("rdx_context_profile", C.c_int, [C.c_void_p, C.c_int]),
     ("rdx_context_stage_times", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int]),
     ("rdx_sort_keys", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_size_t, _u32p]),

@@ -79,9 +79,27 @@ def build_synth(force: bool = True) -> str:
     return lib
 
 
+NODE_INCLUDE = "/usr/include/node"
+
+
+def build_napi(force: bool = True) -> str | None:
+    """The Node N-API addon (radix_amd/napi/radix_napi.node), when Node's headers are present."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        return None
+    src = os.path.join(HERE, "napi", "radix_napi.cc")
+    out = os.path.join(HERE, "napi", "radix_napi.node")
+    if force or not _newer(out, [src, os.path.join(ROOT, "include", "radix_hip.h")]):
+        tmp = out + ".tmp"
+        _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-DNODE_GYP_MODULE_NAME=radix_napi",
+              "-I" + NODE_INCLUDE, "-o", tmp, src, "-L" + HERE, "-lradix_hip", "-Wl,-rpath,$ORIGIN/.."])
+        os.replace(tmp, out)
+    return out
+
+
 def build(force: bool = True) -> None:
     build_synth(force)
     build_hip(force)
+    build_napi(force)
 
 
 if __name__ == "__main__":

@@ -217,6 +217,10 @@ int run_passes(Context* ctx, hipStream_t s, const SortPlan& p, const uint32_t* k
     a.count_out = last ? count_out : nullptr;
     a.draw_args = last ? draw_args : nullptr;
     a.error = hist + HIST_ERROR;
+    a.seg_of_tile = nullptr;
+    a.seg = nullptr;
+    a.hist_stride = 0;
+    a.dst_cap = n_cap;
     char name[32];
     snprintf(name, sizeof name, "sort_pass%d", pass);
     Stage st(ctx, s, name);
@@ -318,6 +322,53 @@ int rdx_context_get_tuning(const rdx_context* ctx, rdx_sort_tuning* out) {
   if (!ctx || !out) return fail(RDX_E_INVALID, "rdx_context_get_tuning: NULL argument");
   *out = reinterpret_cast<const Context*>(ctx)->tuning;
   return RDX_OK;
+}
+
+static __global__ void k_fill_u32(uint32_t* dst, uint32_t value, size_t count) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = value;
+}
+
+int rdx_device_alloc(size_t bytes, void** out) {
+  if (!out) return fail(RDX_E_INVALID, "rdx_device_alloc: out is NULL");
+  *out = nullptr;
+  hipError_t e = hipMalloc(out, std::max<size_t>(bytes, 4));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(RDX_E_NOMEM, "device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+  }
+  return RDX_OK;
+}
+
+int rdx_device_free(void* ptr) {
+  if (!ptr) return RDX_OK;
+  hipError_t e = hipFree(ptr);
+  return e == hipSuccess ? RDX_OK : hip_fail(e, "rdx_device_free");
+}
+
+int rdx_device_upload(void* stream, void* dst, const void* src, size_t bytes) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream);
+  if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+  return e == hipSuccess ? RDX_OK : hip_fail(e, "rdx_device_upload");
+}
+
+int rdx_device_download(void* stream, void* dst, const void* src, size_t bytes) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream);
+  if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+  return e == hipSuccess ? RDX_OK : hip_fail(e, "rdx_device_download");
+}
+
+int rdx_device_fill_u32(void* stream, uint32_t* dst, uint32_t value, size_t count) {
+  if (count == 0) return RDX_OK;
+  const uint32_t grid = (uint32_t)std::min<size_t>((count + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(256), 0, (hipStream_t)stream, dst, value, count);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? RDX_OK : hip_fail(e, "rdx_device_fill_u32");
+}
+
+int rdx_stream_synchronize(void* stream) {
+  hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+  return e == hipSuccess ? RDX_OK : hip_fail(e, "rdx_stream_synchronize");
 }
 
 int rdx_context_profile(rdx_context* ctx, int enable) {

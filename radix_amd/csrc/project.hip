@@ -78,10 +78,8 @@ int make_uniforms(const rdx_splat_table* t, const rdx_camera* cam, const rdx_par
   if (!invert(A, Ainv)) return fail(RDX_E_INVALID, "object to head transform is singular");
   if (!invert(cam->head, headInv)) return fail(RDX_E_INVALID, "head matrix is singular");
   for (int k = 0; k < 3; k++) u->h[k] = cm(Ainv, k, 3);
-  for (int r = 0; r < 3; r++) {
+  for (int r = 0; r < 3; r++)
     for (int c = 0; c < 3; c++) u->H[3 * r + c] = cm(A, r, c);
-    u->Ht[r] = cm(A, r, 3);
-  }
   u->eyes = cam->eyes;
   u->head_is_eye0 = 0;
   for (int e = 0; e < cam->eyes; e++) {
@@ -95,12 +93,13 @@ int make_uniforms(const rdx_splat_table* t, const rdx_camera* cam, const rdx_par
       mul(cam->eye[e].view, headInv, F);
       mul(F, A, G);
     }
+    double tr[3];
     for (int r = 0; r < 3; r++) {
       for (int c = 0; c < 3; c++) E.L[3 * r + c] = cm(G, r, c);
-      E.t[r] = cm(G, r, 3);
+      tr[r] = cm(G, r, 3);
     }
-    const double neg_t[3] = {-E.t[0], -E.t[1], -E.t[2]};
-    solve3(E.L, neg_t, E.e);
+    const double neg_t[3] = {-tr[0], -tr[1], -tr[2]};
+    solve3(E.L, neg_t, E.e);  // the eye in object space: L e + t = 0
     const double dl = det3(E.L);
     if (!(dl != 0.0)) return fail(RDX_E_INVALID, "object to eye transform is singular");
     E.det_l2 = (float)(dl * dl);
@@ -110,9 +109,9 @@ int make_uniforms(const rdx_splat_table* t, const rdx_camera* cam, const rdx_par
     for (int k = 0; k < 3; k++) E.ortho_l[k] = (float)ol[k];
     for (int k = 0; k < 9; k++) E.Lf[k] = (float)E.L[k];
     for (int r = 0; r < 4; r++)
-      for (int c = 0; c < 4; c++) E.P[4 * r + c] = cm(cam->eye[e].projection, r, c);
-    E.fx = (float)(cam->size[0] * 0.5 * E.P[0]);
-    E.fy = (float)(cam->size[1] * 0.5 * E.P[5]);
+      for (int c = 0; c < 4; c++) E.Pf[4 * r + c] = (float)cm(cam->eye[e].projection, r, c);
+    E.fx = (float)(cam->size[0] * 0.5 * cm(cam->eye[e].projection, 0, 0));
+    E.fy = (float)(cam->size[1] * 0.5 * cm(cam->eye[e].projection, 1, 1));
     E.ortho = cam->eye[e].ortho ? 1 : 0;
   }
   u->size[0] = (float)cam->size[0];
@@ -143,7 +142,7 @@ TableP make_table(const rdx_splat_table* t) {
   TableP p;
   memset(&p, 0, sizeof p);
   p.count = t->count;
-  p.stride = t->plane_stride ? t->plane_stride : t->count;
+  p.stride = (uint32_t)(t->plane_stride ? t->plane_stride : t->count);
   p.center = t->center;
   p.log_scale = t->log_scale;
   p.quat = t->quat;
@@ -168,6 +167,10 @@ static int check_table(const rdx_splat_table* t) {
       if (deg > 0 && (t->sh_codec != RDX_SH_F32 || !t->sh))
         return fail(RDX_E_INVALID, "f32 table: SH must be f32 planes");
       if (t->plane_stride && t->plane_stride < t->count) return fail(RDX_E_INVALID, "f32 table: plane stride below count");
+      // planes are read with 32-bit offsets: the SH group (45 planes at degree 3) within 4 GiB
+      if ((uint64_t)(t->plane_stride ? t->plane_stride : t->count) * 4u * 45u > 0xffffffffull)
+        return fail(RDX_E_UNSUPPORTED, "f32 table: %u splats per plane exceed the 4 GiB plane group limit",
+                    (unsigned)(t->plane_stride ? t->plane_stride : t->count));
       break;
     case RDX_LAYOUT_EXT:
       if (t->count && (!t->ext_a || !t->ext_b)) return fail(RDX_E_INVALID, "extSplats table: missing words");
@@ -304,14 +307,17 @@ int rdx_project_sort(rdx_context* ctx, void* stream, const rdx_splat_table* tabl
   o.keys = keys;
   o.at = 0;
   uint32_t* hist = (uint32_t*)c->hist.ptr;
-  o.hist = hist;
+  const uint32_t invalid = key_invalid(params->sort_bits);
+  // With float outputs the projection runs its checking variant and the
+  // histograms come from a separate read of the keys.
+  o.hist = side ? nullptr : hist;
   fill_side(o, side, n);
   launch_clear(c, s, hist, pass_grid(p, n));
   {
     Stage st(c, s, "project");
     if (int rc = launch_project(s, table, u, o, side != nullptr, 0)) return rc;
   }
-  const uint32_t invalid = key_invalid(params->sort_bits);
+  if (side) launch_histogram(c, s, p, keys, nullptr, n, hist, 1, invalid);
   return run_passes(c, s, p, keys, nullptr, nullptr, out_indices, n, nullptr, true, invalid, hist, out_count, out_args);
 }
 

@@ -3,27 +3,25 @@
 
 namespace rdx {
 
-template <int SHC, int EYES, bool SIDE, bool HIST>
-static void go(hipStream_t s, dim3 grid, const TableP& t, const ProjU& u, const OutP& o) {
-  hipLaunchKernelGGL((k_project<LAYOUT_EXT, SHC, EYES, SIDE, HIST>), grid, dim3(256), 0, s, t, u, o);
-}
-
-template <int SHC>
-static void pick(hipStream_t s, dim3 grid, const TableP& t, const ProjU& u, const OutP& o, int eyes, bool side,
-                 bool hist) {
-  if (eyes == 1) {
-    if (side) hist ? go<SHC, 1, true, true>(s, grid, t, u, o) : go<SHC, 1, true, false>(s, grid, t, u, o);
-    else hist ? go<SHC, 1, false, true>(s, grid, t, u, o) : go<SHC, 1, false, false>(s, grid, t, u, o);
-  } else {
-    if (side) hist ? go<SHC, 2, true, true>(s, grid, t, u, o) : go<SHC, 2, true, false>(s, grid, t, u, o);
-    else hist ? go<SHC, 2, false, true>(s, grid, t, u, o) : go<SHC, 2, false, false>(s, grid, t, u, o);
-  }
+template <int SHC, int EYES>
+static void pick(hipStream_t s, dim3 grid, const TableP& t, const ProjU& u, const OutP& o, bool side, bool hist) {
+  if (side)
+    hipLaunchKernelGGL((k_project<LAYOUT_EXT, SHC, -1, EYES, true, false>), grid, dim3(256), 0, s, t, u, o);
+  else if (hist)
+    hipLaunchKernelGGL((k_project<LAYOUT_EXT, SHC, -1, EYES, false, true>), grid, dim3(256), 0, s, t, u, o);
+  else
+    hipLaunchKernelGGL((k_project<LAYOUT_EXT, SHC, -1, EYES, false, false>), grid, dim3(256), 0, s, t, u, o);
 }
 
 void launch_project_ext(hipStream_t s, dim3 grid, int shc, const TableP& t, const ProjU& u, const OutP& o, int eyes,
                          bool side, bool hist) {
-  if (shc == SHC_PACKED) pick<SHC_PACKED>(s, grid, t, u, o, eyes, side, hist);
-  else pick<SHC_EXT>(s, grid, t, u, o, eyes, side, hist);
+  if (shc == SHC_PACKED) {
+    if (eyes == 1) pick<SHC_PACKED, 1>(s, grid, t, u, o, side, hist);
+    else pick<SHC_PACKED, 2>(s, grid, t, u, o, side, hist);
+  } else {
+    if (eyes == 1) pick<SHC_EXT, 1>(s, grid, t, u, o, side, hist);
+    else pick<SHC_EXT, 2>(s, grid, t, u, o, side, hist);
+  }
 }
 
 }  // namespace rdx

@@ -1,21 +1,31 @@
-// Projection kernels for f32 structure-of-arrays tables.
+// Projection kernels for f32 structure-of-arrays tables, one per SH degree.
 #include "project_host.hpp"
 
 namespace rdx {
 
-template <int EYES, bool SIDE, bool HIST>
-static void go(hipStream_t s, dim3 grid, const TableP& t, const ProjU& u, const OutP& o) {
-  hipLaunchKernelGGL((k_project<LAYOUT_F32, SHC_F32, EYES, SIDE, HIST>), grid, dim3(256), 0, s, t, u, o);
+template <int DEG, int EYES>
+static void pick(hipStream_t s, dim3 grid, const TableP& t, const ProjU& u, const OutP& o, bool side, bool hist) {
+  if (side)
+    hipLaunchKernelGGL((k_project<LAYOUT_F32, SHC_F32, DEG, EYES, true, false>), grid, dim3(256), 0, s, t, u, o);
+  else if (hist)
+    hipLaunchKernelGGL((k_project<LAYOUT_F32, SHC_F32, DEG, EYES, false, true>), grid, dim3(256), 0, s, t, u, o);
+  else
+    hipLaunchKernelGGL((k_project<LAYOUT_F32, SHC_F32, DEG, EYES, false, false>), grid, dim3(256), 0, s, t, u, o);
 }
 
-void launch_project_f32(hipStream_t s, dim3 grid, const TableP& t, const ProjU& u, const OutP& o, int eyes, bool side,
+template <int DEG>
+static void eyes(hipStream_t s, dim3 grid, const TableP& t, const ProjU& u, const OutP& o, int n, bool side, bool hist) {
+  if (n == 1) pick<DEG, 1>(s, grid, t, u, o, side, hist);
+  else pick<DEG, 2>(s, grid, t, u, o, side, hist);
+}
+
+void launch_project_f32(hipStream_t s, dim3 grid, const TableP& t, const ProjU& u, const OutP& o, int n_eyes, bool side,
                         bool hist) {
-  if (eyes == 1) {
-    if (side) hist ? go<1, true, true>(s, grid, t, u, o) : go<1, true, false>(s, grid, t, u, o);
-    else hist ? go<1, false, true>(s, grid, t, u, o) : go<1, false, false>(s, grid, t, u, o);
-  } else {
-    if (side) hist ? go<2, true, true>(s, grid, t, u, o) : go<2, true, false>(s, grid, t, u, o);
-    else hist ? go<2, false, true>(s, grid, t, u, o) : go<2, false, false>(s, grid, t, u, o);
+  switch (u.sh_degree) {
+    case 0: eyes<0>(s, grid, t, u, o, n_eyes, side, hist); break;
+    case 1: eyes<1>(s, grid, t, u, o, n_eyes, side, hist); break;
+    case 2: eyes<2>(s, grid, t, u, o, n_eyes, side, hist); break;
+    default: eyes<3>(s, grid, t, u, o, n_eyes, side, hist); break;
   }
 }
 

@@ -4,10 +4,11 @@
 // count the key's digits for the radix sort, in one pass over the table.
 //
 // Reads are structure-of-arrays: lane i of a wave reads splat i of each
-// plane (f32) or its uint32x4 words (ext, packed), so every load is one
-// contiguous run per wave. Positions, the eye-space centre, clip
-// coordinates and the key distance are computed in double (a few dozen
-// FMAs); the rest in float. The 2D determinant uses the closed form
+// plane (f32, as buffer loads with one 32-bit lane offset shared by every
+// plane) or its uint32x4 words (ext, packed), so every load is one
+// contiguous run per wave, and all of a splat's loads are issued before any
+// is used. The eye-space centre and the key distance are computed in double
+// (a dozen FMAs); the rest in float. The 2D determinant uses the closed form
 //   det(J S' J^T) = (fx fy / z^3)^2 det(L)^2 (sy^2 sz^2 lx^2 + sx^2 sz^2 ly^2 + sx^2 sy^2 lz^2),
 // l = R^T (c - eye) (Cauchy-Binet with J0 x J1 = fx fy c / z^3), a sum of
 // squares that stays accurate for thin splats seen edge-on, where a*d - b^2
@@ -21,16 +22,16 @@ namespace rdx {
 
 constexpr int LAYOUT_F32 = 0, LAYOUT_EXT = 1, LAYOUT_PACKED = 2;
 constexpr int SHC_NONE = 0, SHC_F32 = 1, SHC_EXT = 2, SHC_PACKED = 3;
+constexpr uint32_t BUFFER_DWORD3 = 0x00020000u;  // raw buffer resource, gfx9 family
 
 struct EyeU {
-  double L[9];  // object -> eye linear part, row-major
-  double t[3];  // object -> eye translation
-  double e[3];  // the eye in object space
-  double P[16]; // projection, row-major
+  double L[9];      // object -> eye linear part, row-major
+  double e[3];      // the eye in object space (eye-space centre = L (c - e))
   float Lf[9];
-  float fx, fy;    // focal length in pixels: size / 2 * (P00, P11)
-  float det_l2;    // det(L)^2
-  float ortho_l[3];// L^-1 (0, 0, 1) (ortho only)
+  float Pf[16];     // projection, row-major
+  float fx, fy;     // focal length in pixels: size / 2 * (P00, P11)
+  float det_l2;     // det(L)^2
+  float ortho_l[3]; // L^-1 (0, 0, 1) (ortho only)
   int ortho;
 };
 
@@ -39,8 +40,7 @@ struct ProjU {
   int eyes;
   int head_is_eye0;   // mono: the head is eye 0 (the key distance is eye 0's)
   double h[3];        // the head in object space: SH origin
-  double H[9];        // object -> head linear part (key distance when not eye 0)
-  double Ht[3];
+  double H[9];        // object -> head linear part (head-space centre = H (c - h))
   float size[2];
   float ndc_z_min;
   float max_std_dev, min_alpha, blur, pre_blur, clip_xy, min_pr, max_pr, min_contrib;
@@ -53,7 +53,7 @@ struct ProjU {
 
 struct TableP {
   uint32_t count;
-  size_t stride;
+  uint32_t stride;         // floats per plane
   const float* center;     // 3 planes
   const float* log_scale;  // 3 planes
   const float* quat;       // 4 planes
@@ -103,18 +103,23 @@ __device__ __forceinline__ uint32_t encode_ext_rgb(float r, float g, float b) {
   if (mx > 0.0f) {
     const uint32_t u = fbits(mx);
     const int ex = (int)(u >> 23);
-    if (ex == 0) {
-      cl = -126;  // subnormal: below 2^-126, the base clamps to 0 anyway
-    } else {
-      cl = ex - 127 + ((u & 0x7fffffu) ? 1 : 0);
-    }
+    cl = ex == 0 ? -126 : ex - 127 + ((u & 0x7fffffu) ? 1 : 0);  // subnormal: the base clamps to 0 anyway
   }
   const int base = min(max(cl + 15, 0), 31);
   // 255 * 2^(15 - base) from the exponent field (142 - base): exact
   const float scale = bitsf((uint32_t)(142 - base) << 23) * 255.0f;
-  const uint32_t mr = (uint32_t)fminf(floorf(ar * scale + 0.5f), 255.0f);
-  const uint32_t mg = (uint32_t)fminf(floorf(ag * scale + 0.5f), 255.0f);
-  const uint32_t mb = (uint32_t)fminf(floorf(ab * scale + 0.5f), 255.0f);
+  // Math.round(|v| * scale) of the exact product: the float product p is off
+  // by e = fma(|v|, scale, -p), |e| <= ulp(p) / 2; its fraction f = p - floor(p)
+  // is exact and on p's grid, as is 0.5, so only f == 0.5 can be decided by e.
+  auto mag = [&](float v) {
+    const float p = v * scale;
+    const float e = fmaf(v, scale, -p);
+    float rr = floorf(p);
+    const float f = p - rr;
+    if (f > 0.5f || (f == 0.5f && e >= 0.0f)) rr += 1.0f;
+    return (uint32_t)fminf(rr, 255.0f);
+  };
+  const uint32_t mr = mag(ar), mg = mag(ag), mb = mag(ab);
   const uint32_t signs = (r < 0.0f ? 1u : 0u) | (g < 0.0f ? 2u : 0u) | (b < 0.0f ? 4u : 0u);
   return mr | (mg << 8) | (mb << 16) | ((((uint32_t)base << 3) | signs) << 24);
 }
@@ -169,16 +174,14 @@ __device__ __forceinline__ uint32_t encode_opacity8(float o) {
 
 // ---- SH (src/gpu/sh.tsl.ts:60-139; basis constants of src/core/sh.ts:381-405) ----
 
-// Adds bands 1..deg of coefficients coef(k) (float3) along unit d.
-template <typename Coef>
-__device__ __forceinline__ float3 sh_eval(float3 d, int deg, Coef coef) {
+// Bands 1..deg of coefficients coef[k] (float3) along unit d.
+__device__ __forceinline__ float3 sh_eval(float3 d, int deg, const float3* coef) {
   const float x = d.x, y = d.y, z = d.z;
   float3 s = make_float3(0.0f, 0.0f, 0.0f);
   auto acc = [&](int k, float w) {
-    const float3 c = coef(k);
-    s.x = fmaf(c.x, w, s.x);
-    s.y = fmaf(c.y, w, s.y);
-    s.z = fmaf(c.z, w, s.z);
+    s.x = fmaf(coef[k].x, w, s.x);
+    s.y = fmaf(coef[k].y, w, s.y);
+    s.z = fmaf(coef[k].z, w, s.z);
   };
   if (deg >= 1) {
     acc(0, -0.4886025f * y);
@@ -220,101 +223,133 @@ struct Splat {
   float op;
 };
 
-template <int LAYOUT>
-__device__ __forceinline__ void load_geometry(const TableP& t, const ProjU& u, uint32_t i, Splat& sp) {
+// Buffer resources of an f32 table's plane groups (one 32-bit lane offset
+// i * 4 serves every plane; a plane's start is a scalar offset).
+struct F32Rsrc {
+  __amdgpu_buffer_rsrc_t c, l, q, o, g, h;
+  uint32_t st4;  // bytes per plane
+};
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const float* p, uint32_t planes, uint32_t st4) {
+  const uint64_t bytes = (uint64_t)planes * st4;
+  return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)(bytes > 0xffffffffull ? 0xffffffffu : (uint32_t)bytes),
+                                           (int)BUFFER_DWORD3);
+}
+
+__device__ __forceinline__ F32Rsrc f32_rsrc(const TableP& t, int deg) {
+  F32Rsrc r;
+  r.st4 = t.stride * 4u;
+  r.c = plane_rsrc(t.center, 3, r.st4);
+  r.l = plane_rsrc(t.log_scale, 3, r.st4);
+  r.q = plane_rsrc(t.quat, 4, r.st4);
+  r.o = plane_rsrc(t.opacity, 1, r.st4);
+  r.g = plane_rsrc(t.rgb, 3, r.st4);
+  const int K = deg >= 3 ? 15 : (deg == 2 ? 8 : (deg == 1 ? 3 : 0));
+  r.h = plane_rsrc(t.sh ? t.sh : t.rgb, (uint32_t)(3 * K), r.st4);
+  return r;
+}
+
+__device__ __forceinline__ float ldp(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __uint_as_float((uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+
+// All of splat i: geometry, base colour and SH bands 1..deg into coef.
+// DEG >= 0: a compile-time degree (f32 planes); DEG < 0: the runtime `deg`.
+template <int LAYOUT, int SHC, int DEG>
+__device__ __forceinline__ void load_splat(const TableP& t, const F32Rsrc& rs, const float* enc, uint32_t i, int deg,
+                                           Splat& sp, float3* coef) {
   if constexpr (LAYOUT == LAYOUT_F32) {
-    const size_t st = t.stride;
-    sp.c = make_float3(t.center[i], t.center[st + i], t.center[2 * st + i]);
-    sp.s = make_float3(__expf(t.log_scale[i]), __expf(t.log_scale[st + i]), __expf(t.log_scale[2 * st + i]));
-    sp.q = make_float4(t.quat[i], t.quat[st + i], t.quat[2 * st + i], t.quat[3 * st + i]);
-    sp.op = t.opacity[i];
-  } else if constexpr (LAYOUT == LAYOUT_EXT) {
-    const uint4 A = t.ext_a[i], B = t.ext_b[i];
-    sp.c = make_float3(bitsf(A.x), bitsf(A.y), bitsf(A.z));
-    sp.op = (float)(A.w >> 24) * (1.0f / 128.0f);
-    sp.rgb = decode_ext_rgb(B.x);
-    sp.s = make_float3(__expf(half_to_float(B.y)), __expf(half_to_float(B.y >> 16)), __expf(half_to_float(B.z)));
-    sp.q = quat_oct_decode(B.w, 10, 12);
+    constexpr int K = DEG >= 3 ? 15 : (DEG == 2 ? 8 : (DEG == 1 ? 3 : 0));
+    const uint32_t v = i * 4u, s = rs.st4;
+    sp.c = make_float3(ldp(rs.c, v, 0), ldp(rs.c, v, s), ldp(rs.c, v, 2 * s));
+    const float l0 = ldp(rs.l, v, 0), l1 = ldp(rs.l, v, s), l2 = ldp(rs.l, v, 2 * s);
+    sp.q = make_float4(ldp(rs.q, v, 0), ldp(rs.q, v, s), ldp(rs.q, v, 2 * s), ldp(rs.q, v, 3 * s));
+    sp.op = ldp(rs.o, v, 0);
+    sp.rgb = make_float3(ldp(rs.g, v, 0), ldp(rs.g, v, s), ldp(rs.g, v, 2 * s));
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      coef[k] = make_float3(ldp(rs.h, v, (3 * k) * s), ldp(rs.h, v, (3 * k + 1) * s), ldp(rs.h, v, (3 * k + 2) * s));
+    sp.s = make_float3(__expf(l0), __expf(l1), __expf(l2));
   } else {
-    const uint4 W = t.packed[i];
-    const float lo = u.enc[0], range = u.enc[1] - u.enc[0];
-    sp.rgb = make_float3(lo + (float)(W.x & 0xffu) * (1.0f / 255.0f) * range,
-                         lo + (float)((W.x >> 8) & 0xffu) * (1.0f / 255.0f) * range,
-                         lo + (float)((W.x >> 16) & 0xffu) * (1.0f / 255.0f) * range);
-    sp.op = (float)(W.x >> 24) * (1.0f / 255.0f);
-    sp.c = make_float3(half_to_float(W.y), half_to_float(W.y >> 16), half_to_float(W.z));
-    const float step = (u.enc[3] - u.enc[2]) * (1.0f / 254.0f);
-    auto sc = [&](uint32_t code) { return code == 0u ? 0.0f : __expf(u.enc[2] + ((float)code - 1.0f) * step); };
-    sp.s = make_float3(sc(W.w & 0xffu), sc((W.w >> 8) & 0xffu), sc((W.w >> 16) & 0xffu));
-    sp.q = quat_oct_decode((W.z >> 16) | ((W.w >> 24) << 16), 8, 8);
+    // uint32x4 words: issue the SH texels' loads with the splat's
+    uint4 tex[4];
+    if constexpr (SHC == SHC_EXT) {
+      const int texels = deg <= 0 ? 0 : (deg == 1 ? 1 : (deg == 2 ? 2 : 4));
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+        if (x < texels) tex[x] = t.shc[x][i];
+    } else if constexpr (SHC == SHC_PACKED) {
+      const int texels = deg <= 0 ? 0 : deg;
+#pragma unroll
+      for (int x = 0; x < 3; x++)
+        if (x < texels) tex[x] = t.shc[x][i];
+    }
+    if constexpr (LAYOUT == LAYOUT_EXT) {
+      const uint4 A = t.ext_a[i], B = t.ext_b[i];
+      sp.c = make_float3(bitsf(A.x), bitsf(A.y), bitsf(A.z));
+      sp.op = (float)(A.w >> 24) * (1.0f / 128.0f);
+      sp.rgb = decode_ext_rgb(B.x);
+      sp.s = make_float3(__expf(half_to_float(B.y)), __expf(half_to_float(B.y >> 16)), __expf(half_to_float(B.z)));
+      sp.q = quat_oct_decode(B.w, 10, 12);
+    } else {
+      const uint4 W = t.packed[i];
+      const float lo = enc[0], range = enc[1] - enc[0];
+      sp.rgb = make_float3(lo + (float)(W.x & 0xffu) * (1.0f / 255.0f) * range,
+                           lo + (float)((W.x >> 8) & 0xffu) * (1.0f / 255.0f) * range,
+                           lo + (float)((W.x >> 16) & 0xffu) * (1.0f / 255.0f) * range);
+      sp.op = (float)(W.x >> 24) * (1.0f / 255.0f);
+      sp.c = make_float3(half_to_float(W.y), half_to_float(W.y >> 16), half_to_float(W.z));
+      const float step = (enc[3] - enc[2]) * (1.0f / 254.0f);
+      auto sc = [&](uint32_t code) { return code == 0u ? 0.0f : __expf(enc[2] + ((float)code - 1.0f) * step); };
+      sp.s = make_float3(sc(W.w & 0xffu), sc((W.w >> 8) & 0xffu), sc((W.w >> 16) & 0xffu));
+      sp.q = quat_oct_decode((W.z >> 16) | ((W.w >> 24) << 16), 8, 8);
+    }
+    if constexpr (SHC == SHC_EXT) {
+      const int K = deg >= 3 ? 15 : (deg == 2 ? 8 : (deg == 1 ? 3 : 0));
+#pragma unroll
+      for (int k = 0; k < 15; k++) {
+        if (k < K) {
+          const uint4 v = tex[k >> 2];
+          const uint32_t w = (k & 3) == 0 ? v.x : ((k & 3) == 1 ? v.y : ((k & 3) == 2 ? v.z : v.w));
+          coef[k] = decode_ext_rgb(w);
+        }
+      }
+    } else if constexpr (SHC == SHC_PACKED) {
+      if (deg > 0) {
+        uint32_t w[12];
+#pragma unroll
+        for (int x = 0; x < 3; x++) {
+          const bool have = x < deg;
+          w[4 * x] = have ? tex[x].x : 0u;
+          w[4 * x + 1] = have ? tex[x].y : 0u;
+          w[4 * x + 2] = have ? tex[x].z : 0u;
+          w[4 * x + 3] = have ? tex[x].w : 0u;
+        }
+        const float sc[3] = {sh_scale(sh_byte(w, 0)), deg >= 2 ? sh_scale(sh_byte(w, 10)) : 0.0f,
+                             deg >= 3 ? sh_scale(sh_byte(w, 26)) : 0.0f};
+#pragma unroll
+        for (int k = 0; k < 15; k++) {
+          const int band = k < 3 ? 0 : (k < 8 ? 1 : 2);
+          const int start = band == 0 ? 0 : (band == 1 ? 10 : 26);
+          const int first = band == 0 ? 0 : (band == 1 ? 3 : 8);
+          const int at = start + 1 + (k - first) * 3;
+          coef[k] = make_float3((float)(int8_t)sh_byte(w, at) * sc[band], (float)(int8_t)sh_byte(w, at + 1) * sc[band],
+                                (float)(int8_t)sh_byte(w, at + 2) * sc[band]);
+        }
+      }
+    }
   }
 }
 
-// SH colour of splat i (base + bands), clamped at 0 (sh.tsl.ts:255-266)
-template <int LAYOUT, int SHC>
-__device__ __forceinline__ float3 shade(const TableP& t, const ProjU& u, uint32_t i, const Splat& sp) {
-  float3 rgb = sp.rgb;
-  if constexpr (LAYOUT == LAYOUT_F32) {
-    const size_t st = t.stride;
-    rgb = make_float3(t.rgb[i], t.rgb[st + i], t.rgb[2 * st + i]);
-  }
-  const int deg = u.sh_degree;
-  if constexpr (SHC == SHC_NONE) {
-    return rgb;
-  } else {
-    if (deg <= 0) return rgb;
-    // direction from the head (object space) to the centre
-    const float dx = (float)((double)sp.c.x - u.h[0]);
-    const float dy = (float)((double)sp.c.y - u.h[1]);
-    const float dz = (float)((double)sp.c.z - u.h[2]);
-    const float rl = rsqrtf(dx * dx + dy * dy + dz * dz);
-    const float3 d = make_float3(dx * rl, dy * rl, dz * rl);
-    float3 s;
-    if constexpr (SHC == SHC_F32) {
-      const size_t st = t.stride;
-      const float* p = t.sh + i;
-      s = sh_eval(d, deg, [&](int k) { return make_float3(p[(3 * k) * st], p[(3 * k + 1) * st], p[(3 * k + 2) * st]); });
-    } else if constexpr (SHC == SHC_EXT) {
-      uint32_t w[16];
-      const int texels = deg == 1 ? 1 : (deg == 2 ? 2 : 4);
-#pragma unroll
-      for (int x = 0; x < 4; x++) {
-        if (x < texels) {
-          const uint4 v = t.shc[x][i];
-          w[4 * x] = v.x;
-          w[4 * x + 1] = v.y;
-          w[4 * x + 2] = v.z;
-          w[4 * x + 3] = v.w;
-        }
-      }
-      s = sh_eval(d, deg, [&](int k) { return decode_ext_rgb(w[k]); });
-    } else {
-      uint32_t w[12];
-      const int texels = deg == 1 ? 1 : (deg == 2 ? 2 : 3);
-#pragma unroll
-      for (int x = 0; x < 3; x++) {
-        if (x < texels) {
-          const uint4 v = t.shc[x][i];
-          w[4 * x] = v.x;
-          w[4 * x + 1] = v.y;
-          w[4 * x + 2] = v.z;
-          w[4 * x + 3] = v.w;
-        }
-      }
-      const float sc1 = sh_scale(sh_byte(w, 0));
-      const float sc2 = deg >= 2 ? sh_scale(sh_byte(w, 10)) : 0.0f;
-      const float sc3 = deg >= 3 ? sh_scale(sh_byte(w, 26)) : 0.0f;
-      s = sh_eval(d, deg, [&](int k) {
-        const int band_start = k < 3 ? 0 : (k < 8 ? 10 : 26);
-        const int first = k < 3 ? 0 : (k < 8 ? 3 : 8);
-        const float sc = k < 3 ? sc1 : (k < 8 ? sc2 : sc3);
-        const int at = band_start + 1 + (k - first) * 3;
-        return make_float3((float)(int8_t)sh_byte(w, at) * sc, (float)(int8_t)sh_byte(w, at + 1) * sc,
-                           (float)(int8_t)sh_byte(w, at + 2) * sc);
-      });
-    }
-    return make_float3(fmaxf(rgb.x + s.x, 0.0f), fmaxf(rgb.y + s.y, 0.0f), fmaxf(rgb.z + s.z, 0.0f));
-  }
+// base + SH(normalize(c - origin)), clamped at 0 (sh.tsl.ts:255-266)
+__device__ __forceinline__ float3 sh_colour(const Splat& sp, const double* origin, int deg, const float3* coef) {
+  if (deg <= 0) return sp.rgb;
+  const float dx = (float)((double)sp.c.x - origin[0]);
+  const float dy = (float)((double)sp.c.y - origin[1]);
+  const float dz = (float)((double)sp.c.z - origin[2]);
+  const float rl = rsqrtf(dx * dx + dy * dy + dz * dz);
+  const float3 s = sh_eval(make_float3(dx * rl, dy * rl, dz * rl), deg, coef);
+  return make_float3(fmaxf(sp.rgb.x + s.x, 0.0f), fmaxf(sp.rgb.y + s.y, 0.0f), fmaxf(sp.rgb.z + s.z, 0.0f));
 }
 
 // Rotation of a normalised quaternion, row-major (math.ts:29-45, math.tsl.ts:21-40)
@@ -339,12 +374,172 @@ __device__ __forceinline__ void quat_matrix(float4 q, float R[9]) {
   R[8] = 1.0f - 2.0f * (x * x + y * y);
 }
 
-struct EyeResult {
-  bool visible;
+// A splat's centre seen from one eye: object-space offset from the eye, the
+// eye-space centre (double, and its distance), clip coordinates, and the
+// centre culls (cullDepth, cullClipXY, projection.tsl.ts:132-140).
+struct EyeCentre {
+  double ce[3];
+  double dist;
+  float3 d;      // c - eye (object space)
+  float clip[4];
+  bool pass;
+};
+
+__device__ __forceinline__ EyeCentre eye_centre(const EyeU& E, float3 c, float ndc_z_min, float clip_xy) {
+  EyeCentre r;
+  const double dx = (double)c.x - E.e[0], dy = (double)c.y - E.e[1], dz = (double)c.z - E.e[2];
+#pragma unroll
+  for (int i = 0; i < 3; i++) r.ce[i] = fma(E.L[3 * i], dx, fma(E.L[3 * i + 1], dy, E.L[3 * i + 2] * dz));
+  r.dist = sqrt(fma(r.ce[0], r.ce[0], fma(r.ce[1], r.ce[1], r.ce[2] * r.ce[2])));
+  r.d = make_float3((float)dx, (float)dy, (float)dz);
+  const float x = (float)r.ce[0], y = (float)r.ce[1], z = (float)r.ce[2];
+#pragma unroll
+  for (int i = 0; i < 4; i++) r.clip[i] = fmaf(E.Pf[4 * i], x, fmaf(E.Pf[4 * i + 1], y, fmaf(E.Pf[4 * i + 2], z, E.Pf[4 * i + 3])));
+  const bool cull_depth = z >= 0.0f || r.clip[2] < ndc_z_min * r.clip[3] || r.clip[2] >= r.clip[3];
+  const float limit = fminf(clip_xy, 2.0f) * r.clip[3];
+  const bool cull_xy = fabsf(r.clip[0]) > limit || fabsf(r.clip[1]) > limit;
+  r.pass = !(cull_depth || cull_xy);
+  return r;
+}
+
+struct EwaResult {
+  bool ok;
+  float a, b, d, det, opc, sc1, sc2, angle, z;
+  float2 ndc;
   uint4 words;
 };
 
-template <int LAYOUT, int SHC, int EYES, bool SIDE, bool HIST>
+// The ewa method for one eye (projections.tsl.ts:95-148 with boundFromCov,
+// projection.tsl.ts:215-240, and the word codecs): `pass` is the eye's
+// centre culls, R the rotation, s the (inflated) scales. U supplies the
+// params (min_alpha, blur, ...).
+template <typename U>
+__device__ __forceinline__ EwaResult ewa_eye(const EyeU& E, const EyeCentre& C, bool pass, const float* R, float3 s,
+                                             float op, float std_dev, float3 rgb, const U& u) {
+  EwaResult r;
+  const float x = (float)C.ce[0], y = (float)C.ce[1], z = (float)C.ce[2];
+  r.z = z;
+  // Q = L R, J rows (projections.tsl.ts:104-117)
+  float Q[9];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      Q[3 * i + j] = fmaf(E.Lf[3 * i], R[j], fmaf(E.Lf[3 * i + 1], R[3 + j], E.Lf[3 * i + 2] * R[6 + j]));
+  float j00, j02, j11, j12, kdet, l0, l1, l2;
+  if (E.ortho) {
+    j00 = E.fx;
+    j02 = 0.0f;
+    j11 = E.fy;
+    j12 = 0.0f;
+    kdet = E.fx * E.fy;
+    l0 = R[0] * E.ortho_l[0] + R[3] * E.ortho_l[1] + R[6] * E.ortho_l[2];
+    l1 = R[1] * E.ortho_l[0] + R[4] * E.ortho_l[1] + R[7] * E.ortho_l[2];
+    l2 = R[2] * E.ortho_l[0] + R[5] * E.ortho_l[1] + R[8] * E.ortho_l[2];
+  } else {
+    const float iz = __builtin_amdgcn_rcpf(z);
+    j00 = E.fx * iz;
+    j11 = E.fy * iz;
+    j02 = -j00 * x * iz;
+    j12 = -j11 * y * iz;
+    kdet = j00 * j11 * iz;  // fx fy / z^3
+    l0 = R[0] * C.d.x + R[3] * C.d.y + R[6] * C.d.z;
+    l1 = R[1] * C.d.x + R[4] * C.d.y + R[7] * C.d.z;
+    l2 = R[2] * C.d.x + R[5] * C.d.y + R[8] * C.d.z;
+  }
+  // T rows = (J Q) diag(s); cov2D = T T^T
+  const float t00 = (j00 * Q[0] + j02 * Q[6]) * s.x, t01 = (j00 * Q[1] + j02 * Q[7]) * s.y,
+              t02 = (j00 * Q[2] + j02 * Q[8]) * s.z;
+  const float t10 = (j11 * Q[3] + j12 * Q[6]) * s.x, t11 = (j11 * Q[4] + j12 * Q[7]) * s.y,
+              t12 = (j11 * Q[5] + j12 * Q[8]) * s.z;
+  const float a0 = t00 * t00 + t01 * t01 + t02 * t02;
+  const float d0 = t10 * t10 + t11 * t11 + t12 * t12;
+  const float b0 = t00 * t10 + t01 * t11 + t02 * t12;
+  const float sxx = s.x * s.x, syy = s.y * s.y, szz = s.z * s.z;
+  const float det0 = kdet * kdet * E.det_l2 * (syy * szz * l0 * l0 + sxx * szz * l1 * l1 + sxx * syy * l2 * l2);
+  // blur and opacity compensation (projections.tsl.ts:119-129), depth of field (projection.tsl.ts:118-126)
+  float blur = u.blur;
+  if (u.focal_distance > 0.0f && u.aperture_tan > 0.0f && z < 0.0f) {
+    const float depth = -z;
+    const float coc_view = fabsf(depth - u.focal_distance) * u.aperture_tan;
+    const float focal_pixels = u.size[1] * 0.5f * E.Pf[5];
+    const float coc = E.ortho ? coc_view * focal_pixels : coc_view * focal_pixels / depth;
+    blur = fminf(fmaxf(coc * coc, blur), u.max_pr * u.max_pr);
+  }
+  const float pre = u.pre_blur;
+  const float ap = a0 + pre, dp = d0 + pre;
+  const float det_orig = det0 + pre * (a0 + d0) + pre * pre;
+  r.a = ap + blur;
+  r.d = dp + blur;
+  r.b = b0;
+  r.det = det_orig + blur * (ap + dp) + blur * blur;
+  r.opc = op * sqrtf(fmaxf(0.0f, det_orig * __builtin_amdgcn_rcpf(r.det)));
+  const float contribution = r.opc * 6.283185307179586f * sqrtf(fmaxf(0.0f, r.det));
+  // boundFromCov (projection.tsl.ts:223-240)
+  const float half = (r.a - r.d) * 0.5f;
+  const float eig1 = (r.a + r.d) * 0.5f + sqrtf(half * half + r.b * r.b);
+  const float eig2 = eig1 > 0.0f ? fmaxf(r.det * __builtin_amdgcn_rcpf(eig1), 0.0f) : 0.0f;
+  r.sc1 = fminf(u.max_pr, std_dev * sqrtf(fmaxf(eig1, 0.0f)));
+  r.sc2 = fminf(u.max_pr, std_dev * sqrtf(eig2));
+  const bool bound_ok = r.sc1 >= u.min_pr || r.sc2 >= u.min_pr;
+  {
+    float vx, vy;
+    if (fabsf(r.a - eig1) >= fabsf(r.d - eig1)) {
+      vx = r.b;
+      vy = eig1 - r.a;
+    } else {
+      vx = eig1 - r.d;
+      vy = r.b;
+    }
+    const float len2 = vx * vx + vy * vy;
+    if (len2 > 0.0f) {
+      const float rl = rsqrtf(len2);
+      vx *= rl;
+      vy *= rl;
+    } else {
+      vx = 1.0f;
+      vy = 0.0f;
+    }
+    r.angle = atan2f(vy, vx);
+    if (r.angle < 0.0f) r.angle += 3.14159265358979323846f;
+  }
+  r.ok = pass && r.opc >= u.min_alpha && contribution >= u.min_contrib && bound_ok;
+  r.ndc = make_float2(0.0f, 0.0f);
+  if (C.clip[3] > 0.0f) {
+    const float iw = 1.0f / C.clip[3];
+    r.ndc = make_float2(C.clip[0] * iw, C.clip[1] * iw);
+  }
+  r.words = make_uint4(0u, 0u, 0u, 0u);
+  if (r.ok) {
+    const float stored = r.opc > 1.0f ? (r.opc + 3.0f) * 0.25f : r.opc;
+    r.words = make_uint4(encode_fixed16(r.ndc.x) | (encode_fixed16(r.ndc.y) << 16),
+                         encode_depth20(z) | (encode_angle12(r.angle) << 20), encode_ext_rgb(rgb.x, rgb.y, rgb.z),
+                         encode_opacity8(stored) | (encode_log_scale12(r.sc1) << 8) | (encode_log_scale12(r.sc2) << 20));
+  }
+  return r;
+}
+
+// prepareSplat (projection.tsl.ts:42-64) and cullInvisible (66-69)
+template <typename U>
+__device__ __forceinline__ void prepare(const Splat& sp, const U& u, float& op, float3& s, float& std_dev,
+                                        bool& invisible) {
+  op = sp.op;
+  float inflate = 1.0f;
+  if (op > 1.0f) {
+    op = fminf(op * 4.0f - 3.0f, 5.0f);
+    if (u.lod_inflate) {
+      inflate = cbrtf(__expf((op * op - 1.0f) * 0.36787944117144233f));
+      op = 1.0f;
+    }
+  }
+  s = make_float3(sp.s.x * inflate, sp.s.y * inflate, sp.s.z * inflate);
+  std_dev = op > 1.0f ? u.max_std_dev + (op - 1.0f) * 0.7f : u.max_std_dev;
+  invisible = op <= 0.0f || op < u.min_alpha || (s.x == 0.0f && s.y == 0.0f && s.z == 0.0f);
+}
+
+// The projection kernel. DEG: the f32 table's SH degree (compile time), or
+// -1 for word tables (u.sh_degree at run time).
+template <int LAYOUT, int SHC, int DEG, int EYES, bool SIDE, bool HIST>
 __global__ __launch_bounds__(256) void k_project(TableP t, ProjU u, OutP o) {
   constexpr int BLOCK = 256;
   constexpr int HEYES = HIST ? EYES : 1;
@@ -356,200 +551,83 @@ __global__ __launch_bounds__(256) void k_project(TableP t, ProjU u, OutP o) {
   const uint32_t n = t.count;
   const uint32_t invalid = key_invalid(u.sort_bits);
   const bool per_eye = EYES == 2 && u.key_per_eye;
+  const int deg = DEG >= 0 ? DEG : u.sh_degree;
+  F32Rsrc rs;
+  if constexpr (LAYOUT == LAYOUT_F32) rs = f32_rsrc(t, deg);
 
   for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
     Splat sp;
-    load_geometry<LAYOUT>(t, u, i, sp);
+    float3 coef[15];
+    load_splat<LAYOUT, SHC, DEG>(t, rs, u.enc, i, deg, sp, coef);
+    float op, std_dev;
+    float3 s;
+    bool invisible;
+    prepare(sp, u, op, s, std_dev, invisible);
 
-    // prepareSplat (projection.tsl.ts:42-64)
-    float op = sp.op, inflate = 1.0f;
-    if (op > 1.0f) {
-      op = fminf(op * 4.0f - 3.0f, 5.0f);
-      if (u.lod_inflate) {
-        inflate = cbrtf(__expf((op * op - 1.0f) * 0.36787944117144233f));
-        op = 1.0f;
-      }
-    }
-    const float3 s = make_float3(sp.s.x * inflate, sp.s.y * inflate, sp.s.z * inflate);
-    const float std_dev = op > 1.0f ? u.max_std_dev + (op - 1.0f) * 0.7f : u.max_std_dev;
-    // cullInvisible (projection.tsl.ts:66-69)
-    const bool invisible = op <= 0.0f || op < u.min_alpha || (s.x == 0.0f && s.y == 0.0f && s.z == 0.0f);
-
-    // Eye-space centres and clip coordinates, in double.
-    double ce[EYES][3], clip[EYES][4];
-    bool pass[EYES];
+    EyeCentre C[EYES];
     bool any = false;
 #pragma unroll
     for (int e = 0; e < EYES; e++) {
-      const EyeU& E = u.eye[e];
-      const double px = (double)sp.c.x, py = (double)sp.c.y, pz = (double)sp.c.z;
-      for (int r = 0; r < 3; r++)
-        ce[e][r] = fma(E.L[3 * r], px, fma(E.L[3 * r + 1], py, fma(E.L[3 * r + 2], pz, E.t[r])));
-      for (int r = 0; r < 4; r++)
-        clip[e][r] = fma(E.P[4 * r], ce[e][0], fma(E.P[4 * r + 1], ce[e][1], fma(E.P[4 * r + 2], ce[e][2], E.P[4 * r + 3])));
-      // cullDepth, cullClipXY (projection.tsl.ts:132-140)
-      const bool cull_depth = ce[e][2] >= 0.0 || clip[e][2] < (double)u.ndc_z_min * clip[e][3] || clip[e][2] >= clip[e][3];
-      const double limit = (double)fminf(u.clip_xy, 2.0f) * clip[e][3];
-      const bool cull_xy = fabs(clip[e][0]) > limit || fabs(clip[e][1]) > limit;
-      pass[e] = !(invisible || cull_depth || cull_xy);
-      any |= pass[e];
+      C[e] = eye_centre(u.eye[e], sp.c, u.ndc_z_min, u.clip_xy);
+      C[e].pass = C[e].pass && !invisible;
+      any |= C[e].pass;
     }
-
     // The key distance: |head-space centre| (per eye with key_per_eye)
-    double dist[EYES];
-#pragma unroll
-    for (int e = 0; e < EYES; e++)
-      dist[e] = sqrt(fma(ce[e][0], ce[e][0], fma(ce[e][1], ce[e][1], ce[e][2] * ce[e][2])));
-    double head_dist = dist[0];
-    double hc[3] = {ce[0][0], ce[0][1], ce[0][2]};
+    double head_dist = C[0].dist;
+    double hc[3] = {C[0].ce[0], C[0].ce[1], C[0].ce[2]};
     if (!u.head_is_eye0) {
-      const double px = (double)sp.c.x, py = (double)sp.c.y, pz = (double)sp.c.z;
-      for (int r = 0; r < 3; r++) hc[r] = fma(u.H[3 * r], px, fma(u.H[3 * r + 1], py, fma(u.H[3 * r + 2], pz, u.Ht[r])));
+      const double dx = (double)sp.c.x - u.h[0], dy = (double)sp.c.y - u.h[1], dz = (double)sp.c.z - u.h[2];
+      for (int r = 0; r < 3; r++) hc[r] = fma(u.H[3 * r], dx, fma(u.H[3 * r + 1], dy, u.H[3 * r + 2] * dz));
       head_dist = sqrt(fma(hc[0], hc[0], fma(hc[1], hc[1], hc[2] * hc[2])));
     }
 
-    EyeResult res[EYES];
-    float R[9];
-    float3 rgb = make_float3(0.0f, 0.0f, 0.0f);
+    uint4 words[EYES];
+    bool vis[EYES];
     bool visible = false;
 #pragma unroll
     for (int e = 0; e < EYES; e++) {
-      res[e].visible = false;
-      res[e].words = make_uint4(0u, 0u, 0u, 0u);
+      words[e] = make_uint4(0u, 0u, 0u, 0u);
+      vis[e] = false;
     }
-
+    float3 rgb = sp.rgb;
+    float R[9];
     if (any || SIDE) {
       quat_matrix(sp.q, R);
-      rgb = shade<LAYOUT, SHC>(t, u, i, sp);
-    }
-    if (any || SIDE) {
+      rgb = sh_colour(sp, u.h, deg, coef);
 #pragma unroll
       for (int e = 0; e < EYES; e++) {
-        if (!pass[e] && !SIDE) continue;
-        const EyeU& E = u.eye[e];
-        const float x = (float)ce[e][0], y = (float)ce[e][1], z = (float)ce[e][2];
-        // Q = L R, J rows (projections.tsl.ts:104-117)
-        float Q[9];
-#pragma unroll
-        for (int r = 0; r < 3; r++)
-#pragma unroll
-          for (int c = 0; c < 3; c++)
-            Q[3 * r + c] = fmaf(E.Lf[3 * r], R[c], fmaf(E.Lf[3 * r + 1], R[3 + c], E.Lf[3 * r + 2] * R[6 + c]));
-        float j00, j02, j11, j12, kdet;
-        float l0, l1, l2;
-        if (E.ortho) {
-          j00 = E.fx; j02 = 0.0f; j11 = E.fy; j12 = 0.0f;
-          kdet = E.fx * E.fy;
-          l0 = R[0] * E.ortho_l[0] + R[3] * E.ortho_l[1] + R[6] * E.ortho_l[2];
-          l1 = R[1] * E.ortho_l[0] + R[4] * E.ortho_l[1] + R[7] * E.ortho_l[2];
-          l2 = R[2] * E.ortho_l[0] + R[5] * E.ortho_l[1] + R[8] * E.ortho_l[2];
-        } else {
-          const float iz = 1.0f / z;
-          j00 = E.fx * iz;
-          j11 = E.fy * iz;
-          j02 = -j00 * x * iz;
-          j12 = -j11 * y * iz;
-          kdet = j00 * j11 * iz;  // fx fy / z^3
-          const float dx = (float)((double)sp.c.x - E.e[0]);
-          const float dy = (float)((double)sp.c.y - E.e[1]);
-          const float dz = (float)((double)sp.c.z - E.e[2]);
-          l0 = R[0] * dx + R[3] * dy + R[6] * dz;
-          l1 = R[1] * dx + R[4] * dy + R[7] * dz;
-          l2 = R[2] * dx + R[5] * dy + R[8] * dz;
-        }
-        // T rows = (J Q) diag(s)
-        const float t00 = (j00 * Q[0] + j02 * Q[6]) * s.x, t01 = (j00 * Q[1] + j02 * Q[7]) * s.y,
-                    t02 = (j00 * Q[2] + j02 * Q[8]) * s.z;
-        const float t10 = (j11 * Q[3] + j12 * Q[6]) * s.x, t11 = (j11 * Q[4] + j12 * Q[7]) * s.y,
-                    t12 = (j11 * Q[5] + j12 * Q[8]) * s.z;
-        const float a0 = t00 * t00 + t01 * t01 + t02 * t02;
-        const float d0 = t10 * t10 + t11 * t11 + t12 * t12;
-        const float b0 = t00 * t10 + t01 * t11 + t02 * t12;
-        const float sxx = s.x * s.x, syy = s.y * s.y, szz = s.z * s.z;
-        const float det0 = kdet * kdet * E.det_l2 * (syy * szz * l0 * l0 + sxx * szz * l1 * l1 + sxx * syy * l2 * l2);
-        // blur and opacity compensation (projections.tsl.ts:119-129)
-        float blur = u.blur;
-        if (u.focal_distance > 0.0f && u.aperture_tan > 0.0f && z < 0.0f) {
-          const float depth = -z;
-          const float coc_view = fabsf(depth - u.focal_distance) * u.aperture_tan;
-          const float focal_pixels = u.size[1] * 0.5f * (float)E.P[5];
-          const float coc = E.ortho ? coc_view * focal_pixels : coc_view * focal_pixels / depth;
-          blur = fminf(fmaxf(coc * coc, blur), u.max_pr * u.max_pr);
-        }
-        const float pre = u.pre_blur;
-        const float ap = a0 + pre, dp = d0 + pre;
-        const float det_orig = det0 + pre * (a0 + d0) + pre * pre;
-        const float a = ap + blur, d = dp + blur, b = b0;
-        const float det = det_orig + blur * (ap + dp) + blur * blur;
-        const float opc = op * sqrtf(fmaxf(0.0f, det_orig / det));
-        const float contribution = opc * 6.283185307179586f * sqrtf(fmaxf(0.0f, det));
-        // boundFromCov (projection.tsl.ts:223-240)
-        const float half = (a - d) * 0.5f;
-        const float eig1 = (a + d) * 0.5f + sqrtf(half * half + b * b);
-        const float eig2 = eig1 > 0.0f ? fmaxf(det / eig1, 0.0f) : 0.0f;
-        const float sc1 = fminf(u.max_pr, std_dev * sqrtf(fmaxf(eig1, 0.0f)));
-        const float sc2 = fminf(u.max_pr, std_dev * sqrtf(eig2));
-        const bool bound_ok = sc1 >= u.min_pr || sc2 >= u.min_pr;
-        float angle = 0.0f;
-        {
-          float vx, vy;
-          if (fabsf(a - eig1) >= fabsf(d - eig1)) {
-            vx = b;
-            vy = eig1 - a;
-          } else {
-            vx = eig1 - d;
-            vy = b;
-          }
-          const float len2 = vx * vx + vy * vy;
-          if (len2 > 0.0f) {
-            const float rl = rsqrtf(len2);
-            vx *= rl;
-            vy *= rl;
-          } else {
-            vx = 1.0f;
-            vy = 0.0f;
-          }
-          angle = atan2f(vy, vx);
-          if (angle < 0.0f) angle += 3.14159265358979323846f;
-        }
-        const bool ok = pass[e] && opc >= u.min_alpha && contribution >= u.min_contrib && bound_ok;
-        float2 ndc = make_float2(0.0f, 0.0f);
-        if (clip[e][3] > 0.0) ndc = make_float2((float)(clip[e][0] / clip[e][3]), (float)(clip[e][1] / clip[e][3]));
-        if (ok) {
-          const float stored = opc > 1.0f ? (opc + 3.0f) * 0.25f : opc;
-          res[e].words = make_uint4(encode_fixed16(ndc.x) | (encode_fixed16(ndc.y) << 16),
-                                    encode_depth20(z) | (encode_angle12(angle) << 20),
-                                    encode_ext_rgb(rgb.x, rgb.y, rgb.z),
-                                    encode_opacity8(stored) | (encode_log_scale12(sc1) << 8) |
-                                        (encode_log_scale12(sc2) << 20));
-          res[e].visible = true;
+        if (!C[e].pass && !SIDE) continue;
+        const EwaResult r = ewa_eye(u.eye[e], C[e], C[e].pass, R, s, op, std_dev, rgb, u);
+        if (r.ok) {
+          words[e] = r.words;
+          vis[e] = true;
           visible = true;
         }
         if constexpr (SIDE) {
           const size_t ss = o.sstride;
           if (o.conic) {
             float* cp = o.conic + (size_t)e * 3 * ss;
-            cp[i] = d / det;
-            cp[ss + i] = -b / det;
-            cp[2 * ss + i] = a / det;
+            cp[i] = r.d / r.det;
+            cp[ss + i] = -r.b / r.det;
+            cp[2 * ss + i] = r.a / r.det;
           }
           if (o.cov2d) {
             float* cp = o.cov2d + (size_t)e * 3 * ss;
-            cp[i] = a;
-            cp[ss + i] = b;
-            cp[2 * ss + i] = d;
+            cp[i] = r.a;
+            cp[ss + i] = r.b;
+            cp[2 * ss + i] = r.d;
           }
-          if (o.opac) o.opac[(size_t)e * ss + i] = opc;
+          if (o.opac) o.opac[(size_t)e * ss + i] = r.opc;
           if (o.ndc) {
-            o.ndc[(size_t)e * 2 * ss + i] = ndc.x;
-            o.ndc[(size_t)e * 2 * ss + ss + i] = ndc.y;
+            o.ndc[(size_t)e * 2 * ss + i] = r.ndc.x;
+            o.ndc[(size_t)e * 2 * ss + ss + i] = r.ndc.y;
           }
-          if (o.vz) o.vz[(size_t)e * ss + i] = z;
+          if (o.vz) o.vz[(size_t)e * ss + i] = r.z;
           if (o.bound) {
             float* bp = o.bound + (size_t)e * 3 * ss;
-            bp[i] = sc1;
-            bp[ss + i] = sc2;
-            bp[2 * ss + i] = angle;
+            bp[i] = r.sc1;
+            bp[ss + i] = r.sc2;
+            bp[2 * ss + i] = r.angle;
           }
         }
       }
@@ -572,15 +650,16 @@ __global__ __launch_bounds__(256) void k_project(TableP t, ProjU u, OutP o) {
         diag[r] = acc;
       }
       const float radius = sqrtf(fmaxf(fmaxf(diag[0], diag[1]), diag[2])) * u.max_std_dev;
-      const double* P0 = u.eye[0].P;
-      const double hw = fma(P0[12], hc[0], fma(P0[13], hc[1], fma(P0[14], hc[2], P0[15])));
-      const double hx = fma(P0[0], hc[0], fma(P0[1], hc[1], fma(P0[2], hc[2], P0[3])));
-      const double hy = fma(P0[4], hc[0], fma(P0[5], hc[1], fma(P0[6], hc[2], P0[7])));
-      const float w = fmaxf((float)hw, 1e-6f);
-      const float rx = (float)P0[0] * radius / w, ry = (float)P0[5] * radius / w;
-      const bool inside = fabsf((float)hx / w) - rx <= u.sort_guard + 1.0f && fabsf((float)hy / w) - ry <= u.sort_guard + 1.0f;
-      const bool near = -(float)hc[2] <= radius;
-      guarded = (float)hc[2] < radius && (inside || near);
+      const float* P0 = u.eye[0].Pf;
+      const float hx0 = (float)hc[0], hy0 = (float)hc[1], hz0 = (float)hc[2];
+      const float hw = P0[12] * hx0 + P0[13] * hy0 + P0[14] * hz0 + P0[15];
+      const float hx = P0[0] * hx0 + P0[1] * hy0 + P0[2] * hz0 + P0[3];
+      const float hy = P0[4] * hx0 + P0[5] * hy0 + P0[6] * hz0 + P0[7];
+      const float w = fmaxf(hw, 1e-6f);
+      const float rx = P0[0] * radius / w, ry = P0[5] * radius / w;
+      const bool inside = fabsf(hx / w) - rx <= u.sort_guard + 1.0f && fabsf(hy / w) - ry <= u.sort_guard + 1.0f;
+      const bool near = -hz0 <= radius;
+      guarded = hz0 < radius && (inside || near);
     }
 
     // Keys (SplatProjection.ts:143-151)
@@ -589,8 +668,8 @@ __global__ __launch_bounds__(256) void k_project(TableP t, ProjU u, OutP o) {
     if (per_eye) {
 #pragma unroll
       for (int e = 0; e < EYES; e++) {
-        const bool ke = u.cull_keys ? res[e].visible : keyed;
-        key[e] = ke ? sort_key(fbits((float)dist[e]), u.sort_bits) : invalid;
+        const bool ke = u.cull_keys ? vis[e] : keyed;
+        key[e] = ke ? sort_key(fbits((float)C[e].dist), u.sort_bits) : invalid;
       }
     } else {
       key[0] = keyed ? sort_key(fbits((float)head_dist), u.sort_bits) : invalid;
@@ -600,7 +679,7 @@ __global__ __launch_bounds__(256) void k_project(TableP t, ProjU u, OutP o) {
     const uint32_t at = o.at + i;
 #pragma unroll
     for (int e = 0; e < EYES; e++)
-      if (o.words[e]) o.words[e][at] = res[e].words;
+      if (o.words[e]) o.words[e][at] = words[e];
     if (o.keys) {
       o.keys[at] = key[0];
       if (per_eye) o.keys[o.key_stride + at] = key[EYES - 1];
@@ -619,7 +698,7 @@ __global__ __launch_bounds__(256) void k_project(TableP t, ProjU u, OutP o) {
       }
       if (o.dist) {
         if (per_eye) {
-          for (int e = 0; e < EYES; e++) o.dist[(size_t)e * ss + i] = (float)dist[e];
+          for (int e = 0; e < EYES; e++) o.dist[(size_t)e * ss + i] = (float)C[e].dist;
         } else {
           o.dist[i] = (float)head_dist;
         }

@@ -39,6 +39,7 @@ constexpr int HIST_WORDS = HIST_RULE + 8;
 // Look-back polls before giving up (each poll sleeps ~64 clocks): far above
 // any real wait, a bound that keeps a broken run from hanging the device.
 constexpr uint32_t SPIN_LIMIT = 1u << 22;
+constexpr int LOOKBACK = 8;  // predecessors read together in the look-back
 
 struct PassArgs {
   const uint32_t* keys_in;
@@ -59,6 +60,15 @@ struct PassArgs {
   uint32_t* count_out;       // tile 0 writes the sorted count here (may be NULL)
   uint32_t* draw_args;       // tile 0 writes [6 * 1024, ceil(count / 1024), 0, 0, 0] (may be NULL)
   uint32_t* error;           // set to 1 if a look-back wait ran out
+  // Segments (independent sorts in one launch; NULL: one segment): tile t
+  // belongs to segment seg_of_tile[t], whose keys start at seg[4 s + 1] in
+  // the input and output (both: the same offset), first tile seg[4 s],
+  // count seg[4 s + 2]; its histograms at hist + s * hist_stride. count_out
+  // and draw_args are per segment (1 and 5 words apart).
+  const uint32_t* seg_of_tile;
+  const uint32_t* seg;
+  uint32_t hist_stride;
+  uint32_t dst_cap;          // output capacity (a guard on every write)
 };
 
 // Zero the histograms, tickets and counters, and the first `rows` status
@@ -140,41 +150,55 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(PassArgs a) {
   const int w = tid / RDX_WAVE;
   const int lane = tid & (RDX_WAVE - 1);
 
-  if (tid == 0) s_misc[0] = add_agent(a.ticket, 1u);
+  if (tid == 0) {
+    const uint32_t t = add_agent(a.ticket, 1u);
+    s_misc[0] = t;
+    s_misc[1] = a.seg_of_tile ? a.seg_of_tile[t] : 0u;
+  }
   for (int d = tid; d < WAVES * BINS; d += BLOCK) (&s_wave[0][0])[d] = 0u;
+  __syncthreads();
+  const uint32_t tile = s_misc[0];
+  const uint32_t sg = s_misc[1];
+  const uint32_t tile0 = a.seg ? a.seg[4 * sg] : 0u;
+  const uint32_t off = a.seg ? a.seg[4 * sg + 1] : 0u;
+  const uint32_t* hist = a.hist + (size_t)sg * a.hist_stride;
 
-  // Global digit starts: exclusive scan of this pass's histogram.
+  // Digit starts of the segment: exclusive scan of this pass's histogram.
   uint32_t total_valid = 0;
   for (int base = 0; base < BINS; base += BLOCK) {
     const int d = base + tid;
-    uint32_t c = d < BINS ? a.hist[d] : 0u;
+    uint32_t c = d < BINS ? hist[d] : 0u;
     uint32_t tot;
     uint32_t ex = block_exclusive_sum<BLOCK>(c, s_scan, &tot);
-    if (d < BINS) s_digit_base[d] = total_valid + ex;
+    if (d < BINS) s_digit_base[d] = off + total_valid + ex;
     total_valid += tot;
   }
   __syncthreads();
-  const uint32_t tile = s_misc[0];
 
   // Zero this tile's row of the other status region for the next pass.
   if (a.status_next)
     for (int d = tid; d < BINS; d += BLOCK) a.status_next[(size_t)tile * BINS + d] = 0u;
 
-  uint32_t n = a.n_dev ? min(*a.n_dev, a.n) : a.n;
-  if (a.use_hist_total) n = min(total_valid, a.n);
+  const uint32_t seg_n = a.seg ? a.seg[4 * sg + 2] : a.n;
+  uint32_t n = a.n_dev ? min(*a.n_dev, seg_n) : seg_n;
+  if (a.use_hist_total) n = min(total_valid, seg_n);
   const uint32_t sorted_count = a.drop || a.use_hist_total ? total_valid : n;
-  if (tile == 0 && tid == 0) {
-    if (a.count_out) *a.count_out = sorted_count;
+  const uint32_t local_tile = tile - tile0;
+  if (local_tile == 0 && tid == 0) {
+    if (a.count_out) a.count_out[sg] = sorted_count;
     if (a.draw_args) {
-      a.draw_args[0] = 6u * 1024u;
-      a.draw_args[1] = (sorted_count + 1023u) / 1024u;
-      a.draw_args[2] = 0u;
-      a.draw_args[3] = 0u;
-      a.draw_args[4] = 0u;
+      uint32_t* args = a.draw_args + 5 * sg;
+      args[0] = 6u * 1024u;
+      args[1] = (sorted_count + 1023u) / 1024u;
+      args[2] = 0u;
+      args[3] = 0u;
+      args[4] = 0u;
     }
   }
-  const uint32_t tile_base = tile * (uint32_t)TILE;
+  const uint32_t tile_base = local_tile * (uint32_t)TILE;
   if (tile_base >= n) return;
+  const uint32_t* keys_in = a.keys_in + off;
+  const uint32_t* vals_in = a.vals_in ? a.vals_in + off : nullptr;
 
   // Load and rank.
   uint32_t keys[KPT], vals[KPT], rank[KPT];
@@ -183,8 +207,8 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(PassArgs a) {
   for (int j = 0; j < KPT; j++) {
     const uint32_t i = wbase + (uint32_t)(j * RDX_WAVE);
     const bool in = i < n;
-    keys[j] = in ? a.keys_in[i] : 0u;
-    vals[j] = in ? (a.vals_in ? a.vals_in[i] : i) : 0u;
+    keys[j] = in ? keys_in[i] : 0u;
+    vals[j] = in ? (vals_in ? vals_in[i] : i) : 0u;
     rank[j] = in && !(a.drop && keys[j] == a.invalid) ? 0u : 0xffffffffu;  // 0xffffffff: not kept
   }
 #pragma unroll
@@ -228,15 +252,33 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(PassArgs a) {
     const uint32_t cnt = s_local[d];
     uint32_t* st = a.status + (size_t)tile * BINS + d;
     uint32_t prefix = 0;
-    if (tile == 0) {
+    if (local_tile == 0) {
       store_agent(st, ST_INC | cnt);
     } else {
       store_agent(st, ST_AGG | cnt);
+      // Look back LOOKBACK predecessors at a time (their loads in flight
+      // together): every tile is resident at once, so the walk to the
+      // nearest inclusive prefix is the latency chain of the pass.
       int k = (int)tile - 1;
       uint32_t spins = 0;
-      while (k >= 0) {
-        const uint32_t v = load_agent(a.status + (size_t)k * BINS + d);
-        if ((v & ~ST_MASK) == 0u) {
+      bool done = false;
+      while (!done && k >= (int)tile0) {
+        uint32_t v[LOOKBACK];
+#pragma unroll
+        for (int j = 0; j < LOOKBACK; j++)
+          v[j] = k - j >= (int)tile0 ? load_agent(a.status + (size_t)(k - j) * BINS + d) : ST_INC;
+        int used = 0;
+#pragma unroll
+        for (int j = 0; j < LOOKBACK; j++) {
+          if (done || used < j) continue;  // stop at the first not-yet-published status
+          const uint32_t f = v[j] & ~ST_MASK;
+          if (f == 0u) continue;
+          prefix += v[j] & ST_MASK;
+          used = j + 1;
+          if (f == ST_INC) done = true;
+        }
+        k -= used;
+        if (!done && used == 0) {
           // Bounded wait: a predecessor that never publishes (a broken
           // invariant, not a normal run) flags an error instead of hanging.
           if (++spins > SPIN_LIMIT) {
@@ -244,11 +286,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(PassArgs a) {
             break;
           }
           __builtin_amdgcn_s_sleep(1);
-          continue;
         }
-        prefix += v & ST_MASK;
-        if ((v & ~ST_MASK) == ST_INC) break;
-        k--;
       }
       store_agent(st, ST_INC | (prefix + cnt));
     }
@@ -287,7 +325,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(PassArgs a) {
   for (uint32_t i = tid; i < tile_kept; i += BLOCK) {
     const uint32_t k = s_keys[i];
     const uint32_t dst = s_out[((k & a.key_mask) >> a.shift) & DMASK] + i;
-    if (dst < a.n) {  // always true when the histograms are consistent
+    if (dst < a.dst_cap) {  // always true when the histograms are consistent
       if (a.keys_out) a.keys_out[dst] = k;
       a.vals_out[dst] = s_vals[i];
     }

@@ -1,0 +1,169 @@
+'use strict';
+// The Node host side of the splat pipeline: the reference's interfaces
+// (src/core/sort.ts SortKeys, src/worker/splatSort.ts sortSplatKeys,
+// src/wgpu/WgpuRadixSort.ts, src/wgpu/WgpuSplatSorter.ts and the
+// projection writer of src/render/SplatProjection.ts) over the N-API
+// binding of libradix_hip.so. Plain ES2018 so it runs on Node 12 as well
+// as under TypeScript (types in index.d.ts).
+const path = require('path');
+
+const native = require(path.join(__dirname, 'radix_napi.node'));
+
+const ROW_ELEMENTS = 1024 * 4;
+
+// TextureLayout.alignCapacity (src/core/layout.ts:70-72)
+function alignCapacity(count) {
+  return Math.max(1, Math.ceil(count / ROW_ELEMENTS)) * ROW_ELEMENTS;
+}
+
+// src/core/sort.ts: keys of distances and their stable sort (on the GPU)
+class SortKeys {
+  static invalid(bits) {
+    return native.keyInvalid(bits);
+  }
+
+  static validMax(bits) {
+    return native.keyValidMax(bits);
+  }
+
+  static compute(distance, bits) {
+    return native.keyCompute(distance, bits);
+  }
+
+  // {keys, count, bits, ordering} -> {ordering, visible}; throws the
+  // reference's Error messages (sort.ts:70-72)
+  static sort({ keys, count, bits, ordering }) {
+    const n = count === undefined ? keys.length : count;
+    const out = ordering || new Uint32Array(alignCapacity(n));
+    const visible = native.sortKeys(keys, n, bits, out);
+    return { ordering: out, visible };
+  }
+}
+SortKeys.KEY16_LO = 95 << 23;
+SortKeys.KEY16_HI = (159 << 23) - 1;
+
+// sortSplatKeys (src/worker/splatSort.ts:17-30): off the main thread (the
+// libuv pool runs the call), the arrays handed back for reuse
+function sortSplatKeys(keys, { bits, ordering } = {}) {
+  const count = keys.length;
+  const reuse = ordering && ordering.length >= alignCapacity(count) ? ordering : new Uint32Array(alignCapacity(count));
+  return native.sortKeysAsync(keys, count, bits, reuse).then((visible) => ({ ordering: reuse, visible, keys }));
+}
+
+// Device memory (the storage of a WgpuBuffer, src/wgpu/WgpuBuffer.ts)
+class DeviceBuffer {
+  constructor(bytes) {
+    this.bytes = bytes;
+    this.handle = native.deviceAlloc(bytes);
+  }
+
+  static fromArray(array) {
+    const b = new DeviceBuffer(array.byteLength);
+    b.upload(array);
+    return b;
+  }
+
+  upload(array, byteOffset = 0) {
+    native.deviceUpload(this.handle, array, byteOffset);
+  }
+
+  // The first `count` uint32 from element `start`
+  read(start = 0, count = this.bytes / 4 - start, Type = Uint32Array) {
+    const out = new Type(count);
+    native.deviceDownload(this.handle, start * out.BYTES_PER_ELEMENT, out);
+    return out;
+  }
+
+  fill(value, count = this.bytes / 4) {
+    native.deviceFillU32(this.handle, value, count);
+  }
+
+  dispose() {
+    native.deviceFree(this.handle);
+  }
+}
+
+// WgpuRadixSort: stable LSD sort of device keys and values, in place
+class RadixSort {
+  constructor({ keys, values }) {
+    this.keys = keys;
+    this.values = values;
+  }
+
+  get capacity() {
+    return Math.min(this.keys.bytes, this.values.bytes) / 4;
+  }
+
+  sort(count, { bits = 32 } = {}) {
+    if (count <= 0) return;
+    native.sortPairs(this.keys.handle, this.values.handle, count, bits);
+  }
+
+  sortIndirect(countBuffer, { bits = 32 } = {}) {
+    native.sortPairsIndirect(this.keys.handle, this.values.handle, countBuffer.handle, this.capacity, bits);
+  }
+
+  dispose() {}
+}
+
+// WgpuSplatSorter: drop invalid keys, order the rest back to front, write
+// the drawIndexedIndirect arguments
+class SplatSorter {
+  constructor(capacity = 1) {
+    this.values = new DeviceBuffer(Math.max(capacity, 1) * 4);
+    this.count = new DeviceBuffer(4);
+    this.indirect = new DeviceBuffer(20);
+  }
+
+  sort(keys, count, bits) {
+    if (this.values.bytes < count * 4) {
+      this.values.dispose();
+      this.values = new DeviceBuffer(count * 4);
+    }
+    native.sortSplats(keys.handle, count, bits, this.values.handle, this.count.handle, this.indirect.handle);
+  }
+
+  readCount() {
+    return Promise.resolve(this.count.read(0, 1)[0]);
+  }
+
+  dispose() {
+    [this.values, this.count, this.indirect].forEach((b) => b.dispose());
+  }
+}
+SplatSorter.QUADS_PER_INSTANCE = 1024;
+
+const LAYOUTS = { f32: 0, extSplats: 1, packedSplats: 2 };
+const SH_CODECS = { none: 0, f32: 1, ext: 2, packed: 3 };
+
+function tableDesc(table) {
+  const d = Object.assign({}, table);
+  d.layout = typeof table.layout === 'string' ? LAYOUTS[table.layout] : table.layout || 0;
+  d.shCodec = typeof table.shCodec === 'string' ? SH_CODECS[table.shCodec] : table.shCodec || 0;
+  const unwrap = (b) => (b && b.handle !== undefined ? b.handle : b);
+  ['planes', 'extA', 'extB', 'packed'].forEach((k) => {
+    if (d[k]) d[k] = unwrap(d[k]);
+  });
+  if (d.shChannels) d.shChannels = d.shChannels.map(unwrap);
+  return d;
+}
+
+// The ewa projection writer (SplatProjection.write) and the per-frame
+// project + GPU sort (SplatRenderer.update): camera as SplatCamera.setViews
+// takes it ({head, eyes: [{view, projection, ortho}], size, ndcZMin},
+// matrices as Matrix4.elements), params as SplatParams' fields
+class SplatProjector {
+  project(table, camera, params, { words, keys, at = 0 }) {
+    native.projectEwa(tableDesc(table), camera, params || {}, words.map((w) => w.handle), keys ? keys.handle : null, at);
+  }
+
+  projectSort(table, camera, params, { words, keys, indices, count, indirect }) {
+    native.projectSort(tableDesc(table), camera, params || {}, (words || []).map((w) => w.handle),
+      keys ? keys.handle : null, indices.handle, count ? count.handle : null, indirect ? indirect.handle : null);
+  }
+}
+
+module.exports = {
+  alignCapacity, SortKeys, sortSplatKeys, DeviceBuffer, RadixSort, SplatSorter, SplatProjector,
+  version: native.version,
+};

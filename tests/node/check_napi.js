@@ -1,0 +1,169 @@
+'use strict';
+// Checks of the Node binding, written like the reference's own checks
+// (test/gpu/renderTests.ts, scripts/check-core.ts): each check returns
+// undefined or a failure message. `node check_napi.js cpu` runs the checks
+// that make no GPU call; `gpu` adds the GPU sorts and projection.
+const path = require('path');
+const radix = require(path.join(__dirname, '..', '..', 'radix_amd', 'napi', 'index.js'));
+const { SortKeys, sortSplatKeys, DeviceBuffer, RadixSort, SplatSorter, SplatProjector, alignCapacity } = radix;
+
+const failures = [];
+async function check(name, fn) {
+  let detail;
+  try {
+    detail = await fn();
+  } catch (error) {
+    detail = String(error && error.message ? error.message : error);
+  }
+  console.log(`${detail === undefined ? 'ok  ' : 'FAIL'}  ${name}${detail ? `  (${detail})` : ''}`);
+  if (detail !== undefined) failures.push(name);
+}
+
+// mulberry32 (test/gpu/renderUtils.ts:23-30)
+function rng(seed) {
+  return () => {
+    seed = (seed + 0x6d2b79f5) | 0;
+    let t = Math.imul(seed ^ (seed >>> 15), 1 | seed);
+    t = (t + Math.imul(t ^ (t >>> 7), 61 | t)) ^ t;
+    return ((t ^ (t >>> 14)) >>> 0) / 4294967296;
+  };
+}
+
+// The stable order of keys (the low `bits` bits; 16 bits ignore the rest) by
+// Array.prototype.sort, stable since V8 7.0
+function stableOrder(keys, count, bits) {
+  const mask = bits === 32 ? 0xffffffff : 2 ** bits - 1;
+  const idx = Array.from({ length: count }, (_, i) => i);
+  idx.sort((a, b) => ((keys[a] & mask) >>> 0) - ((keys[b] & mask) >>> 0));
+  return idx;
+}
+
+function firstMismatch(actual, expected, n) {
+  for (let i = 0; i < n; i += 1) {
+    if (actual[i] !== expected[i]) return `index ${i} is ${actual[i]}, expected ${expected[i]}`;
+  }
+  return undefined;
+}
+
+async function cpuChecks() {
+  await check('sort keys: known answer and invalid keys', () => {
+    if (SortKeys.compute(1, 32) !== 0xc07fffff) return `key of 1: ${SortKeys.compute(1, 32)}`;
+    for (const bits of [16, 24, 32]) {
+      if ([Infinity, NaN, -Infinity].some((d) => SortKeys.compute(d, bits) !== SortKeys.invalid(bits))) return `${bits} bits`;
+    }
+    return SortKeys.validMax(32) === 0xfeffffff && alignCapacity(4097) === 8192 ? undefined : 'validMax / alignCapacity';
+  });
+  await check('SortKeys.sort: the reference errors', () => {
+    try {
+      SortKeys.sort({ keys: new Uint32Array(10), count: 10, bits: 24, ordering: new Uint32Array(3) });
+      return 'no error';
+    } catch (e) {
+      return e.message === 'SortKeys.sort: ordering holds 3 of 10 indices' ? undefined : e.message;
+    }
+  });
+}
+
+async function gpuChecks() {
+  await check('SortKeys.sort equals the stable order: 16/24/32 bits, counts 0, 1, block edges, large', () => {
+    const random = rng(7);
+    for (const bits of [16, 24, 32]) {
+      for (const count of [0, 1, 4095, 4096, 4097, 70000]) {
+        const keys = new Uint32Array(Math.max(count, 1)).map(() => Math.floor(random() * 2 ** bits) >>> 0);
+        const { ordering } = SortKeys.sort({ keys, count, bits });
+        const m = firstMismatch(ordering, stableOrder(keys, count, bits), count);
+        if (m) return `${bits} bits, ${count} keys: ${m}`;
+      }
+    }
+    return undefined;
+  });
+
+  await check('RadixSort (WgpuRadixSort) equals the stable order', () => {
+    const random = rng(8);
+    for (const bits of [16, 24, 32]) {
+      for (const count of [1, 4097, 70000]) {
+        const keys = new Uint32Array(count).map(() => Math.floor(random() * 2 ** bits) >>> 0);
+        const kb = DeviceBuffer.fromArray(keys);
+        const vb = DeviceBuffer.fromArray(new Uint32Array(count).map((_, i) => i));
+        new RadixSort({ keys: kb, values: vb }).sort(count, { bits });
+        const m = firstMismatch(vb.read(0, count), stableOrder(keys, count, bits), count);
+        kb.dispose();
+        vb.dispose();
+        if (m) return `${bits} bits, ${count}: ${m}`;
+      }
+    }
+    return undefined;
+  });
+
+  await check('SplatSorter (WgpuSplatSorter): invalid keys dropped, draw arguments', async () => {
+    const random = rng(11);
+    for (const bits of [16, 24, 32]) {
+      const count = 20000;
+      const keys = new Uint32Array(count).map(() => (random() < 0.3 ? SortKeys.invalid(bits) : Math.floor(random() * 0xff * 2 ** (bits - 8))) >>> 0);
+      const kb = DeviceBuffer.fromArray(keys);
+      const sorter = new SplatSorter(count);
+      sorter.sort(kb, count, bits);
+      const { ordering, visible } = SortKeys.sort({ keys, count, bits });
+      const got = await sorter.readCount();
+      const args = Array.from(sorter.indirect.read(0, 5));
+      const expected = [6 * 1024, Math.ceil(visible / 1024), 0, 0, 0];
+      const m = firstMismatch(sorter.values.read(0, visible), ordering, visible);
+      kb.dispose();
+      sorter.dispose();
+      if (got !== visible || args.join() !== expected.join()) return `${bits}: count ${got} vs ${visible}, args ${args}`;
+      if (m) return `${bits}: ${m}`;
+    }
+    return undefined;
+  });
+
+  await check('sortSplatKeys: the pool and this thread agree', async () => {
+    const random = rng(9);
+    const keys = new Uint32Array(50000).map(() => (random() < 0.1 ? 0xffffff : Math.floor(random() * 0xfeffff)));
+    const here = SortKeys.sort({ keys: keys.slice(), count: keys.length, bits: 24 });
+    const there = await sortSplatKeys(keys.slice(), { bits: 24 });
+    if (here.visible !== there.visible) return `visible ${here.visible} vs ${there.visible}`;
+    return firstMismatch(there.ordering, here.ordering, keys.length) || (there.keys.length === keys.length ? undefined : 'keys not returned');
+  });
+
+  await check('SplatProjector.projectSort: the order of SortKeys.sort on its keys', () => {
+    // 4096 splats in front of a camera at the origin (renderUtils.ts:49-62 ranges)
+    const random = rng(3);
+    const n = 4096;
+    const planes = new Float32Array(14 * n);
+    for (let i = 0; i < n; i += 1) {
+      planes[i] = (random() - 0.5) * 4;
+      planes[n + i] = (random() - 0.5) * 3;
+      planes[2 * n + i] = -2 - random() * 6;
+      for (let k = 3; k < 6; k += 1) planes[k * n + i] = Math.log(0.01 + random() * 0.1);
+      planes[9 * n + i] = 1; // quaternion (0, 0, 0, 1)
+      planes[10 * n + i] = 0.2 + random() * 0.8;
+      for (let k = 11; k < 14; k += 1) planes[k * n + i] = random();
+    }
+    const f = 1 / Math.tan(Math.PI / 6);
+    const near = 0.1;
+    const far = 100;
+    const aspect = 2;
+    // three.js makePerspective, WebGPU clip convention, column-major
+    const projection = [f / aspect, 0, 0, 0, 0, f, 0, 0, 0, 0, -far / (far - near), -1, 0, 0, (-far * near) / (far - near), 0];
+    const view = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1];
+    const camera = { head: view, eyes: [{ view, projection }], size: [128, 64], ndcZMin: 0 };
+    const table = { layout: 'f32', count: n, planes: DeviceBuffer.fromArray(planes) };
+    const words = [new DeviceBuffer(16 * n)];
+    const keys = new DeviceBuffer(4 * n);
+    const indices = new DeviceBuffer(4 * n);
+    const count = new DeviceBuffer(4);
+    const indirect = new DeviceBuffer(20);
+    new SplatProjector().projectSort(table, camera, { sortBits: 32 }, { words, keys, indices, count, indirect });
+    const k = keys.read(0, n);
+    const { ordering, visible } = SortKeys.sort({ keys: k, count: n, bits: 32 });
+    const got = count.read(0, 1)[0];
+    if (got !== visible || visible < n / 2) return `count ${got}, visible ${visible}`;
+    return firstMismatch(indices.read(0, visible), ordering, visible);
+  });
+}
+
+(async () => {
+  await cpuChecks();
+  if (process.argv[2] === 'gpu') await gpuChecks();
+  console.log(failures.length ? `${failures.length} failed` : 'all passed');
+  process.exit(failures.length ? 1 : 0);
+})();

@@ -40,8 +40,11 @@ def _decode_rgb(words):
 
 
 def compare(gpu: dict, ref: dict, eyes: int, per_eye: bool = False, check_rgb_words: bool = True,
-            rgb_sample: int = 20000) -> dict:
-    """Assert parity; returns counts of what was compared and excused."""
+            rgb_sample: int = 20000, exact_rgb: bool = False) -> dict:
+    """Assert parity; returns counts of what was compared and excused.
+
+    exact_rgb: the colours are the stored base colours (no SH), so the
+    encoder sees the same floats and the rgb words must be bit-exact."""
     report = {}
     margin = ref["margin"]
     near = margin < TOL
@@ -77,8 +80,12 @@ def compare(gpu: dict, ref: dict, eyes: int, per_eye: bool = False, check_rgb_wo
         report[f"eye{e}_flips"] = int(flip.size)
         if check_rgb_words and both.size:
             sel = both[:: max(1, both.size // rgb_sample)]
+            if exact_rgb:
+                assert np.array_equal(gw[sel, 2], rw[sel, 2]), f"eye {e}: rgb words differ"
             gr, rr = _decode_rgb(gw[sel]), _decode_rgb(rw[sel])
-            step = np.maximum(np.abs(rr).max(axis=1, keepdims=True), np.abs(gr).max(axis=1, keepdims=True)) / 255
+            # one step of the coarser of the two codes: 2^(base - 15) / 255
+            base = np.maximum(gw[sel, 2] >> 27, rw[sel, 2] >> 27).astype(np.float64)
+            step = np.exp2(base - 15)[:, None] / 255
             assert np.all(np.abs(gr - rr) <= step * 1.0001 + 1e-12), f"eye {e}: rgb word beyond one step"
         # floats
         side = gpu["side"]

@@ -46,7 +46,7 @@ def test_plain_small_parity(cuda):
     print("plain_small 65,536 SH0 32-bit", flush=True)
     g = run(table, rig, prm)
     ref = O.project_f32(planes, 0, O.view_from_rig(rig), O.params(sort_bits=32))
-    rep = compare(g, ref, 1)
+    rep = compare(g, ref, 1, exact_rgb=True)
     assert rep["eye0_compared"] > 30000
     check_order(table, rig, prm, keys_expected=g["keys"][: spec.count])
 
