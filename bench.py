@@ -5,7 +5,7 @@ fraction of HBM peak. Prints one JSON line.
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 
 A step is one rdx_project_sort over the table already resident in HBM:
-fused projection kernel (words, keys, digit histograms), then the radix
+fused projection kernel (words, keys, first-pass tile counts), then the radix
 passes (compaction in the first), writing the visible indices back to front.
 """
 from __future__ import annotations
@@ -186,7 +186,7 @@ def main() -> int:
         "pipeline_bytes_per_splat": B_IN_SH3 + B_OUT + B_SORT,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_project (fused projection + SH + key + digit histograms)",
+                     "kernel": "k_project (fused projection + SH + key + first-pass tile counts)",
                      "bytes_per_splat": B_IN_SH3 + B_OUT},
         "stages_ms": {k: round(v["ms_avg"], 5) for k, v in stages.items()},
         "wall_s_timed": round(wall, 4),

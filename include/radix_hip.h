@@ -233,22 +233,66 @@ typedef struct rdx_float_out {
  *                 projected bundle's channel w<e>, SplatProjection.ts:48-54)
  *   keys_dev      the sort key per splat at [at + i] (sortKeys.key, little-endian)
  *                 (with key_per_eye, eye e's keys at keys_dev + e * key_stride)
- *   hist_dev      optional: per-digit histograms of the valid keys
- *                 (rdx_sort_hist_words() words), the front of a fused sort
  *   side          optional float outputs (NULL: none). */
 int rdx_project_ewa(rdx_context* ctx, void* stream, const rdx_splat_table* table, const rdx_camera* camera,
                     const rdx_params* params, uint32_t at, uint32_t* const* words_dev, uint32_t* keys_dev,
                     size_t key_stride, const rdx_float_out* side);
 
 /* Project and sort in one stream-ordered sequence: the fused kernel writes
- * words, keys and the digit histograms of the valid keys; then the first
- * radix pass drops invalid keys and every pass scatters with a single-pass
- * look-back. Output as rdx_sort_splats for the keys it made. The headline
- * path (SplatRenderer.update's project + sort, SplatRenderer.ts:412-456). */
+ * words, keys and the first radix pass's tile counts of the valid keys;
+ * the first pass drops invalid keys. Output as rdx_sort_splats for the
+ * keys it made (keys_dev NULL: context scratch). The headline path
+ * (SplatRenderer.update's project + sort, SplatRenderer.ts:412-456).
+ * With params->key_per_eye on a stereo camera each eye sorts its own keys
+ * (two segments of n = table->count): eye e's keys at keys_dev + e n, its
+ * indices at out_indices_dev + e n, its count at out_count_dev[e] and its
+ * draw arguments at out_draw_args_dev + 5 e; the buffers hold 2 n, 2 n, 2
+ * and 10 words. */
 int rdx_project_sort(rdx_context* ctx, void* stream, const rdx_splat_table* table, const rdx_camera* camera,
                      const rdx_params* params, uint32_t* const* words_dev, uint32_t* keys_dev,
                      uint32_t* out_indices_dev, uint32_t* out_count_dev, uint32_t* out_draw_args_dev,
                      const rdx_float_out* side);
+
+/* ---- Batched views: many (view, collection) segments in one sequence -----
+ * No reference counterpart: the reference re-projects and re-sorts per
+ * render and per object (SplatProjector.ts:127-137); here every view of a
+ * batch is a segment of one keys array, each table is read once per group
+ * of up to 8 views drawing it, one projection launch covers every table of
+ * a kind and each radix pass sorts every segment at once. Segments follow
+ * the offsets of SplatCollection (SplatCollection.ts:54-65): view v's keys
+ * and indices are [off_v, off_v + n_v) with n_v its table's count. */
+typedef struct rdx_batch rdx_batch;
+#define RDX_BATCH_MAX_VIEWS 4096
+typedef struct rdx_batch_view {
+  uint32_t* words_dev;  /* the view's ewa words, 4 per splat of its table (NULL: not written) */
+  uint32_t table;       /* index into the batch's tables */
+  uint32_t reserved;
+  rdx_camera camera;    /* one eye */
+} rdx_batch_view;
+/* A batch over `tables` and `views` with one rdx_params (sortGuard is not
+ * applied: keys follow cullKeys) and the context's sort tuning at creation.
+ * keys_dev: rdx_batch_total() words for the keys (NULL: the batch owns
+ * them). The tables' device memory must outlive the batch. */
+int rdx_batch_create(rdx_context* ctx, const rdx_splat_table* tables, int num_tables, const rdx_batch_view* views,
+                     int num_views, const rdx_params* params, uint32_t* keys_dev, rdx_batch** out);
+int rdx_batch_destroy(rdx_batch* batch);
+/* New cameras for every view (one each, in view order); waits for the
+ * batch's last run to finish before the device copy changes. */
+int rdx_batch_set_cameras(rdx_batch* batch, const rdx_camera* cameras, int num_views);
+/* Sum of the views' splat counts: the length of the keys and indices. */
+uint64_t rdx_batch_total(const rdx_batch* batch);
+/* View v's first key and index (its segment offset). */
+uint64_t rdx_batch_segment_offset(const rdx_batch* batch, int view);
+/* The device keys of the last run (view v's at rdx_batch_segment_offset(v)). */
+const uint32_t* rdx_batch_keys(const rdx_batch* batch);
+/* Project and sort every view: view v's visible indices back to front at
+ * out_indices_dev + off_v, its count at out_counts_dev[v] and its draw
+ * arguments at out_draw_args_dev + 5 v (either may be NULL). flags
+ * RDX_BATCH_GRAPH: capture the sequence once into a HIP graph (per output
+ * set) and replay it. */
+#define RDX_BATCH_GRAPH 1
+int rdx_batch_run(rdx_batch* batch, void* stream, uint32_t* out_indices_dev, uint32_t* out_counts_dev,
+                  uint32_t* out_draw_args_dev, int flags);
 
 #ifdef __cplusplus
 }

@@ -108,6 +108,19 @@ class FloatOut(C.Structure):
     ]
 
 
+class BatchView(C.Structure):
+    _fields_ = [
+        ("words_dev", C.c_void_p),
+        ("table", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("camera", Camera),
+    ]
+
+
+BATCH_GRAPH = 1
+BATCH_MAX_VIEWS = 4096
+
+
 class RadixError(RuntimeError):
     """An error of the native library, with the reference's message text."""
 
@@ -150,6 +163,15 @@ _SIGNATURES = [
     ("rdx_project_sort", C.c_int,
      [C.c_void_p, C.c_void_p, C.POINTER(SplatTable), C.POINTER(Camera), C.POINTER(Params),
       C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(FloatOut)]),
+    ("rdx_batch_create", C.c_int,
+     [C.c_void_p, C.POINTER(SplatTable), C.c_int, C.POINTER(BatchView), C.c_int, C.POINTER(Params), C.c_void_p,
+      C.POINTER(C.c_void_p)]),
+    ("rdx_batch_destroy", C.c_int, [C.c_void_p]),
+    ("rdx_batch_set_cameras", C.c_int, [C.c_void_p, C.POINTER(Camera), C.c_int]),
+    ("rdx_batch_total", C.c_uint64, [C.c_void_p]),
+    ("rdx_batch_segment_offset", C.c_uint64, [C.c_void_p, C.c_int]),
+    ("rdx_batch_keys", C.c_void_p, [C.c_void_p]),
+    ("rdx_batch_run", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
 ]
 
 _lib = None

@@ -1,0 +1,82 @@
+"""Batched views: many (view, table) segments projected and sorted in one
+sequence (rdx_batch_*). No reference counterpart: the reference re-projects
+and re-sorts per render and per object (src/render/SplatProjector.ts:127-137);
+a segment is what one SplatProjector.project + WgpuSplatSorter.sort of that
+view would produce, at the offsets SplatCollection gives its objects
+(src/render/SplatCollection.ts:54-65).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Optional, Sequence, Tuple
+
+from . import _native as N
+from .camera import CameraRig
+from .projection import DeviceTable, Params
+from .sort import Context, _stream, default_context
+
+
+class Batch:
+    """views: (table index, mono CameraRig) pairs. Outputs per view v:
+    words[v] (its table's count x 4 int32, if words=True), and after run()
+    indices[off_v : off_v + counts[v]] (back to front), counts[v] and
+    args[5 v : 5 v + 5]; keys[off_v : off_v + n_v] hold its sort keys."""
+
+    def __init__(self, tables: Sequence[DeviceTable], views: Sequence[Tuple[int, CameraRig]], params: Params,
+                 words: bool = True, context: Optional[Context] = None, device="cuda"):
+        import torch
+
+        self.context = context or default_context()
+        self.tables = list(tables)
+        self.views = [(int(t), rig) for t, rig in views]
+        self.params = params
+        dev = torch.device(device)
+        self.counts_of_view = [self.tables[t].count for t, _ in self.views]
+        self.total = sum(self.counts_of_view)
+        self.keys = torch.zeros(max(self.total, 1), dtype=torch.int32, device=dev)
+        self.words = ([torch.zeros((max(n, 1), 4), dtype=torch.int32, device=dev) for n in self.counts_of_view]
+                      if words else None)
+        self._tdesc = (N.SplatTable * len(self.tables))(*[t.desc() for t in self.tables])
+        bv = (N.BatchView * len(self.views))()
+        for v, (t, rig) in enumerate(self.views):
+            if len(rig.eyes) != 1:
+                raise ValueError(f"batch view {v}: one eye per view")
+            bv[v].words_dev = self.words[v].data_ptr() if words else None
+            bv[v].table = t
+            bv[v].camera = rig.desc()
+        self._views_desc = bv
+        h = C.c_void_p()
+        N.check(N.lib().rdx_batch_create(self.context.handle, self._tdesc, len(self.tables), bv, len(self.views),
+                                          C.byref(params.desc()), C.c_void_p(self.keys.data_ptr()), C.byref(h)))
+        self._h = h
+        lib = N.lib()
+        assert lib.rdx_batch_total(h) == self.total
+        self.offsets = [int(lib.rdx_batch_segment_offset(h, v)) for v in range(len(self.views))]
+        self.indices = torch.zeros(max(self.total, 1), dtype=torch.int32, device=dev)
+        self.counts = torch.zeros(len(self.views), dtype=torch.int32, device=dev)
+        self.args = torch.zeros(5 * len(self.views), dtype=torch.int32, device=dev)
+
+    def segment(self, v: int) -> Tuple[int, int]:
+        """(offset, splat count) of view v's segment."""
+        return self.offsets[v], self.counts_of_view[v]
+
+    def set_cameras(self, rigs: Sequence[CameraRig]) -> None:
+        cams = (N.Camera * len(rigs))(*[r.desc() for r in rigs])
+        N.check(N.lib().rdx_batch_set_cameras(self._h, cams, len(rigs)))
+        self.views = [(t, r) for (t, _), r in zip(self.views, rigs)]
+
+    def run(self, graph: bool = True, stream=None) -> None:
+        N.check(N.lib().rdx_batch_run(self._h, _stream(stream), C.c_void_p(self.indices.data_ptr()),
+                                      C.c_void_p(self.counts.data_ptr()), C.c_void_p(self.args.data_ptr()),
+                                      N.BATCH_GRAPH if graph else 0))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            N.lib().rdx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -20,13 +20,13 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("RDX_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["api.hip", "project.hip", "project_f32.hip", "project_ext.hip", "project_packed.hip"]
+HIP_SOURCES = ["api.hip", "project.hip", "project_f32.hip", "project_ext.hip", "project_packed.hip", "batch.hip"]
 SYNTH_SOURCES = ["synth.cpp"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 # Packed fp32 ops stay off: ordinary float code only.
 HIP_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-ffp-contract=on",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + os.path.join(ROOT, "include"),
 ]

@@ -116,115 +116,159 @@ bool tuning_supported(const rdx_sort_tuning& t) {
   return rb && shape;
 }
 
+// ---- launches of the pass kernels ---------------------------------------------------
+
+enum class PassKernel { count, scatter };
+
 template <int RB, int BLOCK, int KPT>
-static void launch_pass(const SortPlan& p, uint32_t grid, hipStream_t s, const PassArgs& a) {
-  hipLaunchKernelGGL((k_onesweep<RB, BLOCK, KPT>), dim3(grid), dim3(BLOCK), 0, s, a);
+static void launch_kpass(PassKernel k, uint32_t grid, hipStream_t s, const PassArgs& a) {
+  if (k == PassKernel::count)
+    hipLaunchKernelGGL((k_count<RB, BLOCK, KPT>), dim3(grid), dim3(BLOCK), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_scatter<RB, BLOCK, KPT>), dim3(grid), dim3(BLOCK), 0, s, a);
 }
 
 template <int RB>
-static bool launch_pass_rb(const SortPlan& p, uint32_t grid, hipStream_t s, const PassArgs& a) {
+static bool launch_rb(const SortPlan& p, PassKernel k, uint32_t grid, hipStream_t s, const PassArgs& a) {
   if (p.block == 256) {
     switch (p.kpt) {
-      case 8: launch_pass<RB, 256, 8>(p, grid, s, a); return true;
-      case 12: launch_pass<RB, 256, 12>(p, grid, s, a); return true;
-      case 16: launch_pass<RB, 256, 16>(p, grid, s, a); return true;
-      case 20: launch_pass<RB, 256, 20>(p, grid, s, a); return true;
+      case 8: launch_kpass<RB, 256, 8>(k, grid, s, a); return true;
+      case 12: launch_kpass<RB, 256, 12>(k, grid, s, a); return true;
+      case 16: launch_kpass<RB, 256, 16>(k, grid, s, a); return true;
+      case 20: launch_kpass<RB, 256, 20>(k, grid, s, a); return true;
     }
   } else if (p.block == 512) {
     switch (p.kpt) {
-      case 8: launch_pass<RB, 512, 8>(p, grid, s, a); return true;
-      case 12: launch_pass<RB, 512, 12>(p, grid, s, a); return true;
-      case 16: launch_pass<RB, 512, 16>(p, grid, s, a); return true;
+      case 8: launch_kpass<RB, 512, 8>(k, grid, s, a); return true;
+      case 12: launch_kpass<RB, 512, 12>(k, grid, s, a); return true;
+      case 16: launch_kpass<RB, 512, 16>(k, grid, s, a); return true;
     }
   }
   return false;
 }
 
-static bool launch_onesweep(const SortPlan& p, uint32_t grid, hipStream_t s, const PassArgs& a) {
+static bool launch_pass_kernel(const SortPlan& p, PassKernel k, uint32_t grid, hipStream_t s, const PassArgs& a) {
   switch (p.rb) {
-    case 4: return launch_pass_rb<4>(p, grid, s, a);
-    case 6: return launch_pass_rb<6>(p, grid, s, a);
-    case 8: return launch_pass_rb<8>(p, grid, s, a);
+    case 4: return launch_rb<4>(p, k, grid, s, a);
+    case 6: return launch_rb<6>(p, k, grid, s, a);
+    case 8: return launch_rb<8>(p, k, grid, s, a);
   }
   return false;
 }
 
-uint32_t pass_grid(const SortPlan& p, uint32_t n_cap) {
-  return std::max<uint32_t>(1u, (uint32_t)(((uint64_t)n_cap + p.tile - 1) / p.tile));
+uint32_t tiles_of(const SortPlan& p, uint32_t n) {
+  return std::max<uint32_t>(1u, (uint32_t)(((uint64_t)n + p.tile - 1) / p.tile));
 }
 
-// Scratch for a sort of up to n keys (and `segments` histogram blocks).
-int Context::reserve(const SortPlan& p, uint32_t n, int segments) {
+int SortScratch::reserve(uint32_t n, uint32_t segments, uint32_t tiles) {
   const size_t words = std::max<size_t>(n, 1);
   int rc;
   for (int i = 0; i < 2; i++) {
     if ((rc = keys[i].ensure(words * 4))) return rc;
     if ((rc = vals[i].ensure(words * 4))) return rc;
   }
-  if ((rc = hist.ensure((size_t)HIST_WORDS * 4 * std::max(segments, 1)))) return rc;
-  const size_t rows = pass_grid(p, n);
-  if ((rc = status.ensure(2 * rows * HIST_BINS * 4))) return rc;
-  status_rows = std::max(status_rows, rows);
-  status_rows = status.bytes / (2 * HIST_BINS * 4);
+  if ((rc = cnt.ensure((size_t)(MAX_BINS + 1) * tiles * 4))) return rc;
+  if ((rc = tot.ensure((size_t)TOT_STRIDE * std::max<uint32_t>(segments, 1) * 4))) return rc;
+  if ((rc = nvalid.ensure((size_t)std::max<uint32_t>(segments, 1) * 4))) return rc;
   return RDX_OK;
 }
 
-void launch_clear(Context* ctx, hipStream_t s, uint32_t* hist, uint32_t rows) {
-  Stage st(ctx, s, "clear");
-  const uint32_t words = rows * HIST_BINS;
-  const uint32_t grid = std::min<uint32_t>(std::max<uint32_t>((words + 255) / 256, 1u), 1024u);
-  hipLaunchKernelGGL(k_clear, dim3(grid), dim3(256), 0, s, hist, (uint32_t*)ctx->status.ptr, words);
+struct SmallSegs {
+  uint32_t count;
+  uint32_t tile0[SMALL_SEGMENTS], off[SMALL_SEGMENTS], n[SMALL_SEGMENTS], ntiles[SMALL_SEGMENTS];
+};
+
+static __global__ void k_small_segments(SmallSegs q, uint32_t* seg, uint32_t* seg_of_tile, uint32_t tiles) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < q.count) {
+    seg[4 * i] = q.tile0[i];
+    seg[4 * i + 1] = q.off[i];
+    seg[4 * i + 2] = q.n[i];
+    seg[4 * i + 3] = q.ntiles[i];
+  }
+  for (uint32_t t = i; t < tiles; t += gridDim.x * blockDim.x) {
+    uint32_t s = 0;
+    for (uint32_t k = 1; k < q.count; k++)
+      if (q.tile0[k] <= t) s = k;
+    seg_of_tile[t] = s;
+  }
 }
 
-void launch_histogram(Context* ctx, hipStream_t s, const SortPlan& p, const uint32_t* keys, const uint32_t* n_dev, uint32_t n_cap,
-                      uint32_t* hist, int drop, uint32_t invalid) {
-  Stage st(ctx, s, "histogram");
-  const uint32_t per_block = 256 * 16;
-  const uint32_t grid = std::min<uint32_t>(std::max<uint32_t>((n_cap + per_block - 1) / per_block, 1u), 2048u);
-  hipLaunchKernelGGL(k_histogram<256>, dim3(grid), dim3(256), 0, s, keys, n_dev, n_cap, hist, p.passes, p.rb, drop,
-                     invalid, p.bits);
+int small_segments(SortScratch& sc, hipStream_t s, const SortPlan& p, const uint32_t* off, const uint32_t* n, int count,
+                   SegLayout* out) {
+  if (count < 1 || count > SMALL_SEGMENTS) return fail(RDX_E_INVALID, "small_segments: %d segments", count);
+  SmallSegs q;
+  memset(&q, 0, sizeof q);
+  q.count = (uint32_t)count;
+  uint32_t tiles = 0;
+  for (int k = 0; k < count; k++) {
+    q.tile0[k] = tiles;
+    q.off[k] = off[k];
+    q.n[k] = n[k];
+    q.ntiles[k] = tiles_of(p, n[k]);
+    tiles += q.ntiles[k];
+  }
+  if (int rc = sc.seg.ensure((size_t)(4 * SMALL_SEGMENTS + tiles) * 4)) return rc;
+  uint32_t* seg = (uint32_t*)sc.seg.ptr;
+  const uint32_t grid = std::min<uint32_t>((tiles + 255) / 256, 1024u);
+  hipLaunchKernelGGL(k_small_segments, dim3(grid), dim3(256), 0, s, q, seg, seg + 4 * SMALL_SEGMENTS, tiles);
+  out->seg = seg;
+  out->seg_of_tile = seg + 4 * SMALL_SEGMENTS;
+  out->segments = (uint32_t)count;
+  out->tiles = tiles;
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? RDX_OK : hip_fail(e, "segment table");
 }
 
-// The P digit passes after the histograms are in `hist`: pass 0 reads
-// (keys_in, vals_in or the index), the last writes (final_keys, final_vals);
-// between them the context's two scratch pairs alternate.
-int run_passes(Context* ctx, hipStream_t s, const SortPlan& p, const uint32_t* keys_in, const uint32_t* vals_in,
-               uint32_t* final_keys, uint32_t* final_vals, uint32_t n_cap, const uint32_t* n_dev, bool drop,
-               uint32_t invalid, uint32_t* hist, uint32_t* count_out, uint32_t* draw_args) {
-  const uint32_t grid = pass_grid(p, n_cap);
-  uint32_t* status = (uint32_t*)ctx->status.ptr;
-  const size_t region = ctx->status_rows * HIST_BINS;
+int run_sort(Context* ctx, SortScratch& sc, hipStream_t s, const SortPlan& p, const SegLayout& L,
+             const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* final_keys, uint32_t* final_vals, uint32_t n_cap,
+             const uint32_t* n_dev, bool drop, uint32_t invalid, bool counted0, bool rule, uint32_t* count_out,
+             uint32_t* draw_args) {
+  const int bins = 1 << p.rb;
   const uint32_t* kin = keys_in;
   const uint32_t* vin = vals_in;
   for (int pass = 0; pass < p.passes; pass++) {
     const bool last = pass == p.passes - 1;
     PassArgs a;
+    memset(&a, 0, sizeof a);
     a.keys_in = kin;
     a.vals_in = vin;
-    a.keys_out = last ? final_keys : (uint32_t*)ctx->keys[pass & 1].ptr;
-    a.vals_out = last ? final_vals : (uint32_t*)ctx->vals[pass & 1].ptr;
-    a.n_dev = pass == 0 ? n_dev : (drop ? nullptr : n_dev);
+    a.keys_out = last ? final_keys : (uint32_t*)sc.keys[pass & 1].ptr;
+    a.vals_out = last ? final_vals : (uint32_t*)sc.vals[pass & 1].ptr;
+    a.n_dev = n_dev;
     a.n = n_cap;
-    a.hist = hist + pass * HIST_BINS;
-    a.ticket = hist + HIST_TICKETS + pass;
-    a.status = status + (pass & 1) * region;
-    a.status_next = last ? nullptr : status + ((pass + 1) & 1) * region;
-    a.invalid = invalid;
-    a.drop = pass == 0 && drop;
-    a.use_hist_total = pass > 0 && drop;
+    a.cnt = (uint32_t*)sc.cnt.ptr;
+    a.tot = (uint32_t*)sc.tot.ptr;
+    a.nvalid = (uint32_t*)sc.nvalid.ptr;
+    a.tiles = L.tiles;
+    a.rows = bins + (rule && pass == 0 ? 1 : 0);
     a.shift = pass * p.rb;
     a.key_mask = key_invalid(p.bits);
+    a.invalid = invalid;
+    a.drop = pass == 0 && drop;
+    a.compacted = pass > 0 && drop;
+    a.bits = p.bits;
     a.count_out = last ? count_out : nullptr;
     a.draw_args = last ? draw_args : nullptr;
-    a.error = hist + HIST_ERROR;
-    a.seg_of_tile = nullptr;
-    a.seg = nullptr;
-    a.hist_stride = 0;
+    a.seg_of_tile = L.seg_of_tile;
+    a.seg = L.seg;
     a.dst_cap = n_cap;
     char name[32];
-    snprintf(name, sizeof name, "sort_pass%d", pass);
-    Stage st(ctx, s, name);
-    if (!launch_onesweep(p, grid, s, a)) return fail(RDX_E_INVALID, "unsupported sort tuning");
+    if (!(pass == 0 && counted0)) {
+      snprintf(name, sizeof name, "sort_count%d", pass);
+      Stage st(ctx, s, name);
+      if (!launch_pass_kernel(p, PassKernel::count, L.tiles, s, a)) return fail(RDX_E_INVALID, "unsupported sort tuning");
+    }
+    {
+      snprintf(name, sizeof name, "sort_scan%d", pass);
+      Stage st(ctx, s, name);
+      hipLaunchKernelGGL(k_scan, dim3(L.segments * (uint32_t)a.rows), dim3(SCAN_BLOCK), 0, s, a, bins);
+    }
+    {
+      snprintf(name, sizeof name, "sort_scatter%d", pass);
+      Stage st(ctx, s, name);
+      if (!launch_pass_kernel(p, PassKernel::scatter, L.tiles, s, a)) return fail(RDX_E_INVALID, "unsupported sort tuning");
+    }
     kin = a.keys_out;
     vin = a.vals_out;
   }
@@ -301,7 +345,7 @@ int rdx_context_destroy(rdx_context* ctx) {
 int rdx_context_reserve(rdx_context* ctx, uint32_t max_keys) {
   if (!ctx) return fail(RDX_E_INVALID, "rdx_context_reserve: no context");
   Context* c = reinterpret_cast<Context*>(ctx);
-  return c->reserve(make_plan(c->tuning, 32), max_keys, 1);
+  return c->sort.reserve(max_keys, 1, tiles_of(make_plan(c->tuning, 32), max_keys));
 }
 
 int rdx_context_set_tuning(rdx_context* ctx, const rdx_sort_tuning* t) {
@@ -425,23 +469,24 @@ int rdx_sort_keys(rdx_context* ctx, const uint32_t* keys, uint32_t count, int bi
     return RDX_OK;
   }
   const SortPlan p = make_plan(c->tuning, bits);
+  SegLayout L;
+  L.tiles = tiles_of(p, count);
   int rc;
-  if ((rc = c->reserve(p, count, 1))) return rc;
+  if ((rc = c->sort.reserve(count, 1, L.tiles))) return rc;
   if ((rc = c->io_keys.ensure((size_t)count * 4))) return rc;
   if ((rc = c->io_vals.ensure((size_t)count * 4))) return rc;
   hipStream_t s = nullptr;
   uint32_t* dkeys = (uint32_t*)c->io_keys.ptr;
   uint32_t* dord = (uint32_t*)c->io_vals.ptr;
-  uint32_t* hist = (uint32_t*)c->hist.ptr;
   hipError_t e = hipMemcpyAsync(dkeys, keys, (size_t)count * 4, hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return hip_fail(e, "SortKeys.sort upload");
-  launch_clear(c, s, hist, pass_grid(p, count));
-  launch_histogram(c, s, p, dkeys, nullptr, count, hist, 0, 0u);
-  if ((rc = run_passes(c, s, p, dkeys, nullptr, nullptr, dord, count, nullptr, false, 0u, hist, nullptr, nullptr)))
+  if ((rc = run_sort(c, c->sort, s, p, L, dkeys, nullptr, nullptr, dord, count, nullptr, false, 0u, false, true, nullptr,
+                     nullptr)))
     return rc;
   uint32_t hidden = 0;
   e = hipMemcpyAsync(ordering, dord, (size_t)count * 4, hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(&hidden, hist + HIST_RULE, 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&hidden, (uint32_t*)c->sort.tot.ptr + MAX_BINS, 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) return hip_fail(e, "SortKeys.sort download");
   if (visible) *visible = count - hidden;
@@ -455,12 +500,11 @@ static int sort_pairs_impl(rdx_context* ctx, void* stream, uint32_t* keys, uint3
   Context* c = reinterpret_cast<Context*>(ctx);
   if (n == 0) return RDX_OK;
   const SortPlan p = make_plan(c->tuning, bits);
-  if (int rc = c->reserve(p, n, 1)) return rc;
-  hipStream_t s = (hipStream_t)stream;
-  uint32_t* hist = (uint32_t*)c->hist.ptr;
-  launch_clear(c, s, hist, pass_grid(p, n));
-  launch_histogram(c, s, p, keys, count_dev, n, hist, 0, 0u);
-  return run_passes(c, s, p, keys, vals, keys, vals, n, count_dev, false, 0u, hist, nullptr, nullptr);
+  SegLayout L;
+  L.tiles = tiles_of(p, n);
+  if (int rc = c->sort.reserve(n, 1, L.tiles)) return rc;
+  return run_sort(c, c->sort, (hipStream_t)stream, p, L, keys, vals, keys, vals, n, count_dev, false, 0u, false, false,
+                  nullptr, nullptr);
 }
 
 int rdx_sort_pairs(rdx_context* ctx, void* stream, uint32_t* keys, uint32_t* vals, uint32_t count, int bits) {
@@ -485,12 +529,11 @@ int rdx_sort_splats(rdx_context* ctx, void* stream, const uint32_t* keys, uint32
     return e == hipSuccess ? RDX_OK : hip_fail(e, "WgpuSplatSorter");
   }
   const SortPlan p = make_plan(c->tuning, bits);
-  if (int rc = c->reserve(p, n, 1)) return rc;
-  uint32_t* hist = (uint32_t*)c->hist.ptr;
-  const uint32_t invalid = key_invalid(bits);
-  launch_clear(c, s, hist, pass_grid(p, n));
-  launch_histogram(c, s, p, keys, nullptr, n, hist, 1, invalid);
-  return run_passes(c, s, p, keys, nullptr, nullptr, out_indices, n, nullptr, true, invalid, hist, out_count, out_args);
+  SegLayout L;
+  L.tiles = tiles_of(p, n);
+  if (int rc = c->sort.reserve(n, 1, L.tiles)) return rc;
+  return run_sort(c, c->sort, s, p, L, keys, nullptr, nullptr, out_indices, n, nullptr, true, key_invalid(bits), false,
+                  false, out_count, out_args);
 }
 
 }  // extern "C"

@@ -29,25 +29,37 @@ struct StageRecord {
   hipEvent_t start, end;
 };
 
-struct Context {
-  rdx_sort_tuning tuning;
-  bool profiling = false;
-  std::vector<StageRecord> records;  // recorded, not yet read
-  std::vector<hipEvent_t> event_pool;
-  Buffer keys[2], vals[2];  // the passes' alternating scratch pairs
-  Buffer hist;              // digit histograms, tickets, counters
-  Buffer status;            // two look-back status regions
-  size_t status_rows = 0;   // rows per region
-  Buffer io_keys, io_vals;  // host-array sorts: device copies
-  Buffer proj;              // projection scratch (keys of the fused path)
-  int reserve(const SortPlan& p, uint32_t n, int segments);
+// Device scratch of sorts: the passes' alternating (key, value) buffers
+// and the tile counts, digit totals and kept counts.
+struct SortScratch {
+  Buffer keys[2], vals[2];
+  Buffer cnt;     // [MAX_BINS + 1][tiles]
+  Buffer tot;     // [segments][TOT_STRIDE]
+  Buffer nvalid;  // [segments]
+  Buffer seg;     // segment table [segments][4] and tile map [tiles] (small segmented sorts)
+  int reserve(uint32_t n, uint32_t segments, uint32_t tiles);
   void release() {
     for (int i = 0; i < 2; i++) {
       keys[i].release();
       vals[i].release();
     }
-    hist.release();
-    status.release();
+    cnt.release();
+    tot.release();
+    nvalid.release();
+    seg.release();
+  }
+};
+
+struct Context {
+  rdx_sort_tuning tuning;
+  bool profiling = false;
+  std::vector<StageRecord> records;  // recorded, not yet read
+  std::vector<hipEvent_t> event_pool;
+  SortScratch sort;
+  Buffer io_keys, io_vals;  // host-array sorts: device copies
+  Buffer proj;              // projection scratch (keys of the fused path)
+  void release() {
+    sort.release();
     io_keys.release();
     io_vals.release();
     proj.release();
@@ -60,6 +72,14 @@ struct Context {
     event_pool.clear();
   }
   hipEvent_t take_event();
+};
+
+// Segments of a sort on the device (NULL tables: one segment of all tiles).
+struct SegLayout {
+  const uint32_t* seg_of_tile = nullptr;
+  const uint32_t* seg = nullptr;  // [segments][4]: tile0, off, n, ntiles
+  uint32_t segments = 1;
+  uint32_t tiles = 1;
 };
 
 // Brackets the kernels launched in its scope with events when profiling.
@@ -77,12 +97,20 @@ int hip_fail(hipError_t e, const char* what);
 int check_bits(int bits, const char* who);
 SortPlan make_plan(const rdx_sort_tuning& t, int bits);
 bool tuning_supported(const rdx_sort_tuning& t);
-uint32_t pass_grid(const SortPlan& p, uint32_t n_cap);
-void launch_clear(Context* ctx, hipStream_t s, uint32_t* hist, uint32_t rows);
-void launch_histogram(Context* ctx, hipStream_t s, const SortPlan& p, const uint32_t* keys, const uint32_t* n_dev, uint32_t n_cap,
-                      uint32_t* hist, int drop, uint32_t invalid);
-int run_passes(Context* ctx, hipStream_t s, const SortPlan& p, const uint32_t* keys_in, const uint32_t* vals_in,
-               uint32_t* final_keys, uint32_t* final_vals, uint32_t n_cap, const uint32_t* n_dev, bool drop,
-               uint32_t invalid, uint32_t* hist, uint32_t* count_out, uint32_t* draw_args);
+// Tiles of a segment of n keys (at least one, which writes its counts).
+uint32_t tiles_of(const SortPlan& p, uint32_t n);
+// A segment table of up to 8 segments written on the device by a kernel
+// (safe under stream capture) into the scratch's seg buffer.
+constexpr int SMALL_SEGMENTS = 8;
+int small_segments(SortScratch& sc, hipStream_t s, const SortPlan& p, const uint32_t* off, const uint32_t* n, int count,
+                   SegLayout* out);
+// The P digit passes: pass 0 reads (keys_in, vals_in or the index), the last
+// writes (final_keys, final_vals); between them the scratch pairs alternate.
+// counted0: pass 0's tile counts are already in sc.cnt (the fused
+// projection); rule: count the host sort's hidden keys into tot[s][MAX_BINS].
+int run_sort(Context* ctx, SortScratch& sc, hipStream_t s, const SortPlan& p, const SegLayout& L,
+             const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* final_keys, uint32_t* final_vals, uint32_t n_cap,
+             const uint32_t* n_dev, bool drop, uint32_t invalid, bool counted0, bool rule, uint32_t* count_out,
+             uint32_t* draw_args);
 
 }  // namespace rdx

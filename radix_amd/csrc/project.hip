@@ -156,7 +156,7 @@ TableP make_table(const rdx_splat_table* t) {
   return p;
 }
 
-static int check_table(const rdx_splat_table* t) {
+int check_table(const rdx_splat_table* t) {
   if (!t) return fail(RDX_E_INVALID, "no splat table");
   const int deg = t->sh_codec == RDX_SH_NONE ? 0 : t->sh_degree;
   if (deg < 0 || deg > 3) return fail(RDX_E_UNSUPPORTED, "SH degree %d: degrees 0 to 3 are supported", t->sh_degree);
@@ -195,16 +195,18 @@ static int check_table(const rdx_splat_table* t) {
 
 int launch_project(hipStream_t s, const rdx_splat_table* t, const ProjU& u, const OutP& o, bool side, int blocks) {
   const uint32_t n = t->count;
-  if (n == 0) return RDX_OK;
-  const bool hist = o.hist != nullptr;
-  uint32_t grid = (n + 255) / 256;
-  grid = std::min<uint32_t>(grid, (uint32_t)(blocks > 0 ? blocks : (hist ? 1024 : 4096)));
+  const bool count = o.cnt != nullptr;
+  if (n == 0 && !count) return RDX_OK;
+  // Counting: one block per sort tile (at least one, which writes its zero
+  // counts); otherwise a grid-stride launch.
+  uint32_t grid = count ? std::max<uint32_t>(1u, (uint32_t)(((uint64_t)n + u.tile - 1) / u.tile))
+                        : std::min<uint32_t>((n + 255) / 256, (uint32_t)(blocks > 0 ? blocks : 4096));
   const TableP tp = make_table(t);
   const int shc = t->sh_codec == RDX_SH_PACKED ? SHC_PACKED : SHC_EXT;
   switch (t->layout) {
-    case RDX_LAYOUT_F32_SOA: launch_project_f32(s, dim3(grid), tp, u, o, u.eyes, side, hist); break;
-    case RDX_LAYOUT_EXT: launch_project_ext(s, dim3(grid), shc, tp, u, o, u.eyes, side, hist); break;
-    case RDX_LAYOUT_PACKED: launch_project_packed(s, dim3(grid), shc, tp, u, o, u.eyes, side, hist); break;
+    case RDX_LAYOUT_F32_SOA: launch_project_f32(s, dim3(grid), tp, u, o, u.eyes, side, count); break;
+    case RDX_LAYOUT_EXT: launch_project_ext(s, dim3(grid), shc, tp, u, o, u.eyes, side, count); break;
+    case RDX_LAYOUT_PACKED: launch_project_packed(s, dim3(grid), shc, tp, u, o, u.eyes, side, count); break;
   }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? RDX_OK : hip_fail(e, "projection launch");
@@ -280,45 +282,49 @@ int rdx_project_sort(rdx_context* ctx, void* stream, const rdx_splat_table* tabl
   if (!camera || !params) return fail(RDX_E_INVALID, "rdx_project_sort: NULL camera or params");
   if (int rc = check_table(table)) return rc;
   if (int rc = check_bits(params->sort_bits, "WgpuRadixSort")) return rc;
-  if (params->key_per_eye && camera->eyes == 2)
-    return fail(RDX_E_UNSUPPORTED, "rdx_project_sort: per-eye keys sort as segments (rdx_batch_*)");
   if (!out_indices) return fail(RDX_E_INVALID, "rdx_project_sort: no output indices");
   Context* c = reinterpret_cast<Context*>(ctx);
   hipStream_t s = (hipStream_t)stream;
   const uint32_t n = table->count;
-  if (n == 0) {
-    hipLaunchKernelGGL(k_empty_outputs, dim3(1), dim3(64), 0, s, out_count, out_args);
-    hipError_t e = hipGetLastError();
-    return e == hipSuccess ? RDX_OK : hip_fail(e, "rdx_project_sort");
-  }
+  // Per-eye keys (stereo): two segments, eye e's keys, indices, count and
+  // draw arguments at keys + e n, out_indices + e n, out_count + e and
+  // out_args + 5 e.
+  const int segs = params->key_per_eye && camera->eyes == 2 ? 2 : 1;
+  if ((uint64_t)n * segs > 0xffffffffull) return fail(RDX_E_CAPACITY, "rdx_project_sort: %u splats per eye", n);
   ProjU u;
   if (int rc = make_uniforms(table, camera, params, &u)) return rc;
   const SortPlan p = make_plan(c->tuning, params->sort_bits);
-  if (int rc = c->reserve(p, n, 1)) return rc;
+  SegLayout L;
+  const uint32_t nt = tiles_of(p, n);
+  if (int rc = c->sort.reserve(n * segs, segs, nt * segs)) return rc;
+  if (segs == 2) {
+    const uint32_t off[2] = {0u, n}, cnt[2] = {n, n};
+    if (int rc = small_segments(c->sort, s, p, off, cnt, 2, &L)) return rc;
+  } else {
+    L.tiles = nt;
+  }
   if (!keys) {
-    if (int rc = c->proj.ensure((size_t)n * 4)) return rc;
+    if (int rc = c->proj.ensure((size_t)n * segs * 4)) return rc;
     keys = (uint32_t*)c->proj.ptr;
   }
-  u.passes = p.passes;
   u.rb = p.rb;
+  u.tile = (uint32_t)p.tile;
   OutP o;
   memset(&o, 0, sizeof o);
   for (int e = 0; e < camera->eyes; e++) o.words[e] = words ? (uint4*)words[e] : nullptr;
   o.keys = keys;
+  o.key_stride = n;
   o.at = 0;
-  uint32_t* hist = (uint32_t*)c->hist.ptr;
-  const uint32_t invalid = key_invalid(params->sort_bits);
-  // With float outputs the projection runs its checking variant and the
-  // histograms come from a separate read of the keys.
-  o.hist = side ? nullptr : hist;
+  o.cnt = (uint32_t*)c->sort.cnt.ptr;
+  o.tiles = L.tiles;
+  o.eye_tile = nt;
   fill_side(o, side, n);
-  launch_clear(c, s, hist, pass_grid(p, n));
   {
     Stage st(c, s, "project");
     if (int rc = launch_project(s, table, u, o, side != nullptr, 0)) return rc;
   }
-  if (side) launch_histogram(c, s, p, keys, nullptr, n, hist, 1, invalid);
-  return run_passes(c, s, p, keys, nullptr, nullptr, out_indices, n, nullptr, true, invalid, hist, out_count, out_args);
+  return run_sort(c, c->sort, s, p, L, keys, nullptr, nullptr, out_indices, n * segs, nullptr, true,
+                  key_invalid(params->sort_bits), true, false, out_count, out_args);
 }
 
 }  // extern "C"

@@ -15,15 +15,19 @@ int make_uniforms(const rdx_splat_table* t, const rdx_camera* cam, const rdx_par
 // Kernel pointers of a table.
 TableP make_table(const rdx_splat_table* t);
 
-// Launch the projection of a table: `hist` NULL for no histograms.
+// A table descriptor's checks (layout, pointers, SH codec, sizes).
+int check_table(const rdx_splat_table* t);
+
+// Launch the projection of a table; with o.cnt, one block per sort tile
+// writing the sort's first-pass tile counts.
 int launch_project(hipStream_t s, const rdx_splat_table* t, const ProjU& u, const OutP& o, bool side, int blocks);
 
 // The kernels, instantiated per layout in their own translation units.
 void launch_project_f32(hipStream_t s, dim3 grid, const TableP& t, const ProjU& u, const OutP& o, int eyes, bool side,
-                        bool hist);
+                        bool count);
 void launch_project_ext(hipStream_t s, dim3 grid, int shc, const TableP& t, const ProjU& u, const OutP& o, int eyes,
-                        bool side, bool hist);
+                        bool side, bool count);
 void launch_project_packed(hipStream_t s, dim3 grid, int shc, const TableP& t, const ProjU& u, const OutP& o, int eyes,
-                           bool side, bool hist);
+                           bool side, bool count);
 
 }  // namespace rdx

@@ -182,27 +182,34 @@ class Projector:
 
     def project_sort(self, table: DeviceTable, rig: CameraRig, params: Params, words=None, keys=None, indices=None,
                      count=None, args=None, side: bool = False, stream=None):
-        """The fused path: words, keys, the visible indices back to front, count and draw arguments."""
+        """The fused path: words, keys, the visible indices back to front, count and draw arguments.
+
+        With per-eye keys (stereo, params.key_per_eye) each eye is its own
+        sort: eye e's keys and indices at [e n, (e + 1) n), its count at
+        count[e] and its draw arguments at args[5 e:5 e + 5]."""
         import torch
 
         n = table.count
         eyes = len(rig.eyes)
+        segs = 2 if (eyes == 2 and params.key_per_eye) else 1
         dev = torch.device("cuda")
         if words is None:
             words = [torch.zeros((max(n, 1), 4), dtype=torch.int32, device=dev) for _ in range(eyes)]
         if keys is None:
-            keys = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
+            keys = torch.zeros(max(n * segs, 1), dtype=torch.int32, device=dev)
         if indices is None:
-            indices = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
+            indices = torch.zeros(max(n * segs, 1), dtype=torch.int32, device=dev)
         if count is None:
-            count = torch.zeros(1, dtype=torch.int32, device=dev)
+            count = torch.zeros(segs, dtype=torch.int32, device=dev)
         if args is None:
-            args = torch.zeros(5, dtype=torch.int32, device=dev)
+            args = torch.zeros(5 * segs, dtype=torch.int32, device=dev)
+        if keys.numel() < n * segs or indices.numel() < n * segs or count.numel() < segs or args.numel() < 5 * segs:
+            raise ValueError(f"project_sort: outputs too small for {segs} segment(s) of {n} splats")
         wp = (C.c_void_p * 2)(*[w.data_ptr() for w in words] + [None] * (2 - eyes))
         fo = None
         out = {}
         if side:
-            sd, fo = self._side(n, eyes, False, dev)
+            sd, fo = self._side(n, eyes, segs == 2, dev)
             out["side"] = sd
         N.check(N.lib().rdx_project_sort(self.context.handle, _stream(stream), C.byref(table.desc()),
                                          C.byref(rig.desc()), C.byref(params.desc()), wp, _ptr(keys), _ptr(indices),

@@ -67,3 +67,40 @@ def test_tuning_validation():
 def test_align_capacity():
     # TextureLayout.alignCapacity: whole rows of 4096, at least one
     assert [S.align_capacity(c) for c in (0, 1, 4096, 4097)] == [4096, 4096, 4096, 8192]
+
+
+def _fake_f32_table(n=10):
+    # A descriptor whose pointers are never dereferenced (argument checks only).
+    t = N.SplatTable()
+    t.layout = N.LAYOUT_F32_SOA
+    t.count = n
+    t.plane_stride = n
+    for name in ("center", "log_scale", "quat", "opacity", "rgb"):
+        setattr(t, name, 0x1000)
+    t.sh_codec = N.SH_NONE
+    t.object_to_world[:] = [1.0 if i % 5 == 0 else 0.0 for i in range(16)]
+    return t
+
+
+def test_batch_argument_errors_before_any_device_work():
+    from radix_amd import Params
+    from radix_amd import camera as cam
+
+    ctx = S.Context()
+    tables = (N.SplatTable * 1)(_fake_f32_table())
+    views = (N.BatchView * 2)()
+    views[0].table = 0
+    views[0].camera = cam.mono(np.eye(4), 60, 64, 64).desc()
+    views[1].table = 3
+    views[1].camera = cam.mono(np.eye(4), 60, 64, 64).desc()
+    h = C.c_void_p()
+    lib = N.lib()
+    rc = lib.rdx_batch_create(ctx.handle, tables, 1, views, 2, C.byref(Params(sort_bits=32).desc()), None, C.byref(h))
+    assert rc == N.RDX_E_INVALID and b"table 3 of 1" in lib.rdx_last_error()
+    views[1].table = 0
+    views[1].camera = cam.stereo(np.eye(4), 0.06, 60, 64, 64).desc()
+    rc = lib.rdx_batch_create(ctx.handle, tables, 1, views, 2, C.byref(Params(sort_bits=32).desc()), None, C.byref(h))
+    assert rc == N.RDX_E_INVALID and b"one eye per view" in lib.rdx_last_error()
+    rc = lib.rdx_batch_create(ctx.handle, tables, 1, views, 1, C.byref(Params(sort_bits=12).desc()), None, C.byref(h))
+    assert rc == N.RDX_E_INVALID and b"bits must be 16, 24 or 32 (got 12)" in lib.rdx_last_error()
+    assert not h.value
