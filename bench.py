@@ -5,7 +5,7 @@ fraction of HBM peak. Prints one JSON line.
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 
 A step is one rdx_project_sort over the table already resident in HBM:
-fused projection kernel (words, keys, first-pass tile counts), then the radix
+projection kernel (words, keys), then the radix
 passes (compaction in the first), writing the visible indices back to front.
 """
 from __future__ import annotations
@@ -93,6 +93,20 @@ def oracle_projection_rate(planes, rig, count: int) -> dict:
             "sample": f"{count:,} splats of the same table, FP64 projection + SH3 + key, 1 thread"}
 
 
+def measured_traffic(n: int):
+    """HBM bytes per k_project launch at this size, from the committed
+    rocprofv3 counter runs (FETCH_SIZE and WRITE_SIZE, each scaled by the
+    same counter over a copy of known size), or None if not measured."""
+    path = os.path.join(ROOT, "profiles", "plain_4m_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if t.get("splats") != n:
+        return None
+    return t.get("k_project_bytes")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +114,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--count", type=int, default=4194304)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of 512 MiB five times (the reference kernel that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -108,6 +125,15 @@ def main() -> int:
 
     import numpy as np
     import torch
+
+    if args.copy_calibration:
+        src = torch.ones(512 << 18, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4, "copies": 5}))
+        return 0
 
     from radix_amd import DeviceTable, Params, Projector, workloads as W
     from radix_amd.sort import Context
@@ -185,8 +211,8 @@ def main() -> int:
         "hbm_frac_pipeline": round(pipeline_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "pipeline_bytes_per_splat": B_IN_SH3 + B_OUT + B_SORT,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_project (fused projection + SH + key + first-pass tile counts)",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n),
+                     "kernel": "k_project (projection + SH + key)",
                      "bytes_per_splat": B_IN_SH3 + B_OUT},
         "stages_ms": {k: round(v["ms_avg"], 5) for k, v in stages.items()},
         "wall_s_timed": round(wall, 4),

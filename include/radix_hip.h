@@ -73,6 +73,16 @@ typedef struct rdx_sort_tuning {
 } rdx_sort_tuning;
 int rdx_context_set_tuning(rdx_context* ctx, const rdx_sort_tuning* tuning);
 int rdx_context_get_tuning(const rdx_context* ctx, rdx_sort_tuning* out);
+/* Autotuner (no reference counterpart: the reference fixes 8-bit digits,
+ * WgpuRadixSort.ts:13-15, and a keysPerThread per tier): times
+ * rdx_sort_splats of `n` keys under every supported tuning (the first
+ * `count` of them, max 64, in the order of rdx_sort_tuning_candidate) and
+ * keeps the fastest on the context. keys_dev: the keys to sort (NULL:
+ * uniform random keys of `bits` bits made on the device). Writes the chosen
+ * tuning and its time per sort in milliseconds. Synchronous. */
+int rdx_sort_tuning_candidate(int index, rdx_sort_tuning* out);  /* RDX_E_INVALID past the last */
+int rdx_context_autotune(rdx_context* ctx, void* stream, const uint32_t* keys_dev, uint32_t n, int bits, int reps,
+                         rdx_sort_tuning* out, double* ms_out);
 
 /* Stage timing: with profiling on, each kernel a call on this context
  * launches is bracketed by HIP events on that call's stream. No reference
@@ -238,10 +248,9 @@ int rdx_project_ewa(rdx_context* ctx, void* stream, const rdx_splat_table* table
                     const rdx_params* params, uint32_t at, uint32_t* const* words_dev, uint32_t* keys_dev,
                     size_t key_stride, const rdx_float_out* side);
 
-/* Project and sort in one stream-ordered sequence: the fused kernel writes
- * words, keys and the first radix pass's tile counts of the valid keys;
- * the first pass drops invalid keys. Output as rdx_sort_splats for the
- * keys it made (keys_dev NULL: context scratch). The headline path
+/* Project and sort in one stream-ordered sequence: the projection kernel
+ * writes words and keys; the first radix pass drops invalid keys. Output
+ * as rdx_sort_splats for the keys it made (keys_dev NULL: context scratch). The headline path
  * (SplatRenderer.update's project + sort, SplatRenderer.ts:412-456).
  * With params->key_per_eye on a stereo camera each eye sorts its own keys
  * (two segments of n = table->count): eye e's keys at keys_dev + e n, its

@@ -142,6 +142,10 @@ _SIGNATURES = [
     ("rdx_context_reserve", C.c_int, [C.c_void_p, C.c_uint32]),
     ("rdx_context_set_tuning", C.c_int, [C.c_void_p, C.POINTER(SortTuning)]),
     ("rdx_context_get_tuning", C.c_int, [C.c_void_p, C.POINTER(SortTuning)]),
+    ("rdx_sort_tuning_candidate", C.c_int, [C.c_int, C.POINTER(SortTuning)]),
+    ("rdx_context_autotune", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.POINTER(SortTuning),
+      C.POINTER(C.c_double)]),
     ("rdx_device_alloc", C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
     ("rdx_device_free", C.c_int, [C.c_void_p]),
     ("rdx_device_upload", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),

@@ -222,7 +222,7 @@ int small_segments(SortScratch& sc, hipStream_t s, const SortPlan& p, const uint
 
 int run_sort(Context* ctx, SortScratch& sc, hipStream_t s, const SortPlan& p, const SegLayout& L,
              const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* final_keys, uint32_t* final_vals, uint32_t n_cap,
-             const uint32_t* n_dev, bool drop, uint32_t invalid, bool counted0, bool rule, uint32_t* count_out,
+             const uint32_t* n_dev, bool drop, uint32_t invalid, bool rule, uint32_t* count_out,
              uint32_t* draw_args) {
   const int bins = 1 << p.rb;
   const uint32_t* kin = keys_in;
@@ -254,7 +254,7 @@ int run_sort(Context* ctx, SortScratch& sc, hipStream_t s, const SortPlan& p, co
     a.seg = L.seg;
     a.dst_cap = n_cap;
     char name[32];
-    if (!(pass == 0 && counted0)) {
+    {
       snprintf(name, sizeof name, "sort_count%d", pass);
       Stage st(ctx, s, name);
       if (!launch_pass_kernel(p, PassKernel::count, L.tiles, s, a)) return fail(RDX_E_INVALID, "unsupported sort tuning");
@@ -480,7 +480,7 @@ int rdx_sort_keys(rdx_context* ctx, const uint32_t* keys, uint32_t count, int bi
   uint32_t* dord = (uint32_t*)c->io_vals.ptr;
   hipError_t e = hipMemcpyAsync(dkeys, keys, (size_t)count * 4, hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return hip_fail(e, "SortKeys.sort upload");
-  if ((rc = run_sort(c, c->sort, s, p, L, dkeys, nullptr, nullptr, dord, count, nullptr, false, 0u, false, true, nullptr,
+  if ((rc = run_sort(c, c->sort, s, p, L, dkeys, nullptr, nullptr, dord, count, nullptr, false, 0u, true, nullptr,
                      nullptr)))
     return rc;
   uint32_t hidden = 0;
@@ -503,8 +503,8 @@ static int sort_pairs_impl(rdx_context* ctx, void* stream, uint32_t* keys, uint3
   SegLayout L;
   L.tiles = tiles_of(p, n);
   if (int rc = c->sort.reserve(n, 1, L.tiles)) return rc;
-  return run_sort(c, c->sort, (hipStream_t)stream, p, L, keys, vals, keys, vals, n, count_dev, false, 0u, false, false,
-                  nullptr, nullptr);
+  return run_sort(c, c->sort, (hipStream_t)stream, p, L, keys, vals, keys, vals, n, count_dev, false, 0u, false, nullptr,
+                  nullptr);
 }
 
 int rdx_sort_pairs(rdx_context* ctx, void* stream, uint32_t* keys, uint32_t* vals, uint32_t count, int bits) {
@@ -533,7 +533,99 @@ int rdx_sort_splats(rdx_context* ctx, void* stream, const uint32_t* keys, uint32
   L.tiles = tiles_of(p, n);
   if (int rc = c->sort.reserve(n, 1, L.tiles)) return rc;
   return run_sort(c, c->sort, s, p, L, keys, nullptr, nullptr, out_indices, n, nullptr, true, key_invalid(bits), false,
-                  false, out_count, out_args);
+                  out_count, out_args);
+}
+
+static const rdx_sort_tuning k_candidates[] = {
+    {8, 16, 256}, {8, 8, 256}, {8, 12, 256}, {8, 20, 256}, {8, 8, 512}, {8, 12, 512}, {8, 16, 512},
+    {6, 16, 256}, {6, 8, 256}, {6, 12, 256}, {6, 20, 256}, {6, 8, 512}, {6, 12, 512}, {6, 16, 512},
+    {4, 16, 256}, {4, 8, 256}, {4, 12, 256}, {4, 20, 256}, {4, 8, 512}, {4, 12, 512}, {4, 16, 512},
+};
+
+int rdx_sort_tuning_candidate(int index, rdx_sort_tuning* out) {
+  const int n = (int)(sizeof k_candidates / sizeof k_candidates[0]);
+  if (index < 0 || index >= n || !out) return fail(RDX_E_INVALID, "rdx_sort_tuning_candidate: %d of %d", index, n);
+  *out = k_candidates[index];
+  return RDX_OK;
+}
+
+static __global__ void k_random_keys(uint32_t* keys, uint32_t n, uint32_t mask, uint32_t seed) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    // splitmix32-style mixing of (seed, index)
+    uint32_t x = i * 0x9e3779b9u + seed;
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    keys[i] = x & mask;
+  }
+}
+
+int rdx_context_autotune(rdx_context* ctx, void* stream, const uint32_t* keys_dev, uint32_t n, int bits, int reps,
+                         rdx_sort_tuning* out, double* ms_out) {
+  if (!ctx) return fail(RDX_E_INVALID, "rdx_context_autotune: no context");
+  if (int rc = check_bits(bits, "WgpuRadixSort")) return rc;
+  if (n == 0) return fail(RDX_E_INVALID, "rdx_context_autotune: no keys");
+  Context* c = reinterpret_cast<Context*>(ctx);
+  hipStream_t s = (hipStream_t)stream;
+  reps = std::max(1, std::min(reps, 100));
+  Buffer keys, out_idx, counts;
+  int rc;
+  if ((rc = out_idx.ensure((size_t)n * 4)) || (rc = counts.ensure(64))) {
+    keys.release();
+    out_idx.release();
+    counts.release();
+    return rc;
+  }
+  if (!keys_dev) {
+    if ((rc = keys.ensure((size_t)n * 4))) {
+      out_idx.release();
+      counts.release();
+      return rc;
+    }
+    // below the invalid key: every key is sorted
+    hipLaunchKernelGGL(k_random_keys, dim3(std::min<uint32_t>((n + 255) / 256, 4096u)), dim3(256), 0, s,
+                       (uint32_t*)keys.ptr, n, key_invalid(bits) >> 1, 0x2545f491u);
+    keys_dev = (const uint32_t*)keys.ptr;
+  }
+  const rdx_sort_tuning saved = c->tuning;
+  const bool prof = c->profiling;
+  c->profiling = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  double best = 1e30;
+  rdx_sort_tuning best_t = saved;
+  const int ncand = (int)(sizeof k_candidates / sizeof k_candidates[0]);
+  for (int k = 0; k < ncand && e == hipSuccess && rc == RDX_OK; k++) {
+    c->tuning = k_candidates[k];
+    rc = rdx_sort_splats(ctx, stream, keys_dev, n, bits, (uint32_t*)out_idx.ptr, (uint32_t*)counts.ptr, nullptr);
+    if (rc) break;
+    e = hipEventRecord(e0, s);
+    for (int r = 0; r < reps && rc == RDX_OK && e == hipSuccess; r++)
+      rc = rdx_sort_splats(ctx, stream, keys_dev, n, bits, (uint32_t*)out_idx.ptr, (uint32_t*)counts.ptr, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess && rc == RDX_OK && ms / reps < best) {
+      best = ms / reps;
+      best_t = k_candidates[k];
+    }
+  }
+  c->profiling = prof;
+  c->tuning = rc == RDX_OK && e == hipSuccess ? best_t : saved;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  keys.release();
+  out_idx.release();
+  counts.release();
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_fail(e, "rdx_context_autotune");
+  if (out) *out = best_t;
+  if (ms_out) *ms_out = best;
+  return RDX_OK;
 }
 
 }  // extern "C"

@@ -106,11 +106,10 @@ int small_segments(SortScratch& sc, hipStream_t s, const SortPlan& p, const uint
                    SegLayout* out);
 // The P digit passes: pass 0 reads (keys_in, vals_in or the index), the last
 // writes (final_keys, final_vals); between them the scratch pairs alternate.
-// counted0: pass 0's tile counts are already in sc.cnt (the fused
-// projection); rule: count the host sort's hidden keys into tot[s][MAX_BINS].
+// rule: count the host sort's hidden keys into tot[s][MAX_BINS].
 int run_sort(Context* ctx, SortScratch& sc, hipStream_t s, const SortPlan& p, const SegLayout& L,
              const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* final_keys, uint32_t* final_vals, uint32_t n_cap,
-             const uint32_t* n_dev, bool drop, uint32_t invalid, bool counted0, bool rule, uint32_t* count_out,
+             const uint32_t* n_dev, bool drop, uint32_t invalid, bool rule, uint32_t* count_out,
              uint32_t* draw_args);
 
 }  // namespace rdx

@@ -87,41 +87,38 @@ static int view_uniforms(Batch* b, int v) {
   bv.eye = pu.eye[0];
   for (int k = 0; k < 3; k++) bv.h[k] = pu.h[k];
   bv.seg_off = (uint32_t)b->seg_off[v];
-  bv.tile0 = b->tile0[v];
+  bv.pad = 0;
   bv.words = (uint4*)V.words_dev;
   return RDX_OK;
 }
 
 template <int LAYOUT, int SHC, int DEG>
-static void launch_kind(hipStream_t s, uint32_t blocks, const BatchDesc& d, const BatchU& u, uint32_t* keys, uint32_t* cnt,
-                        uint32_t tiles) {
-  hipLaunchKernelGGL((k_project_batch<LAYOUT, SHC, DEG>), dim3(blocks), dim3(256), 0, s, d, u, keys, cnt, tiles);
+static void launch_kind(hipStream_t s, uint32_t blocks, const BatchDesc& d, const BatchU& u, uint32_t* keys) {
+  hipLaunchKernelGGL((k_project_batch<LAYOUT, SHC, DEG>), dim3(blocks), dim3(256), 0, s, d, u, keys);
 }
 
-static bool launch_batch_kind(hipStream_t s, const BatchKind& k, const BatchDesc& d, const BatchU& u, uint32_t* keys,
-                              uint32_t* cnt, uint32_t tiles) {
+static bool launch_batch_kind(hipStream_t s, const BatchKind& k, const BatchDesc& d, const BatchU& u, uint32_t* keys) {
   if (k.layout == LAYOUT_F32) {
     switch (k.deg) {
-      case 0: launch_kind<LAYOUT_F32, SHC_F32, 0>(s, k.blocks, d, u, keys, cnt, tiles); return true;
-      case 1: launch_kind<LAYOUT_F32, SHC_F32, 1>(s, k.blocks, d, u, keys, cnt, tiles); return true;
-      case 2: launch_kind<LAYOUT_F32, SHC_F32, 2>(s, k.blocks, d, u, keys, cnt, tiles); return true;
-      case 3: launch_kind<LAYOUT_F32, SHC_F32, 3>(s, k.blocks, d, u, keys, cnt, tiles); return true;
+      case 0: launch_kind<LAYOUT_F32, SHC_F32, 0>(s, k.blocks, d, u, keys); return true;
+      case 1: launch_kind<LAYOUT_F32, SHC_F32, 1>(s, k.blocks, d, u, keys); return true;
+      case 2: launch_kind<LAYOUT_F32, SHC_F32, 2>(s, k.blocks, d, u, keys); return true;
+      case 3: launch_kind<LAYOUT_F32, SHC_F32, 3>(s, k.blocks, d, u, keys); return true;
     }
     return false;
   }
   if (k.layout == LAYOUT_EXT) {
-    if (k.shc == SHC_PACKED) launch_kind<LAYOUT_EXT, SHC_PACKED, -1>(s, k.blocks, d, u, keys, cnt, tiles);
-    else launch_kind<LAYOUT_EXT, SHC_EXT, -1>(s, k.blocks, d, u, keys, cnt, tiles);
+    if (k.shc == SHC_PACKED) launch_kind<LAYOUT_EXT, SHC_PACKED, -1>(s, k.blocks, d, u, keys);
+    else launch_kind<LAYOUT_EXT, SHC_EXT, -1>(s, k.blocks, d, u, keys);
     return true;
   }
-  if (k.shc == SHC_PACKED) launch_kind<LAYOUT_PACKED, SHC_PACKED, -1>(s, k.blocks, d, u, keys, cnt, tiles);
-  else launch_kind<LAYOUT_PACKED, SHC_EXT, -1>(s, k.blocks, d, u, keys, cnt, tiles);
+  if (k.shc == SHC_PACKED) launch_kind<LAYOUT_PACKED, SHC_PACKED, -1>(s, k.blocks, d, u, keys);
+  else launch_kind<LAYOUT_PACKED, SHC_EXT, -1>(s, k.blocks, d, u, keys);
   return true;
 }
 
 // The whole batch on one stream: projections, then the segmented sort.
 static int run_batch(Batch* b, hipStream_t s, uint32_t* out_indices, uint32_t* out_counts, uint32_t* out_args) {
-  uint32_t* cnt = (uint32_t*)b->scratch.cnt.ptr;
   {
     Stage st(b->ctx, s, "batch_project");
     for (auto& k : b->kinds) {
@@ -132,12 +129,12 @@ static int run_batch(Batch* b, hipStream_t s, uint32_t* out_indices, uint32_t* o
       d.table_sh_degree = (const int*)b->d_deg.ptr;
       d.table_enc = (const float*)b->d_enc.ptr;
       d.num_groups = (int)k.groups.size();
-      if (!launch_batch_kind(s, k, d, b->u, b->keys, cnt, b->tiles))
+      if (!launch_batch_kind(s, k, d, b->u, b->keys))
         return fail(RDX_E_INVALID, "batch: no kernel for layout %d", k.layout);
     }
   }
   return run_sort(b->ctx, b->scratch, s, b->plan, b->layout, b->keys, nullptr, nullptr, out_indices, b->total, nullptr,
-                  true, key_invalid(b->params.sort_bits), true, false, out_counts, out_args);
+                  true, key_invalid(b->params.sort_bits), false, out_counts, out_args);
 }
 
 }  // namespace rdx
@@ -228,7 +225,7 @@ int rdx_batch_create(rdx_context* ctx, const rdx_splat_table* tables, int num_ta
         g.num_views++;
         i++;
       }
-      k.blocks += tiles_of(b->plan, tables[t].count);
+      k.blocks += std::max<uint32_t>(1u, (tables[t].count + 255u) / 256u);
       k.groups.push_back(g);
     }
     b->kinds.push_back(std::move(k));
@@ -254,8 +251,6 @@ int rdx_batch_create(rdx_context* ctx, const rdx_splat_table* tables, int num_ta
     u.aperture_tan = pu.aperture_tan;
     u.sort_bits = pu.sort_bits;
     u.cull_keys = pu.cull_keys;
-    u.rb = b->plan.rb;
-    u.tile = (uint32_t)b->plan.tile;
   }
   for (int v = 1; v < num_views; v++)
     if (views[v].camera.ndc_z_min != views[0].camera.ndc_z_min)

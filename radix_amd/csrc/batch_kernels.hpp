@@ -1,9 +1,7 @@
 // Batched projection: many views over many tables in one launch. A block
-// takes one sort tile of one table's splats and projects each of them for
-// every view of that table in its group (up to BATCH_MAX_VIEWS), so a table
-// is read once for all the views that draw it. Each view is a segment of the
-// batch's keys; the block writes the tile's first-pass counts for each view's
-// segmented sort.
+// takes 256 splats of one table and projects each of them for every view of
+// that table in its group (up to BATCH_MAX_VIEWS), so a table is read once
+// for all the views that draw it. Each view is a segment of the batch's keys.
 #pragma once
 
 #include "project_kernels.hpp"
@@ -16,7 +14,7 @@ struct BatchView {
   EyeU eye;          // object (its table) -> eye
   double h[3];       // the eye in object space: the SH origin
   uint32_t seg_off;  // the view's first key in the batch
-  uint32_t tile0;    // the view's first sort tile
+  uint32_t pad;
   uint4* words;      // its ewa words (NULL: not written)
 };
 
@@ -33,8 +31,7 @@ struct BatchU {
   float ndc_z_min, clip_xy;
   float max_std_dev, min_alpha, blur, pre_blur, min_pr, max_pr, min_contrib;
   float focal_distance, aperture_tan;
-  int sort_bits, cull_keys, rb;
-  uint32_t tile;  // splats per block: one sort tile
+  int sort_bits, cull_keys;
 };
 
 struct BatchDesc {
@@ -47,10 +44,8 @@ struct BatchDesc {
 };
 
 template <int LAYOUT, int SHC, int DEG>
-__global__ __launch_bounds__(256) void k_project_batch(BatchDesc D, BatchU u, uint32_t* keys, uint32_t* cnt,
-                                                       uint32_t tiles) {
+__global__ __launch_bounds__(256) void k_project_batch(BatchDesc D, BatchU u, uint32_t* keys) {
   constexpr int BLOCK = 256;
-  __shared__ uint32_t s_cnt[BATCH_MAX_VIEWS * MAX_BINS];
   __shared__ uint32_t s_group;
 
   // The block's group: the last group whose first block is <= blockIdx.x.
@@ -60,7 +55,6 @@ __global__ __launch_bounds__(256) void k_project_batch(BatchDesc D, BatchU u, ui
       if (D.groups[k].block0 <= blockIdx.x) g = (uint32_t)k;
     s_group = g;
   }
-  for (int x = threadIdx.x; x < BATCH_MAX_VIEWS * MAX_BINS; x += BLOCK) s_cnt[x] = 0u;
   __syncthreads();
   const BatchGroup G = D.groups[s_group];
   const TableP& t = D.tables[G.table];
@@ -69,14 +63,12 @@ __global__ __launch_bounds__(256) void k_project_batch(BatchDesc D, BatchU u, ui
   const uint32_t chunk = blockIdx.x - G.block0;
   const float* enc = D.table_enc + 4 * G.table;
   const uint32_t invalid = key_invalid(u.sort_bits);
-  const uint32_t bins = 1u << u.rb;
-  const uint32_t lo = chunk * u.tile;
-  const uint32_t hi = min(lo + u.tile, t.count);
 
   F32Rsrc rs;
   if constexpr (LAYOUT == LAYOUT_F32) rs = f32_rsrc(t, deg);
 
-  for (uint32_t i = lo + threadIdx.x; i < hi; i += BLOCK) {
+  const uint32_t i = chunk * BLOCK + threadIdx.x;
+  if (i < t.count) {
     Splat sp;
     float3 coef[15];
     load_splat<LAYOUT, SHC, DEG>(t, rs, enc, i, deg, sp, coef);
@@ -109,13 +101,7 @@ __global__ __launch_bounds__(256) void k_project_batch(BatchDesc D, BatchU u, ui
       const bool keyed = u.cull_keys ? visible : !invisible;
       const uint32_t key = keyed ? sort_key(fbits((float)C.dist), u.sort_bits) : invalid;
       keys[V.seg_off + i] = key;
-      if (key != invalid) atomicAdd(&s_cnt[v * MAX_BINS + (key & (bins - 1u))], 1u);
     }
-  }
-  __syncthreads();
-  for (uint32_t v = 0; v < nv; v++) {
-    const uint32_t col = D.views[G.first_view + v].tile0 + chunk;
-    for (uint32_t d = threadIdx.x; d < bins; d += BLOCK) cnt[(size_t)d * tiles + col] = s_cnt[v * MAX_BINS + d];
   }
 }
 

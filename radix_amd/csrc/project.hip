@@ -195,18 +195,16 @@ int check_table(const rdx_splat_table* t) {
 
 int launch_project(hipStream_t s, const rdx_splat_table* t, const ProjU& u, const OutP& o, bool side, int blocks) {
   const uint32_t n = t->count;
-  const bool count = o.cnt != nullptr;
-  if (n == 0 && !count) return RDX_OK;
-  // Counting: one block per sort tile (at least one, which writes its zero
-  // counts); otherwise a grid-stride launch.
-  uint32_t grid = count ? std::max<uint32_t>(1u, (uint32_t)(((uint64_t)n + u.tile - 1) / u.tile))
-                        : std::min<uint32_t>((n + 255) / 256, (uint32_t)(blocks > 0 ? blocks : 4096));
+  if (n == 0) return RDX_OK;
+  // One thread per splat: many short blocks keep the machine full to the
+  // end (a grid of a few resident rounds leaves a tail).
+  const uint32_t grid = std::min<uint32_t>((n + 255) / 256, (uint32_t)(blocks > 0 ? blocks : (1 << 20)));
   const TableP tp = make_table(t);
   const int shc = t->sh_codec == RDX_SH_PACKED ? SHC_PACKED : SHC_EXT;
   switch (t->layout) {
-    case RDX_LAYOUT_F32_SOA: launch_project_f32(s, dim3(grid), tp, u, o, u.eyes, side, count); break;
-    case RDX_LAYOUT_EXT: launch_project_ext(s, dim3(grid), shc, tp, u, o, u.eyes, side, count); break;
-    case RDX_LAYOUT_PACKED: launch_project_packed(s, dim3(grid), shc, tp, u, o, u.eyes, side, count); break;
+    case RDX_LAYOUT_F32_SOA: launch_project_f32(s, dim3(grid), tp, u, o, u.eyes, side); break;
+    case RDX_LAYOUT_EXT: launch_project_ext(s, dim3(grid), shc, tp, u, o, u.eyes, side); break;
+    case RDX_LAYOUT_PACKED: launch_project_packed(s, dim3(grid), shc, tp, u, o, u.eyes, side); break;
   }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? RDX_OK : hip_fail(e, "projection launch");
@@ -307,24 +305,19 @@ int rdx_project_sort(rdx_context* ctx, void* stream, const rdx_splat_table* tabl
     if (int rc = c->proj.ensure((size_t)n * segs * 4)) return rc;
     keys = (uint32_t*)c->proj.ptr;
   }
-  u.rb = p.rb;
-  u.tile = (uint32_t)p.tile;
   OutP o;
   memset(&o, 0, sizeof o);
   for (int e = 0; e < camera->eyes; e++) o.words[e] = words ? (uint4*)words[e] : nullptr;
   o.keys = keys;
   o.key_stride = n;
   o.at = 0;
-  o.cnt = (uint32_t*)c->sort.cnt.ptr;
-  o.tiles = L.tiles;
-  o.eye_tile = nt;
   fill_side(o, side, n);
   {
     Stage st(c, s, "project");
     if (int rc = launch_project(s, table, u, o, side != nullptr, 0)) return rc;
   }
   return run_sort(c, c->sort, s, p, L, keys, nullptr, nullptr, out_indices, n * segs, nullptr, true,
-                  key_invalid(params->sort_bits), true, false, out_count, out_args);
+                  key_invalid(params->sort_bits), false, out_count, out_args);
 }
 
 }  // extern "C"

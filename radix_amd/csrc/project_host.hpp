@@ -18,16 +18,15 @@ TableP make_table(const rdx_splat_table* t);
 // A table descriptor's checks (layout, pointers, SH codec, sizes).
 int check_table(const rdx_splat_table* t);
 
-// Launch the projection of a table; with o.cnt, one block per sort tile
-// writing the sort's first-pass tile counts.
+// Launch the projection of a table (one thread per splat, grid-stride
+// beyond 2^20 blocks).
 int launch_project(hipStream_t s, const rdx_splat_table* t, const ProjU& u, const OutP& o, bool side, int blocks);
 
 // The kernels, instantiated per layout in their own translation units.
-void launch_project_f32(hipStream_t s, dim3 grid, const TableP& t, const ProjU& u, const OutP& o, int eyes, bool side,
-                        bool count);
+void launch_project_f32(hipStream_t s, dim3 grid, const TableP& t, const ProjU& u, const OutP& o, int eyes, bool side);
 void launch_project_ext(hipStream_t s, dim3 grid, int shc, const TableP& t, const ProjU& u, const OutP& o, int eyes,
-                        bool side, bool count);
+                        bool side);
 void launch_project_packed(hipStream_t s, dim3 grid, int shc, const TableP& t, const ProjU& u, const OutP& o, int eyes,
-                           bool side, bool count);
+                           bool side);
 
 }  // namespace rdx

@@ -46,8 +46,6 @@ struct ProjU {
   float max_std_dev, min_alpha, blur, pre_blur, clip_xy, min_pr, max_pr, min_contrib;
   float focal_distance, aperture_tan, sort_guard;
   int lod_inflate, sh_degree, sort_bits, cull_keys, key_per_eye;
-  int rb;             // digit width of the sort's first pass (fused tile counts)
-  uint32_t tile;      // splats per block (one sort tile) when counting
   float enc[4];       // packed encoding: rgbMin, rgbMax, lnScaleMin, lnScaleMax
 };
 
@@ -71,9 +69,6 @@ struct OutP {
   uint32_t* keys;
   size_t key_stride;
   uint32_t at;
-  uint32_t* cnt;         // the sort's first-pass tile counts [digit][tiles], or NULL
-  uint32_t tiles;        // their row length
-  uint32_t eye_tile;     // eye 1's first tile (key_per_eye)
   // side outputs (float planes of `sstride`), any may be NULL
   size_t sstride;
   float* conic;
@@ -540,31 +535,17 @@ __device__ __forceinline__ void prepare(const Splat& sp, const U& u, float& op, 
 
 // The projection kernel. DEG: the f32 table's SH degree (compile time), or
 // -1 for word tables (u.sh_degree at run time).
-template <int LAYOUT, int SHC, int DEG, int EYES, bool SIDE, bool COUNT>
+template <int LAYOUT, int SHC, int DEG, int EYES, bool SIDE>
 __global__ __launch_bounds__(256) void k_project(TableP t, ProjU u, OutP o) {
   constexpr int BLOCK = 256;
-  // The checking variant (SIDE) counts when given a count buffer.
-  constexpr bool MAY_COUNT = COUNT || SIDE;
-  constexpr int CEYES = MAY_COUNT ? EYES : 1;
-  __shared__ uint32_t s_cnt[MAY_COUNT ? CEYES * MAX_BINS : 1];
-  const bool counting = COUNT || (SIDE && o.cnt != nullptr);
   const uint32_t n = t.count;
-  const uint32_t bins = 1u << u.rb;
-  if (counting) {
-    for (uint32_t x = threadIdx.x; x < CEYES * MAX_BINS; x += BLOCK) s_cnt[x] = 0u;
-    __syncthreads();
-  }
   const uint32_t invalid = key_invalid(u.sort_bits);
   const bool per_eye = EYES == 2 && u.key_per_eye;
   const int deg = DEG >= 0 ? DEG : u.sh_degree;
   F32Rsrc rs;
   if constexpr (LAYOUT == LAYOUT_F32) rs = f32_rsrc(t, deg);
 
-  // Counting: block b is sort tile b, splats [b * tile, (b + 1) * tile).
-  const uint32_t i0 = counting ? blockIdx.x * u.tile + threadIdx.x : blockIdx.x * BLOCK + threadIdx.x;
-  const uint32_t i_end = counting ? min(n, (blockIdx.x + 1) * u.tile) : n;
-  const uint32_t i_step = counting ? BLOCK : gridDim.x * BLOCK;
-  for (uint32_t i = i0; i < i_end; i += i_step) {
+  for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
     Splat sp;
     float3 coef[15];
     load_splat<LAYOUT, SHC, DEG>(t, rs, u.enc, i, deg, sp, coef);
@@ -693,10 +674,6 @@ __global__ __launch_bounds__(256) void k_project(TableP t, ProjU u, OutP o) {
       o.keys[at] = key[0];
       if (per_eye) o.keys[o.key_stride + at] = key[EYES - 1];
     }
-    if (MAY_COUNT && counting) {
-      if (key[0] != invalid) atomicAdd(&s_cnt[key[0] & (bins - 1u)], 1u);
-      if (EYES == 2 && per_eye && key[EYES - 1] != invalid) atomicAdd(&s_cnt[(CEYES - 1) * MAX_BINS + (key[EYES - 1] & (bins - 1u))], 1u);
-    }
     if constexpr (SIDE) {
       const size_t ss = o.sstride;
       if (o.rgb) {
@@ -711,13 +688,6 @@ __global__ __launch_bounds__(256) void k_project(TableP t, ProjU u, OutP o) {
           o.dist[i] = (float)head_dist;
         }
       }
-    }
-  }
-  if (MAY_COUNT && counting) {
-    __syncthreads();
-    for (uint32_t d = threadIdx.x; d < bins; d += BLOCK) {
-      o.cnt[(size_t)d * o.tiles + blockIdx.x] = s_cnt[d];
-      if (EYES == 2 && per_eye) o.cnt[(size_t)d * o.tiles + o.eye_tile + blockIdx.x] = s_cnt[(CEYES - 1) * MAX_BINS + d];
     }
   }
 }

@@ -4,23 +4,21 @@
 namespace rdx {
 
 template <int SHC, int EYES>
-static void pick(hipStream_t s, dim3 grid, const TableP& t, const ProjU& u, const OutP& o, bool side, bool count) {
+static void pick(hipStream_t s, dim3 grid, const TableP& t, const ProjU& u, const OutP& o, bool side) {
   if (side)
-    hipLaunchKernelGGL((k_project<LAYOUT_PACKED, SHC, -1, EYES, true, false>), grid, dim3(256), 0, s, t, u, o);
-  else if (count)
-    hipLaunchKernelGGL((k_project<LAYOUT_PACKED, SHC, -1, EYES, false, true>), grid, dim3(256), 0, s, t, u, o);
+    hipLaunchKernelGGL((k_project<LAYOUT_PACKED, SHC, -1, EYES, true>), grid, dim3(256), 0, s, t, u, o);
   else
-    hipLaunchKernelGGL((k_project<LAYOUT_PACKED, SHC, -1, EYES, false, false>), grid, dim3(256), 0, s, t, u, o);
+    hipLaunchKernelGGL((k_project<LAYOUT_PACKED, SHC, -1, EYES, false>), grid, dim3(256), 0, s, t, u, o);
 }
 
 void launch_project_packed(hipStream_t s, dim3 grid, int shc, const TableP& t, const ProjU& u, const OutP& o, int eyes,
-                         bool side, bool count) {
+                         bool side) {
   if (shc == SHC_PACKED) {
-    if (eyes == 1) pick<SHC_PACKED, 1>(s, grid, t, u, o, side, count);
-    else pick<SHC_PACKED, 2>(s, grid, t, u, o, side, count);
+    if (eyes == 1) pick<SHC_PACKED, 1>(s, grid, t, u, o, side);
+    else pick<SHC_PACKED, 2>(s, grid, t, u, o, side);
   } else {
-    if (eyes == 1) pick<SHC_EXT, 1>(s, grid, t, u, o, side, count);
-    else pick<SHC_EXT, 2>(s, grid, t, u, o, side, count);
+    if (eyes == 1) pick<SHC_EXT, 1>(s, grid, t, u, o, side);
+    else pick<SHC_EXT, 2>(s, grid, t, u, o, side);
   }
 }
 

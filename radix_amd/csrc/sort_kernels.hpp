@@ -3,13 +3,12 @@
 // A sort of P digit passes (P = ceil(bits / RB), RB = 4, 6 or 8 bits) runs,
 // per pass, three kernels over tiles of TILE = BLOCK * KPT keys:
 //   count    per tile, the number of keys of each digit -> cnt[digit][tile]
-//            (the fused projection kernel produces pass 0's counts itself)
 //   scan     per (segment, digit), an exclusive scan of its row over the
 //            segment's tiles, in place, and the digit's total -> tot
-//   scatter  per tile: rank every key within the tile with wave64 ballots
-//            (match-any over the digit's bits, mbcnt for the rank), stage the
-//            tile in LDS in digit order and write it out in runs starting at
-//            base(digit) + cnt[digit][tile]
+//   scatter  per tile: rank every key within the tile (wave64 ballots for a
+//            round's most present digits, LDS lane masks for the rest, mbcnt
+//            for the rank), stage the tile in LDS in digit order and write it
+//            out in runs starting at base(digit) + cnt[digit][tile]
 // No kernel waits on another block: a pass costs one extra read of the keys
 // (4 B per key) instead of a cross-tile look-back. Every count word is
 // written before it is read, so nothing needs clearing between sorts.
@@ -27,6 +26,13 @@ constexpr int MAX_BINS = 256;
 constexpr int TOT_STRIDE = MAX_BINS + 1;  // tot words per segment: digits, then the hidden-rule count
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_ITEMS = 8;             // tiles per thread per round of the scan
+// Digits a wave takes out of a round with ballots before LDS atomics take
+// the rest: a round whose keys share a few digits (the top digit of
+// clustered distances) would otherwise serialise its atomics on one address.
+// Peeling stops at the first digit held by fewer than PEEL_MIN lanes (a
+// round of spread digits then costs one ballot).
+constexpr int PEEL = 3;
+constexpr uint32_t PEEL_MIN = 8;
 
 // One pass of a sort: what the count, scan and scatter kernels share.
 struct PassArgs {
@@ -93,22 +99,6 @@ __device__ __forceinline__ bool hidden_rule(uint32_t k, int bits) {
   return (k >> 16) == 0xffffu;
 }
 
-// Match-any over a digit of RB bits: the lanes of `m` whose digit equals
-// this lane's. Per bit: the wave's ballot B, and this lane's bit as a sign
-// mask s (0 or all ones): m &= ~(B ^ s), one three-input bit op per half.
-template <int RB>
-__device__ __forceinline__ uint64_t match_digit(uint64_t m, uint32_t d) {
-  uint32_t lo = (uint32_t)m, hi = (uint32_t)(m >> 32);
-#pragma unroll
-  for (int b = 0; b < RB; b++) {
-    const uint64_t B = ballot((d >> b) & 1u);
-    const uint32_t sgn = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);
-    lo &= ~((uint32_t)B ^ sgn);
-    hi &= ~((uint32_t)(B >> 32) ^ sgn);
-  }
-  return ((uint64_t)hi << 32) | lo;
-}
-
 // ---- count ---------------------------------------------------------------------
 
 // Tile counts of one pass's digit. Wave w holds keys [w * 64 * KPT, ...) of
@@ -152,7 +142,21 @@ __global__ __launch_bounds__(BLOCK) void k_count(PassArgs a) {
     if (rule && in && hidden_rule(k[j], a.bits)) hidden++;
     const bool keep = in && !(a.drop && k[j] == a.invalid);
     const uint32_t d = ((k[j] & a.key_mask) >> a.shift) & DMASK;
-    if (keep) atomicAdd(&s_wave[w][d], 1u);
+    bool done = !keep;
+    uint64_t rem = ballot(keep);
+#pragma unroll
+    for (int q = 0; q < PEEL; q++) {
+      if (rem == 0ull) break;
+      const int first = __builtin_ctzll(rem);
+      const uint32_t d0 = (uint32_t)__builtin_amdgcn_readlane((int)d, first);
+      const uint64_t m = ballot(!done && d == d0);
+      const uint32_t c = (uint32_t)__builtin_popcountll(m);
+      if (lane == first) atomicAdd(&s_wave[w][d0], c);  // no return: no wait
+      done = done || d == d0;
+      rem &= ~m;
+      if (c < PEEL_MIN) break;
+    }
+    if (!done) atomicAdd(&s_wave[w][d], 1u);
   }
   if (rule && hidden) atomicAdd(&s_rule[w], hidden);
   __syncthreads();
@@ -221,18 +225,24 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
   static_assert(BLOCK % RDX_WAVE == 0, "block of whole waves");
   static_assert(BINS <= BLOCK, "a thread per digit");
 
+  static_assert(WAVES * BINS * 8 <= TILE * 8, "lane masks fit in the staging area");
   __shared__ uint32_t s_wave[WAVES][BINS];  // per-wave digit counts, then exclusive offsets
   __shared__ uint32_t s_local[BINS];        // start of each digit within the tile
   __shared__ uint32_t s_out[BINS];          // global start - local start, for the write-out
-  __shared__ uint32_t s_keys[TILE];
-  __shared__ uint32_t s_vals[TILE];
+  __shared__ uint64_t s_stage[TILE];        // ranking: lane masks; then the staged keys and values
   __shared__ uint32_t s_scan[WAVES];
+  uint64_t (*s_match)[BINS] = reinterpret_cast<uint64_t (*)[BINS]>(s_stage);  // per-wave lane masks of a round
+  uint32_t* s_keys = reinterpret_cast<uint32_t*>(s_stage);
+  uint32_t* s_vals = s_keys + TILE;
 
   const int tid = threadIdx.x;
   const int w = tid / RDX_WAVE;
   const int lane = tid & (RDX_WAVE - 1);
   const uint32_t tile = blockIdx.x;
-  for (int d = tid; d < WAVES * BINS; d += BLOCK) (&s_wave[0][0])[d] = 0u;
+  for (int d = tid; d < WAVES * BINS; d += BLOCK) {
+    (&s_wave[0][0])[d] = 0u;
+    (&s_match[0][0])[d] = 0ull;
+  }
   const SegInfo g = seg_info(a, a.seg_of_tile ? a.seg_of_tile[tile] : 0u);
   const uint32_t local_tile = tile - g.tile0;
   const uint32_t tile_base = local_tile * (uint32_t)TILE;
@@ -280,16 +290,46 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
     const uint32_t i = wbase + (uint32_t)(j * RDX_WAVE);
     rank[j] = i < n && !(a.drop && keys[j] == a.invalid) ? 0u : 0xffffffffu;  // 0xffffffff: not kept
   }
-  __syncthreads();  // s_wave zeroed
+  __syncthreads();  // s_wave, s_match zeroed
+  // Rank each round of 64 keys in input order. Up to PEEL digits of the
+  // round (those of its first remaining lanes) are ranked by one ballot each.
+  // The other lanes OR their bit into their digit's LDS mask, read the mask
+  // back (the lanes of the round with that digit) and rank themselves by the
+  // set lanes below them; the lowest adds the round's count to the wave's
+  // digit count and clears the mask. A wave's LDS operations complete in
+  // order, so the read sees every lane's OR.
+  const uint64_t lane_bit = 1ull << lane;
 #pragma unroll
   for (int j = 0; j < KPT; j++) {
     const bool keep = rank[j] == 0u;
     const uint32_t d = ((keys[j] & a.key_mask) >> a.shift) & DMASK;
-    const uint64_t m = match_digit<RB>(ballot(keep), d);
-    if (keep) {
+    bool done = !keep;
+    uint64_t rem = ballot(keep);
+#pragma unroll
+    for (int q = 0; q < PEEL; q++) {
+      if (rem == 0ull) break;
+      const int first = __builtin_ctzll(rem);
+      const uint32_t d0 = (uint32_t)__builtin_amdgcn_readlane((int)d, first);
+      const uint64_t m = ballot(!done && d == d0);
+      const uint32_t old0 = s_wave[w][d0];
+      if (!done && d == d0) {
+        rank[j] = old0 + count_below(m);
+        done = true;
+      }
+      const uint32_t c = (uint32_t)__builtin_popcountll(m);
+      if (lane == first) s_wave[w][d0] = old0 + c;
+      rem &= ~m;
+      if (c < PEEL_MIN) break;
+    }
+    if (!done) {
+      __hip_atomic_fetch_or(&s_match[w][d], lane_bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      const uint64_t m = __hip_atomic_load(&s_match[w][d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       const uint32_t below = count_below(m);
       const uint32_t old = s_wave[w][d];
-      if (below == 0u) s_wave[w][d] = old + (uint32_t)__builtin_popcountll(m);
+      if (below == 0u) {
+        s_wave[w][d] = old + (uint32_t)__builtin_popcountll(m);
+        __hip_atomic_store(&s_match[w][d], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
       rank[j] = old + below;
     }
   }

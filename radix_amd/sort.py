@@ -52,6 +52,52 @@ class Context:
         N.check(self._lib.rdx_context_get_tuning(self._h, C.byref(t)))
         return (t.digit_bits, t.keys_per_thread, t.block_threads)
 
+    def autotune(self, count: int, bits: int = 32, keys=None, reps: int = 5, stream=None,
+                 cache: Optional[str] = None) -> Tuple[Tuple[int, int, int], float]:
+        """Time every supported tuning on `count` keys (`keys`: a device
+        tensor, or None for uniform random keys) and keep the fastest.
+        Returns ((digit_bits, keys_per_thread, block_threads), ms per sort).
+        cache: a JSON file of earlier choices per (device, bits, count
+        rounded up to a power of two), read before and written after."""
+        import json
+        import os
+
+        import torch
+
+        slot = f"{torch.cuda.get_device_name()}|{bits}|{1 << max(0, (int(count) - 1).bit_length())}"
+        if cache and keys is None and os.path.exists(cache):
+            with open(cache) as f:
+                known = json.load(f)
+            if slot in known:
+                t, ms = known[slot]
+                self.set_tuning(*t)
+                return tuple(t), float(ms)
+        out = N.SortTuning()
+        ms = C.c_double()
+        N.check(self._lib.rdx_context_autotune(self._h, _stream(stream), _ptr(keys), count, bits, reps,
+                                               C.byref(out), C.byref(ms)))
+        chosen = (out.digit_bits, out.keys_per_thread, out.block_threads)
+        if cache and keys is None:
+            known = {}
+            if os.path.exists(cache):
+                with open(cache) as f:
+                    known = json.load(f)
+            known[slot] = [list(chosen), ms.value]
+            with open(cache, "w") as f:
+                json.dump(known, f, indent=1, sort_keys=True)
+        return chosen, ms.value
+
+    @staticmethod
+    def candidates():
+        """The tunings the library supports, in autotuner order."""
+        lib = N.lib()
+        out, i = [], 0
+        t = N.SortTuning()
+        while lib.rdx_sort_tuning_candidate(i, C.byref(t)) == N.RDX_OK:
+            out.append((t.digit_bits, t.keys_per_thread, t.block_threads))
+            i += 1
+        return out
+
     def reserve(self, max_keys: int) -> None:
         N.check(self._lib.rdx_context_reserve(self._h, max_keys))
 
@@ -159,8 +205,10 @@ def sort_splat_keys(keys: np.ndarray, bits: int, ordering: Optional[np.ndarray] 
 class RadixSort:
     """WgpuRadixSort: stable LSD sort of device keys and values, in place."""
 
-    def __init__(self, keys, values, context: Optional[Context] = None, keys_per_thread: int = 16,
-                 digit_bits: int = 8, block_threads: int = 256):
+    def __init__(self, keys, values, context: Optional[Context] = None, keys_per_thread: Optional[int] = None,
+                 digit_bits: Optional[int] = None, block_threads: Optional[int] = None):
+        """keys_per_thread (WgpuRadixSort's keysPerThread), digit_bits and
+        block_threads override the context's tuning where given."""
         import torch
 
         for name, t in (("keys", keys), ("values", values)):
@@ -169,7 +217,9 @@ class RadixSort:
         self.keys = keys
         self.values = values
         self.context = context or Context()
-        self.context.set_tuning(digit_bits, keys_per_thread, block_threads)
+        if keys_per_thread is not None or digit_bits is not None or block_threads is not None:
+            rb, kpt, bt = self.context.tuning()
+            self.context.set_tuning(digit_bits or rb, keys_per_thread or kpt, block_threads or bt)
 
     @property
     def capacity(self) -> int:

@@ -262,3 +262,28 @@ def test_stereo_per_eye_segmented_sort(cuda):
         near = ref["margin"] < 1e-5
         bad = np.nonzero(keys[e] != ref["keys"][e])[0]
         assert np.all(near[bad])
+
+
+def test_plain_small_against_golden_fixture(cuda):
+    # The committed vectors (tests/golden/plain_small.npz): keys bit-exact
+    # away from cull thresholds, and then the order and visible count exactly.
+    import os
+
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "plain_small.npz"))
+    spec = W.plain_small_spec()
+    planes = W.generate(spec)
+    table = DeviceTable.from_f32_planes(planes, spec.count, 0)
+    out = Projector().project_sort(table, W.plain_small_camera(), Params(sort_bits=32))
+    torch.cuda.synchronize()
+    keys = out["keys"].cpu().numpy().view(np.uint32)[:spec.count]
+    bad = np.nonzero(keys != g["keys"])[0]
+    assert np.all(g["near"][bad]), f"{np.count_nonzero(~g['near'][bad])} keys differ away from cull thresholds"
+    c = int(out["count"].cpu().item())
+    if bad.size == 0:
+        assert c == int(g["visible"][0])
+        assert np.array_equal(out["indices"].cpu().numpy().view(np.uint32)[:c], g["ordering"][:c])
+    else:  # a flipped cull moves the order: check it against the oracle sort of these keys
+        ordering, visible = O.sort(keys, 32)
+        assert c == visible and np.array_equal(out["indices"].cpu().numpy().view(np.uint32)[:c], ordering[:c])
+    words = out["words"][0].cpu().numpy().view(np.uint32)[:4096]
+    assert np.array_equal((words[:, 3] & 0xFF) != 0, (g["words"][:, 3] & 0xFF) != 0)

@@ -131,3 +131,43 @@ def test_compute_keys_device_matches_host(cuda):
         k = host_u32(SortKeys.compute_device(d, bits))
         exp = np.array([O.key_compute(float(x), bits) for x in f.view(np.float32)], dtype=np.uint32)
         assert first_mismatch(k, exp) is None, bits
+
+
+@pytest.mark.parametrize("tuning", [(8, 16, 256), (6, 12, 512), (4, 8, 256)])
+def test_skewed_digits_equal_sortkeys(cuda, tuning):
+    # Rounds whose keys share one, two or a few digits (the ballot-ranked
+    # digits) next to rounds of many digits (the LDS-mask ranked ones).
+    random = Rng(11)
+    n = 150000
+    dist = 1.0 + 15.0 * random.random(n)
+    cases = {
+        "distances": (~dist.astype(np.float32).view(np.uint32)).astype(np.uint32),
+        "equal": np.full(n, 0x3C00_1234, dtype=np.uint32),
+        "two": np.where(random.random(n) < 0.5, 7, 0xBEEF_0000).astype(np.uint32),
+        "three_runs": np.repeat(np.array([5, 9, 5, 300, 9], dtype=np.uint32), n // 5),
+    }
+    ctx = Context(tuning)
+    for name, keys in cases.items():
+        print(f"skewed {name} tuning={tuning}", flush=True)
+        kd = dev_u32(keys)
+        vd = dev_u32(np.arange(keys.size, dtype=np.uint32))
+        RadixSort(kd, vd, context=ctx, digit_bits=tuning[0], keys_per_thread=tuning[1],
+                  block_threads=tuning[2]).sort(keys.size, bits=32)
+        torch.cuda.synchronize()
+        expected, _ = O.sort(keys, 32)
+        assert first_mismatch(host_u32(vd), expected) is None, name
+
+
+def test_autotune_picks_a_supported_tuning(cuda):
+    ctx = Context()
+    chosen, ms = ctx.autotune(1 << 18, bits=24, reps=2)
+    assert chosen in Context.candidates()
+    assert ms > 0.0
+    assert ctx.tuning() == chosen
+    random = Rng(5)
+    keys = np.floor(random.random(1 << 18) * 2.0 ** 24).astype(np.uint32)
+    kd, vd = dev_u32(keys), dev_u32(np.arange(keys.size, dtype=np.uint32))
+    RadixSort(kd, vd, context=ctx).sort(keys.size, bits=24)
+    torch.cuda.synchronize()
+    expected, _ = O.sort(keys, 24)
+    assert first_mismatch(host_u32(vd), expected) is None
