@@ -93,6 +93,93 @@ def oracle_projection_rate(planes, rig, count: int) -> dict:
             "sample": f"{count:,} splats of the same table, FP64 projection + SH3 + key, 1 thread"}
 
 
+def time_events(fn, steps: int, warmup: int = 2) -> float:
+    """ms per call of fn() on the current stream, HIP events around `steps` calls."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def bench_batched(steps: int) -> dict:
+    """Batched views (8 collections, 32 views at 1280x720): one graph replay of
+    the batch against 32 plain rdx_project_sort calls, both as splat-views/s
+    and as a fraction of HBM peak over sum_c 236 n_c + sum_v 80 n_v bytes."""
+    import torch
+
+    from radix_amd import Batch, DeviceTable, Params, Projector, workloads as W
+    from radix_amd.sort import Context
+
+    wl = W.batched()
+    tables = [DeviceTable.from_f32_planes(W.generate(sp), sp.count, 3) for sp in wl.specs]
+    views = list(zip(wl.view_table, wl.cameras))
+    prm = Params(sort_bits=32)
+    ctx = Context()
+    b = Batch(tables, views, prm, context=ctx)
+    splat_views = b.total
+    bytes_ = 236 * sum(sp.count for sp in wl.specs) + 80 * splat_views
+    ms_b = time_events(lambda: b.run(graph=True), steps)
+    proj = Projector(ctx)
+    outs = []
+    for t, rig in views:
+        n = tables[t].count
+        outs.append(dict(words=[torch.empty((n, 4), dtype=torch.int32, device="cuda")],
+                         keys=torch.empty(n, dtype=torch.int32, device="cuda"),
+                         indices=torch.empty(n, dtype=torch.int32, device="cuda"),
+                         count=torch.zeros(1, dtype=torch.int32, device="cuda"),
+                         args=torch.zeros(5, dtype=torch.int32, device="cuda")))
+
+    def plain_all():
+        for (t, rig), o in zip(views, outs):
+            proj.project_sort(tables[t], rig, prm, **o)
+
+    ms_p = time_events(plain_all, steps)
+    plain_bytes = sum((236 + 80) * tables[t].count for t, _ in views)
+    b.close()
+    return {"workload": "8 collections (log-uniform 1e4..6e6 splats, SH3 f32), 32 views 1280x720 on radius 4-12 orbits",
+            "collections": [sp.count for sp in wl.specs], "splat_views": splat_views,
+            "batched": {"ms": round(ms_b, 4), "Gsplat_views_per_s": round(splat_views / ms_b / 1e6, 3),
+                        "hbm_frac": round(bytes_ / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "bytes": bytes_, "form": "one HIP graph: 1 projection launch + 4 x (count, scan, scatter)"},
+            "plain": {"ms": round(ms_p, 4), "Gsplat_views_per_s": round(splat_views / ms_p / 1e6, 3),
+                      "hbm_frac_of_batched_bytes": round(bytes_ / (ms_p * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                      "bytes_read_and_written": plain_bytes, "form": "32 rdx_project_sort calls"},
+            "speedup": round(ms_p / ms_b, 3)}
+
+
+def bench_stereo(steps: int, count: int = 16777216) -> dict:
+    """Quantized stereo: packedSplats + packedSh SH3, two 2064x2208 eyes,
+    per-eye keys sorted as two segments; 64 + 2 x (20 + 60) B per splat."""
+    from radix_amd import DeviceTable, Params, Projector, workloads as W
+
+    spec = W.stereo_16m_spec(count=count)
+    planes = W.generate(spec)
+    words, chans = W.pack(planes, 3)
+    del planes
+    table = DeviceTable.from_packed(words, spec.count, (0.0, 1.0, -10.0, 2.0), chans, 3)
+    del words, chans
+    rig = W.stereo_camera()
+    prm = Params(sort_bits=32, key_per_eye=True)
+    proj = Projector()
+    out = proj.project_sort(table, rig, prm)
+    ms = time_events(lambda: proj.project_sort(table, rig, prm, words=out["words"], keys=out["keys"],
+                                               indices=out["indices"], count=out["count"], args=out["args"]), steps)
+    n = spec.count
+    bytes_ = (64 + 2 * (20 + 60)) * n
+    return {"workload": "16,777,216 packedSplats + packedSh SH3, stereo 2064x2208 (64 mm), per-eye segmented sort",
+            "ms": round(ms, 4), "Gsplats_per_s": round(n / ms / 1e6, 3),
+            "hbm_frac": round(bytes_ / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "bytes_per_splat": 224,
+            "visible_per_eye": [int(v) for v in out["count"].cpu().tolist()]}
+
+
 def measured_traffic(n: int):
     """HBM bytes per k_project launch at this size, from the committed
     rocprofv3 counter runs (FETCH_SIZE and WRITE_SIZE, each scaled by the
@@ -114,6 +201,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--count", type=int, default=4194304)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched and stereo workloads")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 512 MiB five times (the reference kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -217,6 +305,13 @@ def main() -> int:
         "stages_ms": {k: round(v["ms_avg"], 5) for k, v in stages.items()},
         "wall_s_timed": round(wall, 4),
     }
+    if not args.no_extra:
+        log("batched workload")
+        out["batched_vs_plain"] = bench_batched(max(3, args.steps // 5))
+        torch.cuda.empty_cache()
+        log("stereo 16M workload")
+        out["stereo_16m"] = bench_stereo(max(3, args.steps // 5))
+        torch.cuda.empty_cache()
     if not args.no_cpu:
         log("cpu baseline")
         keys_h = keys.cpu().numpy().view(np.uint32)

@@ -202,6 +202,26 @@ double orc_raster_loss(const orc_diff_params* P, const orc_view* v, const orc_pa
 void orc_raster_footprint(const orc_diff_params* P, const orc_view* v, const orc_params* p, const orc_raster_cfg* c,
                           uint32_t i, int* box);
 
+/* ---- The differentiable compositor and its adjoint (raster.c) ------------- */
+#define ORC_DIFF_PARAMS 59 /* centre 3, log-scale 3, quat 4, opacity logit 1, rgb 3, SH 45 */
+typedef struct OrcRaster {
+  int width, height;
+  double view[16]; /* world -> view, column-major */
+  double proj[16]; /* column-major */
+  double ndc_z_min, max_std_dev, min_alpha, blur, clip_xy, t_stop;
+  int sh_degree;
+} OrcRaster;
+typedef struct OrcDiffFrozen OrcDiffFrozen;
+int orc_diff_render(const double* theta, int n, const OrcRaster* r, double* image, double* alpha);
+double orc_diff_loss(const double* theta, int n, const OrcRaster* r, const double* target);
+double orc_diff_grad(const double* theta, int n, const OrcRaster* r, const double* target, double* grad);
+/* the loss with every pixel's contributor list fixed at theta (smooth in theta) */
+OrcDiffFrozen* orc_diff_freeze(const double* theta, int n, const OrcRaster* r, const double* target);
+double orc_diff_loss_frozen(const OrcDiffFrozen* f, const double* theta, int n, const OrcRaster* r,
+                            const double* target);
+void orc_diff_free(OrcDiffFrozen* f);
+void orc_sh_basis_grad(const double* dir, double (*grad)[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,6 +66,15 @@ class OrcBatchOut(C.Structure):
                 ("margin", C.c_void_p), ("visible", C.c_void_p)]
 
 
+class OrcRaster(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("view", _d * 16), ("proj", _d * 16),
+                ("ndc_z_min", _d), ("max_std_dev", _d), ("min_alpha", _d), ("blur", _d), ("clip_xy", _d),
+                ("t_stop", _d), ("sh_degree", C.c_int)]
+
+
+DIFF_PARAMS = 59
+
+
 def build() -> str:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
     return LIB
@@ -117,6 +126,13 @@ def lib() -> C.CDLL:
                                              C.POINTER(OrcView), C.POINTER(OrcParams), C.POINTER(OrcBatchOut)]),
             "orc_view_from_camera": (None, [_dp, _dp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, _d, _d, _d,
                                             C.POINTER(OrcView)]),
+            "orc_diff_render": (C.c_int, [_dp, C.c_int, C.POINTER(OrcRaster), _dp, _dp]),
+            "orc_diff_loss": (_d, [_dp, C.c_int, C.POINTER(OrcRaster), _dp]),
+            "orc_diff_grad": (_d, [_dp, C.c_int, C.POINTER(OrcRaster), _dp, _dp]),
+            "orc_diff_freeze": (C.c_void_p, [_dp, C.c_int, C.POINTER(OrcRaster), _dp]),
+            "orc_diff_loss_frozen": (_d, [C.c_void_p, _dp, C.c_int, C.POINTER(OrcRaster), _dp]),
+            "orc_diff_free": (None, [C.c_void_p]),
+            "orc_sh_basis_grad": (None, [_dp, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(h, name)
@@ -262,3 +278,59 @@ def _run_batch(call, n, eyes, per_eye):
         out["keys"] = out["keys"][0]
         out["distance"] = out["distance"][0]
     return out
+
+
+# ---- the differentiable compositor (raster.c) ---------------------------------
+
+
+def raster(width: int, height: int, view, proj, sh_degree: int = 3, ndc_z_min: float = 0.0,
+           max_std_dev: float = float(np.sqrt(8.0)), min_alpha: float = 0.5 / 255.0, blur: float = 0.3,
+           clip_xy: float = 1.4, t_stop: float = 1e-4) -> OrcRaster:
+    """view: world -> view 4x4 (row-major numpy); proj: 4x4 (row-major)."""
+    r = OrcRaster()
+    r.width, r.height = width, height
+    r.view[:] = list(np.asarray(view, dtype=np.float64).T.reshape(16))
+    r.proj[:] = list(np.asarray(proj, dtype=np.float64).T.reshape(16))
+    r.ndc_z_min, r.max_std_dev, r.min_alpha, r.blur, r.clip_xy, r.t_stop = ndc_z_min, max_std_dev, min_alpha, blur, clip_xy, t_stop
+    r.sh_degree = sh_degree
+    return r
+
+
+def diff_render(theta: np.ndarray, r: OrcRaster):
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    n = theta.shape[0]
+    img = np.zeros((3, r.height, r.width))
+    alpha = np.zeros((r.height, r.width))
+    drawn = lib().orc_diff_render(dptr(theta), n, C.byref(r), dptr(img), dptr(alpha))
+    return img, alpha, drawn
+
+
+def diff_loss(theta: np.ndarray, r: OrcRaster, target: np.ndarray) -> float:
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    return lib().orc_diff_loss(dptr(theta), theta.shape[0], C.byref(r), dptr(np.ascontiguousarray(target)))
+
+
+def diff_grad(theta: np.ndarray, r: OrcRaster, target: np.ndarray):
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    g = np.zeros_like(theta)
+    loss = lib().orc_diff_grad(dptr(theta), theta.shape[0], C.byref(r), dptr(np.ascontiguousarray(target)), dptr(g))
+    return loss, g
+
+
+class Frozen:
+    """The loss with every pixel's contributor list fixed at theta0."""
+
+    def __init__(self, theta0: np.ndarray, r: OrcRaster, target: np.ndarray):
+        self.r = r
+        self.target = np.ascontiguousarray(target, dtype=np.float64)
+        t0 = np.ascontiguousarray(theta0, dtype=np.float64)
+        self.h = lib().orc_diff_freeze(dptr(t0), t0.shape[0], C.byref(r), dptr(self.target))
+
+    def loss(self, theta: np.ndarray) -> float:
+        theta = np.ascontiguousarray(theta, dtype=np.float64)
+        return lib().orc_diff_loss_frozen(self.h, dptr(theta), theta.shape[0], C.byref(self.r), dptr(self.target))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_diff_free(self.h)
+            self.h = None

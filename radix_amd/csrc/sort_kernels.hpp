@@ -26,11 +26,11 @@ constexpr int MAX_BINS = 256;
 constexpr int TOT_STRIDE = MAX_BINS + 1;  // tot words per segment: digits, then the hidden-rule count
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_ITEMS = 8;             // tiles per thread per round of the scan
-// Digits a wave takes out of a round with ballots before LDS atomics take
-// the rest: a round whose keys share a few digits (the top digit of
-// clustered distances) would otherwise serialise its atomics on one address.
-// Peeling stops at the first digit held by fewer than PEEL_MIN lanes (a
-// round of spread digits then costs one ballot).
+// Digits a wave ranks in a round with one ballot each before the LDS lane
+// masks take the rest: a round whose keys share a few digits (the top
+// digit of clustered distances) would otherwise serialise its LDS atomics
+// on one address. Peeling stops at the first digit held by fewer than
+// PEEL_MIN lanes (a round of spread digits then costs one ballot).
 constexpr int PEEL = 3;
 constexpr uint32_t PEEL_MIN = 8;
 
@@ -102,8 +102,10 @@ __device__ __forceinline__ bool hidden_rule(uint32_t k, int bits) {
 // ---- count ---------------------------------------------------------------------
 
 // Tile counts of one pass's digit. Wave w holds keys [w * 64 * KPT, ...) of
-// the tile and counts them with LDS atomics into its own row (no contention
-// between waves); the rows are then summed.
+// the tile and counts them with LDS atomics into SUBS private rows per wave
+// (by lane), so a round whose keys share a digit (the top digit of
+// clustered distances) contends SUBS ways less on one address; the rows
+// (padded by one word: distinct banks) are summed at the end.
 template <int RB, int BLOCK, int KPT>
 __global__ __launch_bounds__(BLOCK) void k_count(PassArgs a) {
   constexpr int BINS = 1 << RB;
@@ -111,14 +113,16 @@ __global__ __launch_bounds__(BLOCK) void k_count(PassArgs a) {
   constexpr int WAVES = BLOCK / RDX_WAVE;
   constexpr int WTILE = RDX_WAVE * KPT;
   constexpr int TILE = BLOCK * KPT;
-  __shared__ uint32_t s_wave[WAVES][BINS];
+  constexpr int SUBS = 4;
+  constexpr int ROW = BINS + 1;
+  __shared__ uint32_t s_rows[WAVES * SUBS * ROW];
   __shared__ uint32_t s_rule[WAVES];
 
   const int tid = threadIdx.x;
   const int w = tid / RDX_WAVE;
   const int lane = tid & (RDX_WAVE - 1);
   const uint32_t tile = blockIdx.x;
-  for (int d = tid; d < WAVES * BINS; d += BLOCK) (&s_wave[0][0])[d] = 0u;
+  for (int d = tid; d < WAVES * SUBS * ROW; d += BLOCK) s_rows[d] = 0u;
   if (tid < WAVES) s_rule[tid] = 0u;
   const SegInfo g = seg_info(a, a.seg_of_tile ? a.seg_of_tile[tile] : 0u);
   const uint32_t base = (tile - g.tile0) * (uint32_t)TILE;
@@ -133,37 +137,22 @@ __global__ __launch_bounds__(BLOCK) void k_count(PassArgs a) {
     k[j] = i < g.cap ? keys[i] : 0u;
   }
   const uint32_t n = seg_count(a, g);
-  __syncthreads();  // s_wave zeroed
+  __syncthreads();  // rows zeroed
+  uint32_t* row = s_rows + (w * SUBS + (lane & (SUBS - 1))) * ROW;
   uint32_t hidden = 0;
 #pragma unroll
   for (int j = 0; j < KPT; j++) {
     const uint32_t i = wbase + (uint32_t)(j * RDX_WAVE);
     const bool in = i < n;
     if (rule && in && hidden_rule(k[j], a.bits)) hidden++;
-    const bool keep = in && !(a.drop && k[j] == a.invalid);
-    const uint32_t d = ((k[j] & a.key_mask) >> a.shift) & DMASK;
-    bool done = !keep;
-    uint64_t rem = ballot(keep);
-#pragma unroll
-    for (int q = 0; q < PEEL; q++) {
-      if (rem == 0ull) break;
-      const int first = __builtin_ctzll(rem);
-      const uint32_t d0 = (uint32_t)__builtin_amdgcn_readlane((int)d, first);
-      const uint64_t m = ballot(!done && d == d0);
-      const uint32_t c = (uint32_t)__builtin_popcountll(m);
-      if (lane == first) atomicAdd(&s_wave[w][d0], c);  // no return: no wait
-      done = done || d == d0;
-      rem &= ~m;
-      if (c < PEEL_MIN) break;
-    }
-    if (!done) atomicAdd(&s_wave[w][d], 1u);
+    if (in && !(a.drop && k[j] == a.invalid)) atomicAdd(&row[((k[j] & a.key_mask) >> a.shift) & DMASK], 1u);
   }
   if (rule && hidden) atomicAdd(&s_rule[w], hidden);
   __syncthreads();
   for (int d = tid; d < BINS; d += BLOCK) {
     uint32_t c = 0;
 #pragma unroll
-    for (int v = 0; v < WAVES; v++) c += s_wave[v][d];
+    for (int r = 0; r < WAVES * SUBS; r++) c += s_rows[r * ROW + d];
     a.cnt[(size_t)d * a.tiles + tile] = c;
   }
   if (rule && tid == 0) {
