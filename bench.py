@@ -180,6 +180,35 @@ def bench_stereo(steps: int, count: int = 16777216) -> dict:
             "visible_per_eye": [int(v) for v in out["count"].cpu().tolist()]}
 
 
+def bench_backward(steps: int) -> dict:
+    """Differentiable compositor: 262,144 splats, 512x512, 16x16 tiles, L2
+    loss against the perturbed scene; bwd Gsplats/s = n / backward time."""
+    import torch
+
+    from radix_amd import Params
+    from radix_amd import diff as D
+
+    theta, target_theta, rig = D.workload()
+    rast = D.Rasterizer(sh_degree=3)
+    target, _ = rast.forward(D.params_tensor(target_theta), rig, Params())
+    target = target.clone()
+    p = D.params_tensor(theta)
+    prm = Params()
+    img, _ = rast.forward(p, rig, prm)
+    gi = 2.0 * (img - target) / img.numel()
+    ms_f = time_events(lambda: rast.forward(p, rig, prm), steps)
+    rast.forward(p, rig, prm)
+    ms_b = time_events(lambda: rast.backward(gi), steps)
+    st = rast.stats()
+    n = p.shape[1]
+    px = rig.width * rig.height
+    bytes_ = (236 + 236 + 40) * n + 20 * px
+    return {"workload": "262,144 splats (SH3), 512x512, 16x16 tiles, T < 1e-4 stop, L2 loss; step = rdx_raster_backward",
+            "bwd_ms": round(ms_b, 4), "bwd_Gsplats_per_s": round(n / ms_b / 1e6, 3),
+            "fwd_ms": round(ms_f, 4), "fwd_Gsplats_per_s": round(n / ms_f / 1e6, 3),
+            "entries": st["entries"], "hbm_frac_lower_bound": round(bytes_ / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
 def measured_traffic(n: int):
     """HBM bytes per k_project launch at this size, from the committed
     rocprofv3 counter runs (FETCH_SIZE and WRITE_SIZE, each scaled by the
@@ -311,6 +340,9 @@ def main() -> int:
         torch.cuda.empty_cache()
         log("stereo 16M workload")
         out["stereo_16m"] = bench_stereo(max(3, args.steps // 5))
+        torch.cuda.empty_cache()
+        log("differentiable compositor")
+        out["differentiable"] = bench_backward(max(3, args.steps // 5))
         torch.cuda.empty_cache()
     if not args.no_cpu:
         log("cpu baseline")

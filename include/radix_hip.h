@@ -303,6 +303,41 @@ const uint32_t* rdx_batch_keys(const rdx_batch* batch);
 int rdx_batch_run(rdx_batch* batch, void* stream, uint32_t* out_indices_dev, uint32_t* out_counts_dev,
                   uint32_t* out_draw_args_dev, int flags);
 
+/* ---- Differentiable compositor ------------------------------------------------
+ * No reference counterpart: the reference draws the sorted splats with
+ * rasterised quads and has no backward (splatDraw.ts). This composites the
+ * ewa splats front to back over 16 x 16 pixel tiles with the draw's fragment
+ * (alpha = opacity exp(-z^2 / 2), discarded beyond the cutoff and below
+ * minAlpha: projection.tsl.ts:444-455, splatDraw.ts:136-174) and
+ * premultiplied "over" (splatDraw.ts:62-78), stopping before the splat that
+ * would take transmittance below t_stop, and differentiates it.
+ * Parameters: 59 planes of n floats (params_dev[k * n + i]): centre 3,
+ * log-scale 3, quaternion 4 (x, y, z, w), opacity LOGIT 1, base colour 3,
+ * SH bands 1-3 45 (coefficient-major). Images: 3 planes of width x height
+ * floats, row 0 at the bottom (pixel centres (x + 0.5, y + 0.5), y up);
+ * alpha = 1 - final transmittance. */
+typedef struct rdx_raster rdx_raster;
+typedef struct rdx_raster_settings {
+  int sh_degree;                  /* SH bands used, 0..3 */
+  float t_stop;                   /* transmittance stop (0: 1e-4) */
+  uint32_t max_entries_per_splat; /* (splat, tile) entry capacity per splat (0: 16) */
+} rdx_raster_settings;
+int rdx_raster_create(rdx_context* ctx, const rdx_raster_settings* settings, rdx_raster** out);
+int rdx_raster_destroy(rdx_raster* raster);
+/* The forward on a mono camera (eye 0; renderSize = camera->size) with the
+ * params' maxStdDev, minAlpha, blurAmount and clipXY. Keeps what the
+ * backward needs; params_dev must stay unchanged until then. */
+int rdx_raster_forward(rdx_raster* raster, void* stream, const float* params_dev, uint32_t n, const rdx_camera* camera,
+                       const rdx_params* params, float* image_dev, float* alpha_dev);
+/* The adjoint of the last forward: dL/dimage (3 planes) -> dL/dparams (59
+ * planes of n, overwritten). Transmittance is recomputed from each pixel's
+ * final one, back to front over its contributors. */
+int rdx_raster_backward(rdx_raster* raster, void* stream, const float* grad_image_dev, float* grad_params_dev);
+/* The last forward's drawn splats, (splat, tile) entries, entry capacity and
+ * whether they overflowed it (then nothing was drawn). Synchronizes. */
+int rdx_raster_stats(rdx_raster* raster, void* stream, uint32_t* visible, uint32_t* entries, uint32_t* capacity,
+                     int* overflowed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -117,6 +117,10 @@ class BatchView(C.Structure):
     ]
 
 
+class RasterSettings(C.Structure):
+    _fields_ = [("sh_degree", C.c_int), ("t_stop", C.c_float), ("max_entries_per_splat", C.c_uint32)]
+
+
 BATCH_GRAPH = 1
 BATCH_MAX_VIEWS = 4096
 
@@ -176,6 +180,13 @@ _SIGNATURES = [
     ("rdx_batch_segment_offset", C.c_uint64, [C.c_void_p, C.c_int]),
     ("rdx_batch_keys", C.c_void_p, [C.c_void_p]),
     ("rdx_batch_run", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    ("rdx_raster_create", C.c_int, [C.c_void_p, C.POINTER(RasterSettings), C.POINTER(C.c_void_p)]),
+    ("rdx_raster_destroy", C.c_int, [C.c_void_p]),
+    ("rdx_raster_forward", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(Camera), C.POINTER(Params), C.c_void_p, C.c_void_p]),
+    ("rdx_raster_backward", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rdx_raster_stats", C.c_int,
+     [C.c_void_p, C.c_void_p, _u32p, _u32p, _u32p, C.POINTER(C.c_int)]),
 ]
 
 _lib = None

@@ -1,0 +1,668 @@
+// The differentiable compositor for gfx950: front-to-back alpha compositing
+// of the ewa splats over 16 x 16 pixel tiles, and its adjoint.
+//
+// No reference counterpart as a whole: the reference draws the sorted splats
+// with rasterised quads and premultiplied "over" blending and has no
+// backward (src/render/splatDraw.ts). The compositor follows its fragment:
+// alpha = opacity * exp(-z^2 / 2) at falloff 1 (projection.tsl.ts:444-455),
+// z^2 the squared Mahalanobis distance under the blurred 2D covariance
+// (projections.tsl.ts:155-167), discarded where z^2 > cutoff^2 or alpha <
+// minAlpha, cutoff = min(maxStdDev, sqrt(2 ln(opacity / minAlpha)))
+// (splatDraw.ts:136-145, 165-174), splats in the sort's order (keys
+// ascending = back to front, composited here front to back). Compositing
+// stops before the splat that would bring transmittance below t_stop.
+//
+// Parameters: 59 planes of n floats (the f32 table's planes, with the
+// opacity plane holding the LOGIT): centre 3, log-scale 3, quaternion 4
+// (x, y, z, w), opacity logit 1, base colour 3, SH bands 1-3 45
+// (coefficient-major). Pixels (x + 0.5, y + 0.5), y up; images planar
+// [channel][y][x].
+//
+// Pipeline (raster.hip): k_diff_pre (per splat: centre, conic, opacity,
+// cutoff, colour, key, tile rectangle) -> radix sort of the keys (back to
+// front) -> scan of the tile counts in front-to-back order -> k_diff_emit
+// (one entry per (splat, tile), in front-to-back order) -> stable radix sort
+// of the entries by tile -> k_diff_ranges -> k_diff_render (a block per
+// tile). Backward: k_diff_render_bwd (a block per tile, back to front from
+// each pixel's last contributor, transmittance recomputed by division from
+// the final one), k_diff_gather (per splat, the sum of its entries'
+// partials) and k_diff_pre_bwd (the chain rule to the 59 parameters).
+#pragma once
+
+#include "common.hpp"
+
+namespace rdx {
+
+constexpr int DIFF_PARAMS = 59;
+constexpr int DIFF_TILE = 16;
+constexpr int DIFF_BLOCK = DIFF_TILE * DIFF_TILE;
+constexpr int DIFF_PIX_GRADS = 9;  // mu2 2, conic 3, opacity 1, colour 3
+
+struct DiffU {
+  double Vl[9], Vt[3], cam[3];  // world -> view (row-major linear part), camera position
+  double P[16];                 // projection, row-major
+  float fx, fy;                 // renderSize / 2 * (P00, P11)
+  int width, height, tiles_x, tiles_y;
+  float ndc_z_min, max_std_dev, min_alpha, blur, clip_xy, t_stop;
+  int sh_degree;
+};
+
+// ---- SH basis (sh.ts:376-407, the reference's constants) and its gradient ----
+
+__device__ __forceinline__ void sh_basis15(float x, float y, float z, float* o) {
+  const float xx = x * x, yy = y * y, zz = z * z;
+  o[0] = -0.4886025f * y;
+  o[1] = 0.4886025f * z;
+  o[2] = -0.4886025f * x;
+  o[3] = 1.0925484f * x * y;
+  o[4] = -1.0925484f * y * z;
+  o[5] = 0.3153916f * (2.0f * zz - xx - yy);
+  o[6] = -1.0925484f * x * z;
+  o[7] = 0.5462742f * (xx - yy);
+  o[8] = -0.5900436f * y * (3.0f * xx - yy);
+  o[9] = 2.8906114f * x * y * z;
+  o[10] = -0.4570458f * y * (4.0f * zz - xx - yy);
+  o[11] = 0.3731763f * z * (2.0f * zz - 3.0f * xx - 3.0f * yy);
+  o[12] = -0.4570458f * x * (4.0f * zz - xx - yy);
+  o[13] = 1.4453057f * z * (xx - yy);
+  o[14] = -0.5900436f * x * (xx - 3.0f * yy);
+}
+
+// d(basis)/d(direction): g[k] = (d/dx, d/dy, d/dz) of o[k].
+__device__ __forceinline__ void sh_basis15_grad(float x, float y, float z, float3* g) {
+  const float xx = x * x, yy = y * y, zz = z * z;
+  g[0] = make_float3(0.0f, -0.4886025f, 0.0f);
+  g[1] = make_float3(0.0f, 0.0f, 0.4886025f);
+  g[2] = make_float3(-0.4886025f, 0.0f, 0.0f);
+  g[3] = make_float3(1.0925484f * y, 1.0925484f * x, 0.0f);
+  g[4] = make_float3(0.0f, -1.0925484f * z, -1.0925484f * y);
+  g[5] = make_float3(-0.6307832f * x, -0.6307832f * y, 1.2615664f * z);
+  g[6] = make_float3(-1.0925484f * z, 0.0f, -1.0925484f * x);
+  g[7] = make_float3(1.0925484f * x, -1.0925484f * y, 0.0f);
+  g[8] = make_float3(-0.5900436f * 6.0f * x * y, -0.5900436f * (3.0f * xx - 3.0f * yy), 0.0f);
+  g[9] = make_float3(2.8906114f * y * z, 2.8906114f * x * z, 2.8906114f * x * y);
+  g[10] = make_float3(0.9140916f * x * y, -0.4570458f * (4.0f * zz - xx - 3.0f * yy), -3.6563664f * y * z);
+  g[11] = make_float3(-2.2390578f * x * z, -2.2390578f * y * z, 0.3731763f * (6.0f * zz - 3.0f * xx - 3.0f * yy));
+  g[12] = make_float3(-0.4570458f * (4.0f * zz - 3.0f * xx - yy), 0.9140916f * x * y, -3.6563664f * x * z);
+  g[13] = make_float3(2.8906114f * x * z, -2.8906114f * y * z, 1.4453057f * (xx - yy));
+  g[14] = make_float3(-0.5900436f * (3.0f * xx - 3.0f * yy), 3.5402616f * x * y, 0.0f);
+}
+
+// ---- one splat's forward -------------------------------------------------------
+
+struct DiffSplat {
+  bool ok;
+  double t[3], dist, clip[4];
+  float mu2x, mu2y;
+  float qv[4], qlen, R[9], s[3], M[9], Sig[9];
+  float J[6], T[6];
+  float a, b, d, ap, dp, det_o, det_p, ratio, op, opc, cut;
+  float Q0, Q1, Q2;
+  float dir[3], dlen, Y[15], col_raw[3], col[3];
+};
+
+__device__ __forceinline__ float plane(const float* th, uint32_t n, int k, uint32_t i) { return th[(size_t)k * n + i]; }
+
+__device__ inline void diff_splat_forward(const float* th, uint32_t n, uint32_t i, const DiffU& u, DiffSplat& s) {
+  const double mu[3] = {plane(th, n, 0, i), plane(th, n, 1, i), plane(th, n, 2, i)};
+#pragma unroll
+  for (int r = 0; r < 3; r++) s.t[r] = u.Vl[3 * r] * mu[0] + u.Vl[3 * r + 1] * mu[1] + u.Vl[3 * r + 2] * mu[2] + u.Vt[r];
+  s.dist = sqrt(s.t[0] * s.t[0] + s.t[1] * s.t[1] + s.t[2] * s.t[2]);
+#pragma unroll
+  for (int r = 0; r < 4; r++) s.clip[r] = u.P[4 * r] * s.t[0] + u.P[4 * r + 1] * s.t[1] + u.P[4 * r + 2] * s.t[2] + u.P[4 * r + 3];
+  const double w = s.clip[3];
+  const bool cull_depth = s.t[2] >= 0.0 || s.clip[2] < (double)u.ndc_z_min * w || s.clip[2] >= w;
+  const double limit = fmin((double)u.clip_xy, 2.0) * w;
+  const bool cull_xy = fabs(s.clip[0]) > limit || fabs(s.clip[1]) > limit;
+  s.mu2x = (float)((s.clip[0] / w + 1.0) * 0.5 * u.width);
+  s.mu2y = (float)((s.clip[1] / w + 1.0) * 0.5 * u.height);
+  // covariance
+  float q[4] = {plane(th, n, 6, i), plane(th, n, 7, i), plane(th, n, 8, i), plane(th, n, 9, i)};
+  s.qlen = sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+#pragma unroll
+  for (int k = 0; k < 4; k++) s.qv[k] = s.qlen > 0.0f ? q[k] / s.qlen : (k == 3 ? 1.0f : 0.0f);
+  const float qx = s.qv[0], qy = s.qv[1], qz = s.qv[2], qw = s.qv[3];
+  s.R[0] = 1.0f - 2.0f * (qy * qy + qz * qz);
+  s.R[1] = 2.0f * (qx * qy - qw * qz);
+  s.R[2] = 2.0f * (qx * qz + qw * qy);
+  s.R[3] = 2.0f * (qx * qy + qw * qz);
+  s.R[4] = 1.0f - 2.0f * (qx * qx + qz * qz);
+  s.R[5] = 2.0f * (qy * qz - qw * qx);
+  s.R[6] = 2.0f * (qx * qz - qw * qy);
+  s.R[7] = 2.0f * (qy * qz + qw * qx);
+  s.R[8] = 1.0f - 2.0f * (qx * qx + qy * qy);
+#pragma unroll
+  for (int k = 0; k < 3; k++) s.s[k] = __expf(plane(th, n, 3 + k, i));
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) s.M[3 * r + c] = s.R[3 * r + c] * s.s[c];
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      s.Sig[3 * r + c] = s.M[3 * r] * s.M[3 * c] + s.M[3 * r + 1] * s.M[3 * c + 1] + s.M[3 * r + 2] * s.M[3 * c + 2];
+  const float x = (float)s.t[0], y = (float)s.t[1], z = (float)s.t[2];
+  const float iz = 1.0f / z;
+  s.J[0] = u.fx * iz;
+  s.J[1] = 0.0f;
+  s.J[2] = -u.fx * x * iz * iz;
+  s.J[3] = 0.0f;
+  s.J[4] = u.fy * iz;
+  s.J[5] = -u.fy * y * iz * iz;
+#pragma unroll
+  for (int r = 0; r < 2; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      s.T[3 * r + c] = s.J[3 * r] * (float)u.Vl[c] + s.J[3 * r + 1] * (float)u.Vl[3 + c] + s.J[3 * r + 2] * (float)u.Vl[6 + c];
+  float TS[6];
+#pragma unroll
+  for (int r = 0; r < 2; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) TS[3 * r + c] = s.T[3 * r] * s.Sig[c] + s.T[3 * r + 1] * s.Sig[3 + c] + s.T[3 * r + 2] * s.Sig[6 + c];
+  s.a = TS[0] * s.T[0] + TS[1] * s.T[1] + TS[2] * s.T[2];
+  s.b = TS[0] * s.T[3] + TS[1] * s.T[4] + TS[2] * s.T[5];
+  s.d = TS[3] * s.T[3] + TS[4] * s.T[4] + TS[5] * s.T[5];
+  s.ap = s.a + u.blur;
+  s.dp = s.d + u.blur;
+  s.det_o = s.a * s.d - s.b * s.b;
+  s.det_p = s.ap * s.dp - s.b * s.b;
+  s.ratio = s.det_o > 0.0f ? s.det_o / s.det_p : 0.0f;
+  s.op = 1.0f / (1.0f + __expf(-plane(th, n, 10, i)));
+  s.opc = s.op * sqrtf(s.ratio);
+  const float idet = 1.0f / s.det_p;
+  s.Q0 = s.dp * idet;
+  s.Q1 = -s.b * idet;
+  s.Q2 = s.ap * idet;
+  s.cut = fminf(u.max_std_dev, sqrtf(2.0f * __logf(fmaxf(s.opc, u.min_alpha) / u.min_alpha)));
+  // colour: base + SH along the direction from the camera, clamped at 0
+  const float dv[3] = {(float)(mu[0] - u.cam[0]), (float)(mu[1] - u.cam[1]), (float)(mu[2] - u.cam[2])};
+  s.dlen = sqrtf(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+#pragma unroll
+  for (int k = 0; k < 3; k++) s.dir[k] = s.dlen > 0.0f ? dv[k] / s.dlen : 0.0f;
+  sh_basis15(s.dir[0], s.dir[1], s.dir[2], s.Y);
+  const int nk = u.sh_degree >= 3 ? 15 : (u.sh_degree == 2 ? 8 : (u.sh_degree == 1 ? 3 : 0));
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    float v = plane(th, n, 11 + c, i);
+    for (int k = 0; k < nk; k++) v = fmaf(plane(th, n, 14 + 3 * k + c, i), s.Y[k], v);
+    s.col_raw[c] = v;
+    s.col[c] = fmaxf(v, 0.0f);
+  }
+  s.ok = !cull_depth && !cull_xy && s.op >= u.min_alpha && s.opc >= u.min_alpha && s.det_p > 0.0f;
+}
+
+// ---- per splat: what the pixels need, the key and the tiles -------------------
+
+// pre[3 i .. 3 i + 2] = (mu2x, mu2y, opc, cut), (Q0, Q1, Q2, 0), (r, g, b, 0)
+static __global__ __launch_bounds__(256) void k_diff_pre(const float* __restrict__ th, uint32_t n, DiffU u,
+                                                         float4* __restrict__ pre, uint32_t* __restrict__ keys,
+                                                         uint32_t* __restrict__ touch, uint4* __restrict__ rect) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  DiffSplat s;
+  diff_splat_forward(th, n, i, u, s);
+  pre[3 * (size_t)i] = make_float4(s.mu2x, s.mu2y, s.opc, s.cut);
+  pre[3 * (size_t)i + 1] = make_float4(s.Q0, s.Q1, s.Q2, 0.0f);
+  pre[3 * (size_t)i + 2] = make_float4(s.col[0], s.col[1], s.col[2], 0.0f);
+  uint32_t cnt = 0;
+  uint4 rc = make_uint4(0u, 0u, 0u, 0u);
+  if (s.ok) {
+    // the cutoff ellipse's bounding box: cut * sqrt of the covariance diagonal
+    const float rx = s.cut * sqrtf(s.ap), ry = s.cut * sqrtf(s.dp);
+    const float x0 = s.mu2x - rx, x1 = s.mu2x + rx, y0 = s.mu2y - ry, y1 = s.mu2y + ry;
+    if (x1 >= 0.0f && y1 >= 0.0f && x0 < (float)u.width && y0 < (float)u.height) {
+      const int tx0 = max(0, (int)floorf(x0 / DIFF_TILE)), tx1 = min(u.tiles_x - 1, (int)floorf(x1 / DIFF_TILE));
+      const int ty0 = max(0, (int)floorf(y0 / DIFF_TILE)), ty1 = min(u.tiles_y - 1, (int)floorf(y1 / DIFF_TILE));
+      rc = make_uint4((uint32_t)tx0, (uint32_t)ty0, (uint32_t)tx1, (uint32_t)ty1);
+      cnt = (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1));
+    }
+  }
+  keys[i] = s.ok ? sort_key(fbits((float)s.dist), 32) : 0xffffffffu;
+  touch[i] = cnt;
+  rect[i] = rc;
+}
+
+// ---- exclusive scan of the tile counts in front-to-back order -------------------
+// Position j (front to back) is splat order[m - 1 - j] of the back-to-front
+// order. Blocks of SCAN_TILE positions: partial sums, then the block sums in
+// one block, then the offsets.
+constexpr int DSCAN_BLOCK = 256;
+constexpr int DSCAN_ITEMS = 4;
+constexpr int DSCAN_TILE = DSCAN_BLOCK * DSCAN_ITEMS;
+
+__device__ __forceinline__ uint32_t f2b_touch(const uint32_t* order, const uint32_t* touch, uint32_t m, uint32_t j) {
+  return j < m ? touch[order[m - 1 - j]] : 0u;
+}
+
+static __global__ __launch_bounds__(DSCAN_BLOCK) void k_diff_scan_partial(const uint32_t* order, const uint32_t* touch,
+                                                                          const uint32_t* m_dev, uint32_t* bsum) {
+  __shared__ uint32_t s_scan[DSCAN_BLOCK / RDX_WAVE];
+  const uint32_t m = *m_dev;
+  const uint32_t j0 = blockIdx.x * DSCAN_TILE + threadIdx.x * DSCAN_ITEMS;
+  uint32_t v = 0;
+#pragma unroll
+  for (int k = 0; k < DSCAN_ITEMS; k++) v += f2b_touch(order, touch, m, j0 + k);
+  uint32_t total;
+  (void)block_exclusive_sum<DSCAN_BLOCK>(v, s_scan, &total);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// One block: exclusive scan of the block sums in place; the total to *total,
+// or 0 with *overflow set when the entries would not fit in `cap` (nothing
+// is then drawn: no kernel reads an entry that was not written).
+static __global__ __launch_bounds__(1024) void k_diff_scan_blocks(uint32_t* bsum, uint32_t blocks, uint32_t* total,
+                                                                   uint32_t cap, uint32_t* overflow) {
+  __shared__ uint32_t s_scan[1024 / RDX_WAVE];
+  uint32_t carry = 0;
+  for (uint32_t b0 = 0; b0 < blocks; b0 += 1024) {
+    const uint32_t b = b0 + threadIdx.x;
+    const uint32_t v = b < blocks ? bsum[b] : 0u;
+    uint32_t sum;
+    const uint32_t ex = block_exclusive_sum<1024>(v, s_scan, &sum);
+    if (b < blocks) bsum[b] = carry + ex;
+    carry += sum;
+  }
+  if (threadIdx.x == 0) {
+    *total = carry <= cap ? carry : 0u;
+    if (carry > cap) *overflow = 1u;
+  }
+}
+
+static __global__ __launch_bounds__(DSCAN_BLOCK) void k_diff_scan_final(const uint32_t* order, const uint32_t* touch,
+                                                                        const uint32_t* m_dev, const uint32_t* bsum,
+                                                                        uint32_t* offsets) {
+  __shared__ uint32_t s_scan[DSCAN_BLOCK / RDX_WAVE];
+  const uint32_t m = *m_dev;
+  const uint32_t j0 = blockIdx.x * DSCAN_TILE + threadIdx.x * DSCAN_ITEMS;
+  uint32_t v[DSCAN_ITEMS], sum = 0;
+#pragma unroll
+  for (int k = 0; k < DSCAN_ITEMS; k++) {
+    v[k] = f2b_touch(order, touch, m, j0 + k);
+    sum += v[k];
+  }
+  uint32_t total;
+  uint32_t run = bsum[blockIdx.x] + block_exclusive_sum<DSCAN_BLOCK>(sum, s_scan, &total);
+#pragma unroll
+  for (int k = 0; k < DSCAN_ITEMS; k++) {
+    if (j0 + k < m) offsets[j0 + k] = run;
+    run += v[k];
+  }
+}
+
+// ---- entries: one per (splat, tile), front to back -------------------------------
+
+static __global__ __launch_bounds__(256) void k_diff_emit(const uint32_t* order, const uint32_t* m_dev, const uint32_t* touch,
+                                                          const uint4* rect, const uint32_t* offsets, uint32_t tiles_x,
+                                                          uint32_t cap, uint32_t* tile_key, uint32_t* emit_idx,
+                                                          uint32_t* emit_splat, uint32_t* overflow) {
+  const uint32_t m = *m_dev;
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const uint32_t sp = order[m - 1 - j];
+  const uint32_t cnt = touch[sp];
+  if (cnt == 0u) return;
+  const uint4 r = rect[sp];
+  uint32_t q = offsets[j];
+  if (q + cnt > cap) {
+    store_agent(overflow, 1u);
+    return;
+  }
+  for (uint32_t ty = r.y; ty <= r.w; ty++)
+    for (uint32_t tx = r.x; tx <= r.z; tx++, q++) {
+      tile_key[q] = ty * tiles_x + tx;
+      emit_idx[q] = q;
+      emit_splat[q] = sp;
+    }
+}
+
+// Each tile's [start, end) in the tile-sorted entries (tiles with none keep 0, 0).
+static __global__ void k_diff_ranges(const uint32_t* tile_key, const uint32_t* e_dev, uint32_t cap, uint32_t tiles,
+                                     uint2* range) {
+  const uint32_t e = min(*e_dev, cap);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < e; k += gridDim.x * blockDim.x) {
+    const uint32_t t = tile_key[k];
+    if (t >= tiles) continue;  // never: every entry below e was written with its tile
+    if (k == 0 || tile_key[k - 1] != t) range[t].x = k;
+    if (k == e - 1 || tile_key[k + 1] != t) range[t].y = k + 1;
+  }
+}
+
+// counts: visible splats, entries, overflow, a zero word
+static __global__ void k_fill_counts(uint32_t* counts) {
+  if (threadIdx.x < 4) counts[threadIdx.x] = 0u;
+}
+
+static __global__ void k_diff_clear_ranges(uint2* range, uint32_t tiles) {
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tiles; t += gridDim.x * blockDim.x)
+    range[t] = make_uint2(0u, 0u);
+}
+
+// ---- the tile renderer -----------------------------------------------------------
+
+struct DiffPixel {
+  float dx, dy, z2;
+};
+
+__device__ __forceinline__ float pixel_alpha(float4 p0, float4 p1, float px, float py, float min_alpha, float& dx,
+                                             float& dy, float& g) {
+  dx = px - p0.x;
+  dy = py - p0.y;
+  const float z2 = p1.x * dx * dx + 2.0f * p1.y * dx * dy + p1.z * dy * dy;
+  if (z2 > p0.w * p0.w) return -1.0f;
+  g = __expf(-0.5f * z2);
+  const float a = p0.z * g;
+  return a < min_alpha ? -1.0f : a;
+}
+
+static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render(const uint2* range, const uint32_t* emit_idx,
+                                                                   const uint32_t* emit_splat, const float4* pre, DiffU u,
+                                                                   float* image, float* alpha_out, float* t_final,
+                                                                   uint32_t* n_contrib) {
+  __shared__ float4 s_p0[DIFF_BLOCK], s_p1[DIFF_BLOCK], s_p2[DIFF_BLOCK];
+  const uint32_t tile = blockIdx.x;
+  const int tx = (int)(tile % (uint32_t)u.tiles_x), ty = (int)(tile / (uint32_t)u.tiles_x);
+  const int x = tx * DIFF_TILE + (int)(threadIdx.x % DIFF_TILE), y = ty * DIFF_TILE + (int)(threadIdx.x / DIFF_TILE);
+  const bool inside = x < u.width && y < u.height;
+  const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+  const uint2 rg = range[tile];
+  float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
+  uint32_t last = 0;
+  bool done = !inside;
+  for (uint32_t b = rg.x; b < rg.y; b += DIFF_BLOCK) {
+    if (__syncthreads_count(!done) == 0) break;
+    const uint32_t e = b + threadIdx.x;
+    if (e < rg.y) {
+      const uint32_t sp = emit_splat[emit_idx[e]];
+      s_p0[threadIdx.x] = pre[3 * (size_t)sp];
+      s_p1[threadIdx.x] = pre[3 * (size_t)sp + 1];
+      s_p2[threadIdx.x] = pre[3 * (size_t)sp + 2];
+    }
+    __syncthreads();
+    const uint32_t cnt = min((uint32_t)DIFF_BLOCK, rg.y - b);
+    for (uint32_t k = 0; k < cnt && !done; k++) {
+      float dx, dy, g;
+      const float a = pixel_alpha(s_p0[k], s_p1[k], px, py, u.min_alpha, dx, dy, g);
+      if (a < 0.0f) continue;
+      const float Tn = T * (1.0f - a);
+      if (Tn < u.t_stop) {
+        done = true;
+        break;
+      }
+      const float4 c = s_p2[k];
+      C0 = fmaf(c.x * a, T, C0);
+      C1 = fmaf(c.y * a, T, C1);
+      C2 = fmaf(c.z * a, T, C2);
+      T = Tn;
+      last = b - rg.x + k + 1;
+    }
+    __syncthreads();
+  }
+  if (!inside) return;
+  const size_t p = (size_t)y * u.width + x, plane_px = (size_t)u.width * u.height;
+  image[p] = C0;
+  image[plane_px + p] = C1;
+  image[2 * plane_px + p] = C2;
+  if (alpha_out) alpha_out[p] = 1.0f - T;
+  t_final[p] = T;
+  n_contrib[p] = last;
+}
+
+// Sum over a wave64 to lane 63 (DPP: rows, then row broadcasts).
+__device__ __forceinline__ float wave_sum_to_last(float x) {
+  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x111, 0xf, 0xf, true));
+  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x112, 0xf, 0xf, true));
+  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x114, 0xf, 0xf, true));
+  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x118, 0xf, 0xf, true));
+  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x142, 0xa, 0xf, false));
+  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x143, 0xc, 0xf, false));
+  return x;
+}
+
+// The adjoint of the renderer: per tile, back to front over each pixel's
+// contributors; every entry's pixel-side partials summed over the tile's
+// pixels (per wave with DPP, then the waves in a fixed order: no float
+// atomics, reproducible) into gentry[emission index][9].
+static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render_bwd(const uint2* range, const uint32_t* emit_idx,
+                                                                       const uint32_t* emit_splat, const float4* pre, DiffU u,
+                                                                       const float* t_final, const uint32_t* n_contrib,
+                                                                       const float* grad_image, float* gentry) {
+  constexpr int WAVES = DIFF_BLOCK / RDX_WAVE;
+  __shared__ float4 s_p0[DIFF_BLOCK], s_p1[DIFF_BLOCK], s_p2[DIFF_BLOCK];
+  __shared__ uint32_t s_q[DIFF_BLOCK];
+  __shared__ float s_g[WAVES][DIFF_BLOCK][DIFF_PIX_GRADS];
+  __shared__ uint32_t s_max[WAVES];
+  const uint32_t tile = blockIdx.x;
+  const int lane = threadIdx.x & (RDX_WAVE - 1), w = threadIdx.x / RDX_WAVE;
+  const int tx = (int)(tile % (uint32_t)u.tiles_x), ty = (int)(tile / (uint32_t)u.tiles_x);
+  const int x = tx * DIFF_TILE + (int)(threadIdx.x % DIFF_TILE), y = ty * DIFF_TILE + (int)(threadIdx.x / DIFF_TILE);
+  const bool inside = x < u.width && y < u.height;
+  const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+  const uint2 rg = range[tile];
+  const size_t p = (size_t)y * u.width + x, plane_px = (size_t)u.width * u.height;
+  float T = inside ? t_final[p] : 1.0f;
+  const uint32_t nc = inside ? n_contrib[p] : 0u;
+  float dC0 = 0.0f, dC1 = 0.0f, dC2 = 0.0f;
+  if (inside) {
+    dC0 = grad_image[p];
+    dC1 = grad_image[plane_px + p];
+    dC2 = grad_image[2 * plane_px + p];
+  }
+  float B0 = 0.0f, B1 = 0.0f, B2 = 0.0f;
+  // the furthest any pixel of the tile went
+  uint32_t m = nc;
+  for (int off = 32; off >= 1; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off));
+  if (lane == 0) s_max[w] = m;
+  __syncthreads();
+  uint32_t reach = 0;
+  for (int v = 0; v < WAVES; v++) reach = max(reach, s_max[v]);
+  const uint32_t end = rg.x + reach;
+  // entries never reached get zero partials
+  for (uint32_t e = end + threadIdx.x; e < rg.y; e += DIFF_BLOCK)
+    for (int k = 0; k < DIFF_PIX_GRADS; k++) gentry[(size_t)emit_idx[e] * DIFF_PIX_GRADS + k] = 0.0f;
+  uint32_t hi = end;
+  while (hi > rg.x) {
+    const uint32_t lo = hi - min((uint32_t)DIFF_BLOCK, hi - rg.x);
+    const uint32_t cnt = hi - lo;
+    __syncthreads();  // the previous batch's write-out is done
+    if (threadIdx.x < cnt) {
+      const uint32_t q = emit_idx[lo + threadIdx.x];
+      const uint32_t sp = emit_splat[q];
+      s_q[threadIdx.x] = q;
+      s_p0[threadIdx.x] = pre[3 * (size_t)sp];
+      s_p1[threadIdx.x] = pre[3 * (size_t)sp + 1];
+      s_p2[threadIdx.x] = pre[3 * (size_t)sp + 2];
+    }
+    for (int k = threadIdx.x; k < WAVES * DIFF_BLOCK * DIFF_PIX_GRADS; k += DIFF_BLOCK) (&s_g[0][0][0])[k] = 0.0f;
+    __syncthreads();
+    for (int k = (int)cnt - 1; k >= 0; k--) {
+      const uint32_t idx = lo - rg.x + (uint32_t)k;  // position in the tile's list
+      float gv[DIFF_PIX_GRADS];
+      bool contrib = false;
+      if (idx < nc) {
+        const float4 p0 = s_p0[k], p1 = s_p1[k], c = s_p2[k];
+        float dx, dy, g;
+        const float a = pixel_alpha(p0, p1, px, py, u.min_alpha, dx, dy, g);
+        if (a >= 0.0f) {
+          contrib = true;
+          T = T / (1.0f - a);  // the transmittance in front of this splat
+          const float aT = a * T;
+          gv[6] = aT * dC0;
+          gv[7] = aT * dC1;
+          gv[8] = aT * dC2;
+          const float galpha = T * ((c.x - B0) * dC0 + (c.y - B1) * dC1 + (c.z - B2) * dC2);
+          B0 = fmaf(c.x, a, (1.0f - a) * B0);
+          B1 = fmaf(c.y, a, (1.0f - a) * B1);
+          B2 = fmaf(c.z, a, (1.0f - a) * B2);
+          gv[5] = g * galpha;
+          const float gz2 = -0.5f * a * galpha;
+          gv[0] = gz2 * -2.0f * (p1.x * dx + p1.y * dy);
+          gv[1] = gz2 * -2.0f * (p1.y * dx + p1.z * dy);
+          gv[2] = gz2 * dx * dx;
+          gv[3] = gz2 * 2.0f * dx * dy;
+          gv[4] = gz2 * dy * dy;
+        }
+      }
+      if (!contrib)
+        for (int j = 0; j < DIFF_PIX_GRADS; j++) gv[j] = 0.0f;
+      if (ballot(contrib)) {
+#pragma unroll
+        for (int j = 0; j < DIFF_PIX_GRADS; j++) {
+          const float sum = wave_sum_to_last(gv[j]);
+          if (lane == RDX_WAVE - 1) s_g[w][k][j] = sum;
+        }
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < cnt) {
+      const size_t base = (size_t)s_q[threadIdx.x] * DIFF_PIX_GRADS;
+#pragma unroll
+      for (int j = 0; j < DIFF_PIX_GRADS; j++) {
+        float v = 0.0f;
+        for (int ww = 0; ww < WAVES; ww++) v += s_g[ww][threadIdx.x][j];
+        gentry[base + j] = v;
+      }
+    }
+    hi = lo;
+  }
+}
+
+// Per splat (front-to-back position j): the sum of its entries' partials.
+static __global__ void k_diff_gather(const uint32_t* order, const uint32_t* m_dev, const uint32_t* touch,
+                                     const uint32_t* offsets, const float* gentry, const uint32_t* e_dev, float* gpix,
+                                     uint32_t n) {
+  const uint32_t m = *m_dev;
+  const uint32_t cap = *e_dev;
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  if (j >= m) return;
+  const uint32_t sp = order[m - 1 - j];
+  const uint32_t q0 = offsets[j], cnt = touch[sp];
+  float acc[DIFF_PIX_GRADS];
+  for (int k = 0; k < DIFF_PIX_GRADS; k++) acc[k] = 0.0f;
+  if (q0 + cnt <= cap)
+    for (uint32_t q = q0; q < q0 + cnt; q++)
+      for (int k = 0; k < DIFF_PIX_GRADS; k++) acc[k] += gentry[(size_t)q * DIFF_PIX_GRADS + k];
+  for (int k = 0; k < DIFF_PIX_GRADS; k++) gpix[(size_t)sp * DIFF_PIX_GRADS + k] = acc[k];
+}
+
+static __global__ void k_diff_clear_f32(float* p, size_t count) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0f;
+}
+
+// The chain rule from the pixel-side partials to the 59 parameters.
+static __global__ __launch_bounds__(256) void k_diff_pre_bwd(const float* __restrict__ th, uint32_t n, DiffU u,
+                                                             const float* __restrict__ gpix, float* __restrict__ grad) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  DiffSplat s;
+  diff_splat_forward(th, n, i, u, s);
+  float gout[DIFF_PARAMS];
+  for (int k = 0; k < DIFF_PARAMS; k++) gout[k] = 0.0f;
+  if (s.ok) {
+    const float* G = gpix + (size_t)i * DIFF_PIX_GRADS;
+    const float gmu2x = G[0], gmu2y = G[1], gQ0 = G[2], gQ1 = G[3], gQ2 = G[4], gopc = G[5];
+    // colour
+    const int nk = u.sh_degree >= 3 ? 15 : (u.sh_degree == 2 ? 8 : (u.sh_degree == 1 ? 3 : 0));
+    float graw[3];
+    for (int c = 0; c < 3; c++) graw[c] = s.col_raw[c] > 0.0f ? G[6 + c] : 0.0f;
+    for (int c = 0; c < 3; c++) gout[11 + c] = graw[c];
+    float3 Yg[15];
+    sh_basis15_grad(s.dir[0], s.dir[1], s.dir[2], Yg);
+    float gdir[3] = {0.0f, 0.0f, 0.0f};
+    for (int k = 0; k < nk; k++)
+      for (int c = 0; c < 3; c++) {
+        gout[14 + 3 * k + c] = s.Y[k] * graw[c];
+        const float cg = plane(th, n, 14 + 3 * k + c, i) * graw[c];
+        gdir[0] = fmaf(cg, Yg[k].x, gdir[0]);
+        gdir[1] = fmaf(cg, Yg[k].y, gdir[1]);
+        gdir[2] = fmaf(cg, Yg[k].z, gdir[2]);
+      }
+    float gmu[3] = {0.0f, 0.0f, 0.0f};
+    if (s.dlen > 0.0f) {
+      const float dd = gdir[0] * s.dir[0] + gdir[1] * s.dir[1] + gdir[2] * s.dir[2];
+      for (int e = 0; e < 3; e++) gmu[e] += (gdir[e] - s.dir[e] * dd) / s.dlen;
+    }
+    // conic = inverse(cov2D'): dcov' = -Q G Q
+    const float G00 = gQ0, G01 = 0.5f * gQ1, G11 = gQ2;
+    const float QG00 = s.Q0 * G00 + s.Q1 * G01, QG01 = s.Q0 * G01 + s.Q1 * G11;
+    const float QG10 = s.Q1 * G00 + s.Q2 * G01, QG11 = s.Q1 * G01 + s.Q2 * G11;
+    float ga = -(QG00 * s.Q0 + QG01 * s.Q1);
+    float gd = -(QG10 * s.Q1 + QG11 * s.Q2);
+    float gb = -2.0f * (QG00 * s.Q1 + QG01 * s.Q2);
+    // opacity compensation
+    float gop = 0.0f;
+    if (s.ratio > 0.0f) {
+      const float h = 0.5f * s.opc;
+      ga += gopc * h * (s.d / s.det_o - s.dp / s.det_p);
+      gd += gopc * h * (s.a / s.det_o - s.ap / s.det_p);
+      gb += gopc * h * (-2.0f * s.b / s.det_o + 2.0f * s.b / s.det_p);
+      gop = gopc * sqrtf(s.ratio);
+    }
+    gout[10] = gop * s.op * (1.0f - s.op);
+    // cov2D = T Sigma T^T
+    const float G2[4] = {ga, 0.5f * gb, 0.5f * gb, gd};
+    float gSig[9], gT[6];
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) {
+        float v = 0.0f;
+        for (int k = 0; k < 2; k++)
+          for (int l = 0; l < 2; l++) v += s.T[3 * k + r] * G2[2 * k + l] * s.T[3 * l + c];
+        gSig[3 * r + c] = v;
+      }
+    for (int r = 0; r < 2; r++)
+      for (int c = 0; c < 3; c++) {
+        float v = 0.0f;
+        for (int l = 0; l < 2; l++)
+          for (int k = 0; k < 3; k++) v += G2[2 * r + l] * s.T[3 * l + k] * s.Sig[3 * k + c];
+        gT[3 * r + c] = 2.0f * v;
+      }
+    float gJ[6];
+    for (int r = 0; r < 2; r++)
+      for (int c = 0; c < 3; c++)
+        gJ[3 * r + c] = gT[3 * r] * (float)u.Vl[3 * c] + gT[3 * r + 1] * (float)u.Vl[3 * c + 1] + gT[3 * r + 2] * (float)u.Vl[3 * c + 2];
+    const float x = (float)s.t[0], y = (float)s.t[1], z = (float)s.t[2];
+    const float iz = 1.0f / z, iz2 = iz * iz, iz3 = iz2 * iz;
+    float gt[3];
+    gt[0] = gJ[2] * (-u.fx * iz2);
+    gt[1] = gJ[5] * (-u.fy * iz2);
+    gt[2] = gJ[0] * (-u.fx * iz2) + gJ[2] * (2.0f * u.fx * x * iz3) + gJ[4] * (-u.fy * iz2) + gJ[5] * (2.0f * u.fy * y * iz3);
+    // the centre in pixels from clip
+    const float w = (float)s.clip[3];
+    const float gcx = gmu2x * 0.5f * u.width / w, gcy = gmu2y * 0.5f * u.height / w;
+    const float gcw = -(gmu2x * 0.5f * u.width * (float)s.clip[0] + gmu2y * 0.5f * u.height * (float)s.clip[1]) / (w * w);
+    for (int k = 0; k < 3; k++) gt[k] += gcx * (float)u.P[k] + gcy * (float)u.P[4 + k] + gcw * (float)u.P[12 + k];
+    for (int k = 0; k < 3; k++) gmu[k] += (float)u.Vl[k] * gt[0] + (float)u.Vl[3 + k] * gt[1] + (float)u.Vl[6 + k] * gt[2];
+    for (int k = 0; k < 3; k++) gout[k] = gmu[k];
+    // Sigma = M M^T, M = R diag(s)
+    float gM[9];
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) {
+        float v = 0.0f;
+        for (int k = 0; k < 3; k++) v += (gSig[3 * r + k] + gSig[3 * k + r]) * s.M[3 * k + c];
+        gM[3 * r + c] = v;
+      }
+    float gR[9];
+    for (int c = 0; c < 3; c++) {
+      float gs = 0.0f;
+      for (int r = 0; r < 3; r++) {
+        gs += gM[3 * r + c] * s.R[3 * r + c];
+        gR[3 * r + c] = gM[3 * r + c] * s.s[c];
+      }
+      gout[3 + c] = gs * s.s[c];
+    }
+    const float qx = s.qv[0], qy = s.qv[1], qz = s.qv[2], qw = s.qv[3];
+    float gq[4];
+    gq[0] = 2.0f * (gR[1] * qy + gR[2] * qz + gR[3] * qy - gR[5] * qw + gR[6] * qz + gR[7] * qw) - 4.0f * qx * (gR[4] + gR[8]);
+    gq[1] = 2.0f * (gR[1] * qx + gR[2] * qw + gR[3] * qx + gR[5] * qz - gR[6] * qw + gR[7] * qz) - 4.0f * qy * (gR[0] + gR[8]);
+    gq[2] = 2.0f * (-gR[1] * qw + gR[2] * qx + gR[3] * qw + gR[5] * qy + gR[6] * qx + gR[7] * qy) - 4.0f * qz * (gR[0] + gR[4]);
+    gq[3] = 2.0f * (-gR[1] * qz + gR[2] * qy + gR[3] * qz - gR[5] * qx - gR[6] * qy + gR[7] * qx);
+    if (s.qlen > 0.0f) {
+      const float dq = gq[0] * qx + gq[1] * qy + gq[2] * qz + gq[3] * qw;
+      for (int k = 0; k < 4; k++) gout[6 + k] = (gq[k] - s.qv[k] * dq) / s.qlen;
+    }
+  }
+  for (int k = 0; k < DIFF_PARAMS; k++) grad[(size_t)k * n + i] = gout[k];
+}
+
+}  // namespace rdx

@@ -48,3 +48,90 @@ def workload(count: int = 262144, seed: int = 4):
     spec = W.diff_spec(seed=seed, count=count)
     theta = theta_from_planes(W.generate(spec))
     return theta, perturbed(theta, seed * 31 + 7), W.diff_camera()
+
+
+class Rasterizer:
+    """rdx_raster_*: the tile compositor and its adjoint on the device."""
+
+    def __init__(self, sh_degree: int = 3, t_stop: float = 1e-4, entries_per_splat: int = 16, context=None):
+        from .sort import default_context
+
+        self.context = context or default_context()
+        s = N.RasterSettings(sh_degree, t_stop, entries_per_splat)
+        h = C.c_void_p()
+        N.check(N.lib().rdx_raster_create(self.context.handle, C.byref(s), C.byref(h)))
+        self._h = h
+
+    def forward(self, params, rig, prm, stream=None):
+        """params: (59, n) float32 device tensor. Returns (image (3, H, W), alpha (H, W))."""
+        import torch
+
+        from .sort import _stream
+
+        if params.dtype != torch.float32 or not params.is_cuda or params.dim() != 2 or params.shape[0] != PARAMS:
+            raise TypeError("params must be a (59, n) float32 device tensor")
+        params = params.contiguous()
+        n = params.shape[1]
+        W, H = rig.width, rig.height
+        image = torch.empty((3, H, W), dtype=torch.float32, device=params.device)
+        alpha = torch.empty((H, W), dtype=torch.float32, device=params.device)
+        self._params = params  # the backward reads the forward's parameters
+        N.check(N.lib().rdx_raster_forward(self._h, _stream(stream), C.c_void_p(params.data_ptr()), n,
+                                           C.byref(rig.desc()), C.byref(prm.desc()), C.c_void_p(image.data_ptr()),
+                                           C.c_void_p(alpha.data_ptr())))
+        return image, alpha
+
+    def backward(self, grad_image, stream=None):
+        import torch
+
+        from .sort import _stream
+
+        g = grad_image.contiguous().float()
+        out = torch.empty_like(self._params)
+        N.check(N.lib().rdx_raster_backward(self._h, _stream(stream), C.c_void_p(g.data_ptr()),
+                                            C.c_void_p(out.data_ptr())))
+        return out
+
+    def stats(self, stream=None):
+        from .sort import _stream
+
+        v, e, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        o = C.c_int()
+        N.check(N.lib().rdx_raster_stats(self._h, _stream(stream), C.byref(v), C.byref(e), C.byref(c), C.byref(o)))
+        return {"visible": v.value, "entries": e.value, "capacity": c.value, "overflowed": bool(o.value)}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            N.lib().rdx_raster_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def composite(params, rig, prm, rasterizer: "Rasterizer"):
+    """The image of `params` (59, n) as a differentiable torch function: a
+    torch.autograd.Function whose backward is the hand-written adjoint."""
+    import torch
+
+    class _Composite(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, p):
+            image, _ = rasterizer.forward(p, rig, prm)
+            return image
+
+        @staticmethod
+        def backward(ctx, grad_image):
+            return rasterizer.backward(grad_image)
+
+    return _Composite.apply(params)
+
+
+def params_tensor(theta: np.ndarray, device="cuda"):
+    """(n, 59) float64 -> (59, n) float32 device planes."""
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(theta.T, dtype=np.float32)).to(device)
