@@ -364,13 +364,23 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
   }
   __syncthreads();
 
-  // Write out: runs of one digit go to consecutive addresses.
-  for (uint32_t i = tid; i < tile_kept; i += BLOCK) {
-    const uint32_t k = s_keys[i];
-    const uint32_t dst = s_out[((k & a.key_mask) >> a.shift) & DMASK] + i;
-    if (dst < a.dst_cap) {  // always true when the counts are consistent
-      if (a.keys_out) a.keys_out[dst] = k;
-      a.vals_out[dst] = s_vals[i];
+  // Write out: runs of one digit go to consecutive addresses. Unrolled over
+  // the KPT rounds so the LDS reads of several rounds are in flight at once.
+  uint32_t ok[KPT], ov[KPT], od[KPT];
+#pragma unroll
+  for (int r = 0; r < KPT; r++) {
+    const uint32_t i = (uint32_t)(r * BLOCK + tid);
+    ok[r] = i < tile_kept ? s_keys[i] : 0u;
+    ov[r] = i < tile_kept ? s_vals[i] : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < KPT; r++) od[r] = s_out[((ok[r] & a.key_mask) >> a.shift) & DMASK] + (uint32_t)(r * BLOCK + tid);
+#pragma unroll
+  for (int r = 0; r < KPT; r++) {
+    const uint32_t i = (uint32_t)(r * BLOCK + tid);
+    if (i < tile_kept && od[r] < a.dst_cap) {  // always below when the counts are consistent
+      if (a.keys_out) a.keys_out[od[r]] = ok[r];
+      a.vals_out[od[r]] = ov[r];
     }
   }
 }
