@@ -274,10 +274,12 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
   if (tile_base >= n) return;  // uniform per block
 
   // Rank.
+  uint32_t dig[KPT];
 #pragma unroll
   for (int j = 0; j < KPT; j++) {
     const uint32_t i = wbase + (uint32_t)(j * RDX_WAVE);
     rank[j] = i < n && !(a.drop && keys[j] == a.invalid) ? 0u : 0xffffffffu;  // 0xffffffff: not kept
+    dig[j] = ((keys[j] & a.key_mask) >> a.shift) & DMASK;
   }
   __syncthreads();  // s_wave, s_match zeroed
   // Rank each round of 64 keys in input order. Up to PEEL digits of the
@@ -291,7 +293,7 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
 #pragma unroll
   for (int j = 0; j < KPT; j++) {
     const bool keep = rank[j] == 0u;
-    const uint32_t d = ((keys[j] & a.key_mask) >> a.shift) & DMASK;
+    const uint32_t d = dig[j];
     bool done = !keep;
     uint64_t rem = ballot(keep);
 #pragma unroll
@@ -356,7 +358,7 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
 #pragma unroll
   for (int j = 0; j < KPT; j++) {
     if (rank[j] != 0xffffffffu) {
-      const uint32_t d = ((keys[j] & a.key_mask) >> a.shift) & DMASK;
+      const uint32_t d = dig[j];
       const uint32_t p = s_local[d] + s_wave[w][d] + rank[j];
       s_keys[p] = keys[j];
       s_vals[p] = vals[j];
