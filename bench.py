@@ -174,7 +174,17 @@ def bench_stereo(steps: int, count: int = 16777216) -> dict:
                                                indices=out["indices"], count=out["count"], args=out["args"]), steps)
     n = spec.count
     bytes_ = (64 + 2 * (20 + 60)) * n
+    from radix_amd import _native as N
+
+    N.lib().rdx_context_profile(proj.context.handle, 1)
+    stage_times(proj.context)
+    for _ in range(3):
+        proj.project_sort(table, rig, prm, words=out["words"], keys=out["keys"], indices=out["indices"],
+                          count=out["count"], args=out["args"])
+    stages = stage_times(proj.context)
+    N.lib().rdx_context_profile(proj.context.handle, 0)
     return {"workload": "16,777,216 packedSplats + packedSh SH3, stereo 2064x2208 (64 mm), per-eye segmented sort",
+            "stages_ms": {k: round(v["ms_avg"], 4) for k, v in stages.items()},
             "ms": round(ms, 4), "Gsplats_per_s": round(n / ms / 1e6, 3),
             "hbm_frac": round(bytes_ / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "bytes_per_splat": 224,
             "visible_per_eye": [int(v) for v in out["count"].cpu().tolist()]}
@@ -200,13 +210,23 @@ def bench_backward(steps: int) -> dict:
     rast.forward(p, rig, prm)
     ms_b = time_events(lambda: rast.backward(gi), steps)
     st = rast.stats()
+    from radix_amd import _native as N
+
+    N.lib().rdx_context_profile(rast.context.handle, 1)
+    stage_times(rast.context)
+    for _ in range(3):
+        rast.forward(p, rig, prm)
+        rast.backward(gi)
+    stages = stage_times(rast.context)
+    N.lib().rdx_context_profile(rast.context.handle, 0)
     n = p.shape[1]
     px = rig.width * rig.height
     bytes_ = (236 + 236 + 40) * n + 20 * px
     return {"workload": "262,144 splats (SH3), 512x512, 16x16 tiles, T < 1e-4 stop, L2 loss; step = rdx_raster_backward",
             "bwd_ms": round(ms_b, 4), "bwd_Gsplats_per_s": round(n / ms_b / 1e6, 3),
             "fwd_ms": round(ms_f, 4), "fwd_Gsplats_per_s": round(n / ms_f / 1e6, 3),
-            "entries": st["entries"], "hbm_frac_lower_bound": round(bytes_ / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            "entries": st["entries"], "hbm_frac_lower_bound": round(bytes_ / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "stages_ms": {k: round(v["ms_avg"], 4) for k, v in stages.items()}}
 
 
 def measured_traffic(n: int):
