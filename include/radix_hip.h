@@ -65,7 +65,7 @@ int rdx_context_destroy(rdx_context* ctx);
 int rdx_context_reserve(rdx_context* ctx, uint32_t max_keys);
 
 /* Radix sort tuning: digit width 4, 6 or 8 bits, keys per thread and
- * workgroup size (threads). Zeros pick the built-in default (8, 16, 256). */
+ * workgroup size (threads). Zeros pick the built-in default (8, 16, 512). */
 typedef struct rdx_sort_tuning {
   int digit_bits;
   int keys_per_thread;

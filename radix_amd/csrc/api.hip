@@ -329,7 +329,7 @@ int rdx_context_create(rdx_context** out) {
   Context* c = new Context();
   c->tuning.digit_bits = 8;
   c->tuning.keys_per_thread = 16;
-  c->tuning.block_threads = 256;
+  c->tuning.block_threads = 512;
   *out = reinterpret_cast<rdx_context*>(c);
   return RDX_OK;
 }
@@ -354,7 +354,7 @@ int rdx_context_set_tuning(rdx_context* ctx, const rdx_sort_tuning* t) {
   rdx_sort_tuning v = *t;
   if (!v.digit_bits) v.digit_bits = 8;
   if (!v.keys_per_thread) v.keys_per_thread = 16;
-  if (!v.block_threads) v.block_threads = 256;
+  if (!v.block_threads) v.block_threads = 512;
   if (!tuning_supported(v))
     return fail(RDX_E_INVALID, "unsupported sort tuning: %d-bit digits, %d keys per thread, %d threads", v.digit_bits,
                 v.keys_per_thread, v.block_threads);

@@ -193,12 +193,12 @@ int check_table(const rdx_splat_table* t) {
   return RDX_OK;
 }
 
-int launch_project(hipStream_t s, const rdx_splat_table* t, const ProjU& u, const OutP& o, bool side, int blocks) {
+int launch_project(hipStream_t s, const rdx_splat_table* t, const ProjU& u, const OutP& o, bool side) {
   const uint32_t n = t->count;
   if (n == 0) return RDX_OK;
-  // One thread per splat: many short blocks keep the machine full to the
-  // end (a grid of a few resident rounds leaves a tail).
-  const uint32_t grid = std::min<uint32_t>((n + 255) / 256, (uint32_t)(blocks > 0 ? blocks : (1 << 20)));
+  // One thread per splat, grid x 256 >= n (up to 2^24 blocks): many short
+  // blocks keep the machine full to the end.
+  const uint32_t grid = (uint32_t)(((uint64_t)n + 255) / 256);
   const TableP tp = make_table(t);
   const int shc = t->sh_codec == RDX_SH_PACKED ? SHC_PACKED : SHC_EXT;
   switch (t->layout) {
@@ -270,7 +270,7 @@ int rdx_project_ewa(rdx_context* ctx, void* stream, const rdx_splat_table* table
   if (params->key_per_eye && camera->eyes == 2 && keys && key_stride < (size_t)at + table->count)
     return fail(RDX_E_INVALID, "rdx_project_ewa: key_stride below the keys written");
   Stage st(reinterpret_cast<Context*>(ctx), (hipStream_t)stream, "project");
-  return launch_project((hipStream_t)stream, table, u, o, side != nullptr, 0);
+  return launch_project((hipStream_t)stream, table, u, o, side != nullptr);
 }
 
 int rdx_project_sort(rdx_context* ctx, void* stream, const rdx_splat_table* table, const rdx_camera* camera,
@@ -314,7 +314,7 @@ int rdx_project_sort(rdx_context* ctx, void* stream, const rdx_splat_table* tabl
   fill_side(o, side, n);
   {
     Stage st(c, s, "project");
-    if (int rc = launch_project(s, table, u, o, side != nullptr, 0)) return rc;
+    if (int rc = launch_project(s, table, u, o, side != nullptr)) return rc;
   }
   return run_sort(c, c->sort, s, p, L, keys, nullptr, nullptr, out_indices, n * segs, nullptr, true,
                   key_invalid(params->sort_bits), false, out_count, out_args);

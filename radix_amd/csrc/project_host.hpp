@@ -20,7 +20,7 @@ int check_table(const rdx_splat_table* t);
 
 // Launch the projection of a table (one thread per splat, grid-stride
 // beyond 2^20 blocks).
-int launch_project(hipStream_t s, const rdx_splat_table* t, const ProjU& u, const OutP& o, bool side, int blocks);
+int launch_project(hipStream_t s, const rdx_splat_table* t, const ProjU& u, const OutP& o, bool side);
 
 // The kernels, instantiated per layout in their own translation units.
 void launch_project_f32(hipStream_t s, dim3 grid, const TableP& t, const ProjU& u, const OutP& o, int eyes, bool side);

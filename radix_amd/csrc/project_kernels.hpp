@@ -545,7 +545,11 @@ __global__ __launch_bounds__(256) void k_project(TableP t, ProjU u, OutP o) {
   F32Rsrc rs;
   if constexpr (LAYOUT == LAYOUT_F32) rs = f32_rsrc(t, deg);
 
-  for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+  // One splat per thread, no grid-stride loop: a loop lets the compiler hoist
+  // the per-eye uniforms out of it and keep them all live in SGPRs, which
+  // spills (165 SGPRs for two eyes) and costs occupancy.
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) {
     Splat sp;
     float3 coef[15];
     load_splat<LAYOUT, SHC, DEG>(t, rs, u.enc, i, deg, sp, coef);

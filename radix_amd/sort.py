@@ -43,7 +43,7 @@ class Context:
     def handle(self) -> C.c_void_p:
         return self._h
 
-    def set_tuning(self, digit_bits: int = 8, keys_per_thread: int = 16, block_threads: int = 256) -> None:
+    def set_tuning(self, digit_bits: int = 8, keys_per_thread: int = 16, block_threads: int = 512) -> None:
         t = N.SortTuning(digit_bits, keys_per_thread, block_threads)
         N.check(self._lib.rdx_context_set_tuning(self._h, C.byref(t)))
 

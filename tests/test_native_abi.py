@@ -57,7 +57,7 @@ def test_sort_errors_carry_the_reference_messages():
 
 def test_tuning_validation():
     ctx = S.Context()
-    assert ctx.tuning() == (8, 16, 256)
+    assert ctx.tuning() == (8, 16, 512)
     ctx.set_tuning(6, 12, 512)
     assert ctx.tuning() == (6, 12, 512)
     with pytest.raises(N.RadixError):

@@ -13,7 +13,7 @@ from tests.helpers import Rng, first_mismatch
 pytestmark = pytest.mark.gpu
 BITS = [16, 24, 32]
 COUNTS = [0, 1, 4095, 4096, 4097, 70000]
-TUNINGS = [(8, 16, 256), (8, 8, 256), (8, 12, 512), (6, 16, 256), (4, 8, 256), (8, 20, 256), (6, 8, 512)]
+TUNINGS = [(8, 16, 512), (8, 16, 256), (8, 8, 256), (8, 12, 512), (6, 16, 256), (4, 8, 256), (8, 20, 256), (6, 8, 512)]
 
 
 def dev_u32(a):
