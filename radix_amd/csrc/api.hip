@@ -167,7 +167,7 @@ int SortScratch::reserve(uint32_t n, uint32_t segments, uint32_t tiles) {
     if ((rc = keys[i].ensure(words * 4))) return rc;
     if ((rc = vals[i].ensure(words * 4))) return rc;
   }
-  if ((rc = cnt.ensure((size_t)(MAX_BINS + 1) * tiles * 4))) return rc;
+  if ((rc = cnt.ensure((size_t)count_stride(MAX_BINS + 1) * tiles * 4))) return rc;
   if ((rc = tot.ensure((size_t)TOT_STRIDE * std::max<uint32_t>(segments, 1) * 4))) return rc;
   if ((rc = nvalid.ensure((size_t)std::max<uint32_t>(segments, 1) * 4))) return rc;
   return RDX_OK;
@@ -242,6 +242,7 @@ int run_sort(Context* ctx, SortScratch& sc, hipStream_t s, const SortPlan& p, co
     a.nvalid = (uint32_t*)sc.nvalid.ptr;
     a.tiles = L.tiles;
     a.rows = bins + (rule && pass == 0 ? 1 : 0);
+    a.cstride = count_stride(a.rows);
     a.shift = pass * p.rb;
     a.key_mask = key_invalid(p.bits);
     a.invalid = invalid;
@@ -262,7 +263,13 @@ int run_sort(Context* ctx, SortScratch& sc, hipStream_t s, const SortPlan& p, co
     {
       snprintf(name, sizeof name, "sort_scan%d", pass);
       Stage st(ctx, s, name);
-      hipLaunchKernelGGL(k_scan, dim3(L.segments * (uint32_t)a.rows), dim3(SCAN_BLOCK), 0, s, a, bins);
+      const uint32_t groups = (uint32_t)(a.rows + CGROUP - 1) / CGROUP;
+      // 32 tiles per thread and round from 2048 tiles (one round for a
+      // segment of up to 16M keys at 8192-key tiles), else 8
+      if (L.tiles >= 2048)
+        hipLaunchKernelGGL(k_scan<32>, dim3(L.segments * groups), dim3(SCAN_BLOCK), 0, s, a, bins);
+      else
+        hipLaunchKernelGGL(k_scan<8>, dim3(L.segments * groups), dim3(SCAN_BLOCK), 0, s, a, bins);
     }
     {
       snprintf(name, sizeof name, "sort_scatter%d", pass);

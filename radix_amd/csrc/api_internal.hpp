@@ -33,7 +33,7 @@ struct StageRecord {
 // and the tile counts, digit totals and kept counts.
 struct SortScratch {
   Buffer keys[2], vals[2];
-  Buffer cnt;     // [MAX_BINS + 1][tiles]
+  Buffer cnt;     // [tiles][count_stride(MAX_BINS + 1)]
   Buffer tot;     // [segments][TOT_STRIDE]
   Buffer nvalid;  // [segments]
   Buffer seg;     // segment table [segments][4] and tile map [tiles] (small segmented sorts)

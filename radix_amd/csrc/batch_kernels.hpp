@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void k_project_batch(BatchDesc D, BatchU u, ui
           visible = true;
         }
       }
-      if (V.words) V.words[i] = words;
+      if (V.words) st_stream(V.words + i, words);
       // Mono views: the head is the eye, the key distance the eye's.
       const bool keyed = u.cull_keys ? visible : !invisible;
       const uint32_t key = keyed ? sort_key(fbits((float)C.dist), u.sort_bits) : invalid;

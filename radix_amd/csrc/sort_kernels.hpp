@@ -2,13 +2,14 @@
 //
 // A sort of P digit passes (P = ceil(bits / RB), RB = 4, 6 or 8 bits) runs,
 // per pass, three kernels over tiles of TILE = BLOCK * KPT keys:
-//   count    per tile, the number of keys of each digit -> cnt[digit][tile]
-//   scan     per (segment, digit), an exclusive scan of its row over the
-//            segment's tiles, in place, and the digit's total -> tot
+//   count    per tile, the number of keys of each digit -> cnt[tile][digit]
+//   scan     per (segment, group of 16 digits), an exclusive scan of each
+//            digit's column over the segment's tiles, in place, and the
+//            digit's total -> tot
 //   scatter  per tile: rank every key within the tile (wave64 ballots for a
 //            round's most present digits, LDS lane masks for the rest, mbcnt
 //            for the rank), stage the tile in LDS in digit order and write it
-//            out in runs starting at base(digit) + cnt[digit][tile]
+//            out in runs starting at base(digit) + cnt[tile][digit]
 // No kernel waits on another block: a pass costs one extra read of the keys
 // (4 B per key) instead of a cross-tile look-back. Every count word is
 // written before it is read, so nothing needs clearing between sorts.
@@ -24,8 +25,13 @@ namespace rdx {
 
 constexpr int MAX_BINS = 256;
 constexpr int TOT_STRIDE = MAX_BINS + 1;  // tot words per segment: digits, then the hidden-rule count
-constexpr int SCAN_BLOCK = 256;
-constexpr int SCAN_ITEMS = 8;             // tiles per thread per round of the scan
+constexpr int SCAN_BLOCK = 1024;
+// Tile counts are stored tile-major, cnt[tile][cstride] (a count block
+// writes its row contiguously, a scatter block reads it so): the scan walks
+// them in column groups of CGROUP digits, one 64-byte line per tile.
+constexpr int CGROUP = 16;
+constexpr int SCAN_CHUNKS = SCAN_BLOCK / CGROUP;  // tile ranges per scan block and round
+__host__ __device__ constexpr uint32_t count_stride(int rows) { return (uint32_t)((rows + CGROUP - 1) / CGROUP * CGROUP); }
 // Digits a wave ranks in a round with one ballot each before the LDS lane
 // masks take the rest: a round whose keys share a few digits (the top
 // digit of clustered distances) would otherwise serialise its LDS atomics
@@ -33,6 +39,17 @@ constexpr int SCAN_ITEMS = 8;             // tiles per thread per round of the s
 // PEEL_MIN lanes (a round of spread digits then costs one ballot).
 constexpr int PEEL = 3;
 constexpr uint32_t PEEL_MIN = 8;
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs
+// on XCD b % 8), each with its own L2. The scatter maps them so that each
+// XCD takes one contiguous range of tiles: the digit runs of neighbouring
+// tiles meet inside shared cache lines, which then fill in one L2 instead of
+// being written back in pieces from two. A bijection of [0, tiles).
+constexpr uint32_t XCDS = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t tiles) {
+  const uint32_t x = b % XCDS, q = tiles / XCDS, r = tiles % XCDS;
+  return x * q + (x < r ? x : r) + b / XCDS;
+}
 
 // One pass of a sort: what the count, scan and scatter kernels share.
 struct PassArgs {
@@ -42,7 +59,8 @@ struct PassArgs {
   uint32_t* vals_out;
   const uint32_t* n_dev;     // input count on the device (NULL: n below); one segment only
   uint32_t n;                // input count (or the cap with n_dev)
-  uint32_t* cnt;             // [rows][tiles] tile counts, scanned in place
+  uint32_t* cnt;             // [tiles][cstride] tile counts, scanned in place (per column, over tiles)
+  uint32_t cstride;          // count_stride(rows)
   uint32_t* tot;             // [segments][TOT_STRIDE] digit totals
   uint32_t* nvalid;          // [segments] kept keys (written by a dropping pass)
   uint32_t tiles;            // tiles of all segments (the row length of cnt)
@@ -153,50 +171,75 @@ __global__ __launch_bounds__(BLOCK) void k_count(PassArgs a) {
     uint32_t c = 0;
 #pragma unroll
     for (int r = 0; r < WAVES * SUBS; r++) c += s_rows[r * ROW + d];
-    a.cnt[(size_t)d * a.tiles + tile] = c;
+    a.cnt[(size_t)tile * a.cstride + d] = c;
   }
   if (rule && tid == 0) {
     uint32_t c = 0;
     for (int v = 0; v < WAVES; v++) c += s_rule[v];
-    a.cnt[(size_t)BINS * a.tiles + tile] = c;
+    a.cnt[(size_t)tile * a.cstride + BINS] = c;
   }
 }
 
 // ---- scan ----------------------------------------------------------------------
 
-// Block (s, r): exclusive scan of row r of segment s over its tiles, in
-// place; the total to tot[s][r] (r = bins: the hidden-rule row, to
-// tot[s][MAX_BINS]).
-static __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(PassArgs a, int bins) {
-  __shared__ uint32_t s_scan[SCAN_BLOCK / RDX_WAVE];
-  const uint32_t s = blockIdx.x / (uint32_t)a.rows;
-  const int r = (int)(blockIdx.x % (uint32_t)a.rows);
+// Block (s, g): for the CGROUP digit columns [g * CGROUP, ...) of segment
+// s, an exclusive scan of each column over the segment's tiles, in place,
+// and the column's total to tot[s][r] (r = bins: the hidden-rule row, to
+// tot[s][MAX_BINS]). A round covers SCAN_CHUNKS x ITEMS tiles: thread
+// (c, d) loads column d of the c-th run of ITEMS tiles (the 16 threads
+// of a run read one 64-byte line per tile); wave d scans column d's 64 run
+// sums (one per lane) through LDS; the thread writes its run's prefixes.
+template <int ITEMS>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan(PassArgs a, int bins) {
+  static_assert(SCAN_CHUNKS == RDX_WAVE && SCAN_BLOCK / RDX_WAVE == CGROUP, "a wave per column, a lane per run");
+  __shared__ uint32_t s_sum[SCAN_CHUNKS][CGROUP];
+  __shared__ uint32_t s_tot[CGROUP];
+  const uint32_t groups = (uint32_t)(a.rows + CGROUP - 1) / CGROUP;
+  const uint32_t s = blockIdx.x / groups;
+  const int g = (int)(blockIdx.x % groups);
+  const int d = threadIdx.x % CGROUP;
+  const int c = threadIdx.x / CGROUP;
+  const int r = g * CGROUP + d;
   uint32_t tile0 = 0, ntiles = a.tiles;
   if (a.seg) {
     tile0 = a.seg[4 * s];
     ntiles = a.seg[4 * s + 3];
   }
-  uint32_t* row = a.cnt + (size_t)r * a.tiles + tile0;
+  uint32_t* col = a.cnt + (size_t)tile0 * a.cstride + r;
+  const bool live = r < a.rows;
   uint32_t carry = 0;
-  for (uint32_t c0 = 0; c0 < ntiles; c0 += SCAN_BLOCK * SCAN_ITEMS) {
-    const uint32_t first = c0 + threadIdx.x * SCAN_ITEMS;
-    uint32_t v[SCAN_ITEMS];
+  for (uint32_t r0 = 0; r0 < ntiles; r0 += SCAN_CHUNKS * ITEMS) {
+    const uint32_t t0 = r0 + (uint32_t)c * ITEMS;
+    uint32_t v[ITEMS];
     uint32_t sum = 0;
 #pragma unroll
-    for (int j = 0; j < SCAN_ITEMS; j++) {
-      v[j] = first + j < ntiles ? row[first + j] : 0u;
-      sum += v[j];
+    for (int k = 0; k < ITEMS; k++) {
+      v[k] = live && t0 + k < ntiles ? col[(size_t)(t0 + k) * a.cstride] : 0u;
+      sum += v[k];
     }
-    uint32_t total;
-    uint32_t run = carry + block_exclusive_sum<SCAN_BLOCK>(sum, s_scan, &total);
+    __syncthreads();  // the previous round's s_sum reads are done
+    s_sum[c][d] = sum;
+    __syncthreads();
+    {
+      // wave w scans column w: lane l holds run l's sum
+      const int w = threadIdx.x / RDX_WAVE, l = threadIdx.x % RDX_WAVE;
+      const uint32_t x = s_sum[l][w];
+      const uint32_t inc = wave_inclusive_sum(x);
+      __syncthreads();
+      s_sum[l][w] = inc - x;
+      if (l == RDX_WAVE - 1) s_tot[w] = inc;
+    }
+    __syncthreads();
+    uint32_t run = carry + s_sum[c][d];
+    const uint32_t total = s_tot[d];
 #pragma unroll
-    for (int j = 0; j < SCAN_ITEMS; j++) {
-      if (first + j < ntiles) row[first + j] = run;
-      run += v[j];
+    for (int k = 0; k < ITEMS; k++) {
+      if (live && t0 + k < ntiles) col[(size_t)(t0 + k) * a.cstride] = run;
+      run += v[k];
     }
     carry += total;
   }
-  if (threadIdx.x == 0) a.tot[(size_t)s * TOT_STRIDE + (r < bins ? r : MAX_BINS)] = carry;
+  if (live && c == 0) a.tot[(size_t)s * TOT_STRIDE + (r < bins ? r : MAX_BINS)] = carry;
 }
 
 // ---- scatter --------------------------------------------------------------------
@@ -227,7 +270,7 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
   const int tid = threadIdx.x;
   const int w = tid / RDX_WAVE;
   const int lane = tid & (RDX_WAVE - 1);
-  const uint32_t tile = blockIdx.x;
+  const uint32_t tile = xcd_tile(blockIdx.x, a.tiles);
   for (int d = tid; d < WAVES * BINS; d += BLOCK) {
     (&s_wave[0][0])[d] = 0u;
     (&s_match[0][0])[d] = 0ull;
@@ -245,12 +288,13 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
 #pragma unroll
   for (int j = 0; j < KPT; j++) {
     const uint32_t i = wbase + (uint32_t)(j * RDX_WAVE);
-    keys[j] = i < g.cap ? keys_in[i] : 0u;
-    vals[j] = vals_in ? (i < g.cap ? vals_in[i] : 0u) : i;
+    // the pass's last reads of its input: nontemporal
+    keys[j] = i < g.cap ? __builtin_nontemporal_load(keys_in + i) : 0u;
+    vals[j] = vals_in ? (i < g.cap ? __builtin_nontemporal_load(vals_in + i) : 0u) : i;
   }
   const uint32_t* tot = a.tot + (size_t)g.s * TOT_STRIDE;
   const uint32_t c_tot = tid < BINS ? tot[tid] : 0u;
-  const uint32_t c_pre = tid < BINS ? a.cnt[(size_t)tid * a.tiles + tile] : 0u;
+  const uint32_t c_pre = tid < BINS ? a.cnt[(size_t)tile * a.cstride + tid] : 0u;
   const uint32_t n = seg_count(a, g);
 
   // Digit starts of the segment: exclusive scan of the digit totals; with

@@ -124,6 +124,21 @@ def test_full_size_properties_4m(cuda):
     assert first_mismatch(results[0], np.argsort(keys, kind="stable").astype(np.uint32)) is None
 
 
+@pytest.mark.parametrize("tuning,n", [((8, 16, 256), 1 << 22),          # 1024 tiles: two scan rounds of 512
+                                      ((8, 8, 256), (1 << 22) + 4097),  # 2051 tiles: 32 per thread, two rounds
+                                      ((4, 8, 256), 1 << 22)])          # 2048 tiles, 8 passes
+def test_many_tiles_equal_stable_argsort(cuda, tuning, n):
+    # the tile-count scan over more tiles than one round covers
+    random = Rng(11)
+    keys = np.floor(random.random(n) * 2.0 ** 32).astype(np.uint64).astype(np.uint32)
+    keys[::3] &= np.uint32(0xFF00FF00)
+    ctx = Context(tuning)
+    kd, vd = dev_u32(keys), dev_u32(np.arange(n, dtype=np.uint32))
+    RadixSort(kd, vd, context=ctx).sort(n, bits=32)
+    torch.cuda.synchronize()
+    assert first_mismatch(host_u32(vd), np.argsort(keys, kind="stable").astype(np.uint32)) is None, tuning
+
+
 def test_compute_keys_device_matches_host(cuda):
     f = np.arange(0, 0xFFFFFFFF, 65537, dtype=np.uint64).astype(np.uint32)
     d = torch.from_numpy(f.view(np.float32).copy()).cuda()
