@@ -37,6 +37,7 @@ constexpr int DIFF_PARAMS = 59;
 constexpr int DIFF_TILE = 16;
 constexpr int DIFF_BLOCK = DIFF_TILE * DIFF_TILE;
 constexpr int DIFF_PIX_GRADS = 9;  // mu2 2, conic 3, opacity 1, colour 3
+constexpr int DIFF_BWD_BATCH = 128;  // entries staged per round of the adjoint (its LDS: 25 KB)
 
 struct DiffU {
   double Vl[9], Vt[3], cam[3];  // world -> view (row-major linear part), camera position
@@ -194,7 +195,11 @@ __device__ inline void diff_splat_forward(const float* th, uint32_t n, uint32_t 
 
 // ---- per splat: what the pixels need, the key and the tiles -------------------
 
-// pre[3 i .. 3 i + 2] = (mu2x, mu2y, opc, cut), (Q0, Q1, Q2, 0), (r, g, b, 0)
+// pre[3 i .. 3 i + 2] = (mu2x, mu2y, opc, cut), (Q0, Q1, Q2, ex), (r, g, b, ey): (ex, ey)
+// the half extents of the cutoff ellipse's bounding box, widened by 2 % and
+// half a pixel (the renderer's waves skip entries whose widened box misses
+// their pixels; z2's float error is far below that margin unless the
+// ellipse is extremely elongated, which gets an unbounded box).
 static __global__ __launch_bounds__(256) void k_diff_pre(const float* __restrict__ th, uint32_t n, DiffU u,
                                                          float4* __restrict__ pre, uint32_t* __restrict__ keys,
                                                          uint32_t* __restrict__ touch, uint4* __restrict__ rect) {
@@ -202,9 +207,12 @@ static __global__ __launch_bounds__(256) void k_diff_pre(const float* __restrict
   if (i >= n) return;
   DiffSplat s;
   diff_splat_forward(th, n, i, u, s);
+  float ex = s.cut * sqrtf(s.ap) * 1.02f + 0.5f, ey = s.cut * sqrtf(s.dp) * 1.02f + 0.5f;
+  // condition proxy (trace^2 / det of the conic's inverse): beyond 1e6 the box is not trusted
+  if (!((s.ap + s.dp) * (s.ap + s.dp) <= 1e6f * s.det_p)) ex = ey = INFINITY;
   pre[3 * (size_t)i] = make_float4(s.mu2x, s.mu2y, s.opc, s.cut);
-  pre[3 * (size_t)i + 1] = make_float4(s.Q0, s.Q1, s.Q2, 0.0f);
-  pre[3 * (size_t)i + 2] = make_float4(s.col[0], s.col[1], s.col[2], 0.0f);
+  pre[3 * (size_t)i + 1] = make_float4(s.Q0, s.Q1, s.Q2, ex);
+  pre[3 * (size_t)i + 2] = make_float4(s.col[0], s.col[1], s.col[2], ey);
   uint32_t cnt = 0;
   uint4 rc = make_uint4(0u, 0u, 0u, 0u);
   if (s.ok) {
@@ -340,19 +348,53 @@ static __global__ void k_diff_clear_ranges(uint2* range, uint32_t tiles) {
 
 // ---- the tile renderer -----------------------------------------------------------
 
-struct DiffPixel {
-  float dx, dy, z2;
+// The tile's pixels: wave w takes the 8x8 quadrant (w & 1, w >> 1) of the
+// 16x16 tile, lane l its pixel (l & 7, l >> 3). Entries are staged in LDS
+// 256 at a time (the next batch's indices and pre words are loaded while
+// this one is composited); each entry's widened box is tested against the
+// four quadrants once, and a wave walks only the entries that can reach its
+// pixels, with predicated arithmetic per lane.
+struct DiffQuad {
+  int w, lane, x, y;
+  float px, py;
+  float cx0[4], cy0[4];  // pixel-centre ranges of the four quadrants: [c0, c0 + 7]
 };
 
-__device__ __forceinline__ float pixel_alpha(float4 p0, float4 p1, float px, float py, float min_alpha, float& dx,
-                                             float& dy, float& g) {
-  dx = px - p0.x;
-  dy = py - p0.y;
-  const float z2 = p1.x * dx * dx + 2.0f * p1.y * dx * dy + p1.z * dy * dy;
-  if (z2 > p0.w * p0.w) return -1.0f;
-  g = __expf(-0.5f * z2);
-  const float a = p0.z * g;
-  return a < min_alpha ? -1.0f : a;
+__device__ __forceinline__ DiffQuad diff_quad(uint32_t tile, const DiffU& u) {
+  DiffQuad q;
+  q.lane = threadIdx.x & (RDX_WAVE - 1);
+  q.w = threadIdx.x / RDX_WAVE;
+  const int tx = (int)(tile % (uint32_t)u.tiles_x), ty = (int)(tile / (uint32_t)u.tiles_x);
+  q.x = tx * DIFF_TILE + (q.w & 1) * 8 + (q.lane & 7);
+  q.y = ty * DIFF_TILE + (q.w >> 1) * 8 + (q.lane >> 3);
+  q.px = (float)q.x + 0.5f;
+  q.py = (float)q.y + 0.5f;
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    q.cx0[v] = (float)(tx * DIFF_TILE + (v & 1) * 8) + 0.5f;
+    q.cy0[v] = (float)(ty * DIFF_TILE + (v >> 1) * 8) + 0.5f;
+  }
+  return q;
+}
+
+// Bit v: the entry's widened box holds a pixel centre of quadrant v.
+__device__ __forceinline__ uint32_t quad_mask(const DiffQuad& q, float4 p0, float4 p1, float4 p2) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const bool hx = p0.x + p1.w >= q.cx0[v] && p0.x - p1.w <= q.cx0[v] + 7.0f;
+    const bool hy = p0.y + p2.w >= q.cy0[v] && p0.y - p2.w <= q.cy0[v] + 7.0f;
+    m |= (hx && hy) ? (1u << v) : 0u;
+  }
+  return m;
+}
+
+__device__ __forceinline__ void load_entry(const uint32_t* emit_idx, const uint32_t* emit_splat, const float4* pre,
+                                           uint32_t e, float4& p0, float4& p1, float4& p2) {
+  const uint32_t sp = emit_splat[emit_idx[e]];
+  p0 = pre[3 * (size_t)sp];
+  p1 = pre[3 * (size_t)sp + 1];
+  p2 = pre[3 * (size_t)sp + 2];
 }
 
 static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render(const uint2* range, const uint32_t* emit_idx,
@@ -360,46 +402,57 @@ static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render(const uint2* 
                                                                    float* image, float* alpha_out, float* t_final,
                                                                    uint32_t* n_contrib) {
   __shared__ float4 s_p0[DIFF_BLOCK], s_p1[DIFF_BLOCK], s_p2[DIFF_BLOCK];
+  __shared__ uint32_t s_ov[DIFF_BLOCK];
   const uint32_t tile = blockIdx.x;
-  const int tx = (int)(tile % (uint32_t)u.tiles_x), ty = (int)(tile / (uint32_t)u.tiles_x);
-  const int x = tx * DIFF_TILE + (int)(threadIdx.x % DIFF_TILE), y = ty * DIFF_TILE + (int)(threadIdx.x / DIFF_TILE);
-  const bool inside = x < u.width && y < u.height;
-  const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+  const DiffQuad q = diff_quad(tile, u);
+  const bool inside = q.x < u.width && q.y < u.height;
   const uint2 rg = range[tile];
+  const float cut_alpha = u.min_alpha;
   float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
   uint32_t last = 0;
   bool done = !inside;
+  float4 n0 = make_float4(0.f, 0.f, 0.f, 0.f), n1 = n0, n2 = n0;
+  if (rg.x + threadIdx.x < rg.y) load_entry(emit_idx, emit_splat, pre, rg.x + threadIdx.x, n0, n1, n2);
   for (uint32_t b = rg.x; b < rg.y; b += DIFF_BLOCK) {
     if (__syncthreads_count(!done) == 0) break;
     const uint32_t e = b + threadIdx.x;
-    if (e < rg.y) {
-      const uint32_t sp = emit_splat[emit_idx[e]];
-      s_p0[threadIdx.x] = pre[3 * (size_t)sp];
-      s_p1[threadIdx.x] = pre[3 * (size_t)sp + 1];
-      s_p2[threadIdx.x] = pre[3 * (size_t)sp + 2];
-    }
+    s_p0[threadIdx.x] = n0;
+    s_p1[threadIdx.x] = n1;
+    s_p2[threadIdx.x] = n2;
+    s_ov[threadIdx.x] = e < rg.y ? quad_mask(q, n0, n1, n2) : 0u;
+    if (e + DIFF_BLOCK < rg.y) load_entry(emit_idx, emit_splat, pre, e + DIFF_BLOCK, n0, n1, n2);
     __syncthreads();
     const uint32_t cnt = min((uint32_t)DIFF_BLOCK, rg.y - b);
-    for (uint32_t k = 0; k < cnt && !done; k++) {
-      float dx, dy, g;
-      const float a = pixel_alpha(s_p0[k], s_p1[k], px, py, u.min_alpha, dx, dy, g);
-      if (a < 0.0f) continue;
-      const float Tn = T * (1.0f - a);
-      if (Tn < u.t_stop) {
-        done = true;
-        break;
+    for (uint32_t c0 = 0; c0 < cnt; c0 += RDX_WAVE) {
+      if (!ballot(!done)) break;
+      uint64_t m = ballot(c0 + q.lane < cnt && ((s_ov[c0 + q.lane] >> q.w) & 1u));
+      while (m) {
+        const uint32_t k = c0 + (uint32_t)__builtin_ctzll(m);
+        m &= m - 1;
+        const float4 p0 = s_p0[k], p1 = s_p1[k];
+        const float dx = q.px - p0.x, dy = q.py - p0.y;
+        const float z2 = p1.x * dx * dx + 2.0f * p1.y * dx * dy + p1.z * dy * dy;
+        const float a = p0.z * __expf(-0.5f * z2);
+        const bool hit = !done && z2 <= p0.w * p0.w && a >= cut_alpha;
+        const float Tn = T * (1.0f - a);
+        const bool stop = hit && Tn < u.t_stop;
+        const bool use = hit && !stop;
+        done = done || stop;
+        if (ballot(use)) {
+          const float4 c = s_p2[k];
+          const float wgt = use ? a * T : 0.0f;
+          C0 = fmaf(c.x, wgt, C0);
+          C1 = fmaf(c.y, wgt, C1);
+          C2 = fmaf(c.z, wgt, C2);
+          T = use ? Tn : T;
+          last = use ? b - rg.x + k + 1 : last;
+        }
       }
-      const float4 c = s_p2[k];
-      C0 = fmaf(c.x * a, T, C0);
-      C1 = fmaf(c.y * a, T, C1);
-      C2 = fmaf(c.z * a, T, C2);
-      T = Tn;
-      last = b - rg.x + k + 1;
     }
     __syncthreads();
   }
   if (!inside) return;
-  const size_t p = (size_t)y * u.width + x, plane_px = (size_t)u.width * u.height;
+  const size_t p = (size_t)q.y * u.width + q.x, plane_px = (size_t)u.width * u.height;
   image[p] = C0;
   image[plane_px + p] = C1;
   image[2 * plane_px + p] = C2;
@@ -420,26 +473,29 @@ __device__ __forceinline__ float wave_sum_to_last(float x) {
 }
 
 // The adjoint of the renderer: per tile, back to front over each pixel's
-// contributors; every entry's pixel-side partials summed over the tile's
-// pixels (per wave with DPP, then the waves in a fixed order: no float
-// atomics, reproducible) into gentry[emission index][9].
+// contributors (the pixel layout and the entry culling of k_diff_render);
+// every entry's pixel-side partials summed over the tile's pixels (per wave
+// with DPP, then the waves in a fixed order: no float atomics,
+// reproducible) into gentry[emission index][9].
 static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render_bwd(const uint2* range, const uint32_t* emit_idx,
                                                                        const uint32_t* emit_splat, const float4* pre, DiffU u,
                                                                        const float* t_final, const uint32_t* n_contrib,
                                                                        const float* grad_image, float* gentry) {
   constexpr int WAVES = DIFF_BLOCK / RDX_WAVE;
-  __shared__ float4 s_p0[DIFF_BLOCK], s_p1[DIFF_BLOCK], s_p2[DIFF_BLOCK];
-  __shared__ uint32_t s_q[DIFF_BLOCK];
-  __shared__ float s_g[WAVES][DIFF_BLOCK][DIFF_PIX_GRADS];
+  constexpr int BATCH = DIFF_BWD_BATCH;
+  constexpr int GWORDS = WAVES * BATCH * DIFF_PIX_GRADS;
+  __shared__ float4 s_p0[BATCH], s_p1[BATCH], s_p2[BATCH];
+  __shared__ uint32_t s_q[BATCH];
+  __shared__ uint32_t s_ov[BATCH];
+  __shared__ __align__(16) float s_g[WAVES][BATCH][DIFF_PIX_GRADS];
   __shared__ uint32_t s_max[WAVES];
+  static_assert(GWORDS % 4 == 0, "partials cleared as float4");
+  static_assert(BATCH % RDX_WAVE == 0 && BATCH <= DIFF_BLOCK, "whole waves of entries");
   const uint32_t tile = blockIdx.x;
-  const int lane = threadIdx.x & (RDX_WAVE - 1), w = threadIdx.x / RDX_WAVE;
-  const int tx = (int)(tile % (uint32_t)u.tiles_x), ty = (int)(tile / (uint32_t)u.tiles_x);
-  const int x = tx * DIFF_TILE + (int)(threadIdx.x % DIFF_TILE), y = ty * DIFF_TILE + (int)(threadIdx.x / DIFF_TILE);
-  const bool inside = x < u.width && y < u.height;
-  const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+  const DiffQuad q = diff_quad(tile, u);
+  const bool inside = q.x < u.width && q.y < u.height;
   const uint2 rg = range[tile];
-  const size_t p = (size_t)y * u.width + x, plane_px = (size_t)u.width * u.height;
+  const size_t p = (size_t)q.y * u.width + q.x, plane_px = (size_t)u.width * u.height;
   float T = inside ? t_final[p] : 1.0f;
   const uint32_t nc = inside ? n_contrib[p] : 0u;
   float dC0 = 0.0f, dC1 = 0.0f, dC2 = 0.0f;
@@ -449,10 +505,10 @@ static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render_bwd(const uin
     dC2 = grad_image[2 * plane_px + p];
   }
   float B0 = 0.0f, B1 = 0.0f, B2 = 0.0f;
-  // the furthest any pixel of the tile went
-  uint32_t m = nc;
-  for (int off = 32; off >= 1; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off));
-  if (lane == 0) s_max[w] = m;
+  // the furthest any pixel of the wave, and of the tile, went
+  uint32_t wreach = nc;
+  for (int off = 32; off >= 1; off >>= 1) wreach = max(wreach, (uint32_t)__shfl_xor((int)wreach, off));
+  if (q.lane == 0) s_max[q.w] = wreach;
   __syncthreads();
   uint32_t reach = 0;
   for (int v = 0; v < WAVES; v++) reach = max(reach, s_max[v]);
@@ -460,56 +516,87 @@ static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render_bwd(const uin
   // entries never reached get zero partials
   for (uint32_t e = end + threadIdx.x; e < rg.y; e += DIFF_BLOCK)
     for (int k = 0; k < DIFF_PIX_GRADS; k++) gentry[(size_t)emit_idx[e] * DIFF_PIX_GRADS + k] = 0.0f;
+  float4 n0 = make_float4(0.f, 0.f, 0.f, 0.f), n1 = n0, n2 = n0;
+  uint32_t nq = 0;
+  {
+    const uint32_t lo0 = end - min((uint32_t)BATCH, end - rg.x);
+    if (threadIdx.x < BATCH && lo0 + threadIdx.x < end) {
+      nq = emit_idx[lo0 + threadIdx.x];
+      const uint32_t sp = emit_splat[nq];
+      n0 = pre[3 * (size_t)sp];
+      n1 = pre[3 * (size_t)sp + 1];
+      n2 = pre[3 * (size_t)sp + 2];
+    }
+  }
   uint32_t hi = end;
   while (hi > rg.x) {
-    const uint32_t lo = hi - min((uint32_t)DIFF_BLOCK, hi - rg.x);
+    const uint32_t lo = hi - min((uint32_t)BATCH, hi - rg.x);
     const uint32_t cnt = hi - lo;
     __syncthreads();  // the previous batch's write-out is done
-    if (threadIdx.x < cnt) {
-      const uint32_t q = emit_idx[lo + threadIdx.x];
-      const uint32_t sp = emit_splat[q];
-      s_q[threadIdx.x] = q;
-      s_p0[threadIdx.x] = pre[3 * (size_t)sp];
-      s_p1[threadIdx.x] = pre[3 * (size_t)sp + 1];
-      s_p2[threadIdx.x] = pre[3 * (size_t)sp + 2];
+    if (threadIdx.x < BATCH) {
+      s_q[threadIdx.x] = nq;
+      s_p0[threadIdx.x] = n0;
+      s_p1[threadIdx.x] = n1;
+      s_p2[threadIdx.x] = n2;
+      s_ov[threadIdx.x] = threadIdx.x < cnt ? quad_mask(q, n0, n1, n2) : 0u;
     }
-    for (int k = threadIdx.x; k < WAVES * DIFF_BLOCK * DIFF_PIX_GRADS; k += DIFF_BLOCK) (&s_g[0][0][0])[k] = 0.0f;
+    {
+      float4* g4 = reinterpret_cast<float4*>(&s_g[0][0][0]);
+#pragma unroll
+      for (int k = threadIdx.x; k < GWORDS / 4; k += DIFF_BLOCK) g4[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    if (lo > rg.x) {
+      const uint32_t lo2 = lo - min((uint32_t)BATCH, lo - rg.x);
+      if (threadIdx.x < BATCH && lo2 + threadIdx.x < lo) {
+        nq = emit_idx[lo2 + threadIdx.x];
+        const uint32_t sp = emit_splat[nq];
+        n0 = pre[3 * (size_t)sp];
+        n1 = pre[3 * (size_t)sp + 1];
+        n2 = pre[3 * (size_t)sp + 2];
+      }
+    }
     __syncthreads();
-    for (int k = (int)cnt - 1; k >= 0; k--) {
-      const uint32_t idx = lo - rg.x + (uint32_t)k;  // position in the tile's list
-      float gv[DIFF_PIX_GRADS];
-      bool contrib = false;
-      if (idx < nc) {
-        const float4 p0 = s_p0[k], p1 = s_p1[k], c = s_p2[k];
-        float dx, dy, g;
-        const float a = pixel_alpha(p0, p1, px, py, u.min_alpha, dx, dy, g);
-        if (a >= 0.0f) {
-          contrib = true;
-          T = T / (1.0f - a);  // the transmittance in front of this splat
-          const float aT = a * T;
-          gv[6] = aT * dC0;
-          gv[7] = aT * dC1;
-          gv[8] = aT * dC2;
-          const float galpha = T * ((c.x - B0) * dC0 + (c.y - B1) * dC1 + (c.z - B2) * dC2);
+    // this wave's entries of the batch below its reach, back to front
+    const uint32_t wend = wreach > lo - rg.x ? min(cnt, wreach - (lo - rg.x)) : 0u;
+    for (int c0 = (int)((wend + RDX_WAVE - 1) / RDX_WAVE) * RDX_WAVE - RDX_WAVE; c0 >= 0; c0 -= RDX_WAVE) {
+      uint64_t m = ballot((uint32_t)c0 + q.lane < wend && ((s_ov[c0 + q.lane] >> q.w) & 1u));
+      while (m) {
+        const int hb = 63 - __builtin_clzll(m);
+        m &= ~(1ull << hb);
+        const int k = c0 + hb;
+        const uint32_t idx = lo - rg.x + (uint32_t)k;  // position in the tile's list
+        const float4 p0 = s_p0[k], p1 = s_p1[k];
+        const float dx = q.px - p0.x, dy = q.py - p0.y;
+        const float z2 = p1.x * dx * dx + 2.0f * p1.y * dx * dy + p1.z * dy * dy;
+        const float g = __expf(-0.5f * z2);
+        const float a = p0.z * g;
+        const bool contrib = idx < nc && z2 <= p0.w * p0.w && a >= u.min_alpha;
+        if (!ballot(contrib)) continue;
+        const float4 c = s_p2[k];
+        float gv[DIFF_PIX_GRADS];
+        const float Tb = T * __builtin_amdgcn_rcpf(1.0f - a);  // the transmittance in front of this splat
+        const float aT = a * Tb;
+        const float galpha = Tb * ((c.x - B0) * dC0 + (c.y - B1) * dC1 + (c.z - B2) * dC2);
+        const float gz2 = -0.5f * a * galpha;
+        gv[0] = gz2 * -2.0f * (p1.x * dx + p1.y * dy);
+        gv[1] = gz2 * -2.0f * (p1.y * dx + p1.z * dy);
+        gv[2] = gz2 * dx * dx;
+        gv[3] = gz2 * 2.0f * dx * dy;
+        gv[4] = gz2 * dy * dy;
+        gv[5] = g * galpha;
+        gv[6] = aT * dC0;
+        gv[7] = aT * dC1;
+        gv[8] = aT * dC2;
+        if (contrib) {
+          T = Tb;
           B0 = fmaf(c.x, a, (1.0f - a) * B0);
           B1 = fmaf(c.y, a, (1.0f - a) * B1);
           B2 = fmaf(c.z, a, (1.0f - a) * B2);
-          gv[5] = g * galpha;
-          const float gz2 = -0.5f * a * galpha;
-          gv[0] = gz2 * -2.0f * (p1.x * dx + p1.y * dy);
-          gv[1] = gz2 * -2.0f * (p1.y * dx + p1.z * dy);
-          gv[2] = gz2 * dx * dx;
-          gv[3] = gz2 * 2.0f * dx * dy;
-          gv[4] = gz2 * dy * dy;
         }
-      }
-      if (!contrib)
-        for (int j = 0; j < DIFF_PIX_GRADS; j++) gv[j] = 0.0f;
-      if (ballot(contrib)) {
 #pragma unroll
         for (int j = 0; j < DIFF_PIX_GRADS; j++) {
-          const float sum = wave_sum_to_last(gv[j]);
-          if (lane == RDX_WAVE - 1) s_g[w][k][j] = sum;
+          const float sum = wave_sum_to_last(contrib ? gv[j] : 0.0f);
+          if (q.lane == RDX_WAVE - 1) s_g[q.w][k][j] = sum;
         }
       }
     }
