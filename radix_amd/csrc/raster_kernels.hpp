@@ -37,7 +37,7 @@ constexpr int DIFF_PARAMS = 59;
 constexpr int DIFF_TILE = 16;
 constexpr int DIFF_BLOCK = DIFF_TILE * DIFF_TILE;
 constexpr int DIFF_PIX_GRADS = 9;  // mu2 2, conic 3, opacity 1, colour 3
-constexpr int DIFF_BWD_BATCH = 128;  // entries staged per round of the adjoint (its LDS: 25 KB)
+constexpr int DIFF_BWD_BATCH = 64;  // entries staged per round of the adjoint
 
 struct DiffU {
   double Vl[9], Vt[3], cam[3];  // world -> view (row-major linear part), camera position
@@ -195,7 +195,7 @@ __device__ inline void diff_splat_forward(const float* th, uint32_t n, uint32_t 
 
 // ---- per splat: what the pixels need, the key and the tiles -------------------
 
-// pre[3 i .. 3 i + 2] = (mu2x, mu2y, opc, cut), (Q0, Q1, Q2, ex), (r, g, b, ey): (ex, ey)
+// pre[3 i .. 3 i + 2] = (mu2x, mu2y, opc, cut^2), (Q0, 2 Q1, Q2, ex), (r, g, b, ey): (ex, ey)
 // the half extents of the cutoff ellipse's bounding box, widened by 2 % and
 // half a pixel (the renderer's waves skip entries whose widened box misses
 // their pixels; z2's float error is far below that margin unless the
@@ -210,8 +210,8 @@ static __global__ __launch_bounds__(256) void k_diff_pre(const float* __restrict
   float ex = s.cut * sqrtf(s.ap) * 1.02f + 0.5f, ey = s.cut * sqrtf(s.dp) * 1.02f + 0.5f;
   // condition proxy (trace^2 / det of the conic's inverse): beyond 1e6 the box is not trusted
   if (!((s.ap + s.dp) * (s.ap + s.dp) <= 1e6f * s.det_p)) ex = ey = INFINITY;
-  pre[3 * (size_t)i] = make_float4(s.mu2x, s.mu2y, s.opc, s.cut);
-  pre[3 * (size_t)i + 1] = make_float4(s.Q0, s.Q1, s.Q2, ex);
+  pre[3 * (size_t)i] = make_float4(s.mu2x, s.mu2y, s.opc, s.cut * s.cut);
+  pre[3 * (size_t)i + 1] = make_float4(s.Q0, 2.0f * s.Q1, s.Q2, ex);
   pre[3 * (size_t)i + 2] = make_float4(s.col[0], s.col[1], s.col[2], ey);
   uint32_t cnt = 0;
   uint4 rc = make_uint4(0u, 0u, 0u, 0u);
@@ -348,6 +348,10 @@ static __global__ void k_diff_clear_ranges(uint2* range, uint32_t tiles) {
 
 // ---- the tile renderer -----------------------------------------------------------
 
+// exp(-z2 / 2) as one multiply and v_exp_f32 (2^x): the same value as
+// __expf(-0.5f * z2), whose halving is exact.
+__device__ __forceinline__ float gauss(float z2) { return __builtin_amdgcn_exp2f(z2 * (-0.5f * 1.44269504088896340736f)); }
+
 // The tile's pixels: wave w takes the 8x8 quadrant (w & 1, w >> 1) of the
 // 16x16 tile, lane l its pixel (l & 7, l >> 3). Entries are staged in LDS
 // 256 at a time (the next batch's indices and pre words are loaded while
@@ -431,9 +435,9 @@ static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render(const uint2* 
         m &= m - 1;
         const float4 p0 = s_p0[k], p1 = s_p1[k];
         const float dx = q.px - p0.x, dy = q.py - p0.y;
-        const float z2 = p1.x * dx * dx + 2.0f * p1.y * dx * dy + p1.z * dy * dy;
-        const float a = p0.z * __expf(-0.5f * z2);
-        const bool hit = !done && z2 <= p0.w * p0.w && a >= cut_alpha;
+        const float z2 = p1.x * dx * dx + p1.y * dx * dy + p1.z * dy * dy;
+        const float a = p0.z * gauss(z2);
+        const bool hit = !done && z2 <= p0.w && a >= cut_alpha;
         const float Tn = T * (1.0f - a);
         const bool stop = hit && Tn < u.t_stop;
         const bool use = hit && !stop;
@@ -461,33 +465,42 @@ static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render(const uint2* 
   n_contrib[p] = last;
 }
 
-// Sum over a wave64 to lane 63 (DPP: rows, then row broadcasts).
-__device__ __forceinline__ float wave_sum_to_last(float x) {
-  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x111, 0xf, 0xf, true));
-  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x112, 0xf, 0xf, true));
-  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x114, 0xf, 0xf, true));
-  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x118, 0xf, 0xf, true));
-  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x142, 0xa, 0xf, false));
-  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x143, 0xc, 0xf, false));
-  return x;
+// Sums over each row of 16 lanes to the row's last lane (DPP row_shr 1, 2, 4, 8
+// with zeros shifted in), for N values at once: the steps of different values
+// are independent, so they fill each other's DPP wait states.
+template <int N>
+__device__ __forceinline__ void row_sums_to_last(float* v) {
+#pragma unroll
+  for (int j = 0; j < N; j++)
+    v[j] += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v[j]), 0x111, 0xf, 0xf, true));
+#pragma unroll
+  for (int j = 0; j < N; j++)
+    v[j] += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v[j]), 0x112, 0xf, 0xf, true));
+#pragma unroll
+  for (int j = 0; j < N; j++)
+    v[j] += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v[j]), 0x114, 0xf, 0xf, true));
+#pragma unroll
+  for (int j = 0; j < N; j++)
+    v[j] += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v[j]), 0x118, 0xf, 0xf, true));
 }
 
 // The adjoint of the renderer: per tile, back to front over each pixel's
 // contributors (the pixel layout and the entry culling of k_diff_render);
-// every entry's pixel-side partials summed over the tile's pixels (per wave
-// with DPP, then the waves in a fixed order: no float atomics,
-// reproducible) into gentry[emission index][9].
+// every entry's pixel-side partials summed over the tile's pixels (per row
+// of 16 lanes with DPP, then the 16 rows of the tile in a fixed order: no
+// float atomics, reproducible) into gentry[emission index][9].
 static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render_bwd(const uint2* range, const uint32_t* emit_idx,
                                                                        const uint32_t* emit_splat, const float4* pre, DiffU u,
                                                                        const float* t_final, const uint32_t* n_contrib,
                                                                        const float* grad_image, float* gentry) {
   constexpr int WAVES = DIFF_BLOCK / RDX_WAVE;
   constexpr int BATCH = DIFF_BWD_BATCH;
-  constexpr int GWORDS = WAVES * BATCH * DIFF_PIX_GRADS;
+  constexpr int SLOTS = WAVES * 4;  // a row of 16 lanes per slot
+  constexpr int GWORDS = SLOTS * BATCH * DIFF_PIX_GRADS;
   __shared__ float4 s_p0[BATCH], s_p1[BATCH], s_p2[BATCH];
   __shared__ uint32_t s_q[BATCH];
   __shared__ uint32_t s_ov[BATCH];
-  __shared__ __align__(16) float s_g[WAVES][BATCH][DIFF_PIX_GRADS];
+  __shared__ __align__(16) float s_g[SLOTS][BATCH][DIFF_PIX_GRADS];
   __shared__ uint32_t s_max[WAVES];
   static_assert(GWORDS % 4 == 0, "partials cleared as float4");
   static_assert(BATCH % RDX_WAVE == 0 && BATCH <= DIFF_BLOCK, "whole waves of entries");
@@ -567,10 +580,10 @@ static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render_bwd(const uin
         const uint32_t idx = lo - rg.x + (uint32_t)k;  // position in the tile's list
         const float4 p0 = s_p0[k], p1 = s_p1[k];
         const float dx = q.px - p0.x, dy = q.py - p0.y;
-        const float z2 = p1.x * dx * dx + 2.0f * p1.y * dx * dy + p1.z * dy * dy;
-        const float g = __expf(-0.5f * z2);
+        const float z2 = p1.x * dx * dx + p1.y * dx * dy + p1.z * dy * dy;
+        const float g = gauss(z2);
         const float a = p0.z * g;
-        const bool contrib = idx < nc && z2 <= p0.w * p0.w && a >= u.min_alpha;
+        const bool contrib = idx < nc && z2 <= p0.w && a >= u.min_alpha;
         if (!ballot(contrib)) continue;
         const float4 c = s_p2[k];
         float gv[DIFF_PIX_GRADS];
@@ -578,8 +591,9 @@ static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render_bwd(const uin
         const float aT = a * Tb;
         const float galpha = Tb * ((c.x - B0) * dC0 + (c.y - B1) * dC1 + (c.z - B2) * dC2);
         const float gz2 = -0.5f * a * galpha;
-        gv[0] = gz2 * -2.0f * (p1.x * dx + p1.y * dy);
-        gv[1] = gz2 * -2.0f * (p1.y * dx + p1.z * dy);
+        const float q1 = 0.5f * p1.y;
+        gv[0] = gz2 * -2.0f * (p1.x * dx + q1 * dy);
+        gv[1] = gz2 * -2.0f * (q1 * dx + p1.z * dy);
         gv[2] = gz2 * dx * dx;
         gv[3] = gz2 * 2.0f * dx * dy;
         gv[4] = gz2 * dy * dy;
@@ -594,21 +608,23 @@ static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render_bwd(const uin
           B2 = fmaf(c.z, a, (1.0f - a) * B2);
         }
 #pragma unroll
-        for (int j = 0; j < DIFF_PIX_GRADS; j++) {
-          const float sum = wave_sum_to_last(contrib ? gv[j] : 0.0f);
-          if (q.lane == RDX_WAVE - 1) s_g[q.w][k][j] = sum;
+        for (int j = 0; j < DIFF_PIX_GRADS; j++) gv[j] = contrib ? gv[j] : 0.0f;
+        row_sums_to_last<DIFF_PIX_GRADS>(gv);
+        if ((q.lane & 15) == 15) {
+          float* gs = &s_g[q.w * 4 + (q.lane >> 4)][k][0];
+#pragma unroll
+          for (int j = 0; j < DIFF_PIX_GRADS; j++) gs[j] = gv[j];
         }
       }
     }
     __syncthreads();
-    if (threadIdx.x < cnt) {
-      const size_t base = (size_t)s_q[threadIdx.x] * DIFF_PIX_GRADS;
+    // entry k's value j: the sum of the slots in a fixed order
+    for (uint32_t o = threadIdx.x; o < cnt * DIFF_PIX_GRADS; o += DIFF_BLOCK) {
+      float v = 0.0f;
 #pragma unroll
-      for (int j = 0; j < DIFF_PIX_GRADS; j++) {
-        float v = 0.0f;
-        for (int ww = 0; ww < WAVES; ww++) v += s_g[ww][threadIdx.x][j];
-        gentry[base + j] = v;
-      }
+      for (int sl = 0; sl < SLOTS; sl++) v += (&s_g[sl][0][0])[o];
+      const uint32_t k = o / DIFF_PIX_GRADS, j = o - k * DIFF_PIX_GRADS;
+      gentry[(size_t)s_q[k] * DIFF_PIX_GRADS + j] = v;
     }
     hi = lo;
   }
