@@ -107,6 +107,16 @@ SortPlan make_plan(const rdx_sort_tuning& t, int bits) {
   return p;
 }
 
+SortPlan compact_plan(const rdx_sort_tuning& t, int bits) {
+  SortPlan p = make_plan(t, bits);
+  if (bits == 32 && t.digit_bits == 8 && t.block_threads == 512) {
+    p.rb = RANGED_RB;
+    p.passes = 4;  // the fourth gated on the range
+    p.ranged = true;
+  }
+  return p;
+}
+
 bool tuning_supported(const rdx_sort_tuning& t) {
   const bool rb = t.digit_bits == 4 || t.digit_bits == 6 || t.digit_bits == 8;
   const bool shape = (t.block_threads == 256 && (t.keys_per_thread == 8 || t.keys_per_thread == 12 ||
@@ -128,16 +138,21 @@ static void launch_kpass(PassKernel k, uint32_t grid, hipStream_t s, const PassA
     hipLaunchKernelGGL((k_scatter<RB, BLOCK, KPT>), dim3(grid), dim3(BLOCK), 0, s, a);
 }
 
+// 256-thread blocks for digits of up to 8 bits (a thread per digit), 512 for all.
 template <int RB>
 static bool launch_rb(const SortPlan& p, PassKernel k, uint32_t grid, hipStream_t s, const PassArgs& a) {
-  if (p.block == 256) {
-    switch (p.kpt) {
-      case 8: launch_kpass<RB, 256, 8>(k, grid, s, a); return true;
-      case 12: launch_kpass<RB, 256, 12>(k, grid, s, a); return true;
-      case 16: launch_kpass<RB, 256, 16>(k, grid, s, a); return true;
-      case 20: launch_kpass<RB, 256, 20>(k, grid, s, a); return true;
+  if constexpr (RB <= 8) {
+    if (p.block == 256) {
+      switch (p.kpt) {
+        case 8: launch_kpass<RB, 256, 8>(k, grid, s, a); return true;
+        case 12: launch_kpass<RB, 256, 12>(k, grid, s, a); return true;
+        case 16: launch_kpass<RB, 256, 16>(k, grid, s, a); return true;
+        case 20: launch_kpass<RB, 256, 20>(k, grid, s, a); return true;
+      }
+      return false;
     }
-  } else if (p.block == 512) {
+  }
+  if (p.block == 512) {
     switch (p.kpt) {
       case 8: launch_kpass<RB, 512, 8>(k, grid, s, a); return true;
       case 12: launch_kpass<RB, 512, 12>(k, grid, s, a); return true;
@@ -152,6 +167,7 @@ static bool launch_pass_kernel(const SortPlan& p, PassKernel k, uint32_t grid, h
     case 4: return launch_rb<4>(p, k, grid, s, a);
     case 6: return launch_rb<6>(p, k, grid, s, a);
     case 8: return launch_rb<8>(p, k, grid, s, a);
+    case RANGED_RB: return launch_rb<RANGED_RB>(p, k, grid, s, a);
   }
   return false;
 }
@@ -170,6 +186,7 @@ int SortScratch::reserve(uint32_t n, uint32_t segments, uint32_t tiles) {
   if ((rc = cnt.ensure((size_t)count_stride(MAX_BINS + 1) * tiles * 4))) return rc;
   if ((rc = tot.ensure((size_t)TOT_STRIDE * std::max<uint32_t>(segments, 1) * 4))) return rc;
   if ((rc = nvalid.ensure((size_t)std::max<uint32_t>(segments, 1) * 4))) return rc;
+  if ((rc = mm.ensure((size_t)2 * (tiles + std::max<uint32_t>(segments, 1)) * 4))) return rc;
   return RDX_OK;
 }
 
@@ -227,6 +244,8 @@ int run_sort(Context* ctx, SortScratch& sc, hipStream_t s, const SortPlan& p, co
   const int bins = 1 << p.rb;
   const uint32_t* kin = keys_in;
   const uint32_t* vin = vals_in;
+  uint32_t* tile_mm = (uint32_t*)sc.mm.ptr;
+  uint32_t* seg_mm = tile_mm + 2 * (size_t)L.tiles;
   for (int pass = 0; pass < p.passes; pass++) {
     const bool last = pass == p.passes - 1;
     PassArgs a;
@@ -254,6 +273,33 @@ int run_sort(Context* ctx, SortScratch& sc, hipStream_t s, const SortPlan& p, co
     a.seg_of_tile = L.seg_of_tile;
     a.seg = L.seg;
     a.dst_cap = n_cap;
+    if (p.ranged) {
+      // pass 0 -> scratch 0, 1 -> scratch 1, 2 -> the output (scratch 0 for
+      // segments whose keys reach beyond three passes), 3 (those segments
+      // only): scratch 0 -> the output. The kept count is final after pass 0.
+      a.ranged = pass == 0 ? 1 : 2;
+      a.tile_mm = tile_mm;
+      a.seg_mm = seg_mm;
+      a.gate = pass == 3;
+      a.alt = pass == 2;
+      if (pass < 2) {
+        a.keys_out = (uint32_t*)sc.keys[pass].ptr;
+        a.vals_out = (uint32_t*)sc.vals[pass].ptr;
+      } else {
+        a.keys_out = final_keys;
+        a.vals_out = final_vals;
+      }
+      if (pass == 2) {
+        a.keys_alt = (uint32_t*)sc.keys[0].ptr;
+        a.vals_alt = (uint32_t*)sc.vals[0].ptr;
+      }
+      if (pass == 3) {
+        a.keys_in = (const uint32_t*)sc.keys[0].ptr;
+        a.vals_in = (const uint32_t*)sc.vals[0].ptr;
+      }
+      a.count_out = pass == 0 ? count_out : nullptr;
+      a.draw_args = pass == 0 ? draw_args : nullptr;
+    }
     char name[32];
     {
       snprintf(name, sizeof name, "sort_count%d", pass);
@@ -263,13 +309,14 @@ int run_sort(Context* ctx, SortScratch& sc, hipStream_t s, const SortPlan& p, co
     {
       snprintf(name, sizeof name, "sort_scan%d", pass);
       Stage st(ctx, s, name);
-      const uint32_t groups = (uint32_t)(a.rows + CGROUP - 1) / CGROUP;
+      // column groups per segment, and the range reduction in a ranged sort's first pass
+      const uint32_t per_seg = (uint32_t)(a.rows + CGROUP - 1) / CGROUP + (a.ranged == 1 ? 1u : 0u);
       // 32 tiles per thread and round from 2048 tiles (one round for a
       // segment of up to 16M keys at 8192-key tiles), else 8
       if (L.tiles >= 2048)
-        hipLaunchKernelGGL(k_scan<32>, dim3(L.segments * groups), dim3(SCAN_BLOCK), 0, s, a, bins);
+        hipLaunchKernelGGL(k_scan<32>, dim3(L.segments * per_seg), dim3(SCAN_BLOCK), 0, s, a, bins);
       else
-        hipLaunchKernelGGL(k_scan<8>, dim3(L.segments * groups), dim3(SCAN_BLOCK), 0, s, a, bins);
+        hipLaunchKernelGGL(k_scan<8>, dim3(L.segments * per_seg), dim3(SCAN_BLOCK), 0, s, a, bins);
     }
     {
       snprintf(name, sizeof name, "sort_scatter%d", pass);
@@ -535,7 +582,7 @@ int rdx_sort_splats(rdx_context* ctx, void* stream, const uint32_t* keys, uint32
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? RDX_OK : hip_fail(e, "WgpuSplatSorter");
   }
-  const SortPlan p = make_plan(c->tuning, bits);
+  const SortPlan p = compact_plan(c->tuning, bits);
   SegLayout L;
   L.tiles = tiles_of(p, n);
   if (int rc = c->sort.reserve(n, 1, L.tiles)) return rc;

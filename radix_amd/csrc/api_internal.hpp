@@ -22,6 +22,7 @@ struct Buffer {
 
 struct SortPlan {
   int rb, block, kpt, tile, passes, bits;
+  bool ranged = false;  // digits of key - base (compacting 32-bit sorts; the last pass gated)
 };
 
 struct StageRecord {
@@ -36,6 +37,7 @@ struct SortScratch {
   Buffer cnt;     // [tiles][count_stride(MAX_BINS + 1)]
   Buffer tot;     // [segments][TOT_STRIDE]
   Buffer nvalid;  // [segments]
+  Buffer mm;      // ranged sorts: [tiles][2] then [segments][2] smallest and largest kept key
   Buffer seg;     // segment table [segments][4] and tile map [tiles] (small segmented sorts)
   int reserve(uint32_t n, uint32_t segments, uint32_t tiles);
   void release() {
@@ -46,6 +48,7 @@ struct SortScratch {
     cnt.release();
     tot.release();
     nvalid.release();
+    mm.release();
     seg.release();
   }
 };
@@ -96,6 +99,10 @@ int fail(int code, const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what);
 int check_bits(int bits, const char* who);
 SortPlan make_plan(const rdx_sort_tuning& t, int bits);
+// The plan of a compacting sort (invalid keys dropped): ranged (9-bit
+// digits of key - base, three passes plus one gated) for 32-bit keys under
+// an 8-bit, 512-thread tuning; make_plan otherwise.
+SortPlan compact_plan(const rdx_sort_tuning& t, int bits);
 bool tuning_supported(const rdx_sort_tuning& t);
 // Tiles of a segment of n keys (at least one, which writes its counts).
 uint32_t tiles_of(const SortPlan& p, uint32_t n);

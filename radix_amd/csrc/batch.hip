@@ -156,7 +156,7 @@ int rdx_batch_create(rdx_context* ctx, const rdx_splat_table* tables, int num_ta
   Batch* b = new Batch();
   b->ctx = c;
   b->params = *params;
-  b->plan = make_plan(c->tuning, params->sort_bits);
+  b->plan = compact_plan(c->tuning, params->sort_bits);
   b->tables.assign(tables, tables + num_tables);
   b->views.assign(views, views + num_views);
   auto bail = [&](int rc) {

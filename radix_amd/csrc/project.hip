@@ -291,7 +291,7 @@ int rdx_project_sort(rdx_context* ctx, void* stream, const rdx_splat_table* tabl
   if ((uint64_t)n * segs > 0xffffffffull) return fail(RDX_E_CAPACITY, "rdx_project_sort: %u splats per eye", n);
   ProjU u;
   if (int rc = make_uniforms(table, camera, params, &u)) return rc;
-  const SortPlan p = make_plan(c->tuning, params->sort_bits);
+  const SortPlan p = compact_plan(c->tuning, params->sort_bits);
   SegLayout L;
   const uint32_t nt = tiles_of(p, n);
   if (int rc = c->sort.reserve(n * segs, segs, nt * segs)) return rc;

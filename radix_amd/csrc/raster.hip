@@ -167,7 +167,7 @@ int rdx_raster_forward(rdx_raster* raster, void* stream, const float* params_dev
                          (uint32_t*)r->keys.ptr, (uint32_t*)r->touch.ptr, (uint4*)r->rect.ptr);
   }
   // back to front, invalid keys dropped; the visible count on the device
-  const SortPlan p = make_plan(c->tuning, 32);
+  const SortPlan p = compact_plan(c->tuning, 32);
   SegLayout L;
   L.tiles = tiles_of(p, n);
   if (int rc = run_sort(c, r->sort_splats, s, p, L, (const uint32_t*)r->keys.ptr, nullptr, nullptr, (uint32_t*)r->order.ptr,

@@ -23,7 +23,7 @@
 
 namespace rdx {
 
-constexpr int MAX_BINS = 256;
+constexpr int MAX_BINS = 512;
 constexpr int TOT_STRIDE = MAX_BINS + 1;  // tot words per segment: digits, then the hidden-rule count
 constexpr int SCAN_BLOCK = 1024;
 // Tile counts are stored tile-major, cnt[tile][cstride] (a count block
@@ -39,6 +39,19 @@ __host__ __device__ constexpr uint32_t count_stride(int rows) { return (uint32_t
 // PEEL_MIN lanes (a round of spread digits then costs one ballot).
 constexpr int PEEL = 3;
 constexpr uint32_t PEEL_MIN = 8;
+
+// Ranged sorts: a compacting 32-bit sort sorts key - base, base = the
+// segment's smallest kept key rounded down to a multiple of 2^RANGED_RB, by
+// RANGED_RB-bit digits. The first pass's digit is then the key's own low
+// digit (it needs no base; its count kernel finds each tile's smallest and
+// largest kept key and its scan reduces them per segment); the later passes
+// read the base. Three passes cover RANGED_BITS bits above the base, i.e.
+// kept distances within a factor 2^16 of each other; a fourth pass runs only
+// for segments whose keys reach further (the third pass then writes to the
+// scratch instead of the output).
+constexpr int RANGED_RB = 9;
+constexpr int RANGED_BITS = 3 * RANGED_RB;
+constexpr uint32_t RANGED_LOW = (1u << RANGED_RB) - 1u;
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs
 // on XCD b % 8), each with its own L2. The scatter maps them so that each
@@ -76,6 +89,14 @@ struct PassArgs {
   const uint32_t* seg_of_tile;  // NULL: one segment of all tiles
   const uint32_t* seg;          // [segments][4]: tile0, off, n, ntiles
   uint32_t dst_cap;          // output capacity (a guard on every write)
+  // ranged sorts
+  int ranged;                // 0: digits of the key; 1: first ranged pass; 2: later ranged pass (digits of key - base)
+  uint32_t* tile_mm;         // [tiles][2] smallest and largest kept key (first ranged pass)
+  uint32_t* seg_mm;          // [segments][2] the same per segment (the first pass's scan)
+  int gate;                  // run only for segments whose keys reach beyond RANGED_BITS above the base
+  int alt;                   // segments that reach beyond write to keys_alt / vals_alt instead
+  uint32_t* keys_alt;
+  uint32_t* vals_alt;
 };
 
 struct SegInfo {
@@ -108,6 +129,19 @@ __device__ __forceinline__ uint32_t seg_count(const PassArgs& a, const SegInfo& 
   return n;
 }
 
+// The base of a segment's digits (0 but in a later ranged pass) and whether
+// its kept keys reach beyond RANGED_BITS above the base.
+__device__ __forceinline__ uint32_t seg_base(const PassArgs& a, uint32_t s) {
+  return a.ranged == 2 ? (a.seg_mm[2 * s] & ~RANGED_LOW) : 0u;
+}
+__device__ __forceinline__ bool seg_wide(const PassArgs& a, uint32_t s) {
+  const uint32_t mn = a.seg_mm[2 * s], mx = a.seg_mm[2 * s + 1];
+  return mx >= mn && mx - (mn & ~RANGED_LOW) >= (1u << RANGED_BITS);
+}
+__device__ __forceinline__ uint32_t digit_of(const PassArgs& a, uint32_t k, uint32_t base, uint32_t dmask) {
+  return (((k & a.key_mask) - base) >> a.shift) & dmask;
+}
+
 // The hidden rule of the reference's CPU sort: keys whose top part is all
 // ones sort last and are not counted as visible (src/core/sort.ts:104, 128,
 // 154).
@@ -131,10 +165,11 @@ __global__ __launch_bounds__(BLOCK) void k_count(PassArgs a) {
   constexpr int WAVES = BLOCK / RDX_WAVE;
   constexpr int WTILE = RDX_WAVE * KPT;
   constexpr int TILE = BLOCK * KPT;
-  constexpr int SUBS = 4;
+  constexpr int SUBS = BINS >= 512 ? 2 : 4;
   constexpr int ROW = BINS + 1;
   __shared__ uint32_t s_rows[WAVES * SUBS * ROW];
   __shared__ uint32_t s_rule[WAVES];
+  __shared__ uint32_t s_mm[2];
 
   const int tid = threadIdx.x;
   const int w = tid / RDX_WAVE;
@@ -142,7 +177,13 @@ __global__ __launch_bounds__(BLOCK) void k_count(PassArgs a) {
   const uint32_t tile = blockIdx.x;
   for (int d = tid; d < WAVES * SUBS * ROW; d += BLOCK) s_rows[d] = 0u;
   if (tid < WAVES) s_rule[tid] = 0u;
+  if (tid == 0) {
+    s_mm[0] = 0xffffffffu;
+    s_mm[1] = 0u;
+  }
   const SegInfo g = seg_info(a, a.seg_of_tile ? a.seg_of_tile[tile] : 0u);
+  if (a.gate && !seg_wide(a, g.s)) return;  // uniform per block
+  const uint32_t kbase = seg_base(a, g.s);
   const uint32_t base = (tile - g.tile0) * (uint32_t)TILE;
   const uint32_t* keys = a.keys_in + g.off;
   const bool rule = a.rows > BINS;
@@ -157,16 +198,35 @@ __global__ __launch_bounds__(BLOCK) void k_count(PassArgs a) {
   const uint32_t n = seg_count(a, g);
   __syncthreads();  // rows zeroed
   uint32_t* row = s_rows + (w * SUBS + (lane & (SUBS - 1))) * ROW;
-  uint32_t hidden = 0;
+  uint32_t hidden = 0, kmin = 0xffffffffu, kmax = 0u;
 #pragma unroll
   for (int j = 0; j < KPT; j++) {
     const uint32_t i = wbase + (uint32_t)(j * RDX_WAVE);
     const bool in = i < n;
+    const bool kept = in && !(a.drop && k[j] == a.invalid);
     if (rule && in && hidden_rule(k[j], a.bits)) hidden++;
-    if (in && !(a.drop && k[j] == a.invalid)) atomicAdd(&row[((k[j] & a.key_mask) >> a.shift) & DMASK], 1u);
+    if (kept) {
+      atomicAdd(&row[digit_of(a, k[j], kbase, DMASK)], 1u);
+      kmin = min(kmin, k[j]);
+      kmax = max(kmax, k[j]);
+    }
   }
   if (rule && hidden) atomicAdd(&s_rule[w], hidden);
+  if (a.ranged == 1) {
+    for (int off = 32; off >= 1; off >>= 1) {
+      kmin = min(kmin, (uint32_t)__shfl_xor((int)kmin, off));
+      kmax = max(kmax, (uint32_t)__shfl_xor((int)kmax, off));
+    }
+    if (lane == 0) {
+      atomicMin(&s_mm[0], kmin);
+      atomicMax(&s_mm[1], kmax);
+    }
+  }
   __syncthreads();
+  if (a.ranged == 1 && tid == 0) {
+    a.tile_mm[2 * (size_t)tile] = s_mm[0];
+    a.tile_mm[2 * (size_t)tile + 1] = s_mm[1];
+  }
   for (int d = tid; d < BINS; d += BLOCK) {
     uint32_t c = 0;
 #pragma unroll
@@ -195,8 +255,9 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(PassArgs a, int bins) {
   __shared__ uint32_t s_sum[SCAN_CHUNKS][CGROUP];
   __shared__ uint32_t s_tot[CGROUP];
   const uint32_t groups = (uint32_t)(a.rows + CGROUP - 1) / CGROUP;
-  const uint32_t s = blockIdx.x / groups;
-  const int g = (int)(blockIdx.x % groups);
+  const uint32_t per_seg = groups + (a.ranged == 1 ? 1u : 0u);  // + the range reduction
+  const uint32_t s = blockIdx.x / per_seg;
+  const int g = (int)(blockIdx.x % per_seg);
   const int d = threadIdx.x % CGROUP;
   const int c = threadIdx.x / CGROUP;
   const int r = g * CGROUP + d;
@@ -204,6 +265,33 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(PassArgs a, int bins) {
   if (a.seg) {
     tile0 = a.seg[4 * s];
     ntiles = a.seg[4 * s + 3];
+  }
+  if (a.gate && !seg_wide(a, s)) return;  // uniform per block
+  if (g == (int)groups) {
+    // the segment's smallest and largest kept key
+    uint32_t mn = 0xffffffffu, mx = 0u;
+    for (uint32_t t = threadIdx.x; t < ntiles; t += SCAN_BLOCK) {
+      mn = min(mn, a.tile_mm[2 * (size_t)(tile0 + t)]);
+      mx = max(mx, a.tile_mm[2 * (size_t)(tile0 + t) + 1]);
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+      mn = min(mn, (uint32_t)__shfl_xor((int)mn, off));
+      mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
+    }
+    if (threadIdx.x % RDX_WAVE == 0) {
+      s_sum[threadIdx.x / RDX_WAVE][0] = mn;
+      s_sum[threadIdx.x / RDX_WAVE][1] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int v = 1; v < SCAN_BLOCK / RDX_WAVE; v++) {
+        mn = min(mn, s_sum[v][0]);
+        mx = max(mx, s_sum[v][1]);
+      }
+      a.seg_mm[2 * s] = mn;
+      a.seg_mm[2 * s + 1] = mx;
+    }
+    return;
   }
   uint32_t* col = a.cnt + (size_t)tile0 * a.cstride + r;
   const bool live = r < a.rows;
@@ -258,7 +346,8 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
   static_assert(BINS <= BLOCK, "a thread per digit");
 
   static_assert(WAVES * BINS * 8 <= TILE * 8, "lane masks fit in the staging area");
-  __shared__ uint32_t s_wave[WAVES][BINS];  // per-wave digit counts, then exclusive offsets
+  static_assert(TILE < 65536, "16-bit wave counts");
+  __shared__ uint16_t s_wave[WAVES][BINS];  // per-wave digit counts, then exclusive offsets (< TILE)
   __shared__ uint32_t s_local[BINS];        // start of each digit within the tile
   __shared__ uint32_t s_out[BINS];          // global start - local start, for the write-out
   __shared__ uint64_t s_stage[TILE];        // ranking: lane masks; then the staged keys and values
@@ -276,6 +365,14 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
     (&s_match[0][0])[d] = 0ull;
   }
   const SegInfo g = seg_info(a, a.seg_of_tile ? a.seg_of_tile[tile] : 0u);
+  if (a.gate && !seg_wide(a, g.s)) return;  // uniform per block
+  const uint32_t kbase = seg_base(a, g.s);
+  uint32_t* keys_out = a.keys_out;
+  uint32_t* vals_out = a.vals_out;
+  if (a.alt && seg_wide(a, g.s)) {
+    keys_out = a.keys_alt;
+    vals_out = a.vals_alt;
+  }
   const uint32_t local_tile = tile - g.tile0;
   const uint32_t tile_base = local_tile * (uint32_t)TILE;
 
@@ -323,7 +420,7 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
   for (int j = 0; j < KPT; j++) {
     const uint32_t i = wbase + (uint32_t)(j * RDX_WAVE);
     rank[j] = i < n && !(a.drop && keys[j] == a.invalid) ? 0u : 0xffffffffu;  // 0xffffffff: not kept
-    dig[j] = ((keys[j] & a.key_mask) >> a.shift) & DMASK;
+    dig[j] = digit_of(a, keys[j], kbase, DMASK);
   }
   __syncthreads();  // s_wave, s_match zeroed
   // Rank each round of 64 keys in input order. Up to PEEL digits of the
@@ -420,13 +517,13 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
     ov[r] = i < tile_kept ? s_vals[i] : 0u;
   }
 #pragma unroll
-  for (int r = 0; r < KPT; r++) od[r] = s_out[((ok[r] & a.key_mask) >> a.shift) & DMASK] + (uint32_t)(r * BLOCK + tid);
+  for (int r = 0; r < KPT; r++) od[r] = s_out[digit_of(a, ok[r], kbase, DMASK)] + (uint32_t)(r * BLOCK + tid);
 #pragma unroll
   for (int r = 0; r < KPT; r++) {
     const uint32_t i = (uint32_t)(r * BLOCK + tid);
     if (i < tile_kept && od[r] < a.dst_cap) {  // always below when the counts are consistent
-      if (a.keys_out) a.keys_out[od[r]] = ok[r];
-      a.vals_out[od[r]] = ov[r];
+      if (keys_out) keys_out[od[r]] = ok[r];
+      vals_out[od[r]] = ov[r];
     }
   }
 }
