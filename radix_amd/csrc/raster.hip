@@ -84,8 +84,16 @@ static int make_diff_uniforms(const rdx_camera* cam, const rdx_params* prm, int 
   return RDX_OK;
 }
 
+// The compositor's two sorts have their own tile shapes: a few hundred
+// thousand depth keys and about a million entries fill the machine only with
+// smaller tiles than the context's (8,192 keys by default). Depth: ranged
+// compacting sort, 512 threads x 8 keys; entries (16-bit tile keys, sized by
+// the capacity, the blocks past the device count return at once): 256 x 8.
+static SortPlan depth_plan() { return compact_plan(rdx_sort_tuning{8, 8, 512}, 32); }
+static SortPlan entry_plan() { return make_plan(rdx_sort_tuning{8, 8, 256}, 16); }
+
 static int reserve(Raster* r, uint32_t n, const DiffU& u) {
-  const SortPlan p = make_plan(r->ctx->tuning, 32);
+  const SortPlan p = depth_plan();
   const uint32_t per = r->set.max_entries_per_splat ? r->set.max_entries_per_splat : 16u;
   const uint64_t cap64 = std::min<uint64_t>((uint64_t)std::max<uint32_t>(n, 1) * per, 0x7fffffffull);
   const uint32_t cap = (uint32_t)cap64;
@@ -104,7 +112,7 @@ static int reserve(Raster* r, uint32_t n, const DiffU& u) {
       (rc = r->gpix.ensure(nn * DIFF_PIX_GRADS * 4)))
     return rc;
   if ((rc = r->sort_splats.reserve(n, 1, tiles_of(p, n)))) return rc;
-  const SortPlan pe = make_plan(r->ctx->tuning, 16);
+  const SortPlan pe = entry_plan();
   if ((rc = r->sort_entries.reserve(cap, 1, tiles_of(pe, cap)))) return rc;
   r->cap = cap;
   return RDX_OK;
@@ -167,7 +175,7 @@ int rdx_raster_forward(rdx_raster* raster, void* stream, const float* params_dev
                          (uint32_t*)r->keys.ptr, (uint32_t*)r->touch.ptr, (uint4*)r->rect.ptr);
   }
   // back to front, invalid keys dropped; the visible count on the device
-  const SortPlan p = compact_plan(c->tuning, 32);
+  const SortPlan p = depth_plan();
   SegLayout L;
   L.tiles = tiles_of(p, n);
   if (int rc = run_sort(c, r->sort_splats, s, p, L, (const uint32_t*)r->keys.ptr, nullptr, nullptr, (uint32_t*)r->order.ptr,
@@ -193,7 +201,7 @@ int rdx_raster_forward(rdx_raster* raster, void* stream, const float* params_dev
                        (uint32_t*)r->emit_splat.ptr, counts + C_OVERFLOW);
   }
   // stable sort of the entries by tile: front to back within each tile
-  const SortPlan pe = make_plan(c->tuning, 16);
+  const SortPlan pe = entry_plan();
   SegLayout Le;
   Le.tiles = tiles_of(pe, r->cap);
   if (int rc = run_sort(c, r->sort_entries, s, pe, Le, (const uint32_t*)r->tile_key.ptr, (const uint32_t*)r->emit_idx.ptr,
