@@ -186,7 +186,9 @@ __device__ inline void diff_splat_forward(const float* th, uint32_t n, uint32_t 
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     float v = plane(th, n, 11 + c, i);
-    for (int k = 0; k < nk; k++) v = fmaf(plane(th, n, 14 + 3 * k + c, i), s.Y[k], v);
+#pragma unroll
+    for (int k = 0; k < 15; k++)  // constant indices: Y stays in registers
+      if (k < nk) v = fmaf(plane(th, n, 14 + 3 * k + c, i), s.Y[k], v);
     s.col_raw[c] = v;
     s.col[c] = fmaxf(v, 0.0f);
   }
@@ -673,18 +675,26 @@ static __global__ __launch_bounds__(256) void k_diff_pre_bwd(const float* __rest
     float3 Yg[15];
     sh_basis15_grad(s.dir[0], s.dir[1], s.dir[2], Yg);
     float gdir[3] = {0.0f, 0.0f, 0.0f};
-    for (int k = 0; k < nk; k++)
-      for (int c = 0; c < 3; c++) {
-        gout[14 + 3 * k + c] = s.Y[k] * graw[c];
-        const float cg = plane(th, n, 14 + 3 * k + c, i) * graw[c];
-        gdir[0] = fmaf(cg, Yg[k].x, gdir[0]);
-        gdir[1] = fmaf(cg, Yg[k].y, gdir[1]);
-        gdir[2] = fmaf(cg, Yg[k].z, gdir[2]);
+    // constant indices throughout (the degree guards each band): the arrays
+    // stay in registers
+#pragma unroll
+    for (int k = 0; k < 15; k++) {
+      if (k < nk) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          gout[14 + 3 * k + c] = s.Y[k] * graw[c];
+          const float cg = plane(th, n, 14 + 3 * k + c, i) * graw[c];
+          gdir[0] = fmaf(cg, Yg[k].x, gdir[0]);
+          gdir[1] = fmaf(cg, Yg[k].y, gdir[1]);
+          gdir[2] = fmaf(cg, Yg[k].z, gdir[2]);
+        }
       }
+    }
     float gmu[3] = {0.0f, 0.0f, 0.0f};
     if (s.dlen > 0.0f) {
       const float dd = gdir[0] * s.dir[0] + gdir[1] * s.dir[1] + gdir[2] * s.dir[2];
-      for (int e = 0; e < 3; e++) gmu[e] += (gdir[e] - s.dir[e] * dd) / s.dlen;
+      const float il = __builtin_amdgcn_rcpf(s.dlen);
+      for (int e = 0; e < 3; e++) gmu[e] += (gdir[e] - s.dir[e] * dd) * il;
     }
     // conic = inverse(cov2D'): dcov' = -Q G Q
     const float G00 = gQ0, G01 = 0.5f * gQ1, G11 = gQ2;
@@ -697,9 +707,10 @@ static __global__ __launch_bounds__(256) void k_diff_pre_bwd(const float* __rest
     float gop = 0.0f;
     if (s.ratio > 0.0f) {
       const float h = 0.5f * s.opc;
-      ga += gopc * h * (s.d / s.det_o - s.dp / s.det_p);
-      gd += gopc * h * (s.a / s.det_o - s.ap / s.det_p);
-      gb += gopc * h * (-2.0f * s.b / s.det_o + 2.0f * s.b / s.det_p);
+      const float io = __builtin_amdgcn_rcpf(s.det_o), ip = __builtin_amdgcn_rcpf(s.det_p);
+      ga += gopc * h * (s.d * io - s.dp * ip);
+      gd += gopc * h * (s.a * io - s.ap * ip);
+      gb += gopc * h * (-2.0f * s.b * io + 2.0f * s.b * ip);
       gop = gopc * sqrtf(s.ratio);
     }
     gout[10] = gop * s.op * (1.0f - s.op);
@@ -725,15 +736,15 @@ static __global__ __launch_bounds__(256) void k_diff_pre_bwd(const float* __rest
       for (int c = 0; c < 3; c++)
         gJ[3 * r + c] = gT[3 * r] * (float)u.Vl[3 * c] + gT[3 * r + 1] * (float)u.Vl[3 * c + 1] + gT[3 * r + 2] * (float)u.Vl[3 * c + 2];
     const float x = (float)s.t[0], y = (float)s.t[1], z = (float)s.t[2];
-    const float iz = 1.0f / z, iz2 = iz * iz, iz3 = iz2 * iz;
+    const float iz = __builtin_amdgcn_rcpf(z), iz2 = iz * iz, iz3 = iz2 * iz;
     float gt[3];
     gt[0] = gJ[2] * (-u.fx * iz2);
     gt[1] = gJ[5] * (-u.fy * iz2);
     gt[2] = gJ[0] * (-u.fx * iz2) + gJ[2] * (2.0f * u.fx * x * iz3) + gJ[4] * (-u.fy * iz2) + gJ[5] * (2.0f * u.fy * y * iz3);
     // the centre in pixels from clip
-    const float w = (float)s.clip[3];
-    const float gcx = gmu2x * 0.5f * u.width / w, gcy = gmu2y * 0.5f * u.height / w;
-    const float gcw = -(gmu2x * 0.5f * u.width * (float)s.clip[0] + gmu2y * 0.5f * u.height * (float)s.clip[1]) / (w * w);
+    const float iw = __builtin_amdgcn_rcpf((float)s.clip[3]);
+    const float gcx = gmu2x * 0.5f * u.width * iw, gcy = gmu2y * 0.5f * u.height * iw;
+    const float gcw = -(gmu2x * 0.5f * u.width * (float)s.clip[0] + gmu2y * 0.5f * u.height * (float)s.clip[1]) * (iw * iw);
     for (int k = 0; k < 3; k++) gt[k] += gcx * (float)u.P[k] + gcy * (float)u.P[4 + k] + gcw * (float)u.P[12 + k];
     for (int k = 0; k < 3; k++) gmu[k] += (float)u.Vl[k] * gt[0] + (float)u.Vl[3 + k] * gt[1] + (float)u.Vl[6 + k] * gt[2];
     for (int k = 0; k < 3; k++) gout[k] = gmu[k];
@@ -762,7 +773,8 @@ static __global__ __launch_bounds__(256) void k_diff_pre_bwd(const float* __rest
     gq[3] = 2.0f * (-gR[1] * qz + gR[2] * qy + gR[3] * qz - gR[5] * qx - gR[6] * qy + gR[7] * qx);
     if (s.qlen > 0.0f) {
       const float dq = gq[0] * qx + gq[1] * qy + gq[2] * qz + gq[3] * qw;
-      for (int k = 0; k < 4; k++) gout[6 + k] = (gq[k] - s.qv[k] * dq) / s.qlen;
+      const float iq = __builtin_amdgcn_rcpf(s.qlen);
+      for (int k = 0; k < 4; k++) gout[6 + k] = (gq[k] - s.qv[k] * dq) * iq;
     }
   }
   for (int k = 0; k < DIFF_PARAMS; k++) grad[(size_t)k * n + i] = gout[k];
