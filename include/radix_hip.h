@@ -342,6 +342,42 @@ int rdx_raster_backward(rdx_raster* raster, void* stream, const float* grad_imag
 int rdx_raster_stats(rdx_raster* raster, void* stream, uint32_t* visible, uint32_t* entries, uint32_t* capacity,
                      int* overflowed);
 
+/* ---- The draw (production form) -------------------------------------------- */
+
+/* The splat draw of the ewa method (src/render/splatDraw.ts:98-181 with
+ * projections.tsl.ts:150-167 and fragmentAlpha, projection.tsl.ts:444-455):
+ * for each splat of the sorted order, back to front, the fragments of its
+ * quad, from the eye's ewa words (ndc centre, angle, log scales, extRgb,
+ * opacity byte; zero byte = culled): z^2 = |uv|^2, discarded beyond the
+ * cutoff and below minAlpha, alpha = fragmentAlpha(z^2, opacity, falloff)
+ * (opacities above 1 are LoD stretches), blended premultiplied "over"
+ * (src = color * alpha, dst = src + (1 - alpha) dst). Composited per 16x16
+ * tile front to back (equal to the blend up to rounding). No counterpart
+ * for the rasteriser itself: it replaces the reference's drawIndexed of
+ * quads (WebGPU) for native hosts. */
+typedef struct rdx_draw rdx_draw;
+typedef struct rdx_draw_settings {
+  float t_stop;                   /* 0: every fragment (the reference's blend); t > 0: stop a pixel at transmittance t */
+  int linear_output;              /* 1: colours sRGB-decoded first (an output storing linear colour, splatDraw.ts:176) */
+  uint32_t max_entries_per_splat; /* (splat, tile) entry capacity per drawn splat (0: 16) */
+} rdx_draw_settings;
+int rdx_draw_create(rdx_context* ctx, const rdx_draw_settings* settings, rdx_draw** out);
+int rdx_draw_destroy(rdx_draw* draw);
+/* words_dev: the eye's ewa words of n splats (uint32x4 each, as
+ * rdx_project_ewa writes them); order_dev: `capacity` splat indices, back
+ * to front (rdx_sort_splats' indices), of which *count_dev (device, clamped
+ * to capacity) are drawn; indices >= n are skipped (splatDraw.ts:121-122).
+ * params: max_std_dev, min_alpha, falloff. rgba_dev: 4 planes of width x
+ * height floats, premultiplied r, g, b and alpha over a transparent target,
+ * row 0 at the top (framebuffer rows, pixel centres at + 0.5). */
+int rdx_draw_splats(rdx_draw* draw, void* stream, const uint32_t* words_dev, uint32_t n, const uint32_t* order_dev,
+                    const uint32_t* count_dev, uint32_t capacity, uint32_t width, uint32_t height,
+                    const rdx_params* params, float* rgba_dev);
+/* The last draw's drawn positions, (splat, tile) entries, entry capacity
+ * and whether they overflowed it (then nothing was drawn). Synchronizes. */
+int rdx_draw_stats(rdx_draw* draw, void* stream, uint32_t* drawn, uint32_t* entries, uint32_t* capacity,
+                   int* overflowed);
+
 #ifdef __cplusplus
 }
 #endif

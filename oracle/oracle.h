@@ -222,6 +222,12 @@ double orc_diff_loss_frozen(const OrcDiffFrozen* f, const double* theta, int n, 
 void orc_diff_free(OrcDiffFrozen* f);
 void orc_sh_basis_grad(const double* dir, double (*grad)[3]);
 
+/* The ewa method's splat draw (src/render/splatDraw.ts:98-181): words (n x
+ * 4) of one eye, order[0..count) back to front, premultiplied "over" into
+ * rgba (4 planes of width x height doubles, row 0 at the top). draw.c */
+void orc_draw(const uint32_t* words, uint32_t n, const uint32_t* order, uint32_t count, int width, int height,
+              double max_std_dev, double min_alpha, double falloff, int linear, double* rgba);
+
 #ifdef __cplusplus
 }
 #endif

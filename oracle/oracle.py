@@ -133,6 +133,8 @@ def lib() -> C.CDLL:
             "orc_diff_loss_frozen": (_d, [C.c_void_p, _dp, C.c_int, C.POINTER(OrcRaster), _dp]),
             "orc_diff_free": (None, [C.c_void_p]),
             "orc_sh_basis_grad": (None, [_dp, _dp]),
+            "orc_draw": (None, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_int, C.c_int, _d, _d, _d, C.c_int,
+                                _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(h, name)
@@ -334,3 +336,18 @@ class Frozen:
         if getattr(self, "h", None):
             lib().orc_diff_free(self.h)
             self.h = None
+
+
+def draw(words, order, count: int, width: int, height: int, max_std_dev: float = float(np.sqrt(8.0)),
+         min_alpha: float = 0.5 / 255.0, falloff: float = 1.0, linear: bool = False) -> np.ndarray:
+    """The ewa splat draw (src/render/splatDraw.ts:98-181): words (n, 4)
+    uint32 of one eye, order (back to front) of which `count` are drawn.
+    Returns (4, height, width) float64: premultiplied rgb and alpha, row 0
+    at the top."""
+    w = np.ascontiguousarray(words, dtype=np.uint32).reshape(-1, 4)
+    o = np.ascontiguousarray(order, dtype=np.uint32)
+    count = int(min(count, o.size))
+    out = np.zeros((4, height, width), dtype=np.float64)
+    lib().orc_draw(w.ctypes.data, w.shape[0], o.ctypes.data if o.size else None, count, width, height,
+                   max_std_dev, min_alpha, falloff, 1 if linear else 0, dptr(out))
+    return out

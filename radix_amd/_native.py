@@ -121,6 +121,10 @@ class RasterSettings(C.Structure):
     _fields_ = [("sh_degree", C.c_int), ("t_stop", C.c_float), ("max_entries_per_splat", C.c_uint32)]
 
 
+class DrawSettings(C.Structure):
+    _fields_ = [("t_stop", C.c_float), ("linear_output", C.c_int), ("max_entries_per_splat", C.c_uint32)]
+
+
 BATCH_GRAPH = 1
 BATCH_MAX_VIEWS = 4096
 
@@ -186,6 +190,13 @@ _SIGNATURES = [
      [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(Camera), C.POINTER(Params), C.c_void_p, C.c_void_p]),
     ("rdx_raster_backward", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("rdx_raster_stats", C.c_int,
+     [C.c_void_p, C.c_void_p, _u32p, _u32p, _u32p, C.POINTER(C.c_int)]),
+    ("rdx_draw_create", C.c_int, [C.c_void_p, C.POINTER(DrawSettings), C.POINTER(C.c_void_p)]),
+    ("rdx_draw_destroy", C.c_int, [C.c_void_p]),
+    ("rdx_draw_splats", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+      C.POINTER(Params), C.c_void_p]),
+    ("rdx_draw_stats", C.c_int,
      [C.c_void_p, C.c_void_p, _u32p, _u32p, _u32p, C.POINTER(C.c_int)]),
 ]
 

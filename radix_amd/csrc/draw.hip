@@ -1,0 +1,180 @@
+// Host side of the production draw (draw_kernels.hpp): the sorted splats'
+// ewa words composited per 16x16 tile into a premultiplied RGBA image, on
+// one stream, with buffers sized on demand.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "../../include/radix_hip.h"
+#include "api_internal.hpp"
+#include "draw_kernels.hpp"
+#include "sort_kernels.hpp"
+
+namespace rdx {
+
+struct Draw {
+  Context* ctx = nullptr;
+  rdx_draw_settings set;
+  SortScratch sort_entries;
+  Buffer pre, touch, rect, offsets, bsum, tile_key, emit_idx, emit_splat, range, counts;
+  uint32_t capacity = 0, cap = 0;  // draw positions, entries
+  void release() {
+    sort_entries.release();
+    Buffer* b[] = {&pre, &touch, &rect, &offsets, &bsum, &tile_key, &emit_idx, &emit_splat, &range, &counts};
+    for (Buffer* x : b) x->release();
+  }
+};
+
+// counts words: [0] drawn positions (the clamped count), [1] entries, [2] overflow
+enum { D_DRAWN = 0, D_ENTRIES = 1, D_OVERFLOW = 2, D_WORDS = 4 };
+
+// Entries of 16-bit tile keys, sorted in 2,048-key tiles (as the compositor's).
+static SortPlan draw_entry_plan() { return make_plan(rdx_sort_tuning{8, 8, 256}, 16); }
+
+static int reserve(Draw* d, uint32_t capacity, uint32_t tiles) {
+  const uint32_t per = d->set.max_entries_per_splat ? d->set.max_entries_per_splat : 16u;
+  const size_t nn = std::max<uint32_t>(capacity, 1);
+  const uint32_t cap = (uint32_t)std::min<uint64_t>((uint64_t)nn * per, 0x7fffffffull);
+  int rc;
+  if ((rc = d->pre.ensure(nn * 3 * sizeof(float4))) || (rc = d->touch.ensure(nn * 4)) ||
+      (rc = d->rect.ensure(nn * sizeof(uint4))) || (rc = d->offsets.ensure(nn * 4)) ||
+      (rc = d->bsum.ensure((nn / DSCAN_TILE + 2) * 4)) || (rc = d->counts.ensure(D_WORDS * 4)))
+    return rc;
+  if ((rc = d->tile_key.ensure((size_t)cap * 4)) || (rc = d->emit_idx.ensure((size_t)cap * 4)) ||
+      (rc = d->emit_splat.ensure((size_t)cap * 4)) || (rc = d->range.ensure((size_t)tiles * sizeof(uint2))))
+    return rc;
+  const SortPlan pe = draw_entry_plan();
+  if ((rc = d->sort_entries.reserve(cap, 1, tiles_of(pe, cap)))) return rc;
+  d->capacity = (uint32_t)nn;
+  d->cap = cap;
+  return RDX_OK;
+}
+
+static uint32_t blocks_for(size_t items, uint32_t block = 256) {
+  return (uint32_t)std::max<size_t>(1, (items + block - 1) / block);
+}
+
+}  // namespace rdx
+
+using namespace rdx;
+
+extern "C" {
+
+int rdx_draw_create(rdx_context* ctx, const rdx_draw_settings* s, rdx_draw** out) {
+  if (!out) return fail(RDX_E_INVALID, "rdx_draw_create: out is NULL");
+  *out = nullptr;
+  if (!ctx) return fail(RDX_E_INVALID, "rdx_draw_create: no context");
+  if (s && !(s->t_stop >= 0.0f && s->t_stop < 1.0f))
+    return fail(RDX_E_INVALID, "rdx_draw_create: t_stop must be in [0, 1) (got %g)", (double)s->t_stop);
+  Draw* d = new Draw();
+  d->ctx = reinterpret_cast<Context*>(ctx);
+  if (s) {
+    d->set = *s;
+  } else {
+    memset(&d->set, 0, sizeof d->set);
+  }
+  *out = reinterpret_cast<rdx_draw*>(d);
+  return RDX_OK;
+}
+
+int rdx_draw_destroy(rdx_draw* draw) {
+  if (!draw) return RDX_OK;
+  Draw* d = reinterpret_cast<Draw*>(draw);
+  d->release();
+  delete d;
+  return RDX_OK;
+}
+
+int rdx_draw_splats(rdx_draw* draw, void* stream, const uint32_t* words_dev, uint32_t n, const uint32_t* order_dev,
+                    const uint32_t* count_dev, uint32_t capacity, uint32_t width, uint32_t height,
+                    const rdx_params* params, float* rgba_dev) {
+  if (!draw || !count_dev || !params || !rgba_dev || (n && !words_dev) || (capacity && !order_dev))
+    return fail(RDX_E_INVALID, "rdx_draw_splats: NULL argument");
+  if (width < 1 || height < 1 || width > 16384 || height > 16384)
+    return fail(RDX_E_INVALID, "rdx_draw_splats: render size %ux%u (1..16384)", width, height);
+  Draw* d = reinterpret_cast<Draw*>(draw);
+  hipStream_t s = (hipStream_t)stream;
+  DiffU u;
+  memset(&u, 0, sizeof u);
+  u.width = (int)width;
+  u.height = (int)height;
+  u.tiles_x = (int)((width + DIFF_TILE - 1) / DIFF_TILE);
+  u.tiles_y = (int)((height + DIFF_TILE - 1) / DIFF_TILE);
+  u.max_std_dev = (float)params->max_std_dev;
+  u.min_alpha = (float)params->min_alpha;
+  u.falloff = (float)params->falloff;
+  u.t_stop = d->set.t_stop;
+  if (!(u.min_alpha > 0.0f) || !(u.max_std_dev > 0.0f))
+    return fail(RDX_E_INVALID, "rdx_draw_splats: min_alpha and max_std_dev must be positive");
+  const uint32_t tiles = (uint32_t)(u.tiles_x * u.tiles_y);
+  if (tiles > 65536) return fail(RDX_E_INVALID, "rdx_draw_splats: more than 65536 tiles");
+  if (int rc = reserve(d, capacity, tiles)) return rc;
+  Context* c = d->ctx;
+  uint32_t* counts = (uint32_t*)d->counts.ptr;
+  const uint32_t cap_pos = capacity;  // positions the order holds
+  hipLaunchKernelGGL(k_draw_counts, dim3(1), dim3(64), 0, s, count_dev, cap_pos, counts);
+  {
+    Stage st(c, s, "draw_pre");
+    if (cap_pos)
+      hipLaunchKernelGGL(k_draw_pre, dim3(blocks_for(cap_pos)), dim3(256), 0, s, (const uint4*)words_dev, n, order_dev,
+                         counts + D_DRAWN, u, d->set.linear_output, (float4*)d->pre.ptr, (uint32_t*)d->touch.ptr,
+                         (uint4*)d->rect.ptr);
+  }
+  const uint32_t blocks = (uint32_t)((d->capacity + DSCAN_TILE - 1) / DSCAN_TILE);
+  {
+    Stage st(c, s, "draw_bin");
+    hipLaunchKernelGGL(k_diff_scan_partial, dim3(blocks), dim3(DSCAN_BLOCK), 0, s, nullptr,
+                       (const uint32_t*)d->touch.ptr, counts + D_DRAWN, (uint32_t*)d->bsum.ptr);
+    hipLaunchKernelGGL(k_diff_scan_blocks, dim3(1), dim3(1024), 0, s, (uint32_t*)d->bsum.ptr, blocks, counts + D_ENTRIES,
+                       d->cap, counts + D_OVERFLOW);
+    hipLaunchKernelGGL(k_diff_scan_final, dim3(blocks), dim3(DSCAN_BLOCK), 0, s, nullptr, (const uint32_t*)d->touch.ptr,
+                       counts + D_DRAWN, (const uint32_t*)d->bsum.ptr, (uint32_t*)d->offsets.ptr);
+    hipLaunchKernelGGL(k_diff_emit, dim3(blocks_for(d->capacity)), dim3(256), 0, s, nullptr, counts + D_DRAWN,
+                       (const uint32_t*)d->touch.ptr, (const uint4*)d->rect.ptr, (const uint32_t*)d->offsets.ptr,
+                       (uint32_t)u.tiles_x, d->cap, (uint32_t*)d->tile_key.ptr, (uint32_t*)d->emit_idx.ptr,
+                       (uint32_t*)d->emit_splat.ptr, counts + D_OVERFLOW);
+  }
+  // stable sort of the entries by tile: front to back within each tile
+  const SortPlan pe = draw_entry_plan();
+  SegLayout Le;
+  Le.tiles = tiles_of(pe, d->cap);
+  if (int rc = run_sort(c, d->sort_entries, s, pe, Le, (const uint32_t*)d->tile_key.ptr, (const uint32_t*)d->emit_idx.ptr,
+                        (uint32_t*)d->tile_key.ptr, (uint32_t*)d->emit_idx.ptr, d->cap, counts + D_ENTRIES, false, 0u,
+                        false, nullptr, nullptr))
+    return rc;
+  {
+    Stage st(c, s, "draw_ranges");
+    hipLaunchKernelGGL(k_diff_clear_ranges, dim3(blocks_for(tiles)), dim3(256), 0, s, (uint2*)d->range.ptr, tiles);
+    hipLaunchKernelGGL(k_diff_ranges, dim3(std::min<uint32_t>(blocks_for(d->cap), 4096u)), dim3(256), 0, s,
+                       (const uint32_t*)d->tile_key.ptr, counts + D_ENTRIES, d->cap, tiles, (uint2*)d->range.ptr);
+  }
+  {
+    Stage st(c, s, "draw_render");
+    const size_t plane = (size_t)width * height;
+    hipLaunchKernelGGL(k_diff_render<true>, dim3(tiles), dim3(DIFF_BLOCK), 0, s, (const uint2*)d->range.ptr,
+                       (const uint32_t*)d->emit_idx.ptr, (const uint32_t*)d->emit_splat.ptr, (const float4*)d->pre.ptr, u,
+                       rgba_dev, rgba_dev + 3 * plane, nullptr, nullptr);
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? RDX_OK : hip_fail(e, "rdx_draw_splats");
+}
+
+int rdx_draw_stats(rdx_draw* draw, void* stream, uint32_t* drawn, uint32_t* entries, uint32_t* capacity,
+                   int* overflowed) {
+  if (!draw) return fail(RDX_E_INVALID, "rdx_draw_stats: no draw");
+  Draw* d = reinterpret_cast<Draw*>(draw);
+  uint32_t h[D_WORDS] = {0, 0, 0, 0};
+  if (d->counts.ptr) {
+    hipError_t e = hipMemcpyAsync(h, d->counts.ptr, sizeof h, hipMemcpyDeviceToHost, (hipStream_t)stream);
+    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "rdx_draw_stats");
+  }
+  if (drawn) *drawn = h[D_DRAWN];
+  if (entries) *entries = h[D_ENTRIES];
+  if (capacity) *capacity = d->cap;
+  if (overflowed) *overflowed = h[D_OVERFLOW] != 0;
+  return RDX_OK;
+}
+
+}  // extern "C"

@@ -81,6 +81,7 @@ static int make_diff_uniforms(const rdx_camera* cam, const rdx_params* prm, int 
   u->clip_xy = (float)prm->clip_xy;
   u->t_stop = t_stop;
   u->sh_degree = std::max(0, std::min(3, sh_degree));
+  u->falloff = 1.0f;
   return RDX_OK;
 }
 
@@ -216,7 +217,7 @@ int rdx_raster_forward(rdx_raster* raster, void* stream, const float* params_dev
   }
   {
     Stage st(c, s, "diff_render");
-    hipLaunchKernelGGL(k_diff_render, dim3(tiles), dim3(DIFF_BLOCK), 0, s, (const uint2*)r->range.ptr,
+    hipLaunchKernelGGL(k_diff_render<false>, dim3(tiles), dim3(DIFF_BLOCK), 0, s, (const uint2*)r->range.ptr,
                        (const uint32_t*)r->emit_idx.ptr, (const uint32_t*)r->emit_splat.ptr, (const float4*)r->pre.ptr, u,
                        image_dev, alpha_dev, (float*)r->t_final.ptr, (uint32_t*)r->n_contrib.ptr);
   }

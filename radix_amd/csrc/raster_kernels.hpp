@@ -46,6 +46,7 @@ struct DiffU {
   int width, height, tiles_x, tiles_y;
   float ndc_z_min, max_std_dev, min_alpha, blur, clip_xy, t_stop;
   int sh_degree;
+  float falloff;  // the draw's fragment shaping (SplatParams.falloff); the differentiable form uses 1
 };
 
 // ---- SH basis (sh.ts:376-407, the reference's constants) and its gradient ----
@@ -241,8 +242,14 @@ constexpr int DSCAN_BLOCK = 256;
 constexpr int DSCAN_ITEMS = 4;
 constexpr int DSCAN_TILE = DSCAN_BLOCK * DSCAN_ITEMS;
 
+// Front-to-back position j of m is back-to-front position m - 1 - j: the
+// splat order[m - 1 - j], or that position itself when order is NULL (the
+// draw, whose per-entry records are stored by position).
+__device__ __forceinline__ uint32_t f2b_item(const uint32_t* order, uint32_t m, uint32_t j) {
+  return order ? order[m - 1 - j] : m - 1 - j;
+}
 __device__ __forceinline__ uint32_t f2b_touch(const uint32_t* order, const uint32_t* touch, uint32_t m, uint32_t j) {
-  return j < m ? touch[order[m - 1 - j]] : 0u;
+  return j < m ? touch[f2b_item(order, m, j)] : 0u;
 }
 
 static __global__ __launch_bounds__(DSCAN_BLOCK) void k_diff_scan_partial(const uint32_t* order, const uint32_t* touch,
@@ -309,7 +316,7 @@ static __global__ __launch_bounds__(256) void k_diff_emit(const uint32_t* order,
   const uint32_t m = *m_dev;
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= m) return;
-  const uint32_t sp = order[m - 1 - j];
+  const uint32_t sp = f2b_item(order, m, j);
   const uint32_t cnt = touch[sp];
   if (cnt == 0u) return;
   const uint4 r = rect[sp];
@@ -353,6 +360,16 @@ static __global__ void k_diff_clear_ranges(uint2* range, uint32_t tiles) {
 // exp(-z2 / 2) as one multiply and v_exp_f32 (2^x): the same value as
 // __expf(-0.5f * z2), whose halving is exact.
 __device__ __forceinline__ float gauss(float z2) { return __builtin_amdgcn_exp2f(z2 * (-0.5f * 1.44269504088896340736f)); }
+
+// The draw's fragment alpha (fragmentAlpha, projection.tsl.ts:444-455): op
+// > 0 is the opacity, mixed toward op * g by the falloff; op <= 0 marks an
+// LoD opacity above 1 whose exponent a = exp((alpha^2 - 1) / e) the entry
+// carries as -op: 1 - (1 - g)^a, mixed toward 1.
+__device__ __forceinline__ float draw_alpha(float op, float g, float falloff) {
+  if (op > 0.0f) return op * fmaf(falloff, g, 1.0f - falloff);
+  const float lod = 1.0f - __builtin_amdgcn_exp2f(-op * __builtin_amdgcn_logf(1.0f - g));
+  return fmaf(falloff, lod, 1.0f - falloff);
+}
 
 // The tile's pixels: wave w takes the 8x8 quadrant (w & 1, w >> 1) of the
 // 16x16 tile, lane l its pixel (l & 7, l >> 3). Entries are staged in LDS
@@ -403,7 +420,8 @@ __device__ __forceinline__ void load_entry(const uint32_t* emit_idx, const uint3
   p2 = pre[3 * (size_t)sp + 2];
 }
 
-static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render(const uint2* range, const uint32_t* emit_idx,
+template <bool DRAW>
+__global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render(const uint2* range, const uint32_t* emit_idx,
                                                                    const uint32_t* emit_splat, const float4* pre, DiffU u,
                                                                    float* image, float* alpha_out, float* t_final,
                                                                    uint32_t* n_contrib) {
@@ -438,11 +456,14 @@ static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render(const uint2* 
         const float4 p0 = s_p0[k], p1 = s_p1[k];
         const float dx = q.px - p0.x, dy = q.py - p0.y;
         const float z2 = p1.x * dx * dx + p1.y * dx * dy + p1.z * dy * dy;
-        const float a = p0.z * gauss(z2);
+        const float a = DRAW ? draw_alpha(p0.z, gauss(z2), u.falloff) : p0.z * gauss(z2);
         const bool hit = !done && z2 <= p0.w && a >= cut_alpha;
         const float Tn = T * (1.0f - a);
+        // the compositor stops before the splat that would take the
+        // transmittance below t_stop; the draw blends it and stops after
+        // (what it leaves out is then below t_stop times a colour)
         const bool stop = hit && Tn < u.t_stop;
-        const bool use = hit && !stop;
+        const bool use = DRAW ? hit : hit && !stop;
         done = done || stop;
         if (ballot(use)) {
           const float4 c = s_p2[k];
@@ -463,8 +484,10 @@ static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render(const uint2* 
   image[plane_px + p] = C1;
   image[2 * plane_px + p] = C2;
   if (alpha_out) alpha_out[p] = 1.0f - T;
-  t_final[p] = T;
-  n_contrib[p] = last;
+  if constexpr (!DRAW) {
+    t_final[p] = T;
+    n_contrib[p] = last;
+  }
 }
 
 // Sums over each row of 16 lanes to the row's last lane (DPP row_shr 1, 2, 4, 8
