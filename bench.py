@@ -190,6 +190,38 @@ def bench_stereo(steps: int, count: int = 16777216) -> dict:
             "visible_per_eye": [int(v) for v in out["count"].cpu().tolist()]}
 
 
+def bench_draw(steps: int, frame, words, idx, cnt, width: int, height: int, params, ctx) -> dict:
+    """The production draw of the plain_4m frame: its sorted, visible splats'
+    ewa words composited into a 1920x1080 premultiplied RGBA image (every
+    fragment, as the reference's blend). Also a whole frame: project, sort
+    and draw."""
+    from radix_amd import _native as N
+    from radix_amd.draw import Drawer
+
+    drawer = Drawer(context=ctx)
+    image = drawer.draw(words, idx, cnt, width, height, params)
+    ms = time_events(lambda: drawer.draw(words, idx, cnt, width, height, params, out=image), steps)
+
+    def whole():
+        frame()
+        drawer.draw(words, idx, cnt, width, height, params, out=image)
+
+    ms_frame = time_events(whole, steps)
+    st = drawer.stats()
+    N.lib().rdx_context_profile(ctx.handle, 1)
+    stage_times(ctx)
+    for _ in range(3):
+        drawer.draw(words, idx, cnt, width, height, params, out=image)
+    stages = stage_times(ctx)
+    N.lib().rdx_context_profile(ctx.handle, 0)
+    drawer.close()
+    return {"workload": f"the plain_4m frame's {st['drawn']:,} sorted splats drawn at {width}x{height} "
+                        "(ewa fragments, premultiplied over, no early stop)",
+            "ms": round(ms, 4), "Gsplats_per_s": round(st["drawn"] / ms / 1e6, 3), "entries": st["entries"],
+            "frame_ms": round(ms_frame, 4), "frame_fps": round(1e3 / ms_frame, 1),
+            "stages_ms": {k: round(v["ms_avg"], 4) for k, v in stages.items() if k.startswith("draw")}}
+
+
 def bench_backward(steps: int) -> dict:
     """Differentiable compositor: 262,144 splats, 512x512, 16x16 tiles, L2
     loss against the perturbed scene; bwd Gsplats/s = n / backward time."""
@@ -355,6 +387,10 @@ def main() -> int:
         "wall_s_timed": round(wall, 4),
     }
     if not args.no_extra:
+        log("draw of the plain_4m frame")
+        out["draw_4m"] = bench_draw(max(3, args.steps // 5), step, words[0], idx, cnt, int(rig.width), int(rig.height),
+                                    params, ctx)
+        torch.cuda.empty_cache()
         log("batched workload")
         out["batched_vs_plain"] = bench_batched(max(3, args.steps // 5))
         torch.cuda.empty_cache()

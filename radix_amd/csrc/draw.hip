@@ -18,16 +18,18 @@ struct Draw {
   rdx_draw_settings set;
   SortScratch sort_entries;
   Buffer pre, touch, rect, offsets, bsum, tile_key, emit_idx, emit_splat, range, counts;
-  uint32_t capacity = 0, cap = 0;  // draw positions, entries
+  Buffer item_start, item_tile, item_chunk, partial;  // the render's work items (tile chunks)
+  uint32_t capacity = 0, cap = 0, max_items = 0;     // draw positions, entries, work items
   void release() {
     sort_entries.release();
-    Buffer* b[] = {&pre, &touch, &rect, &offsets, &bsum, &tile_key, &emit_idx, &emit_splat, &range, &counts};
+    Buffer* b[] = {&pre,     &touch,      &rect,       &offsets,    &bsum,      &tile_key, &emit_idx,
+                   &emit_splat, &range, &counts, &item_start, &item_tile, &item_chunk, &partial};
     for (Buffer* x : b) x->release();
   }
 };
 
-// counts words: [0] drawn positions (the clamped count), [1] entries, [2] overflow
-enum { D_DRAWN = 0, D_ENTRIES = 1, D_OVERFLOW = 2, D_WORDS = 4 };
+// counts words: [0] drawn positions (the clamped count), [1] entries, [2] overflow, [3] work items
+enum { D_DRAWN = 0, D_ENTRIES = 1, D_OVERFLOW = 2, D_ITEMS = 3, D_WORDS = 4 };
 
 // Entries of 16-bit tile keys, sorted in 2,048-key tiles (as the compositor's).
 static SortPlan draw_entry_plan() { return make_plan(rdx_sort_tuning{8, 8, 256}, 16); }
@@ -44,6 +46,13 @@ static int reserve(Draw* d, uint32_t capacity, uint32_t tiles) {
   if ((rc = d->tile_key.ensure((size_t)cap * 4)) || (rc = d->emit_idx.ensure((size_t)cap * 4)) ||
       (rc = d->emit_splat.ensure((size_t)cap * 4)) || (rc = d->range.ensure((size_t)tiles * sizeof(uint2))))
     return rc;
+  // a tile of e entries is ceil(e / DRAW_CHUNK) >= 1 items: at most tiles + cap / DRAW_CHUNK
+  const uint32_t max_items = tiles + (cap + DRAW_CHUNK - 1) / DRAW_CHUNK;
+  if ((rc = d->item_start.ensure((size_t)tiles * 4)) || (rc = d->item_tile.ensure((size_t)max_items * 4)) ||
+      (rc = d->item_chunk.ensure((size_t)max_items * 4)) ||
+      (rc = d->partial.ensure((size_t)max_items * DIFF_BLOCK * sizeof(float4))))
+    return rc;
+  d->max_items = max_items;
   const SortPlan pe = draw_entry_plan();
   if ((rc = d->sort_entries.reserve(cap, 1, tiles_of(pe, cap)))) return rc;
   d->capacity = (uint32_t)nn;
@@ -152,9 +161,17 @@ int rdx_draw_splats(rdx_draw* draw, void* stream, const uint32_t* words_dev, uin
   {
     Stage st(c, s, "draw_render");
     const size_t plane = (size_t)width * height;
-    hipLaunchKernelGGL(k_diff_render<true>, dim3(tiles), dim3(DIFF_BLOCK), 0, s, (const uint2*)d->range.ptr,
+    hipLaunchKernelGGL(k_draw_items, dim3(1), dim3(1024), 0, s, (const uint2*)d->range.ptr, tiles,
+                       (uint32_t*)d->item_start.ptr, (uint32_t*)d->item_tile.ptr, (uint32_t*)d->item_chunk.ptr,
+                       counts + D_ITEMS);
+    const DrawItems di{counts + D_ITEMS, (const uint32_t*)d->item_tile.ptr, (const uint32_t*)d->item_chunk.ptr,
+                       (float4*)d->partial.ptr};
+    hipLaunchKernelGGL(k_diff_render<true>, dim3(d->max_items), dim3(DIFF_BLOCK), 0, s, (const uint2*)d->range.ptr,
                        (const uint32_t*)d->emit_idx.ptr, (const uint32_t*)d->emit_splat.ptr, (const float4*)d->pre.ptr, u,
-                       rgba_dev, rgba_dev + 3 * plane, nullptr, nullptr);
+                       rgba_dev, rgba_dev + 3 * plane, nullptr, nullptr, di);
+    hipLaunchKernelGGL(k_draw_combine, dim3(tiles), dim3(DIFF_BLOCK), 0, s, (const uint2*)d->range.ptr,
+                       (const uint32_t*)d->item_start.ptr, (const float4*)d->partial.ptr, u, rgba_dev,
+                       rgba_dev + 3 * plane);
   }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? RDX_OK : hip_fail(e, "rdx_draw_splats");

@@ -97,4 +97,59 @@ static __global__ __launch_bounds__(256) void k_draw_pre(const uint4* __restrict
   touch[i] = (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1));
 }
 
+// One block: each tile's number of chunks (at least one: an empty tile's
+// block writes its zero pixels), their exclusive scan -> item_start, and
+// the work items (tile, chunk) in tile order; the item count to *items.
+static __global__ __launch_bounds__(1024) void k_draw_items(const uint2* range, uint32_t tiles, uint32_t* item_start,
+                                                             uint32_t* item_tile, uint32_t* item_chunk, uint32_t* items) {
+  __shared__ uint32_t s_scan[1024 / RDX_WAVE];
+  uint32_t carry = 0;
+  for (uint32_t b0 = 0; b0 < tiles; b0 += 1024) {
+    const uint32_t t = b0 + threadIdx.x;
+    uint32_t k = 0;
+    if (t < tiles) {
+      const uint2 r = range[t];
+      k = max(1u, (r.y - r.x + DRAW_CHUNK - 1) / DRAW_CHUNK);
+    }
+    uint32_t sum;
+    const uint32_t at = carry + block_exclusive_sum<1024>(k, s_scan, &sum);
+    if (t < tiles) {
+      item_start[t] = at;
+      for (uint32_t c = 0; c < k; c++) {
+        item_tile[at + c] = t;
+        item_chunk[at + c] = c;
+      }
+    }
+    carry += sum;
+  }
+  if (threadIdx.x == 0) *items = carry;
+}
+
+// A tile of several chunks: its pixels are the chunks' partials folded front
+// to back, C = sum_k (prod_{j < k} T_j) C_k, T = prod_k T_k.
+static __global__ __launch_bounds__(DIFF_BLOCK) void k_draw_combine(const uint2* range, const uint32_t* item_start,
+                                                                     const float4* partial, DiffU u, float* image,
+                                                                     float* alpha_out) {
+  const uint32_t tile = blockIdx.x;
+  const uint2 r = range[tile];
+  const uint32_t k = max(1u, (r.y - r.x + DRAW_CHUNK - 1) / DRAW_CHUNK);
+  if (k <= 1) return;
+  const DiffQuad q = diff_quad(tile, u);
+  if (q.x >= u.width || q.y >= u.height) return;
+  float C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, T = 1.0f;
+  const float4* part = partial + (size_t)item_start[tile] * DIFF_BLOCK + threadIdx.x;
+  for (uint32_t c = 0; c < k; c++) {
+    const float4 v = part[(size_t)c * DIFF_BLOCK];
+    C0 = fmaf(T, v.x, C0);
+    C1 = fmaf(T, v.y, C1);
+    C2 = fmaf(T, v.z, C2);
+    T *= v.w;
+  }
+  const size_t p = (size_t)q.y * u.width + q.x, plane_px = (size_t)u.width * u.height;
+  image[p] = C0;
+  image[plane_px + p] = C1;
+  image[2 * plane_px + p] = C2;
+  alpha_out[p] = 1.0f - T;
+}
+
 }  // namespace rdx

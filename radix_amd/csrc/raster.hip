@@ -219,7 +219,7 @@ int rdx_raster_forward(rdx_raster* raster, void* stream, const float* params_dev
     Stage st(c, s, "diff_render");
     hipLaunchKernelGGL(k_diff_render<false>, dim3(tiles), dim3(DIFF_BLOCK), 0, s, (const uint2*)r->range.ptr,
                        (const uint32_t*)r->emit_idx.ptr, (const uint32_t*)r->emit_splat.ptr, (const float4*)r->pre.ptr, u,
-                       image_dev, alpha_dev, (float*)r->t_final.ptr, (uint32_t*)r->n_contrib.ptr);
+                       image_dev, alpha_dev, (float*)r->t_final.ptr, (uint32_t*)r->n_contrib.ptr, DrawItems{});
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "rdx_raster_forward");

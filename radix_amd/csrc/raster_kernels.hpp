@@ -420,17 +420,40 @@ __device__ __forceinline__ void load_entry(const uint32_t* emit_idx, const uint3
   p2 = pre[3 * (size_t)sp + 2];
 }
 
+// The draw splits a tile's entries into chunks of DRAW_CHUNK: one block
+// composites each (a few tiles of a clustered frame hold tens of thousands
+// of entries) into a partial (colour, transmittance), and k_draw_combine
+// folds a tile's partials front to back ("over" is associative).
+constexpr uint32_t DRAW_CHUNK = 1024;
+struct DrawItems {
+  const uint32_t* items;       // [1]: the number of work items
+  const uint32_t* item_tile;   // per item: its tile
+  const uint32_t* item_chunk;  // per item: its chunk of the tile's entries
+  float4* partial;             // per item and pixel slot: (C, T) of a tile of several chunks
+};
+
 template <bool DRAW>
 __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render(const uint2* range, const uint32_t* emit_idx,
                                                                    const uint32_t* emit_splat, const float4* pre, DiffU u,
                                                                    float* image, float* alpha_out, float* t_final,
-                                                                   uint32_t* n_contrib) {
+                                                                   uint32_t* n_contrib, DrawItems di) {
   __shared__ float4 s_p0[DIFF_BLOCK], s_p1[DIFF_BLOCK], s_p2[DIFF_BLOCK];
   __shared__ uint32_t s_ov[DIFF_BLOCK];
-  const uint32_t tile = blockIdx.x;
+  uint32_t tile = blockIdx.x;
+  uint2 rg;
+  uint32_t nchunks = 1;
+  if constexpr (DRAW) {
+    if (blockIdx.x >= *di.items) return;  // uniform per block
+    tile = di.item_tile[blockIdx.x];
+    const uint2 all = range[tile];
+    nchunks = max(1u, (all.y - all.x + DRAW_CHUNK - 1) / DRAW_CHUNK);
+    const uint32_t b0 = all.x + di.item_chunk[blockIdx.x] * DRAW_CHUNK;
+    rg = make_uint2(b0, min(b0 + DRAW_CHUNK, all.y));
+  } else {
+    rg = range[tile];
+  }
   const DiffQuad q = diff_quad(tile, u);
   const bool inside = q.x < u.width && q.y < u.height;
-  const uint2 rg = range[tile];
   const float cut_alpha = u.min_alpha;
   float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
   uint32_t last = 0;
@@ -477,6 +500,12 @@ __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render(const uint2* range, 
       }
     }
     __syncthreads();
+  }
+  if constexpr (DRAW) {
+    if (nchunks > 1) {
+      di.partial[(size_t)blockIdx.x * DIFF_BLOCK + threadIdx.x] = make_float4(C0, C1, C2, T);
+      return;
+    }
   }
   if (!inside) return;
   const size_t p = (size_t)q.y * u.width + q.x, plane_px = (size_t)u.width * u.height;

@@ -126,6 +126,34 @@ def test_early_stop_within_its_bound(cuda):
     assert d.max() > 0.0
 
 
+def test_dense_tile_split_into_chunks(cuda):
+    # 6,000 splats over one 16x16 tile: its entries span several chunks of
+    # 1,024, composited apart and folded front to back
+    Wd, Ht, n = 64, 64, 6000
+    r = Rng(8)
+    u = r.random(6 * n).reshape(6, n)
+    ndc = lambda px, size: (px + 0.5) * 2.0 / size - 1.0
+    fx = np.floor((ndc(16 + 16 * u[0], Wd) + 2.0) / 4.0 * 65535.0).astype(np.uint32)
+    fy = np.floor((-ndc(16 + 16 * u[1], Ht) + 2.0) / 4.0 * 65535.0).astype(np.uint32)
+    sx = np.floor(u[2] * 300 + 1000).astype(np.uint32)  # 1.5 .. 3 px
+    byte = np.floor(u[3] * 40 + 4).astype(np.uint32)
+    rgb = np.array([O.lib().orc_encode_ext_rgb(float(a), float(b), 0.3) for a, b in zip(u[4], u[5])], dtype=np.uint32)
+    words = np.stack([fx | (fy << 16), np.full(n, 7 | (100 << 20), dtype=np.uint32), rgb, byte | (sx << 8) | (sx << 20)],
+                     axis=1).astype(np.uint32)
+    order = np.arange(n, dtype=np.uint32)
+    prm = Params()
+    d = Drawer()
+    img = d.draw(dev_i32(words), dev_i32(order), dev_i32([n]), Wd, Ht, prm)
+    torch.cuda.synchronize()
+    print("stats", d.stats(), flush=True)
+    ref = O.draw(words, order, n, Wd, Ht, prm.max_std_dev, prm.min_alpha, prm.falloff)
+    assert ref[3].max() > 0.99
+    compare(img.cpu().numpy(), ref, frac=0.01)
+    stop = 1e-3
+    early = Drawer(t_stop=stop).draw(dev_i32(words), dev_i32(order), dev_i32([n]), Wd, Ht, prm).cpu().numpy()
+    assert np.abs(early - img.cpu().numpy()).max() <= 1.01 * stop + 1e-5
+
+
 def test_entry_overflow_draws_nothing(cuda):
     Wd, Ht, n = 128, 128, 200
     words = crafted(n, Wd, Ht, 7)
