@@ -273,6 +273,9 @@ int run_sort(Context* ctx, SortScratch& sc, hipStream_t s, const SortPlan& p, co
     a.seg_of_tile = L.seg_of_tile;
     a.seg = L.seg;
     a.dst_cap = n_cap;
+    a.tile_keys = (uint32_t)p.tile;
+    // the top digit (skewed for distance keys) peels its common digits by ballots
+    a.peel = pass == p.passes - 1 || (p.ranged && pass == 2);
     if (p.ranged) {
       // pass 0 -> scratch 0, 1 -> scratch 1, 2 -> the output (scratch 0 for
       // segments whose keys reach beyond three passes), 3 (those segments

@@ -31,8 +31,9 @@ struct Draw {
 // counts words: [0] drawn positions (the clamped count), [1] entries, [2] overflow, [3] work items
 enum { D_DRAWN = 0, D_ENTRIES = 1, D_OVERFLOW = 2, D_ITEMS = 3, D_WORDS = 4 };
 
-// Entries of 16-bit tile keys, sorted in 2,048-key tiles (as the compositor's).
-static SortPlan draw_entry_plan() { return make_plan(rdx_sort_tuning{8, 8, 256}, 16); }
+// Entries of 16-bit tile keys: a frame's millions fill the machine with
+// 8,192-key tiles.
+static SortPlan draw_entry_plan() { return make_plan(rdx_sort_tuning{8, 16, 512}, 16); }
 
 static int reserve(Draw* d, uint32_t capacity, uint32_t tiles) {
   const uint32_t per = d->set.max_entries_per_splat ? d->set.max_entries_per_splat : 16u;

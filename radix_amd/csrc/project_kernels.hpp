@@ -148,6 +148,29 @@ __device__ __forceinline__ float4 quat_oct_decode(uint32_t code, int axis_bits, 
 
 // ---- word codecs (src/render/projection.tsl.ts:277-310) ---------------------
 
+// atan2 for the 12-bit angle code (steps of pi / 4096 = 7.7e-4 rad): the
+// octant reduction a = min / max in [0, 1], atan(a) = a P(a^2) with P of
+// degree 7 fitted here to atan(a) / a on [0, 1] (weighted least squares
+// toward the minimax error; 1.3e-7 rad evaluated in float), then the
+// octant's reflections. Arguments not both zero.
+__device__ __forceinline__ float atan2_code(float y, float x) {
+  const float ax = fabsf(x), ay = fabsf(y);
+  const float a = fminf(ax, ay) * __builtin_amdgcn_rcpf(fmaxf(ax, ay));
+  const float s = a * a;
+  float p = -0.004054663702845573f;
+  p = fmaf(p, s, 0.02186330407857895f);
+  p = fmaf(p, s, -0.055912818759679794f);
+  p = fmaf(p, s, 0.09642232209444046f);
+  p = fmaf(p, s, -0.13908642530441284f);
+  p = fmaf(p, s, 0.19946567714214325f);
+  p = fmaf(p, s, -0.33329862356185913f);
+  p = fmaf(p, s, 0.9999993443489075f);
+  float r = a * p;
+  if (ay > ax) r = 1.57079632679489662f - r;
+  if (x < 0.0f) r = 3.14159265358979324f - r;
+  return y < 0.0f ? -r : r;
+}
+
 __device__ __forceinline__ uint32_t encode_fixed16(float v) {
   return (uint32_t)fminf(fmaxf(floorf((v + 2.0f) * 0.25f * 65535.0f + 0.5f), 0.0f), 65535.0f);
 }
@@ -161,7 +184,8 @@ __device__ __forceinline__ uint32_t encode_angle12(float angle) {
 }
 __device__ __forceinline__ uint32_t encode_log_scale12(float scale) {
   const float top = 4094.0f;
-  const float l = fminf(fmaxf(log2f(fmaxf(scale, 1e-20f)), -4.0f), 12.0f);
+  // v_log_f32 (1 ulp) for the 12-bit code's 1/256-octave steps
+  const float l = fminf(fmaxf(__builtin_amdgcn_logf(fmaxf(scale, 1e-20f)), -4.0f), 12.0f);
   const float n = (l + 4.0f) * (1.0f / 16.0f) * top;
   return (uint32_t)fminf(fmaxf(floorf(n + 0.5f), 0.0f), top) + 1u;
 }
@@ -480,7 +504,7 @@ __device__ __forceinline__ EwaResult ewa_eye(const EyeU& E, const EyeCentre& C, 
     const float depth = -z;
     const float coc_view = fabsf(depth - u.focal_distance) * u.aperture_tan;
     const float focal_pixels = E.fy;  // renderSize.y / 2 * P11
-    const float coc = E.ortho ? coc_view * focal_pixels : coc_view * focal_pixels / depth;
+    const float coc = E.ortho ? coc_view * focal_pixels : coc_view * focal_pixels * __builtin_amdgcn_rcpf(depth);
     blur = fminf(fmaxf(coc * coc, blur), u.max_pr * u.max_pr);
   }
   const float pre = u.pre_blur;
@@ -490,14 +514,15 @@ __device__ __forceinline__ EwaResult ewa_eye(const EyeU& E, const EyeCentre& C, 
   r.d = dp + blur;
   r.b = b0;
   r.det = det_orig + blur * (ap + dp) + blur * blur;
-  r.opc = op * sqrtf(fmaxf(0.0f, det_orig * __builtin_amdgcn_rcpf(r.det)));
-  const float contribution = r.opc * 6.283185307179586f * sqrtf(fmaxf(0.0f, r.det));
+  // v_sqrt_f32 (1 ulp) throughout: the outputs are 8- to 16-bit codes
+  r.opc = op * __builtin_amdgcn_sqrtf(fmaxf(0.0f, det_orig * __builtin_amdgcn_rcpf(r.det)));
+  const float contribution = r.opc * 6.283185307179586f * __builtin_amdgcn_sqrtf(fmaxf(0.0f, r.det));
   // boundFromCov (projection.tsl.ts:223-240)
   const float half = (r.a - r.d) * 0.5f;
-  const float eig1 = (r.a + r.d) * 0.5f + sqrtf(half * half + r.b * r.b);
+  const float eig1 = (r.a + r.d) * 0.5f + __builtin_amdgcn_sqrtf(half * half + r.b * r.b);
   const float eig2 = eig1 > 0.0f ? fmaxf(r.det * __builtin_amdgcn_rcpf(eig1), 0.0f) : 0.0f;
-  r.sc1 = fminf(u.max_pr, std_dev * sqrtf(fmaxf(eig1, 0.0f)));
-  r.sc2 = fminf(u.max_pr, std_dev * sqrtf(eig2));
+  r.sc1 = fminf(u.max_pr, std_dev * __builtin_amdgcn_sqrtf(fmaxf(eig1, 0.0f)));
+  r.sc2 = fminf(u.max_pr, std_dev * __builtin_amdgcn_sqrtf(eig2));
   const bool bound_ok = r.sc1 >= u.min_pr || r.sc2 >= u.min_pr;
   {
     float vx, vy;
@@ -517,13 +542,13 @@ __device__ __forceinline__ EwaResult ewa_eye(const EyeU& E, const EyeCentre& C, 
       vx = 1.0f;
       vy = 0.0f;
     }
-    r.angle = atan2f(vy, vx);
+    r.angle = atan2_code(vy, vx);
     if (r.angle < 0.0f) r.angle += 3.14159265358979323846f;
   }
   r.ok = pass && r.opc >= u.min_alpha && contribution >= u.min_contrib && bound_ok;
   r.ndc = make_float2(0.0f, 0.0f);
   if (C.clip[3] > 0.0f) {
-    const float iw = 1.0f / C.clip[3];
+    const float iw = __builtin_amdgcn_rcpf(C.clip[3]);
     r.ndc = make_float2(C.clip[0] * iw, C.clip[1] * iw);
   }
   r.words = make_uint4(0u, 0u, 0u, 0u);

@@ -36,7 +36,9 @@ __host__ __device__ constexpr uint32_t count_stride(int rows) { return (uint32_t
 // masks take the rest: a round whose keys share a few digits (the top
 // digit of clustered distances) would otherwise serialise its LDS atomics
 // on one address. Peeling stops at the first digit held by fewer than
-// PEEL_MIN lanes (a round of spread digits then costs one ballot).
+// PEEL_MIN lanes (a round of spread digits then costs one ballot). Only the
+// last digit pass peels (PassArgs::peel): the lower digits of distance keys
+// are spread, and there the ballots cost more than they save.
 constexpr int PEEL = 3;
 constexpr uint32_t PEEL_MIN = 8;
 
@@ -89,11 +91,13 @@ struct PassArgs {
   const uint32_t* seg_of_tile;  // NULL: one segment of all tiles
   const uint32_t* seg;          // [segments][4]: tile0, off, n, ntiles
   uint32_t dst_cap;          // output capacity (a guard on every write)
+  uint32_t tile_keys;        // keys per tile (the scan covers the tiles that hold keys)
   // ranged sorts
   int ranged;                // 0: digits of the key; 1: first ranged pass; 2: later ranged pass (digits of key - base)
   uint32_t* tile_mm;         // [tiles][2] smallest and largest kept key (first ranged pass)
   uint32_t* seg_mm;          // [segments][2] the same per segment (the first pass's scan)
   int gate;                  // run only for segments whose keys reach beyond RANGED_BITS above the base
+  int peel;                  // rank a round's most present digits by ballots first (the top digit pass)
   int alt;                   // segments that reach beyond write to keys_alt / vals_alt instead
   uint32_t* keys_alt;
   uint32_t* vals_alt;
@@ -196,6 +200,9 @@ __global__ __launch_bounds__(BLOCK) void k_count(PassArgs a) {
     k[j] = i < g.cap ? keys[i] : 0u;
   }
   const uint32_t n = seg_count(a, g);
+  // a tile past the segment's keys writes nothing: the scan stops before it
+  // (a sort sized by a capacity, its count on the device, launches them all)
+  if (base >= n && base > 0) return;  // uniform per block; a segment's first tile always writes
   __syncthreads();  // rows zeroed
   uint32_t* row = s_rows + (w * SUBS + (lane & (SUBS - 1))) * ROW;
   uint32_t hidden = 0, kmin = 0xffffffffu, kmax = 0u;
@@ -267,6 +274,11 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(PassArgs a, int bins) {
     ntiles = a.seg[4 * s + 3];
   }
   if (a.gate && !seg_wide(a, s)) return;  // uniform per block
+  {
+    // the tiles that hold keys (at least the first): the count kernel wrote only theirs
+    const uint32_t nk = seg_count(a, seg_info(a, s));
+    ntiles = min(ntiles, max(1u, (nk + a.tile_keys - 1) / a.tile_keys));
+  }
   if (g == (int)groups) {
     // the segment's smallest and largest kept key
     uint32_t mn = 0xffffffffu, mx = 0u;
@@ -436,7 +448,7 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
     const bool keep = rank[j] == 0u;
     const uint32_t d = dig[j];
     bool done = !keep;
-    uint64_t rem = ballot(keep);
+    uint64_t rem = a.peel ? ballot(keep) : 0ull;
 #pragma unroll
     for (int q = 0; q < PEEL; q++) {
       if (rem == 0ull) break;
