@@ -222,6 +222,31 @@ def bench_draw(steps: int, frame, words, idx, cnt, width: int, height: int, para
             "stages_ms": {k: round(v["ms_avg"], 4) for k, v in stages.items() if k.startswith("draw")}}
 
 
+def bench_keys(steps: int, proj, table, rig) -> dict:
+    """The keys-only pass (the CPU sort's keys) over the plain_4m table:
+    centre, log-scales, quaternion and opacity in (44 B), one key out (4 B)."""
+    import torch
+
+    from radix_amd import Params
+    from radix_amd import _native as N
+
+    prm = Params(sort_bits=32)
+    n = table.count
+    keys = torch.empty(n, dtype=torch.int32, device="cuda")
+    proj.project_keys(table, rig, prm, keys=keys)
+    ms = time_events(lambda: proj.project_keys(table, rig, prm, keys=keys), steps)
+    N.lib().rdx_context_profile(proj.context.handle, 1)
+    stage_times(proj.context)
+    for _ in range(steps):
+        proj.project_keys(table, rig, prm, keys=keys)
+    k_ms = stage_times(proj.context).get("project_keys", {}).get("ms_avg", float("nan"))
+    N.lib().rdx_context_profile(proj.context.handle, 0)
+    bytes_ = 48 * n
+    return {"workload": "keys-only pass over the plain_4m table (4,194,304 f32 splats), 32-bit keys",
+            "ms": round(ms, 4), "kernel_ms": round(k_ms, 4), "Gsplats_per_s": round(n / ms / 1e6, 3),
+            "bytes_per_splat": 48, "hbm_frac": round(bytes_ / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
 def bench_backward(steps: int) -> dict:
     """Differentiable compositor: 262,144 splats, 512x512, 16x16 tiles, L2
     loss against the perturbed scene; bwd Gsplats/s = n / backward time."""
@@ -390,6 +415,8 @@ def main() -> int:
         log("draw of the plain_4m frame")
         out["draw_4m"] = bench_draw(max(3, args.steps // 5), step, words[0], idx, cnt, int(rig.width), int(rig.height),
                                     params, ctx)
+        log("keys-only pass")
+        out["keys_pass_4m"] = bench_keys(max(3, args.steps // 5), proj, table, rig)
         torch.cuda.empty_cache()
         log("batched workload")
         out["batched_vs_plain"] = bench_batched(max(3, args.steps // 5))

@@ -261,6 +261,17 @@ int rdx_project_ewa(rdx_context* ctx, void* stream, const rdx_splat_table* table
  * indices at out_indices_dev + e n, its count at out_count_dev[e] and its
  * draw arguments at out_draw_args_dev + 5 e; the buffers hold 2 n, 2 n, 2
  * and 10 words. */
+/* The keys-only pass for CPU sorts (SplatProjection.writeKeys,
+ * src/render/SplatProjection.ts:165-172, run by projectCollection's "keys"
+ * pass, SplatProjector.ts:184-189, for SplatView.sort, SplatView.ts:137-185):
+ * keys[at + i] = compute(|head-space centre|, sort_bits), or invalid for
+ * splats cullInvisible drops or the sortGuard excludes (params->sort_guard;
+ * cull_keys does not apply). Reads the centre, scales, rotation and opacity
+ * only (no SH, no EWA); keys are the ones rdx_project_ewa writes with
+ * cull_keys 0. The host then reads them back for rdx_sort_keys. */
+int rdx_project_keys(rdx_context* ctx, void* stream, const rdx_splat_table* table, const rdx_camera* camera,
+                     const rdx_params* params, uint32_t at, uint32_t* keys_dev);
+
 int rdx_project_sort(rdx_context* ctx, void* stream, const rdx_splat_table* table, const rdx_camera* camera,
                      const rdx_params* params, uint32_t* const* words_dev, uint32_t* keys_dev,
                      uint32_t* out_indices_dev, uint32_t* out_count_dev, uint32_t* out_draw_args_dev,

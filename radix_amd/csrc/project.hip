@@ -273,6 +273,28 @@ int rdx_project_ewa(rdx_context* ctx, void* stream, const rdx_splat_table* table
   return launch_project((hipStream_t)stream, table, u, o, side != nullptr);
 }
 
+int rdx_project_keys(rdx_context* ctx, void* stream, const rdx_splat_table* table, const rdx_camera* camera,
+                     const rdx_params* params, uint32_t at, uint32_t* keys) {
+  if (!camera || !params || !keys) return fail(RDX_E_INVALID, "rdx_project_keys: NULL camera, params or keys");
+  if (int rc = check_table(table)) return rc;
+  if (int rc = check_bits(params->sort_bits, "SplatProjection")) return rc;
+  ProjU u;
+  if (int rc = make_uniforms(table, camera, params, &u)) return rc;
+  const uint32_t n = table->count;
+  if (n == 0) return RDX_OK;
+  const TableP tp = make_table(table);
+  const dim3 grid((uint32_t)(((uint64_t)n + 255) / 256));
+  hipStream_t s = (hipStream_t)stream;
+  Stage st(reinterpret_cast<Context*>(ctx), s, "project_keys");
+  switch (table->layout) {
+    case RDX_LAYOUT_F32_SOA: hipLaunchKernelGGL(k_project_keys<LAYOUT_F32>, grid, dim3(256), 0, s, tp, u, keys, at); break;
+    case RDX_LAYOUT_EXT: hipLaunchKernelGGL(k_project_keys<LAYOUT_EXT>, grid, dim3(256), 0, s, tp, u, keys, at); break;
+    case RDX_LAYOUT_PACKED: hipLaunchKernelGGL(k_project_keys<LAYOUT_PACKED>, grid, dim3(256), 0, s, tp, u, keys, at); break;
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? RDX_OK : hip_fail(e, "rdx_project_keys");
+}
+
 int rdx_project_sort(rdx_context* ctx, void* stream, const rdx_splat_table* table, const rdx_camera* camera,
                      const rdx_params* params, uint32_t* const* words, uint32_t* keys, uint32_t* out_indices,
                      uint32_t* out_count, uint32_t* out_args, const rdx_float_out* side) {

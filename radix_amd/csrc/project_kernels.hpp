@@ -579,6 +579,36 @@ __device__ __forceinline__ void prepare(const Splat& sp, const U& u, float& op, 
   invisible = op <= 0.0f || op < u.min_alpha || (s.x == 0.0f && s.y == 0.0f && s.z == 0.0f);
 }
 
+// sortGuard (projection.tsl.ts:144-155): a CPU-sorted splat keeps its key
+// only if its head-space bound (max std devs along the head's axes) reaches
+// the guard-widened screen of eye 0, or the head is inside it.
+__device__ __forceinline__ bool sort_guarded(const Splat& sp, float3 s, const double* hc, const ProjU& u) {
+  float Rg[9];
+  quat_matrix(sp.q, Rg);
+  // head-space cov diagonal: (H3 R S)(H3 R S)^T
+  float diag[3];
+  for (int r = 0; r < 3; r++) {
+    float acc = 0.0f;
+    for (int k = 0; k < 3; k++) {
+      const float hr = (float)u.H[3 * r] * Rg[k] + (float)u.H[3 * r + 1] * Rg[3 + k] + (float)u.H[3 * r + 2] * Rg[6 + k];
+      const float sk = k == 0 ? s.x : (k == 1 ? s.y : s.z);
+      acc += hr * hr * sk * sk;
+    }
+    diag[r] = acc;
+  }
+  const float radius = sqrtf(fmaxf(fmaxf(diag[0], diag[1]), diag[2])) * u.max_std_dev;
+  const float* P0 = u.eye[0].Pf;
+  const float hx0 = (float)hc[0], hy0 = (float)hc[1], hz0 = (float)hc[2];
+  const float hw = P0[12] * hx0 + P0[13] * hy0 + P0[14] * hz0 + P0[15];
+  const float hx = P0[0] * hx0 + P0[1] * hy0 + P0[2] * hz0 + P0[3];
+  const float hy = P0[4] * hx0 + P0[5] * hy0 + P0[6] * hz0 + P0[7];
+  const float w = fmaxf(hw, 1e-6f);
+  const float rx = P0[0] * radius / w, ry = P0[5] * radius / w;
+  const bool inside = fabsf(hx / w) - rx <= u.sort_guard + 1.0f && fabsf(hy / w) - ry <= u.sort_guard + 1.0f;
+  const bool near = -hz0 <= radius;
+  return hz0 < radius && (inside || near);
+}
+
 // The projection kernel. DEG: the f32 table's SH degree (compile time), or
 // -1 for word tables (u.sh_degree at run time).
 template <int LAYOUT, int SHC, int DEG, int EYES, bool SIDE>
@@ -674,33 +704,7 @@ __global__ __launch_bounds__(256) void k_project(TableP t, ProjU u, OutP o) {
     }
 
     // sortGuard for the CPU sort's keys (projection.tsl.ts:144-155)
-    bool guarded = true;
-    if (!u.cull_keys && u.sort_guard > 0.0f) {
-      float Rg[9];
-      quat_matrix(sp.q, Rg);
-      // head-space cov diagonal: (H3 R S)(H3 R S)^T
-      float diag[3];
-      for (int r = 0; r < 3; r++) {
-        float acc = 0.0f;
-        for (int k = 0; k < 3; k++) {
-          const float hr = (float)u.H[3 * r] * Rg[k] + (float)u.H[3 * r + 1] * Rg[3 + k] + (float)u.H[3 * r + 2] * Rg[6 + k];
-          const float sk = k == 0 ? s.x : (k == 1 ? s.y : s.z);
-          acc += hr * hr * sk * sk;
-        }
-        diag[r] = acc;
-      }
-      const float radius = sqrtf(fmaxf(fmaxf(diag[0], diag[1]), diag[2])) * u.max_std_dev;
-      const float* P0 = u.eye[0].Pf;
-      const float hx0 = (float)hc[0], hy0 = (float)hc[1], hz0 = (float)hc[2];
-      const float hw = P0[12] * hx0 + P0[13] * hy0 + P0[14] * hz0 + P0[15];
-      const float hx = P0[0] * hx0 + P0[1] * hy0 + P0[2] * hz0 + P0[3];
-      const float hy = P0[4] * hx0 + P0[5] * hy0 + P0[6] * hz0 + P0[7];
-      const float w = fmaxf(hw, 1e-6f);
-      const float rx = P0[0] * radius / w, ry = P0[5] * radius / w;
-      const bool inside = fabsf(hx / w) - rx <= u.sort_guard + 1.0f && fabsf(hy / w) - ry <= u.sort_guard + 1.0f;
-      const bool near = -hz0 <= radius;
-      guarded = hz0 < radius && (inside || near);
-    }
+    const bool guarded = u.cull_keys || !(u.sort_guard > 0.0f) || sort_guarded(sp, s, hc, u);
 
     // Keys (SplatProjection.ts:143-151)
     const bool keyed = u.cull_keys ? visible : (!invisible && guarded);
@@ -740,6 +744,38 @@ __global__ __launch_bounds__(256) void k_project(TableP t, ProjU u, OutP o) {
       }
     }
   }
+}
+
+// The keys-only pass (SplatProjection.writeKeys, SplatProjection.ts:165-172,
+// the "keys" pass of SplatProjector.ts:184-189): the CPU sort's keys, from
+// the splat's centre, scales, rotation and opacity alone (no SH, no EWA):
+// key = cullInvisible || !guarded ? invalid : compute(|head centre|). The
+// head-space centre and distance are computed as k_project computes them.
+template <int LAYOUT>
+__global__ __launch_bounds__(256) void k_project_keys(TableP t, ProjU u, uint32_t* keys, uint32_t at) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= t.count) return;
+  F32Rsrc rs;
+  if constexpr (LAYOUT == LAYOUT_F32) rs = f32_rsrc(t, 0);
+  Splat sp;
+  float3 coef[1];
+  load_splat<LAYOUT, SHC_NONE, LAYOUT == LAYOUT_F32 ? 0 : -1>(t, rs, u.enc, i, 0, sp, coef);
+  float op, std_dev;
+  float3 s;
+  bool invisible;
+  prepare(sp, u, op, s, std_dev, invisible);
+  double hc[3], head_dist;
+  if (u.head_is_eye0) {
+    const EyeCentre C = eye_centre(u.eye[0], sp.c, u.ndc_z_min, u.clip_xy);
+    for (int r = 0; r < 3; r++) hc[r] = C.ce[r];
+    head_dist = C.dist;
+  } else {
+    const double dx = (double)sp.c.x - u.h[0], dy = (double)sp.c.y - u.h[1], dz = (double)sp.c.z - u.h[2];
+    for (int r = 0; r < 3; r++) hc[r] = fma(u.H[3 * r], dx, fma(u.H[3 * r + 1], dy, u.H[3 * r + 2] * dz));
+    head_dist = sqrt(fma(hc[0], hc[0], fma(hc[1], hc[1], hc[2] * hc[2])));
+  }
+  const bool guarded = !(u.sort_guard > 0.0f) || sort_guarded(sp, s, hc, u);
+  keys[at + i] = !invisible && guarded ? sort_key(fbits((float)head_dist), u.sort_bits) : key_invalid(u.sort_bits);
 }
 
 }  // namespace rdx

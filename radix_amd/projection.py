@@ -180,6 +180,20 @@ class Projector:
         out["keys"] = keys.view(2, -1)[:, :n] if per_eye else keys[:n]
         return out
 
+    def project_keys(self, table: DeviceTable, rig: CameraRig, params: Params, keys=None, at: int = 0, stream=None):
+        """The keys-only pass (SplatProjection.writeKeys, src/render/SplatProjection.ts:165-172): the
+        guarded head-distance key of each splat into keys[at:at + n] ((n,) int32 when keys is None)."""
+        import torch
+
+        n = table.count
+        if keys is None:
+            keys = torch.zeros(max(at + n, 1), dtype=torch.int32, device="cuda")
+        if keys.numel() < at + n:
+            raise ValueError(f"project_keys: keys hold {keys.numel()} of {at + n}")
+        N.check(N.lib().rdx_project_keys(self.context.handle, _stream(stream), C.byref(table.desc()),
+                                         C.byref(rig.desc()), C.byref(params.desc()), at, _ptr(keys)))
+        return keys[at:at + n]
+
     def project_sort(self, table: DeviceTable, rig: CameraRig, params: Params, words=None, keys=None, indices=None,
                      count=None, args=None, side: bool = False, stream=None):
         """The fused path: words, keys, the visible indices back to front, count and draw arguments.
