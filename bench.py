@@ -247,6 +247,41 @@ def bench_keys(steps: int, proj, table, rig) -> dict:
             "bytes_per_splat": 48, "hbm_frac": round(bytes_ / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
+def bench_ray(steps: int, proj, table, rig) -> dict:
+    """The 3dgut and htgs projection words over the plain_4m table: the f32
+    SH3 splat in (236 B), six head words, two eye words and a key out (36 B)."""
+    import torch
+
+    from radix_amd import Params
+    from radix_amd import _native as N
+
+    import ctypes as C
+
+    prm = Params(sort_bits=32)
+    n = table.count
+    chans = [torch.empty((n, 4), dtype=torch.int32, device="cuda") for _ in range(2)]
+    keys = torch.empty(n, dtype=torch.int32, device="cuda")
+    cp = (C.c_void_p * 3)(chans[0].data_ptr(), chans[1].data_ptr(), None)
+    out = {}
+    for name, m in (("3dgut", 1), ("htgs", 2)):
+        def call():
+            N.check(N.lib().rdx_project_ray(proj.context.handle, None, C.byref(table.desc()), C.byref(rig.desc()),
+                                            C.byref(prm.desc()), m, 0, cp, C.c_void_p(keys.data_ptr())))
+        call()
+        ms = time_events(call, steps)
+        N.lib().rdx_context_profile(proj.context.handle, 1)
+        stage_times(proj.context)
+        for _ in range(steps):
+            call()
+        k_ms = stage_times(proj.context).get(f"project_{name}", {}).get("ms_avg", float("nan"))
+        N.lib().rdx_context_profile(proj.context.handle, 0)
+        out[name] = {"ms": round(ms, 4), "kernel_ms": round(k_ms, 4), "Gsplats_per_s": round(n / ms / 1e6, 3),
+                     "hbm_frac": round(272 * n / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    out["workload"] = "3dgut and htgs words of the plain_4m table (4,194,304 f32 SH3 splats, 1920x1080), 32-bit keys"
+    out["bytes_per_splat"] = 272
+    return out
+
+
 def bench_backward(steps: int) -> dict:
     """Differentiable compositor: 262,144 splats, 512x512, 16x16 tiles, L2
     loss against the perturbed scene; bwd Gsplats/s = n / backward time."""
@@ -415,6 +450,8 @@ def main() -> int:
         log("draw of the plain_4m frame")
         out["draw_4m"] = bench_draw(max(3, args.steps // 5), step, words[0], idx, cnt, int(rig.width), int(rig.height),
                                     params, ctx)
+        log("ray projection methods")
+        out["ray_4m"] = bench_ray(max(3, args.steps // 5), proj, table, rig)
         log("keys-only pass")
         out["keys_pass_4m"] = bench_keys(max(3, args.steps // 5), proj, table, rig)
         torch.cuda.empty_cache()

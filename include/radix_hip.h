@@ -252,15 +252,6 @@ int rdx_project_ewa(rdx_context* ctx, void* stream, const rdx_splat_table* table
                     const rdx_params* params, uint32_t at, uint32_t* const* words_dev, uint32_t* keys_dev,
                     size_t key_stride, const rdx_float_out* side);
 
-/* Project and sort in one stream-ordered sequence: the projection kernel
- * writes words and keys; the first radix pass drops invalid keys. Output
- * as rdx_sort_splats for the keys it made (keys_dev NULL: context scratch). The headline path
- * (SplatRenderer.update's project + sort, SplatRenderer.ts:412-456).
- * With params->key_per_eye on a stereo camera each eye sorts its own keys
- * (two segments of n = table->count): eye e's keys at keys_dev + e n, its
- * indices at out_indices_dev + e n, its count at out_count_dev[e] and its
- * draw arguments at out_draw_args_dev + 5 e; the buffers hold 2 n, 2 n, 2
- * and 10 words. */
 /* The keys-only pass for CPU sorts (SplatProjection.writeKeys,
  * src/render/SplatProjection.ts:165-172, run by projectCollection's "keys"
  * pass, SplatProjector.ts:184-189, for SplatView.sort, SplatView.ts:137-185):
@@ -272,6 +263,39 @@ int rdx_project_ewa(rdx_context* ctx, void* stream, const rdx_splat_table* table
 int rdx_project_keys(rdx_context* ctx, void* stream, const rdx_splat_table* table, const rdx_camera* camera,
                      const rdx_params* params, uint32_t at, uint32_t* keys_dev);
 
+/* The ray-evaluated projection methods' words (SplatProjection.write for
+ * "3dgut" and "htgs", src/render/SplatProjection.ts:141-151 with
+ * projections.tsl.ts:170-328): per splat 6 head words, then 2 per eye,
+ *   head [centre direction (octahedral 16 + 16), centre distance (float
+ *         bits), rgb, rotation QuatOct32, half2 ln(scales.xy),
+ *         half ln(scales.z) | opacity8 << 16 | direction's low 4 + 4 bits << 24]
+ *   eye  [bound ndc 2x16, angle10 | log2 scale (11 bits, outward) << 10 | << 21]
+ * in the projected bundle's channels (SplatProjection.ts:46-53):
+ * channels_dev[0], [1] uint32x4 and, with two eyes, channels_dev[2]
+ * uint32x2, each at element at + i. A culled eye's words are 0; a splat
+ * culled in every eye keeps its head words (zero when invisible). keys_dev
+ * (NULL: none) gets the head-distance key at [at + i] (cullKeys: visible in
+ * an eye; else !invisible and the sortGuard). The object transform must be
+ * a rotation times a uniform scale (the reference's fast path,
+ * splatTransform.tsl.ts:142-152; others fail with RDX_E_UNSUPPORTED).
+ * method: RDX_METHOD_3DGUT or RDX_METHOD_HTGS (raygs: RDX_E_UNSUPPORTED). */
+#define RDX_METHOD_EWA 0
+#define RDX_METHOD_3DGUT 1
+#define RDX_METHOD_HTGS 2
+#define RDX_METHOD_RAYGS 3
+int rdx_project_ray(rdx_context* ctx, void* stream, const rdx_splat_table* table, const rdx_camera* camera,
+                    const rdx_params* params, int method, uint32_t at, uint32_t* const* channels_dev,
+                    uint32_t* keys_dev);
+
+/* Project and sort in one stream-ordered sequence: the projection kernel
+ * writes words and keys; the first radix pass drops invalid keys. Output
+ * as rdx_sort_splats for the keys it made (keys_dev NULL: context scratch). The headline path
+ * (SplatRenderer.update's project + sort, SplatRenderer.ts:412-456).
+ * With params->key_per_eye on a stereo camera each eye sorts its own keys
+ * (two segments of n = table->count): eye e's keys at keys_dev + e n, its
+ * indices at out_indices_dev + e n, its count at out_count_dev[e] and its
+ * draw arguments at out_draw_args_dev + 5 e; the buffers hold 2 n, 2 n, 2
+ * and 10 words. */
 int rdx_project_sort(rdx_context* ctx, void* stream, const rdx_splat_table* table, const rdx_camera* camera,
                      const rdx_params* params, uint32_t* const* words_dev, uint32_t* keys_dev,
                      uint32_t* out_indices_dev, uint32_t* out_count_dev, uint32_t* out_draw_args_dev,

@@ -135,6 +135,13 @@ def lib() -> C.CDLL:
             "orc_sh_basis_grad": (None, [_dp, _dp]),
             "orc_draw": (None, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_int, C.c_int, _d, _d, _d, C.c_int,
                                 _dp]),
+            "orc_project_ray": (None, [C.c_int, C.POINTER(OrcRaySplat), C.POINTER(OrcView), C.POINTER(OrcParams),
+                                       C.POINTER(OrcRayOut)]),
+            "orc_project_ray_table_f32": (None, [C.c_int, C.POINTER(OrcTableF32), C.POINTER(OrcView),
+                                                 C.POINTER(OrcParams), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+            "orc_encode_center": (None, [_dp, _u32p, _u32p, _u32p]),
+            "orc_decode_center": (None, [C.c_uint32, C.c_uint32, C.c_uint32, _dp]),
+            "orc_encode_quat_oct32_gpu": (C.c_uint32, [_dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(h, name)
@@ -230,6 +237,45 @@ def project_f32(planes: np.ndarray, sh_degree: int, view: OrcView, prm: OrcParam
                     base + 14 * st if sh_degree > 0 else None, sh_degree)
     return _run_batch(lambda o: lib().orc_project_table_f32(C.byref(t), C.byref(view), C.byref(prm), C.byref(o)),
                       n, eyes, per_eye)
+
+
+class OrcRaySplat(C.Structure):
+    _fields_ = [("center", _d * 3), ("scales", _d * 3), ("quat", _d * 4), ("rgb", _d * 3), ("opacity", _d)]
+
+
+class OrcRayOut(C.Structure):
+    _fields_ = [("words", C.c_uint32 * 10), ("visible", C.c_int), ("invisible", C.c_int), ("guarded", C.c_int),
+                ("eye_visible", C.c_int * 2), ("key", C.c_uint32), ("distance", _d), ("head_scales", _d * 3),
+                ("head_opacity", _d), ("bound", (_d * 5) * 2), ("eye_margin", _d * 2), ("margin", _d)]
+
+
+METHODS = {"3dgut": 1, "htgs": 2}
+
+
+def project_ray(method: str, center, scales, quat, rgb, opacity, view: OrcView, prm: OrcParams) -> OrcRayOut:
+    """One object-space splat (scales, not log-scales) through orc_project_ray."""
+    s = OrcRaySplat((_d * 3)(*center), (_d * 3)(*scales), (_d * 4)(*quat), (_d * 3)(*rgb), float(opacity))
+    o = OrcRayOut()
+    lib().orc_project_ray(METHODS[method], C.byref(s), C.byref(view), C.byref(prm), C.byref(o))
+    return o
+
+
+def project_ray_f32(method: str, planes: np.ndarray, sh_degree: int, view: OrcView, prm: OrcParams):
+    """The ray oracle over an f32-planes table: words (n, 6 + 2 eyes), keys, margins (n,) and (eyes, n)."""
+    n = planes.shape[1]
+    eyes = view.eyes
+    planes = np.ascontiguousarray(planes, dtype=np.float32)
+    base = planes.ctypes.data
+    st = n * 4
+    t = OrcTableF32(n, n, base, base + 3 * st, base + 6 * st, base + 10 * st, base + 11 * st,
+                    base + 14 * st if sh_degree > 0 else None, sh_degree)
+    words = np.zeros((n, 6 + 2 * eyes), dtype=np.uint32)
+    keys = np.zeros(n, dtype=np.uint32)
+    margin = np.zeros(n, dtype=np.float64)
+    eye_margin = np.zeros((eyes, n), dtype=np.float64)
+    lib().orc_project_ray_table_f32(METHODS[method], C.byref(t), C.byref(view), C.byref(prm), words.ctypes.data,
+                                    keys.ctypes.data, margin.ctypes.data, eye_margin.ctypes.data)
+    return {"words": words, "keys": keys, "margin": margin, "eye_margin": eye_margin}
 
 
 def project_packed(words: np.ndarray, enc, sh_channels, sh_degree: int, view: OrcView, prm: OrcParams):

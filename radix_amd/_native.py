@@ -193,6 +193,9 @@ _SIGNATURES = [
      [C.c_void_p, C.c_void_p, _u32p, _u32p, _u32p, C.POINTER(C.c_int)]),
     ("rdx_project_keys", C.c_int,
      [C.c_void_p, C.c_void_p, C.POINTER(SplatTable), C.POINTER(Camera), C.POINTER(Params), C.c_uint32, C.c_void_p]),
+    ("rdx_project_ray", C.c_int,
+     [C.c_void_p, C.c_void_p, C.POINTER(SplatTable), C.POINTER(Camera), C.POINTER(Params), C.c_int, C.c_uint32,
+      C.c_void_p, C.c_void_p]),
     ("rdx_draw_create", C.c_int, [C.c_void_p, C.POINTER(DrawSettings), C.POINTER(C.c_void_p)]),
     ("rdx_draw_destroy", C.c_int, [C.c_void_p]),
     ("rdx_draw_splats", C.c_int,

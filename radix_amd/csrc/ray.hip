@@ -1,0 +1,152 @@
+// The ray projection methods' entry point: uniforms (the object transform
+// split into a uniform scale and rotations, splatTransform.tsl.ts:46-74,
+// 142-152) and the kernels per layout, SH degree and eye count.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "../../include/radix_hip.h"
+#include "api_internal.hpp"
+#include "project_host.hpp"
+#include "ray_kernels.hpp"
+
+namespace rdx {
+
+// The quaternion (x, y, z, w) of a row-major rotation (Shepperd: the
+// largest of the four squared components first).
+static void quat_of(const double* R, float* q) {
+  const double tr = R[0] + R[4] + R[8];
+  double x, y, z, w;
+  if (tr > 0.0) {
+    const double s = 0.5 / std::sqrt(tr + 1.0);
+    w = 0.25 / s;
+    x = (R[7] - R[5]) * s;
+    y = (R[2] - R[6]) * s;
+    z = (R[3] - R[1]) * s;
+  } else if (R[0] > R[4] && R[0] > R[8]) {
+    const double s = 2.0 * std::sqrt(1.0 + R[0] - R[4] - R[8]);
+    w = (R[7] - R[5]) / s;
+    x = 0.25 * s;
+    y = (R[1] + R[3]) / s;
+    z = (R[2] + R[6]) / s;
+  } else if (R[4] > R[8]) {
+    const double s = 2.0 * std::sqrt(1.0 + R[4] - R[0] - R[8]);
+    w = (R[2] - R[6]) / s;
+    x = (R[1] + R[3]) / s;
+    y = 0.25 * s;
+    z = (R[5] + R[7]) / s;
+  } else {
+    const double s = 2.0 * std::sqrt(1.0 + R[8] - R[0] - R[4]);
+    w = (R[3] - R[1]) / s;
+    x = (R[2] + R[6]) / s;
+    y = (R[5] + R[7]) / s;
+    z = 0.25 * s;
+  }
+  const double n = std::sqrt(x * x + y * y + z * z + w * w);
+  q[0] = (float)(x / n);
+  q[1] = (float)(y / n);
+  q[2] = (float)(z / n);
+  q[3] = (float)(w / n);
+}
+
+static double det3r(const double* M) {
+  return M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6]) + M[2] * (M[3] * M[7] - M[4] * M[6]);
+}
+
+// isUniformScale (splatTransform.tsl.ts:46-62) of a row-major 3x3: M^T M
+// within 1e-6 of s^2 I, and det > 0. Returns s, or 0.
+static double uniform_scale(const double* M) {
+  double G[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) G[3 * i + j] = M[i] * M[j] + M[3 + i] * M[3 + j] + M[6 + i] * M[6 + j];
+  const double s2 = (G[0] + G[4] + G[8]) / 3.0;
+  if (!(s2 > 0.0)) return 0.0;
+  for (int i = 0; i < 3; i++)
+    for (int j = i; j < 3; j++)
+      if (std::fabs(G[3 * i + j] - (i == j ? s2 : 0.0)) > 1e-6 * s2) return 0.0;
+  const double d = det3r(M);
+  return d > 0.0 ? std::cbrt(d) : 0.0;
+}
+
+template <int METHOD, int EYES>
+static void launch_layout(hipStream_t st, dim3 grid, const rdx_splat_table* t, const TableP& tp, const RayU& r,
+                          const RayOut& o) {
+  const int shc = t->sh_codec == RDX_SH_PACKED ? SHC_PACKED : SHC_EXT;
+  if (t->layout == RDX_LAYOUT_F32_SOA) {
+    switch (r.u.sh_degree) {
+      case 0: hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_F32, SHC_F32, 0, EYES>), grid, dim3(256), 0, st, tp, r, o); break;
+      case 1: hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_F32, SHC_F32, 1, EYES>), grid, dim3(256), 0, st, tp, r, o); break;
+      case 2: hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_F32, SHC_F32, 2, EYES>), grid, dim3(256), 0, st, tp, r, o); break;
+      default: hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_F32, SHC_F32, 3, EYES>), grid, dim3(256), 0, st, tp, r, o); break;
+    }
+  } else if (t->layout == RDX_LAYOUT_EXT) {
+    if (shc == SHC_PACKED)
+      hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_EXT, SHC_PACKED, -1, EYES>), grid, dim3(256), 0, st, tp, r, o);
+    else
+      hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_EXT, SHC_EXT, -1, EYES>), grid, dim3(256), 0, st, tp, r, o);
+  } else {
+    if (shc == SHC_PACKED)
+      hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_PACKED, SHC_PACKED, -1, EYES>), grid, dim3(256), 0, st, tp, r, o);
+    else
+      hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_PACKED, SHC_EXT, -1, EYES>), grid, dim3(256), 0, st, tp, r, o);
+  }
+}
+
+}  // namespace rdx
+
+using namespace rdx;
+
+extern "C" {
+
+int rdx_project_ray(rdx_context* ctx, void* stream, const rdx_splat_table* table, const rdx_camera* camera,
+                    const rdx_params* params, int method, uint32_t at, uint32_t* const* channels_dev, uint32_t* keys_dev) {
+  if (!camera || !params || !channels_dev) return fail(RDX_E_INVALID, "rdx_project_ray: NULL camera, params or channels");
+  if (method != RDX_METHOD_3DGUT && method != RDX_METHOD_HTGS)
+    return fail(method == RDX_METHOD_RAYGS || method == RDX_METHOD_EWA ? RDX_E_UNSUPPORTED : RDX_E_INVALID,
+                "rdx_project_ray: method %d is not 3dgut (1) or htgs (2)", method);
+  if (int rc = check_table(table)) return rc;
+  if (int rc = check_bits(params->sort_bits, "SplatProjection")) return rc;
+  RayU r;
+  memset(&r, 0, sizeof r);
+  if (int rc = make_uniforms(table, camera, params, &r.u)) return rc;
+  const int channels = camera->eyes == 2 ? 3 : 2;
+  for (int c = 0; c < channels; c++)
+    if (!channels_dev[c]) return fail(RDX_E_INVALID, "rdx_project_ray: channel w%d is NULL", c);
+  // the object transform as a uniform scale times a rotation
+  const double s = uniform_scale(r.u.H);
+  if (!(s > 0.0))
+    return fail(RDX_E_UNSUPPORTED,
+                "rdx_project_ray: the object transform must be a rotation times a uniform scale (3dgut and htgs read scales "
+                "and rotations)");
+  r.oscale = (float)s;
+  double R[9];
+  for (int k = 0; k < 9; k++) R[k] = r.u.H[k] / s;
+  quat_of(R, r.hq);
+  for (int e = 0; e < camera->eyes; e++) {
+    for (int k = 0; k < 9; k++) R[k] = r.u.eye[e].L[k] / s;
+    quat_of(R, r.eq[e]);
+  }
+  const uint32_t n = table->count;
+  if (n == 0) return RDX_OK;
+  RayOut o;
+  memset(&o, 0, sizeof o);
+  for (int c = 0; c < channels; c++) o.ch[c] = channels_dev[c];
+  o.keys = keys_dev;
+  o.at = at;
+  const TableP tp = make_table(table);
+  const dim3 grid((uint32_t)(((uint64_t)n + 255) / 256));
+  hipStream_t st = (hipStream_t)stream;
+  Stage stage(reinterpret_cast<Context*>(ctx), st, method == RDX_METHOD_3DGUT ? "project_3dgut" : "project_htgs");
+  if (method == RDX_METHOD_3DGUT) {
+    if (camera->eyes == 2) launch_layout<METHOD_3DGUT, 2>(st, grid, table, tp, r, o);
+    else launch_layout<METHOD_3DGUT, 1>(st, grid, table, tp, r, o);
+  } else {
+    if (camera->eyes == 2) launch_layout<METHOD_HTGS, 2>(st, grid, table, tp, r, o);
+    else launch_layout<METHOD_HTGS, 1>(st, grid, table, tp, r, o);
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? RDX_OK : hip_fail(e, "rdx_project_ray");
+}
+
+}  // extern "C"

@@ -1,0 +1,439 @@
+// The ray-evaluated projection methods 3dgut and htgs (the projection words
+// of src/render/projections.tsl.ts:170-328 with the helpers of
+// projection.tsl.ts:169-275, 312-360): per splat, six head words (the
+// head-space centre as an octahedral direction and a float distance, the
+// colour, the head-space rotation, the MIP-inflated scales as half floats,
+// the opacity) and two words per eye (a screen bound: ndc 2x16, a 10-bit
+// angle and two 11-bit log2 pixel scales rounded outward).
+#pragma once
+
+#include <hip/hip_fp16.h>
+
+#include "project_kernels.hpp"
+
+namespace rdx {
+
+constexpr int METHOD_EWA = 0, METHOD_3DGUT = 1, METHOD_HTGS = 2, METHOD_RAYGS = 3;
+
+// Uniforms of a ray projection: the ewa uniforms plus the rotations of the
+// object transform (uniform scale times a rotation; splatTransform.tsl.ts:142-152).
+struct RayU {
+  ProjU u;
+  float hq[4];     // object -> head rotation (x, y, z, w)
+  float eq[2][4];  // object -> eye rotations
+  float oscale;    // the object transform's scale
+};
+
+// Word channels: ch[0], ch[1] uint32x4; with two eyes ch[2] uint32x2
+// (SplatProjection.ts:46-53).
+struct RayOut {
+  uint32_t* ch[3];
+  uint32_t* keys;
+  uint32_t at;
+};
+
+__device__ __forceinline__ float4 quat_mul(float4 a, float4 b) {
+  return make_float4(a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y, a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
+                     a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z);
+}
+__device__ __forceinline__ float4 quat_normalize(float4 q) {
+  const float n2 = q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w;
+  if (!(n2 > 0.0f)) return q;
+  const float r = rsqrtf(n2);
+  return make_float4(q.x * r, q.y * r, q.z * r, q.w * r);
+}
+// v rotated by unit q
+__device__ __forceinline__ float3 quat_rotate(float4 q, float3 v) {
+  const float tx = 2.0f * (q.y * v.z - q.z * v.y), ty = 2.0f * (q.z * v.x - q.x * v.z), tz = 2.0f * (q.x * v.y - q.y * v.x);
+  return make_float3(v.x + q.w * tx + (q.y * tz - q.z * ty), v.y + q.w * ty + (q.z * tx - q.x * tz),
+                     v.z + q.w * tz + (q.x * ty - q.y * tx));
+}
+
+// A bound: centre in ndc, angle of the first axis in [0, pi), half extents in pixels
+struct RayBound {
+  float nx, ny, angle, s1, s2;
+};
+
+// boundFromCov (projection.tsl.ts:223-240)
+template <typename U>
+__device__ __forceinline__ bool bound_from_cov(float a, float b, float d, float std_dev, const U& u, RayBound& bd) {
+  const float det = a * d - b * b;
+  const float half = (a - d) * 0.5f;
+  const float eig1 = (a + d) * 0.5f + sqrtf(half * half + b * b);
+  const float eig2 = eig1 > 0.0f ? fmaxf(det / eig1, 0.0f) : 0.0f;
+  const float sc1 = fminf(u.max_pr, std_dev * sqrtf(fmaxf(eig1, 0.0f)));
+  const float sc2 = fminf(u.max_pr, std_dev * sqrtf(eig2));
+  const bool ok = sc1 >= u.min_pr || sc2 >= u.min_pr;
+  if (ok) {
+    float vx, vy;
+    if (fabsf(a - eig1) >= fabsf(d - eig1)) {
+      vx = b;
+      vy = eig1 - a;
+    } else {
+      vx = eig1 - d;
+      vy = b;
+    }
+    const float len = sqrtf(vx * vx + vy * vy);
+    if (len > 0.0f) {
+      vx /= len;
+      vy /= len;
+    } else {
+      vx = 1.0f;
+      vy = 0.0f;
+    }
+    float ang = atan2f(vy, vx);
+    bd.angle = ang < 0.0f ? ang + 3.14159265358979323846f : ang;
+    bd.s1 = sc1;
+    bd.s2 = sc2;
+  }
+  return ok;
+}
+
+// screenBound (projection.tsl.ts:243-248)
+template <typename U>
+__device__ __forceinline__ void screen_bound(const U& u, RayBound& bd) {
+  bd.nx = bd.ny = 0.0f;
+  bd.angle = 0.0f;
+  bd.s1 = bd.s2 = fminf(u.max_pr, sqrtf(u.size[0] * u.size[0] + u.size[1] * u.size[1]) * 0.5f);
+}
+
+// clipBoundToScreen (projection.tsl.ts:250-275)
+__device__ __forceinline__ bool clip_bound_to_screen(const float* size, RayBound& bd) {
+  const float hx = size[0] * 0.5f, hy = size[1] * 0.5f;
+  float sn, cs;
+  sincosf(bd.angle, &sn, &cs);
+  const float cx = cs * bd.s1, cy = cs * bd.s2, sx = sn * bd.s1, sy = sn * bd.s2;
+  const float ex = sqrtf(cx * cx + sy * sy), ey = sqrtf(sx * sx + cy * cy);
+  const float px = bd.nx * hx, py = bd.ny * hy;
+  float lox = px - ex, loy = py - ey, hix = px + ex, hiy = py + ey;
+  if (!(lox >= -hx && loy >= -hy && hix <= hx && hiy <= hy)) {
+    lox = fmaxf(lox, -hx);
+    loy = fmaxf(loy, -hy);
+    hix = fminf(hix, hx);
+    hiy = fminf(hiy, hy);
+    if (lox >= hix || loy >= hiy) return false;
+    bd.nx = (lox + hix) * 0.5f / hx;
+    bd.ny = (loy + hiy) * 0.5f / hy;
+    bd.angle = 0.0f;
+    bd.s1 = (hix - lox) * 0.5f;
+    bd.s2 = (hiy - loy) * 0.5f;
+  }
+  return true;
+}
+
+// The columns of R(q) diag(s), q unit (scaleQuatMatrix, math.tsl.ts)
+__device__ __forceinline__ void scaled_axes(float4 q, float3 s, float3 col[3]) {
+  col[0] = quat_rotate(q, make_float3(s.x, 0.0f, 0.0f));
+  col[1] = quat_rotate(q, make_float3(0.0f, s.y, 0.0f));
+  col[2] = quat_rotate(q, make_float3(0.0f, 0.0f, s.z));
+}
+
+__device__ __forceinline__ float4 clip_point(const float* P, float3 p, float w) {
+  return make_float4(fmaf(P[0], p.x, fmaf(P[1], p.y, fmaf(P[2], p.z, P[3] * w))),
+                     fmaf(P[4], p.x, fmaf(P[5], p.y, fmaf(P[6], p.z, P[7] * w))),
+                     fmaf(P[8], p.x, fmaf(P[9], p.y, fmaf(P[10], p.z, P[11] * w))),
+                     fmaf(P[12], p.x, fmaf(P[13], p.y, fmaf(P[14], p.z, P[15] * w))));
+}
+
+// gutBound: the unscented transform of the 7 sigma points c, c +- sqrt(3) s_i R e_i
+// (projections.tsl.ts:183-222)
+template <typename U>
+__device__ __forceinline__ bool gut_bound(const float* P, const float* size, float3 c, float4 q, float3 s, float std_dev,
+                                          const U& u, RayBound& bd) {
+  const float k = 1.7320508075688772f;
+  float3 col[3];
+  scaled_axes(q, make_float3(s.x * k, s.y * k, s.z * k), col);
+  const float hx = size[0] * 0.5f, hy = size[1] * 0.5f;
+  bool front = true;
+  float px[7], py[7];
+#pragma unroll
+  for (int i = 0; i < 7; i++) {
+    float3 p = c;
+    if (i > 0) {
+      const float3 a = col[(i - 1) >> 1];
+      const float sg = (i & 1) ? 1.0f : -1.0f;
+      p = make_float3(c.x + sg * a.x, c.y + sg * a.y, c.z + sg * a.z);
+    }
+    const float4 cl = clip_point(P, p, 1.0f);
+    front = front && cl.w > 0.0f;
+    px[i] = cl.x / cl.w * hx;
+    py[i] = cl.y / cl.w * hy;
+  }
+  if (!front) {
+    screen_bound(u, bd);
+    return true;
+  }
+  float mx = 0.0f, my = 0.0f;
+#pragma unroll
+  for (int i = 1; i < 7; i++) {
+    mx += px[i] * (1.0f / 6.0f);
+    my += py[i] * (1.0f / 6.0f);
+  }
+  const float d0x = px[0] - mx, d0y = py[0] - my;
+  float a = d0x * d0x * 2.0f, b = d0x * d0y * 2.0f, d = d0y * d0y * 2.0f;
+#pragma unroll
+  for (int i = 1; i < 7; i++) {
+    const float dx = px[i] - mx, dy = py[i] - my;
+    a += dx * dx * (1.0f / 6.0f);
+    b += dx * dy * (1.0f / 6.0f);
+    d += dy * dy * (1.0f / 6.0f);
+  }
+  const bool ok = bound_from_cov(a, b, d, std_dev, u, bd);
+  bd.nx = mx / hx;
+  bd.ny = my / hy;
+  return ok;
+}
+
+// htgsBound: the exact outline of the std-dev ellipsoid from its dual
+// quadric (projections.tsl.ts:224-264)
+template <typename U>
+__device__ __forceinline__ bool htgs_bound(const float* P, const float* size, float3 c, float4 q, float3 s, float std_dev,
+                                           float2 eye_ndc, const U& u, RayBound& bd) {
+  float3 col[3];
+  scaled_axes(q, make_float3(s.x * std_dev, s.y * std_dev, s.z * std_dev), col);
+  float4 T[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) T[i] = clip_point(P, col[i], 0.0f);
+  const float4 c3 = clip_point(P, c, 1.0f);
+  const float xs[3] = {T[0].x, T[1].x, T[2].x}, ys[3] = {T[0].y, T[1].y, T[2].y}, ws[3] = {T[0].w, T[1].w, T[2].w};
+  const float sww = ws[0] * ws[0] + ws[1] * ws[1] + ws[2] * ws[2];
+  const float qww = sww - c3.w * c3.w;
+  bool valid = qww < 0.0f && c3.w > 0.0f;
+  float a = 0.0f, b = 0.0f, d = 0.0f, cx = 0.0f, cy = 0.0f;
+  if (valid) {
+    float uu[3], vv[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      uu[i] = xs[i] * c3.w - ws[i] * c3.x;
+      vv[i] = ys[i] * c3.w - ws[i] * c3.y;
+    }
+    const float xw = xs[0] * ws[0] + xs[1] * ws[1] + xs[2] * ws[2];
+    const float yw = ys[0] * ws[0] + ys[1] * ws[1] + ys[2] * ws[2];
+    const float xx = xs[0] * xs[0] + xs[1] * xs[1] + xs[2] * xs[2];
+    const float yy = ys[0] * ys[0] + ys[1] * ys[1] + ys[2] * ys[2];
+    const float xy = xs[0] * ys[0] + xs[1] * ys[1] + xs[2] * ys[2];
+    const float uu2 = uu[0] * uu[0] + uu[1] * uu[1] + uu[2] * uu[2];
+    const float vv2 = vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2];
+    const float uv = uu[0] * vv[0] + uu[1] * vv[1] + uu[2] * vv[2];
+    const float inv_c2 = 1.0f / (qww * qww);
+    const float hx = size[0] * 0.5f, hy = size[1] * 0.5f;
+    a = (uu2 - (sww * xx - xw * xw)) * inv_c2 * hx * hx;
+    b = (uv - (sww * xy - xw * yw)) * inv_c2 * hx * hy;
+    d = (vv2 - (sww * yy - yw * yw)) * inv_c2 * hy * hy;
+    const float wu = ws[0] * uu[0] + ws[1] * uu[1] + ws[2] * uu[2];
+    const float wv = ws[0] * vv[0] + ws[1] * vv[1] + ws[2] * vv[2];
+    const float den = c3.w * qww;
+    cx = eye_ndc.x + wu / den;
+    cy = eye_ndc.y + wv / den;
+    valid = a > 0.0f && d > 0.0f && a * d - b * b > 0.0f;
+  }
+  if (!valid) {
+    screen_bound(u, bd);
+    return true;
+  }
+  const bool ok = bound_from_cov(a, b, d, 1.0f, u, bd);
+  bd.nx = cx;
+  bd.ny = cy;
+  return ok;
+}
+
+// ---- word codecs (projection.tsl.ts:277-360, encoding.tsl.ts:50-134) ------
+
+// log2 pixel scale in `bits`, rounded outward (encodeLogScale)
+__device__ __forceinline__ uint32_t encode_log_scale_out(float scale, int bits) {
+  const float top = (float)((1u << bits) - 2u);
+  const float nrm = (fminf(fmaxf(log2f(fmaxf(scale, 1e-20f)), -4.0f), 12.0f) + 4.0f) * (1.0f / 16.0f) * top;
+  return (uint32_t)fminf(fmaxf(ceilf(nrm), 0.0f), top) + 1u;
+}
+
+// encodeBound: [ndc 2x16, angle10 | log2 s1 (11, outward) << 10 | log2 s2 << 21]
+__device__ __forceinline__ uint2 encode_bound(const RayBound& bd) {
+  const uint32_t angle = (uint32_t)floorf(bd.angle * (1024.0f / 3.14159265358979323846f) + 0.5f) & 1023u;
+  // sin(pi / 1024 / 2) * 1.001: the angle's half step folded into the scales
+  const float slack = fmaxf(bd.s1, bd.s2) * (0.0015339801862847655f * 1.001f);
+  return make_uint2(encode_fixed16(bd.nx) | (encode_fixed16(bd.ny) << 16),
+                    angle | (encode_log_scale_out(bd.s1 + slack, 11) << 10) | (encode_log_scale_out(bd.s2 + slack, 11) << 21));
+}
+
+// encodeCenter: octahedral direction (20 + 20 bits) and the float distance;
+// low holds the direction's low 4 + 4 bits in bits 24-31
+__device__ __forceinline__ void encode_center(float3 c, uint32_t& dir, uint32_t& dist, uint32_t& low) {
+  dist = fbits(sqrtf(c.x * c.x + c.y * c.y + c.z * c.z));
+  const float l1 = fmaxf(fabsf(c.x) + fabsf(c.y) + fabsf(c.z), 1e-30f);
+  float px = c.x / l1, py = c.y / l1;
+  const float sx = px >= 0.0f ? 1.0f : -1.0f, sy = py >= 0.0f ? 1.0f : -1.0f;
+  if (c.z < 0.0f) {
+    const float fx = (1.0f - fabsf(py)) * sx, fy = (1.0f - fabsf(px)) * sy;
+    px = fx;
+    py = fy;
+  }
+  const uint32_t qx = (uint32_t)floorf((px * 0.5f + 0.5f) * 1048575.0f + 0.5f);
+  const uint32_t qy = (uint32_t)floorf((py * 0.5f + 0.5f) * 1048575.0f + 0.5f);
+  dir = (qx >> 4) | ((qy >> 4) << 16);
+  low = ((qx & 15u) << 24) | (qy << 28);
+}
+
+// Half-float bits (round to nearest even)
+__device__ __forceinline__ uint32_t half_bits(float f) { return (uint32_t)__half_as_ushort(__float2half_rn(f)); }
+
+// encodeQuatOct32: 10 + 10 bit octahedral axis, 12-bit angle, for w >= 0
+__device__ __forceinline__ uint32_t encode_quat_oct32(float4 q) {
+  const float len = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+  if (!(len > 0.0f)) return 0u;
+  const float s = (q.w < 0.0f ? -1.0f : 1.0f) / len;
+  const float angle = acosf(fminf(q.w * s, 1.0f)) * 2.0f;
+  const uint32_t acode = (uint32_t)floorf(angle * (4095.0f / 3.14159265358979323846f) + 0.5f);
+  const float ax = q.x * s, ay = q.y * s, az = q.z * s;
+  uint32_t oct = 0u;
+  const float l1 = fabsf(ax) + fabsf(ay) + fabsf(az);
+  if (l1 > 0.0f) {
+    float px = ax / l1, py = ay / l1;
+    if (az < 0.0f) {
+      const float ox = px;
+      px = (1.0f - fabsf(py)) * (ox >= 0.0f ? 1.0f : -1.0f);
+      py = (1.0f - fabsf(ox)) * (py >= 0.0f ? 1.0f : -1.0f);
+    }
+    const uint32_t cu = (uint32_t)floorf((px * 0.5f + 0.5f) * 1023.0f + 0.5f);
+    const uint32_t cv = (uint32_t)floorf((py * 0.5f + 0.5f) * 1023.0f + 0.5f);
+    oct = cu | (cv << 10);
+    if (oct == 0u) oct = 0xfffffu;
+  }
+  return oct | (acode << 20);
+}
+
+// ---- the kernel ----------------------------------------------------------------
+
+// One splat per thread: prepareSplat, the head basis with MIP inflation
+// (projections.tsl.ts:172-180, projection.tsl.ts:169-197), per eye the
+// bound (cullDepth only), the words and the key (SplatProjection.ts:131-151;
+// a splat-form sortGuard, projection.tsl.ts:145-155).
+template <int METHOD, int LAYOUT, int SHC, int DEG, int EYES>
+__global__ __launch_bounds__(256) void k_project_ray(TableP t, RayU r, RayOut o) {
+  const ProjU& u = r.u;
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= t.count) return;
+  const int deg = DEG >= 0 ? DEG : u.sh_degree;
+  F32Rsrc rs;
+  if constexpr (LAYOUT == LAYOUT_F32) rs = f32_rsrc(t, deg);
+  Splat sp;
+  float3 coef[15];
+  load_splat<LAYOUT, SHC, DEG>(t, rs, u.enc, i, deg, sp, coef);
+  float op, std_dev;
+  float3 s;
+  bool invisible;
+  prepare(sp, u, op, s, std_dev, invisible);
+  const float3 rgb = sh_colour(sp, u.h, deg, coef);
+
+  // head-space centre (double) and its distance, as the ewa kernel takes them
+  double hc[3], head_dist;
+  if (u.head_is_eye0) {
+    const EyeCentre C0 = eye_centre(u.eye[0], sp.c, u.ndc_z_min, u.clip_xy);
+    for (int k = 0; k < 3; k++) hc[k] = C0.ce[k];
+    head_dist = C0.dist;
+  } else {
+    const double dx = (double)sp.c.x - u.h[0], dy = (double)sp.c.y - u.h[1], dz = (double)sp.c.z - u.h[2];
+    for (int k = 0; k < 3; k++) hc[k] = fma(u.H[3 * k], dx, fma(u.H[3 * k + 1], dy, u.H[3 * k + 2] * dz));
+    head_dist = sqrt(fma(hc[0], hc[0], fma(hc[1], hc[1], hc[2] * hc[2])));
+  }
+  const float3 hcf = make_float3((float)hc[0], (float)hc[1], (float)hc[2]);
+
+  // viewBasis in the head frame: the head-space rotation and scales, flat
+  // splats kept a sliver thick
+  const float4 qn = quat_normalize(sp.q);
+  const float4 hq = quat_normalize(quat_mul(make_float4(r.hq[0], r.hq[1], r.hq[2], r.hq[3]), qn));
+  float3 S = make_float3(s.x * r.oscale, s.y * r.oscale, s.z * r.oscale);
+  {
+    const float floor_s = fmaxf(S.x, fmaxf(S.y, S.z)) * 1e-4f;
+    S = make_float3(fmaxf(S.x, floor_s), fmaxf(S.y, floor_s), fmaxf(S.z, floor_s));
+  }
+  const float3 s_guard = make_float3(s.x * r.oscale, s.y * r.oscale, s.z * r.oscale);
+
+  // mipInflate with the head view (eye 0's lens)
+  float oph = op;
+  {
+    const EyeU& E0 = u.eye[0];
+    const float3 local = quat_rotate(make_float4(-hq.x, -hq.y, -hq.z, hq.w), hcf);
+    const float l2x = local.x * local.x, l2y = local.y * local.y, l2z = local.z * local.z;
+    const float pixel = 2.0f / (u.size[1] * E0.Pf[5]);
+    const float footprint = pixel * pixel * (E0.ortho ? 1.0f : hcf.x * hcf.x + hcf.y * hcf.y + hcf.z * hcf.z);
+    float blur = u.blur;
+    if (u.focal_distance > 0.0f && u.aperture_tan > 0.0f && hcf.z < 0.0f) {
+      const float depth = -hcf.z;
+      const float coc_view = fabsf(depth - u.focal_distance) * u.aperture_tan;
+      const float coc = E0.ortho ? coc_view * E0.fy : coc_view * E0.fy / depth;
+      blur = fminf(fmaxf(coc * coc, blur), u.max_pr * u.max_pr);
+    }
+    const float s2x = S.x * S.x + u.pre_blur * footprint, s2y = S.y * S.y + u.pre_blur * footprint,
+                s2z = S.z * S.z + u.pre_blur * footprint;
+    const float bx = s2x + blur * footprint, by = s2y + blur * footprint, bz = s2z + blur * footprint;
+    const float num = l2x * s2y * s2z + l2y * s2x * s2z + l2z * s2x * s2y;
+    const float den = l2x * by * bz + l2y * bx * bz + l2z * bx * by;
+    oph *= den > 0.0f ? sqrtf(fmaxf(num / den, 0.0f)) : 1.0f;
+    S = make_float3(sqrtf(bx), sqrtf(by), sqrtf(bz));
+  }
+  const bool ok = !invisible && oph >= u.min_alpha;
+
+  uint32_t w[6 + 2 * EYES];
+#pragma unroll
+  for (int k = 0; k < 6 + 2 * EYES; k++) w[k] = 0u;
+  bool visible = false;
+#pragma unroll
+  for (int e = 0; e < EYES; e++) {
+    const EyeU& E = u.eye[e];
+    const EyeCentre C = eye_centre(E, sp.c, u.ndc_z_min, u.clip_xy);
+    const float z = (float)C.ce[2];
+    const bool cull_depth = z >= 0.0f || C.clip[2] < u.ndc_z_min * C.clip[3] || C.clip[2] >= C.clip[3];
+    if (ok && !cull_depth) {
+      const float3 c = make_float3((float)C.ce[0], (float)C.ce[1], z);
+      const float4 eq = quat_normalize(quat_mul(make_float4(r.eq[e][0], r.eq[e][1], r.eq[e][2], r.eq[e][3]), qn));
+      RayBound bd;
+      bool bok;
+      if constexpr (METHOD == METHOD_3DGUT) {
+        bok = gut_bound(E.Pf, u.size, c, eq, S, std_dev, u, bd);
+      } else {
+        const float2 ndc = C.clip[3] > 0.0f ? make_float2(C.clip[0] / C.clip[3], C.clip[1] / C.clip[3]) : make_float2(0.0f, 0.0f);
+        bok = htgs_bound(E.Pf, u.size, c, eq, S, std_dev, ndc, u, bd);
+      }
+      if (bok && clip_bound_to_screen(u.size, bd)) {
+        const uint2 bw = encode_bound(bd);
+        w[6 + 2 * e] = bw.x;
+        w[7 + 2 * e] = bw.y;
+        visible = true;
+      }
+    }
+  }
+  if (ok) {
+    uint32_t dir, dist, low;
+    encode_center(hcf, dir, dist, low);
+    const float stored = oph > 1.0f ? (oph + 3.0f) * 0.25f : oph;
+    w[0] = dir;
+    w[1] = dist;
+    w[2] = encode_ext_rgb(rgb.x, rgb.y, rgb.z);
+    w[3] = encode_quat_oct32(hq);
+    w[4] = half_bits(logf(S.x)) | (half_bits(logf(S.y)) << 16);
+    w[5] = half_bits(logf(S.z)) | (encode_opacity8(stored) << 16) | low;
+  }
+  const size_t at = (size_t)o.at + i;
+  reinterpret_cast<uint4*>(o.ch[0])[at] = make_uint4(w[0], w[1], w[2], w[3]);
+  reinterpret_cast<uint4*>(o.ch[1])[at] = make_uint4(w[4], w[5], w[6], w[7]);
+  if constexpr (EYES == 2) reinterpret_cast<uint2*>(o.ch[2])[at] = make_uint2(w[8], w[9]);
+
+  if (o.keys) {
+    bool guarded = true;
+    if (!u.cull_keys && u.sort_guard > 0.0f) {
+      // inSortGuard for the splat form: the reach is the largest scale
+      const float radius = fmaxf(s_guard.x, fmaxf(s_guard.y, s_guard.z)) * u.max_std_dev;
+      const float* P0 = u.eye[0].Pf;
+      const float4 hcl = clip_point(P0, hcf, 1.0f);
+      const float wv = fmaxf(hcl.w, 1e-6f);
+      const float rx = P0[0] * radius / wv, ry = P0[5] * radius / wv;
+      const bool inside = fabsf(hcl.x / wv) - rx <= u.sort_guard + 1.0f && fabsf(hcl.y / wv) - ry <= u.sort_guard + 1.0f;
+      const bool near = -hcf.z <= radius;
+      guarded = hcf.z < radius && (inside || near);
+    }
+    const bool keyed = u.cull_keys ? visible : (!invisible && guarded);
+    o.keys[o.at + i] = keyed ? sort_key(fbits((float)head_dist), u.sort_bits) : key_invalid(u.sort_bits);
+  }
+}
+
+}  // namespace rdx
