@@ -248,8 +248,9 @@ def bench_keys(steps: int, proj, table, rig) -> dict:
 
 
 def bench_ray(steps: int, proj, table, rig) -> dict:
-    """The 3dgut and htgs projection words over the plain_4m table: the f32
-    SH3 splat in (236 B), six head words, two eye words and a key out (36 B)."""
+    """The ray methods' projection words over the plain_4m table: the f32
+    SH3 splat in (236 B), a key and 8 words out (36 B; raygs writes 7 of them
+    and a zero)."""
     import torch
 
     from radix_amd import Params
@@ -263,7 +264,7 @@ def bench_ray(steps: int, proj, table, rig) -> dict:
     keys = torch.empty(n, dtype=torch.int32, device="cuda")
     cp = (C.c_void_p * 3)(chans[0].data_ptr(), chans[1].data_ptr(), None)
     out = {}
-    for name, m in (("3dgut", 1), ("htgs", 2)):
+    for name, m in (("3dgut", 1), ("htgs", 2), ("raygs", 3)):
         def call():
             N.check(N.lib().rdx_project_ray(proj.context.handle, None, C.byref(table.desc()), C.byref(rig.desc()),
                                             C.byref(prm.desc()), m, 0, cp, C.c_void_p(keys.data_ptr())))
@@ -277,7 +278,7 @@ def bench_ray(steps: int, proj, table, rig) -> dict:
         N.lib().rdx_context_profile(proj.context.handle, 0)
         out[name] = {"ms": round(ms, 4), "kernel_ms": round(k_ms, 4), "Gsplats_per_s": round(n / ms / 1e6, 3),
                      "hbm_frac": round(272 * n / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-    out["workload"] = "3dgut and htgs words of the plain_4m table (4,194,304 f32 SH3 splats, 1920x1080), 32-bit keys"
+    out["workload"] = "3dgut, htgs and raygs words of the plain_4m table (4,194,304 f32 SH3 splats, 1920x1080), 32-bit keys"
     out["bytes_per_splat"] = 272
     return out
 

@@ -264,12 +264,15 @@ int rdx_project_keys(rdx_context* ctx, void* stream, const rdx_splat_table* tabl
                      const rdx_params* params, uint32_t at, uint32_t* keys_dev);
 
 /* The ray-evaluated projection methods' words (SplatProjection.write for
- * "3dgut" and "htgs", src/render/SplatProjection.ts:141-151 with
- * projections.tsl.ts:170-328): per splat 6 head words, then 2 per eye,
+ * "3dgut", "htgs" and "raygs", src/render/SplatProjection.ts:141-151 with
+ * projections.tsl.ts:170-429). 3dgut and htgs: 6 head words, then 2 per eye,
  *   head [centre direction (octahedral 16 + 16), centre distance (float
  *         bits), rgb, rotation QuatOct32, half2 ln(scales.xy),
  *         half ln(scales.z) | opacity8 << 16 | direction's low 4 + 4 bits << 24]
  *   eye  [bound ndc 2x16, angle10 | log2 scale (11 bits, outward) << 10 | << 21]
+ * raygs: 4 head words, then 3 per eye,
+ *   head [centre direction, centre distance, rgb, opacity8 | low 4 + 4 bits << 24]
+ *   eye  [quad frame QuatOct32, c^2 (float bits), half2 ln(axis lengths)]
  * in the projected bundle's channels (SplatProjection.ts:46-53):
  * channels_dev[0], [1] uint32x4 and, with two eyes, channels_dev[2]
  * uint32x2, each at element at + i. A culled eye's words are 0; a splat
@@ -278,7 +281,7 @@ int rdx_project_keys(rdx_context* ctx, void* stream, const rdx_splat_table* tabl
  * an eye; else !invisible and the sortGuard). The object transform must be
  * a rotation times a uniform scale (the reference's fast path,
  * splatTransform.tsl.ts:142-152; others fail with RDX_E_UNSUPPORTED).
- * method: RDX_METHOD_3DGUT or RDX_METHOD_HTGS (raygs: RDX_E_UNSUPPORTED). */
+ * method: RDX_METHOD_3DGUT, RDX_METHOD_HTGS or RDX_METHOD_RAYGS. */
 #define RDX_METHOD_EWA 0
 #define RDX_METHOD_3DGUT 1
 #define RDX_METHOD_HTGS 2

@@ -229,7 +229,7 @@ void orc_draw(const uint32_t* words, uint32_t n, const uint32_t* order, uint32_t
               double max_std_dev, double min_alpha, double falloff, int linear, double* rgba);
 
 /* ---- The 3dgut and htgs projection words (ray.c) ------------------------- */
-/* SplatProjection.project / write for "3dgut" (method 1) and "htgs" (2)
+/* SplatProjection.project / write for "3dgut" (method 1), "htgs" (2) and "raygs" (3)
  * (SplatProjection.ts:130-151, projections.tsl.ts:170-300, with
  * projection.tsl.ts:42-360): an object-space splat (scales and rotation,
  * SH colour applied) through the view's object-to-head transform, which
@@ -238,18 +238,21 @@ typedef struct orc_ray_splat {
   double center[3], scales[3], quat[4], rgb[3], opacity;
 } orc_ray_splat;
 typedef struct orc_ray_out {
-  uint32_t words[10];    /* 6 head words, then 2 per eye */
+  uint32_t words[10];    /* 6 head words, then 2 per eye (raygs: 4, then 3 per eye) */
   int visible, invisible, guarded, eye_visible[2];
   uint32_t key;
   double distance;       /* |head-space centre| */
   double head_scales[3]; /* MIP-inflated */
   double head_opacity;   /* MIP-compensated */
   double bound[2][5];    /* per eye after the screen clip: ndc x, y, angle, scale1, scale2 */
+  double frame_axes[2][2][3]; /* raygs: per eye the quad's axes (view space) */
+  double frame_c2[2];         /* raygs: c^2 per eye */
+  double frame_lengths[2][2]; /* raygs: the axes' lengths */
   double eye_margin[2];  /* relative distance to an eye's cull or clip decisions */
   double margin;         /* the smallest over the head and eyes */
 } orc_ray_out;
 void orc_project_ray(int method, const orc_ray_splat* s, const orc_view* v, const orc_params* p, orc_ray_out* out);
-/* A whole f32 table: words n x (6 + 2 eyes), keys, margins (each may be NULL but words) */
+/* A whole f32 table: words n x (6 + 2 eyes; raygs 4 + 3 eyes), keys, margins (each may be NULL but words) */
 void orc_project_ray_table_f32(int method, const orc_table_f32* t, const orc_view* v, const orc_params* p, uint32_t* words,
                                uint32_t* keys, double* margin, double* eye_margin);
 void orc_encode_center(const double* c, uint32_t* dir, uint32_t* dist, uint32_t* low); /* projection.tsl.ts:317-333 */

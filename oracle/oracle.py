@@ -246,10 +246,11 @@ class OrcRaySplat(C.Structure):
 class OrcRayOut(C.Structure):
     _fields_ = [("words", C.c_uint32 * 10), ("visible", C.c_int), ("invisible", C.c_int), ("guarded", C.c_int),
                 ("eye_visible", C.c_int * 2), ("key", C.c_uint32), ("distance", _d), ("head_scales", _d * 3),
-                ("head_opacity", _d), ("bound", (_d * 5) * 2), ("eye_margin", _d * 2), ("margin", _d)]
+                ("head_opacity", _d), ("bound", (_d * 5) * 2), ("frame_axes", ((_d * 3) * 2) * 2), ("frame_c2", _d * 2),
+                ("frame_lengths", (_d * 2) * 2), ("eye_margin", _d * 2), ("margin", _d)]
 
 
-METHODS = {"3dgut": 1, "htgs": 2}
+METHODS = {"3dgut": 1, "htgs": 2, "raygs": 3}
 
 
 def project_ray(method: str, center, scales, quat, rgb, opacity, view: OrcView, prm: OrcParams) -> OrcRayOut:
@@ -261,7 +262,7 @@ def project_ray(method: str, center, scales, quat, rgb, opacity, view: OrcView, 
 
 
 def project_ray_f32(method: str, planes: np.ndarray, sh_degree: int, view: OrcView, prm: OrcParams):
-    """The ray oracle over an f32-planes table: words (n, 6 + 2 eyes), keys, margins (n,) and (eyes, n)."""
+    """The ray oracle over an f32-planes table: words (n, 6 + 2 eyes; raygs 4 + 3 eyes), keys, margins (n,) and (eyes, n)."""
     n = planes.shape[1]
     eyes = view.eyes
     planes = np.ascontiguousarray(planes, dtype=np.float32)
@@ -269,7 +270,7 @@ def project_ray_f32(method: str, planes: np.ndarray, sh_degree: int, view: OrcVi
     st = n * 4
     t = OrcTableF32(n, n, base, base + 3 * st, base + 6 * st, base + 10 * st, base + 11 * st,
                     base + 14 * st if sh_degree > 0 else None, sh_degree)
-    words = np.zeros((n, 6 + 2 * eyes), dtype=np.uint32)
+    words = np.zeros((n, (4 + 3 * eyes) if method == "raygs" else (6 + 2 * eyes)), dtype=np.uint32)
     keys = np.zeros(n, dtype=np.uint32)
     margin = np.zeros(n, dtype=np.float64)
     eye_margin = np.zeros((eyes, n), dtype=np.float64)

@@ -1,8 +1,10 @@
 /*
- * Oracle: the 3dgut and htgs projection words of one splat, restated in
- * double from the reference's TSL: SplatProjection.project / write
- * (src/render/SplatProjection.ts:130-151), headBasis, gutBound, htgsBound
- * and rayProjection.project (src/render/projections.tsl.ts:170-300), and
+ * Oracle: the 3dgut, htgs and raygs projection words of one splat, restated
+ * in double from the reference's TSL: SplatProjection.project / write
+ * (src/render/SplatProjection.ts:130-151), headBasis, gutBound, htgsBound,
+ * rayProjection.project, quadOffScreen, raygsFrame and raygs.project
+ * (src/render/projections.tsl.ts:170-429), basisToQuat
+ * (src/gpu/math.tsl.ts:116-148), and
  * the helpers of src/render/projection.tsl.ts (prepareSplat 42-64,
  * cullInvisible 66-69, splatInView 112-130, cullDepth 132-135,
  * inSortGuard 145-155, viewBasis 169-174, mipInflate 177-197,
@@ -318,6 +320,131 @@ static int htgs_bound(const orc_eye* E, const orc_view* v, const double* c, cons
   return ok;
 }
 
+/* basisToQuat (math.tsl.ts:116-148): the rotation with columns x, y, z */
+static void basis_to_quat(const double* x, const double* y, const double* z, double* q) {
+  double four, inv;
+  if (z[2] <= 0) {
+    double dif10 = y[1] - x[0], omm22 = 1.0 - z[2];
+    if (dif10 <= 0) {
+      four = omm22 - dif10;
+      inv = 0.5 / sqrt(fmax(four, 1e-30));
+      q[0] = four * inv, q[1] = (x[1] + y[0]) * inv, q[2] = (x[2] + z[0]) * inv, q[3] = (y[2] - z[1]) * inv;
+    } else {
+      four = omm22 + dif10;
+      inv = 0.5 / sqrt(fmax(four, 1e-30));
+      q[0] = (x[1] + y[0]) * inv, q[1] = four * inv, q[2] = (y[2] + z[1]) * inv, q[3] = (z[0] - x[2]) * inv;
+    }
+  } else {
+    double sum10 = y[1] + x[0], opm22 = 1.0 + z[2];
+    if (sum10 <= 0) {
+      four = opm22 - sum10;
+      inv = 0.5 / sqrt(fmax(four, 1e-30));
+      q[0] = (x[2] + z[0]) * inv, q[1] = (y[2] + z[1]) * inv, q[2] = four * inv, q[3] = (x[1] - y[0]) * inv;
+    } else {
+      four = opm22 + sum10;
+      inv = 0.5 / sqrt(fmax(four, 1e-30));
+      q[0] = (y[2] - z[1]) * inv, q[1] = (z[0] - x[2]) * inv, q[2] = (x[1] - y[0]) * inv, q[3] = four * inv;
+    }
+  }
+}
+
+/* quadOffScreen (projections.tsl.ts:335-352) */
+static int quad_off_screen(const double* P, const double* mu, const double* a0, const double* a1, double* margin) {
+  double c[4], A0[4], A1[4];
+  point_of(P, mu, 1.0, c);
+  point_of(P, a0, 0.0, A0);
+  point_of(P, a1, 0.0, A1);
+  int front = 1, o[4] = {1, 1, 1, 1};
+  for (int i = 0; i < 4; i++) {
+    double s0 = (i & 1) ? 1.0 : -1.0, s1 = (i & 2) ? 1.0 : -1.0;
+    double x = c[0] + s0 * A0[0] + s1 * A1[0], y = c[1] + s0 * A0[1] + s1 * A1[1], w = c[3] + s0 * A0[3] + s1 * A1[3];
+    front = front && w > 0;
+    o[0] = o[0] && x < -w;
+    o[1] = o[1] && x > w;
+    o[2] = o[2] && y < -w;
+    o[3] = o[3] && y > w;
+    *margin = fmin(*margin, fmin(fabs(fabs(x) - fabs(w)), fabs(fabs(y) - fabs(w))) / fmax(fabs(w), 1e-30));
+  }
+  return front && (o[0] || o[1] || o[2] || o[3]);
+}
+
+/* raygsFrame (projections.tsl.ts:354-396) and the eye words; fills the frame axes, c^2, lengths */
+static int raygs_eye(const orc_eye* E, const double* c, const double* q, const double* s, double std_dev, uint32_t* w,
+                     double axes[2][3], double* c2_out, double* lengths, double* margin) {
+  double kappa = std_dev * std_dev;
+  double qc[4] = {-q[0], -q[1], -q[2], q[3]}, lc[3];
+  qrot(qc, c, lc);
+  for (int k = 0; k < 3; k++) lc[k] /= s[k];
+  double c2 = lc[0] * lc[0] + lc[1] * lc[1] + lc[2] * lc[2];
+  int ok = 1;
+  double mu[3];
+  if (E->ortho) {
+    double dz[3] = {0, 0, -1}, d[3];
+    qrot(qc, dz, d);
+    for (int k = 0; k < 3; k++) d[k] /= s[k];
+    double n = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    for (int k = 0; k < 3; k++) mu[k] = d[k] / n;
+    c2 = 1e18;
+  } else {
+    ok = c2 > kappa * 1.0001;
+    *margin = fmin(*margin, rel_margin(c2, kappa * 1.0001));
+    for (int k = 0; k < 3; k++) mu[k] = lc[k] / sqrt(c2);
+  }
+  double t[3] = {mu[0], mu[1], mu[2] + (mu[2] >= 0 ? 1.0 : -1.0)};
+  double inv_den = 2.0 / (t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+  double v0[3] = {s[0] * (t[0] * t[0] * inv_den - 1.0), s[1] * (t[1] * t[0] * inv_den), s[2] * (t[2] * t[0] * inv_den)};
+  double v1[3] = {s[0] * (t[0] * t[1] * inv_den), s[1] * (t[1] * t[1] * inv_den - 1.0), s[2] * (t[2] * t[1] * inv_den)};
+  double q0[3], q1[3];
+  qrot(q, v0, q0);
+  qrot(q, v1, q1);
+  double bxx = q0[0] * q0[0] + q0[1] * q0[1] + q0[2] * q0[2];
+  double bxy = q0[0] * q1[0] + q0[1] * q1[1] + q0[2] * q1[2];
+  double byy = q1[0] * q1[0] + q1[1] * q1[1] + q1[2] * q1[2];
+  double diff = bxx - byy;
+  double eig1 = (bxx + byy) * 0.5 + sqrt(fmax(0.0, diff * diff * 0.25 + bxy * bxy));
+  double vx, vy;
+  if (fabs(bxx - eig1) >= fabs(byy - eig1)) {
+    vx = bxy;
+    vy = eig1 - bxx;
+  } else {
+    vx = eig1 - byy;
+    vy = bxy;
+  }
+  double vl = sqrt(vx * vx + vy * vy);
+  if (vl > 0) {
+    vx /= vl;
+    vy /= vl;
+  } else {
+    vx = 1;
+    vy = 0;
+  }
+  double u0x = -vy, u0y = vx;
+  double a0[3], a1[3];
+  for (int k = 0; k < 3; k++) {
+    a0[k] = q0[k] * u0x + q1[k] * u0y;
+    a1[k] = q0[k] * vx + q1[k] * vy;
+  }
+  double lx = sqrt(a0[0] * a0[0] + a0[1] * a0[1] + a0[2] * a0[2]), ly = sqrt(a1[0] * a1[0] + a1[1] * a1[1] + a1[2] * a1[2]);
+  ok = ok && lx > 0 && ly > 0;
+  double zs = std_dev / sqrt(fmax(1.0 - kappa / c2, 1e-4));
+  double za0[3] = {a0[0] * zs, a0[1] * zs, a0[2] * zs}, za1[3] = {a1[0] * zs, a1[1] * zs, a1[2] * zs};
+  ok = ok && !quad_off_screen(E->projection, c, za0, za1, margin);
+  memcpy(axes[0], a0, sizeof a0);
+  memcpy(axes[1], a1, sizeof a1);
+  *c2_out = c2;
+  lengths[0] = lx;
+  lengths[1] = ly;
+  if (!ok) return 0;
+  double n0[3] = {a0[0] / lx, a0[1] / lx, a0[2] / lx}, n1[3] = {a1[0] / ly, a1[1] / ly, a1[2] / ly};
+  double n2[3] = {n0[1] * n1[2] - n0[2] * n1[1], n0[2] * n1[0] - n0[0] * n1[2], n0[0] * n1[1] - n0[1] * n1[0]};
+  double fq[4];
+  basis_to_quat(n0, n1, n2, fq);
+  w[0] = orc_encode_quat_oct32_gpu(fq);
+  w[1] = float_bits(c2);
+  w[2] = orc_encode_half(log(lx)) | (orc_encode_half(log(ly)) << 16);
+  return 1;
+}
+
 void orc_project_ray(int method, const orc_ray_splat* src, const orc_view* v, const orc_params* p, orc_ray_out* out) {
   memset(out, 0, sizeof *out);
   double margin = INFINITY;
@@ -401,6 +528,16 @@ void orc_project_ray(int method, const orc_ray_splat* src, const orc_view* v, co
       quat_of_matrix(E->from_head, 1.0, fq);
       qmul(fq, hq, eq);
       qnorm(eq);
+      if (method == 3) {
+        if (raygs_eye(E, c, eq, S, std_dev, out->words + 4 + 3 * e, out->frame_axes[e], &out->frame_c2[e],
+                      out->frame_lengths[e], &em)) {
+          out->eye_visible[e] = 1;
+          out->visible = 1;
+        }
+        out->eye_margin[e] = em;
+        margin = fmin(margin, em);
+        continue;
+      }
       bound_t bd;
       int bok;
       if (method == 1) {
@@ -437,9 +574,13 @@ void orc_project_ray(int method, const orc_ray_splat* src, const orc_view* v, co
     out->words[0] = dir;
     out->words[1] = dist;
     out->words[2] = orc_encode_ext_rgb(src->rgb[0], src->rgb[1], src->rgb[2]);
-    out->words[3] = orc_encode_quat_oct32_gpu(hq);
-    out->words[4] = orc_encode_half(log(S[0])) | (orc_encode_half(log(S[1])) << 16);
-    out->words[5] = orc_encode_half(log(S[2])) | (opacity8(stored) << 16) | low;
+    if (method == 3) {
+      out->words[3] = opacity8(stored) | low;
+    } else {
+      out->words[3] = orc_encode_quat_oct32_gpu(hq);
+      out->words[4] = orc_encode_half(log(S[0])) | (orc_encode_half(log(S[1])) << 16);
+      out->words[5] = orc_encode_half(log(S[2])) | (opacity8(stored) << 16) | low;
+    }
   }
   (void)W;
 
@@ -471,7 +612,7 @@ void orc_project_ray_table_f32(int method, const orc_table_f32* t, const orc_vie
                                uint32_t* keys, double* margin, double* eye_margin) {
   const uint32_t n = t->count;
   const size_t st = t->stride ? t->stride : n;
-  const int W = 6 + 2 * v->eyes;
+  const int W = method == 3 ? 4 + 3 * v->eyes : 6 + 2 * v->eyes;
   for (uint32_t i = 0; i < n; i++) {
     orc_ray_splat s;
     double coeffs[45];

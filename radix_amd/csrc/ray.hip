@@ -102,9 +102,9 @@ extern "C" {
 int rdx_project_ray(rdx_context* ctx, void* stream, const rdx_splat_table* table, const rdx_camera* camera,
                     const rdx_params* params, int method, uint32_t at, uint32_t* const* channels_dev, uint32_t* keys_dev) {
   if (!camera || !params || !channels_dev) return fail(RDX_E_INVALID, "rdx_project_ray: NULL camera, params or channels");
-  if (method != RDX_METHOD_3DGUT && method != RDX_METHOD_HTGS)
-    return fail(method == RDX_METHOD_RAYGS || method == RDX_METHOD_EWA ? RDX_E_UNSUPPORTED : RDX_E_INVALID,
-                "rdx_project_ray: method %d is not 3dgut (1) or htgs (2)", method);
+  if (method != RDX_METHOD_3DGUT && method != RDX_METHOD_HTGS && method != RDX_METHOD_RAYGS)
+    return fail(method == RDX_METHOD_EWA ? RDX_E_UNSUPPORTED : RDX_E_INVALID,
+                "rdx_project_ray: method %d is not 3dgut (1), htgs (2) or raygs (3) (ewa: rdx_project_ewa)", method);
   if (int rc = check_table(table)) return rc;
   if (int rc = check_bits(params->sort_bits, "SplatProjection")) return rc;
   RayU r;
@@ -117,8 +117,8 @@ int rdx_project_ray(rdx_context* ctx, void* stream, const rdx_splat_table* table
   const double s = uniform_scale(r.u.H);
   if (!(s > 0.0))
     return fail(RDX_E_UNSUPPORTED,
-                "rdx_project_ray: the object transform must be a rotation times a uniform scale (3dgut and htgs read scales "
-                "and rotations)");
+                "rdx_project_ray: the object transform must be a rotation times a uniform scale (the ray methods read "
+                "scales and rotations)");
   r.oscale = (float)s;
   double R[9];
   for (int k = 0; k < 9; k++) R[k] = r.u.H[k] / s;
@@ -137,13 +137,17 @@ int rdx_project_ray(rdx_context* ctx, void* stream, const rdx_splat_table* table
   const TableP tp = make_table(table);
   const dim3 grid((uint32_t)(((uint64_t)n + 255) / 256));
   hipStream_t st = (hipStream_t)stream;
-  Stage stage(reinterpret_cast<Context*>(ctx), st, method == RDX_METHOD_3DGUT ? "project_3dgut" : "project_htgs");
+  Stage stage(reinterpret_cast<Context*>(ctx), st,
+              method == RDX_METHOD_3DGUT ? "project_3dgut" : (method == RDX_METHOD_HTGS ? "project_htgs" : "project_raygs"));
   if (method == RDX_METHOD_3DGUT) {
     if (camera->eyes == 2) launch_layout<METHOD_3DGUT, 2>(st, grid, table, tp, r, o);
     else launch_layout<METHOD_3DGUT, 1>(st, grid, table, tp, r, o);
-  } else {
+  } else if (method == RDX_METHOD_HTGS) {
     if (camera->eyes == 2) launch_layout<METHOD_HTGS, 2>(st, grid, table, tp, r, o);
     else launch_layout<METHOD_HTGS, 1>(st, grid, table, tp, r, o);
+  } else {
+    if (camera->eyes == 2) launch_layout<METHOD_RAYGS, 2>(st, grid, table, tp, r, o);
+    else launch_layout<METHOD_RAYGS, 1>(st, grid, table, tp, r, o);
   }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? RDX_OK : hip_fail(e, "rdx_project_ray");

@@ -1,10 +1,12 @@
-// The ray-evaluated projection methods 3dgut and htgs (the projection words
-// of src/render/projections.tsl.ts:170-328 with the helpers of
-// projection.tsl.ts:169-275, 312-360): per splat, six head words (the
-// head-space centre as an octahedral direction and a float distance, the
-// colour, the head-space rotation, the MIP-inflated scales as half floats,
-// the opacity) and two words per eye (a screen bound: ndc 2x16, a 10-bit
-// angle and two 11-bit log2 pixel scales rounded outward).
+// The ray-evaluated projection methods (the projection words of
+// src/render/projections.tsl.ts:170-429 with the helpers of
+// projection.tsl.ts:169-275, 312-360). 3dgut and htgs: per splat, six head
+// words (the head-space centre as an octahedral direction and a float
+// distance, the colour, the head-space rotation, the MIP-inflated scales as
+// half floats, the opacity) and two words per eye (a screen bound: ndc
+// 2x16, a 10-bit angle and two 11-bit log2 pixel scales rounded outward).
+// raygs: four head words (centre, colour, opacity) and three per eye (the
+// frame of a quad facing the eye, c^2, its two axis lengths).
 #pragma once
 
 #include <hip/hip_fp16.h>
@@ -301,6 +303,118 @@ __device__ __forceinline__ uint32_t encode_quat_oct32(float4 q) {
   return oct | (acode << 20);
 }
 
+// ---- raygs: a tilted 3D quad facing the eye (projections.tsl.ts:333-429) ----------
+
+// basisToQuat (math.tsl.ts:116-148): the rotation with columns x, y, z
+__device__ __forceinline__ float4 basis_to_quat(float3 x, float3 y, float3 z) {
+  float4 q;
+  if (z.z <= 0.0f) {
+    const float dif10 = y.y - x.x, omm22 = 1.0f - z.z;
+    if (dif10 <= 0.0f) {
+      const float four = omm22 - dif10, inv = rsqrtf(fmaxf(four, 1e-30f)) * 0.5f;
+      q = make_float4(four * inv, (x.y + y.x) * inv, (x.z + z.x) * inv, (y.z - z.y) * inv);
+    } else {
+      const float four = omm22 + dif10, inv = rsqrtf(fmaxf(four, 1e-30f)) * 0.5f;
+      q = make_float4((x.y + y.x) * inv, four * inv, (y.z + z.y) * inv, (z.x - x.z) * inv);
+    }
+  } else {
+    const float sum10 = y.y + x.x, opm22 = 1.0f + z.z;
+    if (sum10 <= 0.0f) {
+      const float four = opm22 - sum10, inv = rsqrtf(fmaxf(four, 1e-30f)) * 0.5f;
+      q = make_float4((x.z + z.x) * inv, (y.z + z.y) * inv, four * inv, (x.y - y.x) * inv);
+    } else {
+      const float four = opm22 + sum10, inv = rsqrtf(fmaxf(four, 1e-30f)) * 0.5f;
+      q = make_float4((y.z - z.y) * inv, (z.x - x.z) * inv, (x.y - y.x) * inv, four * inv);
+    }
+  }
+  return q;
+}
+
+// quadOffScreen: every corner mu +- a0 +- a1 in front of the camera and
+// all four beyond the same side of the frustum
+__device__ __forceinline__ bool quad_off_screen(const float* P, float3 mu, float3 a0, float3 a1) {
+  const float4 c = clip_point(P, mu, 1.0f), A0 = clip_point(P, a0, 0.0f), A1 = clip_point(P, a1, 0.0f);
+  bool front = true, o0 = true, o1 = true, o2 = true, o3 = true;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const float s0 = (i & 1) ? 1.0f : -1.0f, s1 = (i & 2) ? 1.0f : -1.0f;
+    const float x = c.x + s0 * A0.x + s1 * A1.x, y = c.y + s0 * A0.y + s1 * A1.y, w = c.w + s0 * A0.w + s1 * A1.w;
+    front = front && w > 0.0f;
+    o0 = o0 && x < -w;
+    o1 = o1 && x > w;
+    o2 = o2 && y < -w;
+    o3 = o3 && y > w;
+  }
+  return front && (o0 || o1 || o2 || o3);
+}
+
+// raygsFrame and the eye words [frame QuatOct32, float c^2, half2 ln(axis
+// lengths)]; false when the camera is inside the splat's support or the
+// quad is off screen
+__device__ __forceinline__ bool raygs_eye(const EyeU& E, float3 c, float4 q, float3 s, float std_dev, uint32_t* out) {
+  const float kappa = std_dev * std_dev;
+  const float4 qc = make_float4(-q.x, -q.y, -q.z, q.w);
+  float3 lc = quat_rotate(qc, c);
+  lc = make_float3(lc.x / s.x, lc.y / s.y, lc.z / s.z);
+  float c2 = lc.x * lc.x + lc.y * lc.y + lc.z * lc.z;
+  bool ok = true;
+  float3 mu;
+  if (E.ortho) {
+    float3 d = quat_rotate(qc, make_float3(0.0f, 0.0f, -1.0f));
+    d = make_float3(d.x / s.x, d.y / s.y, d.z / s.z);
+    const float r = rsqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+    mu = make_float3(d.x * r, d.y * r, d.z * r);
+    c2 = 1e18f;
+  } else {
+    ok = c2 > kappa * 1.0001f;
+    const float r = rsqrtf(c2);
+    mu = make_float3(lc.x * r, lc.y * r, lc.z * r);
+  }
+  // orthonormal basis of the plane orthogonal to mu (Householder)
+  const float3 t = make_float3(mu.x, mu.y, mu.z + (mu.z >= 0.0f ? 1.0f : -1.0f));
+  const float inv_den = 2.0f / (t.x * t.x + t.y * t.y + t.z * t.z);
+  const float fx = t.x * inv_den, fy = t.y * inv_den;
+  const float3 q0 = quat_rotate(q, make_float3(s.x * (t.x * fx - 1.0f), s.y * (t.y * fx), s.z * (t.z * fx)));
+  const float3 q1 = quat_rotate(q, make_float3(s.x * (t.x * fy), s.y * (t.y * fy - 1.0f), s.z * (t.z * fy)));
+  // major eigenvector of Q^T Q; u0 is it rotated 90 degrees
+  const float bxx = q0.x * q0.x + q0.y * q0.y + q0.z * q0.z;
+  const float bxy = q0.x * q1.x + q0.y * q1.y + q0.z * q1.z;
+  const float byy = q1.x * q1.x + q1.y * q1.y + q1.z * q1.z;
+  const float diff = bxx - byy;
+  const float eig1 = (bxx + byy) * 0.5f + sqrtf(fmaxf(0.0f, diff * diff * 0.25f + bxy * bxy));
+  float vx, vy;
+  if (fabsf(bxx - eig1) >= fabsf(byy - eig1)) {
+    vx = bxy;
+    vy = eig1 - bxx;
+  } else {
+    vx = eig1 - byy;
+    vy = bxy;
+  }
+  const float vl = sqrtf(vx * vx + vy * vy);
+  if (vl > 0.0f) {
+    vx /= vl;
+    vy /= vl;
+  } else {
+    vx = 1.0f;
+    vy = 0.0f;
+  }
+  const float u0x = -vy, u0y = vx;
+  const float3 a0 = make_float3(q0.x * u0x + q1.x * u0y, q0.y * u0x + q1.y * u0y, q0.z * u0x + q1.z * u0y);
+  const float3 a1 = make_float3(q0.x * vx + q1.x * vy, q0.y * vx + q1.y * vy, q0.z * vx + q1.z * vy);
+  const float lx = sqrtf(a0.x * a0.x + a0.y * a0.y + a0.z * a0.z), ly = sqrtf(a1.x * a1.x + a1.y * a1.y + a1.z * a1.z);
+  ok = ok && lx > 0.0f && ly > 0.0f;
+  const float zs = std_dev * rsqrtf(fmaxf(1.0f - kappa / c2, 1e-4f));
+  ok = ok && !quad_off_screen(E.Pf, c, make_float3(a0.x * zs, a0.y * zs, a0.z * zs),
+                              make_float3(a1.x * zs, a1.y * zs, a1.z * zs));
+  if (!ok) return false;
+  const float3 n0 = make_float3(a0.x / lx, a0.y / lx, a0.z / lx), n1 = make_float3(a1.x / ly, a1.y / ly, a1.z / ly);
+  const float3 n2 = make_float3(n0.y * n1.z - n0.z * n1.y, n0.z * n1.x - n0.x * n1.z, n0.x * n1.y - n0.y * n1.x);
+  out[0] = encode_quat_oct32(basis_to_quat(n0, n1, n2));
+  out[1] = fbits(c2);
+  out[2] = half_bits(logf(lx)) | (half_bits(logf(ly)) << 16);
+  return true;
+}
+
 // ---- the kernel ----------------------------------------------------------------
 
 // One splat per thread: prepareSplat, the head basis with MIP inflation
@@ -373,9 +487,12 @@ __global__ __launch_bounds__(256) void k_project_ray(TableP t, RayU r, RayOut o)
   }
   const bool ok = !invisible && oph >= u.min_alpha;
 
-  uint32_t w[6 + 2 * EYES];
+  // words: 6 head + 2 per eye (3dgut, htgs) or 4 head + 3 per eye (raygs)
+  constexpr int HEAD = METHOD == METHOD_RAYGS ? 4 : 6;
+  constexpr int EW = METHOD == METHOD_RAYGS ? 3 : 2;
+  uint32_t w[10];
 #pragma unroll
-  for (int k = 0; k < 6 + 2 * EYES; k++) w[k] = 0u;
+  for (int k = 0; k < 10; k++) w[k] = 0u;
   bool visible = false;
 #pragma unroll
   for (int e = 0; e < EYES; e++) {
@@ -386,19 +503,24 @@ __global__ __launch_bounds__(256) void k_project_ray(TableP t, RayU r, RayOut o)
     if (ok && !cull_depth) {
       const float3 c = make_float3((float)C.ce[0], (float)C.ce[1], z);
       const float4 eq = quat_normalize(quat_mul(make_float4(r.eq[e][0], r.eq[e][1], r.eq[e][2], r.eq[e][3]), qn));
-      RayBound bd;
-      bool bok;
-      if constexpr (METHOD == METHOD_3DGUT) {
-        bok = gut_bound(E.Pf, u.size, c, eq, S, std_dev, u, bd);
+      if constexpr (METHOD == METHOD_RAYGS) {
+        if (raygs_eye(E, c, eq, S, std_dev, w + HEAD + EW * e)) visible = true;
       } else {
-        const float2 ndc = C.clip[3] > 0.0f ? make_float2(C.clip[0] / C.clip[3], C.clip[1] / C.clip[3]) : make_float2(0.0f, 0.0f);
-        bok = htgs_bound(E.Pf, u.size, c, eq, S, std_dev, ndc, u, bd);
-      }
-      if (bok && clip_bound_to_screen(u.size, bd)) {
-        const uint2 bw = encode_bound(bd);
-        w[6 + 2 * e] = bw.x;
-        w[7 + 2 * e] = bw.y;
-        visible = true;
+        RayBound bd;
+        bool bok;
+        if constexpr (METHOD == METHOD_3DGUT) {
+          bok = gut_bound(E.Pf, u.size, c, eq, S, std_dev, u, bd);
+        } else {
+          const float2 ndc =
+              C.clip[3] > 0.0f ? make_float2(C.clip[0] / C.clip[3], C.clip[1] / C.clip[3]) : make_float2(0.0f, 0.0f);
+          bok = htgs_bound(E.Pf, u.size, c, eq, S, std_dev, ndc, u, bd);
+        }
+        if (bok && clip_bound_to_screen(u.size, bd)) {
+          const uint2 bw = encode_bound(bd);
+          w[HEAD + EW * e] = bw.x;
+          w[HEAD + EW * e + 1] = bw.y;
+          visible = true;
+        }
       }
     }
   }
@@ -409,9 +531,13 @@ __global__ __launch_bounds__(256) void k_project_ray(TableP t, RayU r, RayOut o)
     w[0] = dir;
     w[1] = dist;
     w[2] = encode_ext_rgb(rgb.x, rgb.y, rgb.z);
-    w[3] = encode_quat_oct32(hq);
-    w[4] = half_bits(logf(S.x)) | (half_bits(logf(S.y)) << 16);
-    w[5] = half_bits(logf(S.z)) | (encode_opacity8(stored) << 16) | low;
+    if constexpr (METHOD == METHOD_RAYGS) {
+      w[3] = encode_opacity8(stored) | low;
+    } else {
+      w[3] = encode_quat_oct32(hq);
+      w[4] = half_bits(logf(S.x)) | (half_bits(logf(S.y)) << 16);
+      w[5] = half_bits(logf(S.z)) | (encode_opacity8(stored) << 16) | low;
+    }
   }
   const size_t at = (size_t)o.at + i;
   reinterpret_cast<uint4*>(o.ch[0])[at] = make_uint4(w[0], w[1], w[2], w[3]);

@@ -194,12 +194,12 @@ class Projector:
                                          C.byref(rig.desc()), C.byref(params.desc()), at, _ptr(keys)))
         return keys[at:at + n]
 
-    METHODS = {"3dgut": 1, "htgs": 2}
+    METHODS = {"3dgut": 1, "htgs": 2, "raygs": 3}
 
     def project_ray(self, table: DeviceTable, rig: CameraRig, params: Params, method: str, keys: bool = True,
                     stream=None):
-        """The 3dgut or htgs words (SplatProjection.write, src/render/SplatProjection.ts:141-151):
-        {"channels": [w0 (n, 4), w1 (n, 4)(, w2 (n, 2))] int32, "words": (n, 6 + 2 eyes) view, "keys"}."""
+        """The 3dgut, htgs or raygs words (SplatProjection.write, src/render/SplatProjection.ts:141-151):
+        {"channels": [w0 (n, 4), w1 (n, 4)(, w2 (n, 2))] int32, "words": (n, 6 + 2 eyes; raygs 4 + 3 eyes), "keys"}."""
         import torch
 
         if method not in self.METHODS:
@@ -215,7 +215,7 @@ class Projector:
         cp = (C.c_void_p * 3)(*[c.data_ptr() for c in chans] + [None] * (3 - len(chans)))
         N.check(N.lib().rdx_project_ray(self.context.handle, _stream(stream), C.byref(table.desc()), C.byref(rig.desc()),
                                         C.byref(params.desc()), self.METHODS[method], 0, cp, _ptr(kd) if keys else None))
-        words = torch.cat(chans, dim=1)[:n, : 6 + 2 * eyes]
+        words = torch.cat(chans, dim=1)[:n, : (4 + 3 * eyes) if method == "raygs" else (6 + 2 * eyes)]
         return {"channels": [c[:n] for c in chans], "words": words, "keys": kd[:n] if keys else None}
 
     def project_sort(self, table: DeviceTable, rig: CameraRig, params: Params, words=None, keys=None, indices=None,

@@ -1,4 +1,4 @@
-"""The oracle's 3dgut and htgs projection words (oracle/ray.c), pinned (no GPU).
+"""The oracle's 3dgut, htgs and raygs projection words (oracle/ray.c), pinned (no GPU).
 
 Pins: closed forms of the unscented transform for an axis-aligned ellipsoid
 seen head-on (sigma points c +- sqrt(3) s_i e_i project to +-sqrt(3) s_i f /
@@ -10,7 +10,14 @@ the ray of (x - c)^T Sigma^-1 (x - c) equals stdDev^2); rotation invariance
 of a sphere; the MIP inflation's closed form for a sphere (scales sqrt(s^2 +
 blur fp), opacity times s^2 / (s^2 + blur fp), fp the pixel footprint); the
 centre and rotation codecs' round trips; the screen bound when the camera is
-inside the ellipsoid; and the culls.
+inside the ellipsoid; and the culls. raygs's quad frame by what defines
+it: its two axes are orthonormal in the ellipsoid's metric (Sigma^-1) and
+conjugate to the centre (axis^T Sigma^-1 c = 0: the plane through the centre
+facing the eye), orthogonal to each other (the principal axes of the section),
+the second the longer; c^2 the centre's squared Mahalanobis distance; for a
+sphere both lengths are its scale; the frame rotation's columns are the
+normalised axes and their cross product; the camera inside the support and a
+quad off screen are culled.
 """
 import math
 
@@ -205,3 +212,65 @@ def test_culls():
     # a visible splat's key is the head distance's
     o4 = O.project_ray("htgs", [0, 0, -3.0], [0.2] * 3, [0, 0, 0, 1], [0.5] * 3, 0.9, view, params())
     assert o4.visible and o4.key == O.key_compute(3.0, 32)
+
+
+def sigma_inv(scales, quat):
+    R = Rotation.from_quat(quat).as_matrix()
+    return R @ np.diag(1.0 / np.asarray(scales, dtype=np.float64) ** 2) @ R.T
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_raygs_frame_invariants(seed):
+    _, view = rig_and_view()
+    rng = np.random.default_rng(100 + seed)
+    center = np.array([rng.uniform(-1.5, 1.5), rng.uniform(-1.0, 1.0), -rng.uniform(3.0, 9.0)])
+    scales = rng.uniform(0.05, 0.4, 3)
+    quat = Rotation.random(random_state=100 + seed).as_quat()
+    o = O.project_ray("raygs", center, scales, quat, [0.5] * 3, 0.9, view, params())
+    assert o.eye_visible[0] and o.visible
+    A = sigma_inv(scales, quat)
+    a0, a1 = np.array(o.frame_axes[0][0][:]), np.array(o.frame_axes[0][1][:])
+    G = np.array([[a0 @ A @ a0, a0 @ A @ a1], [a1 @ A @ a0, a1 @ A @ a1]])
+    assert np.allclose(G, np.eye(2), atol=1e-10)
+    assert abs(a0 @ A @ center) < 1e-9 * np.linalg.norm(A @ center) and abs(a1 @ A @ center) < 1e-9 * np.linalg.norm(A @ center)
+    assert abs(a0 @ a1) < 1e-10 * np.linalg.norm(a0) * np.linalg.norm(a1)
+    lx, ly = o.frame_lengths[0][:]
+    assert lx == pytest.approx(np.linalg.norm(a0), rel=1e-12) and ly >= lx
+    assert o.frame_c2[0] == pytest.approx(center @ A @ center, rel=1e-12)
+    # the words: c^2 as float bits, the lengths as half-float logs, the frame rotation
+    w = list(o.words)
+    assert np.float32(np.array([w[5]], dtype=np.uint32).view(np.float32)[0]) == np.float32(center @ A @ center)
+    assert O.lib().orc_decode_half(w[6] & 0xFFFF) == pytest.approx(math.log(lx), abs=2e-3)
+    assert O.lib().orc_decode_half(w[6] >> 16) == pytest.approx(math.log(ly), abs=2e-3)
+    q = np.zeros(4)
+    O.lib().orc_decode_quat_oct(w[4], 10, 12, O.dptr(q))
+    M = Rotation.from_quat(q).as_matrix()
+    n0, n1 = a0 / lx, a1 / ly
+    for col, ref in ((0, n0), (1, n1), (2, np.cross(n0, n1))):
+        assert np.linalg.norm(M[:, col] - ref) < 1e-2  # the codec's 10-bit axis, 12-bit angle
+
+
+def test_raygs_sphere():
+    _, view = rig_and_view()
+    c, s = np.array([0.4, -0.2, -5.0]), 0.3
+    o = O.project_ray("raygs", c, [s, s, s], [0, 0, 0, 1], [0.5] * 3, 0.9, view, params())
+    assert o.frame_lengths[0][0] == pytest.approx(s, rel=1e-12) and o.frame_lengths[0][1] == pytest.approx(s, rel=1e-12)
+    assert o.frame_c2[0] == pytest.approx(c @ c / (s * s), rel=1e-12)
+    # the quad faces the eye: its normal along the centre direction
+    a0, a1 = np.array(o.frame_axes[0][0][:]), np.array(o.frame_axes[0][1][:])
+    nrm = np.cross(a0, a1)
+    assert abs(abs(nrm @ c) / (np.linalg.norm(nrm) * np.linalg.norm(c)) - 1.0) < 1e-12
+
+
+def test_raygs_culls():
+    _, view = rig_and_view()
+    # the camera inside the support (c^2 <= stdDev^2): no eye words, head words kept
+    o = O.project_ray("raygs", [0, 0, -1.0], [0.5] * 3, [0, 0, 0, 1], [0.5] * 3, 0.9, view, params())
+    assert not o.visible and o.words[0] != 0 and o.words[4] == 0 and o.words[5] == 0
+    # far off to the side: every corner beyond the right edge
+    o = O.project_ray("raygs", [40.0, 0, -5.0], [0.2] * 3, [0, 0, 0, 1], [0.5] * 3, 0.9, view, params())
+    assert not o.visible and o.words[5] == 0
+    # head words: opacity byte and the centre's low bits share word 3
+    o = O.project_ray("raygs", [0.3, 0.2, -4.0], [0.2] * 3, [0, 0, 0, 1], [0.5] * 3, 0.9, view, params())
+    assert o.visible and (o.words[3] & 0xFF) == max(1, int(math.floor(o.head_opacity * 128 + 0.5)))
+    assert o.words[3] & 0x00FFFF00 == 0

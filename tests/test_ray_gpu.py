@@ -1,5 +1,5 @@
-"""GPU 3dgut and htgs projection words (rdx_project_ray) against the FP64
-oracle (oracle/ray.c, the reference's projections.tsl.ts:170-300 restated)
+"""GPU 3dgut, htgs and raygs projection words (rdx_project_ray) against the FP64
+oracle (oracle/ray.c, the reference's projections.tsl.ts:170-429 restated)
 on the same tables: f32 planes with SH, two eyes, a uniformly scaled and
 rotated object, ext words, and the keys with cullKeys and the sortGuard.
 
@@ -104,6 +104,55 @@ def compare(g, ref, eyes, scale_frac=0.0005, scale_codes=1):
     return rep
 
 
+def quat_matrix(code):
+    from scipy.spatial.transform import Rotation
+
+    q = np.zeros(4)
+    O.lib().orc_decode_quat_oct(int(code), 10, 12, O.dptr(q))
+    return Rotation.from_quat(q).as_matrix()
+
+
+def compare_raygs(g, ref, eyes, frac=0.0005):
+    """raygs: head [dir, distance, rgb, opacity8 | low], eye [frame quat, c^2, half2 ln lengths]."""
+    o, margin, em = ref["words"], ref["margin"], ref["eye_margin"]
+    rep = {}
+    gh, oh = g[:, 0] != 0, o[:, 0] != 0
+    flips = np.nonzero(gh != oh)[0]
+    assert np.all(margin[flips] < TOL)
+    both = np.nonzero(gh & oh)[0]
+    gb, ob = g[both], o[both]
+    for f in (lambda w: ((w[:, 0] & 0xFFFF) << 4) | ((w[:, 3] >> 24) & 15), lambda w: ((w[:, 0] >> 16) << 4) | (w[:, 3] >> 28)):
+        d = np.abs(f(gb).astype(np.int64) - f(ob).astype(np.int64))
+        assert np.mean(d <= 1) > 0.999 and d.max() <= 4
+    assert np.allclose(gb[:, 1].view(np.float32), ob[:, 1].view(np.float32), rtol=2e-6)
+    assert np.abs((gb[:, 3] & 0xFF).astype(np.int64) - (ob[:, 3] & 0xFF)).max() <= 1
+    rep["heads"] = both.size
+    for e in range(eyes):
+        wq, wc, wl = 4 + 3 * e, 5 + 3 * e, 6 + 3 * e
+        gv, ov = g[:, wc] != 0, o[:, wc] != 0
+        fl = np.nonzero(gv != ov)[0]
+        assert np.all(em[e][fl] < TOL), f"eye {e}: {np.count_nonzero(em[e][fl] >= TOL)} flips away from thresholds"
+        bv = np.nonzero(gv & ov & (em[e] >= TOL))[0]
+        rep[f"eye{e}_visible"] = bv.size
+        ga, oa = g[bv], o[bv]
+        assert np.allclose(ga[:, wc].view(np.float32), oa[:, wc].view(np.float32), rtol=2e-5), "c^2"
+        lg = np.stack([half(ga[::50, wl] & 0xFFFF), half(ga[::50, wl] >> 16)])
+        lo = np.stack([half(oa[::50, wl] & 0xFFFF), half(oa[::50, wl] >> 16)])
+        assert np.all(np.abs(lg - lo) <= 2e-3 * np.maximum(1.0, np.abs(lo))), "half-float log lengths"
+        # the quad's normal always; its in-plane rotation where the section is elongated
+        elong = (lo[1] - lo[0]) > 2e-2
+        bad = 0
+        for k, i in enumerate(range(0, bv.size, 50)):
+            Mg, Mo = quat_matrix(ga[i, wq]), quat_matrix(oa[i, wq])
+            if abs(abs(Mg[:, 2] @ Mo[:, 2]) - 1.0) > 1e-4:
+                bad += 1
+            elif elong[k] and np.linalg.norm(Mg - Mo) > 3e-2:
+                bad += 1
+        rep[f"eye{e}_frame_off"] = bad
+        assert bad <= max(1, int(frac * bv.size / 50)), f"eye {e}: {bad} quad frames differ"
+    return rep
+
+
 def run(table, rig, prm, method):
     out = Projector().project_ray(table, rig, prm, method)
     torch.cuda.synchronize()
@@ -187,3 +236,42 @@ def test_non_uniform_object_transform_is_refused(cuda):
     table = DeviceTable.from_f32_planes(planes, 16, 0, object_to_world=obj)
     with pytest.raises(Exception, match="uniform scale"):
         Projector().project_ray(table, W.plain_small_camera(), Params(), "3dgut")
+
+
+def test_raygs_mono_and_object_transform(cuda):
+    spec = W.plain_small_spec(seed=3)
+    planes = W.generate(spec)
+    rig = W.plain_small_camera()
+    table = DeviceTable.from_f32_planes(planes, spec.count, 0)
+    g, keys = run(table, rig, Params(sort_bits=32), "raygs")
+    assert g.shape[1] == 7
+    ref = O.project_ray_f32("raygs", planes, 0, O.view_from_rig(rig), O.params(sort_bits=32))
+    rep = compare_raygs(g, ref, 1)
+    print("raygs", rep, flush=True)
+    assert rep["eye0_visible"] > 30000
+    bad = np.nonzero(keys != ref["keys"])[0]
+    assert np.all(ref["margin"][bad] < TOL)
+    spec = W.plain_4m_spec(seed=19, count=30000)
+    planes = W.generate(spec)
+    obj = cam.translation([0.3, -0.8, 1.5])
+    obj[:3, :3] = 0.7 * np.array([[0.36, 0.48, -0.8], [-0.8, 0.6, 0.0], [0.48, 0.64, 0.6]])
+    rig = W.orbit_camera(12, 8.0, 1280, 720)
+    table = DeviceTable.from_f32_planes(planes, spec.count, 3, object_to_world=obj)
+    g, _ = run(table, rig, Params(sort_bits=32), "raygs")
+    ref = O.project_ray_f32("raygs", planes, 3, O.view_from_rig(rig, object_to_world=obj), O.params(sort_bits=32))
+    rep = compare_raygs(g, ref, 1)
+    print("raygs sh3 transform", rep, flush=True)
+    assert rep["eye0_visible"] > 8000
+
+
+def test_raygs_stereo(cuda):
+    spec = W.stereo_16m_spec(count=20000, seed=6)
+    planes = W.generate(spec)
+    rig = W.stereo_camera(width=512, height=544)
+    table = DeviceTable.from_f32_planes(planes, spec.count, 3)
+    g, _ = run(table, rig, Params(sort_bits=32), "raygs")
+    assert g.shape[1] == 10
+    ref = O.project_ray_f32("raygs", planes, 3, O.view_from_rig(rig), O.params(sort_bits=32))
+    rep = compare_raygs(g, ref, 2)
+    print("raygs stereo", rep, flush=True)
+    assert rep["eye0_visible"] > 5000 and rep["eye1_visible"] > 5000
