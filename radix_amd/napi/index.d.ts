@@ -105,6 +105,10 @@ export class SplatProjector {
     out: { words: DeviceBuffer[]; keys?: DeviceBuffer; at?: number }): void;
   projectSort(table: SplatTable, camera: SplatCameraViews, params: SplatProjectionParams,
     out: { words?: DeviceBuffer[]; keys?: DeviceBuffer; indices: DeviceBuffer; count?: DeviceBuffer; indirect?: DeviceBuffer }): void;
+  projectKeys(table: SplatTable, camera: SplatCameraViews, params: SplatProjectionParams,
+    out: { keys: DeviceBuffer; at?: number }): void;
+  projectRay(table: SplatTable, camera: SplatCameraViews, params: SplatProjectionParams, method: '3dgut' | 'htgs' | 'raygs',
+    out: { channels: DeviceBuffer[]; keys?: DeviceBuffer; at?: number }): void;
 }
 
 export function version(): string;

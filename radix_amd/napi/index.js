@@ -161,6 +161,19 @@ class SplatProjector {
     native.projectSort(tableDesc(table), camera, params || {}, (words || []).map((w) => w.handle),
       keys ? keys.handle : null, indices.handle, count ? count.handle : null, indirect ? indirect.handle : null);
   }
+
+  // The keys-only pass (SplatProjection.writeKeys, src/render/SplatProjection.ts:165-172)
+  projectKeys(table, camera, params, { keys, at = 0 }) {
+    native.projectKeys(tableDesc(table), camera, params || {}, keys.handle, at);
+  }
+
+  // The words of "3dgut", "htgs" or "raygs" into the projected bundle's
+  // channels w0, w1 (and w2 with two eyes) (SplatProjection.ts:46-53, 141-151)
+  projectRay(table, camera, params, method, { channels, keys, at = 0 }) {
+    const code = { '3dgut': 1, htgs: 2, raygs: 3 }[method];
+    if (code === undefined) throw new Error(`SplatProjection: unknown method ${method}`);
+    native.projectRay(tableDesc(table), camera, params || {}, code, channels.map((c) => c.handle), keys ? keys.handle : null, at);
+  }
 }
 
 module.exports = {

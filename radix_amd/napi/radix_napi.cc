@@ -495,6 +495,41 @@ napi_value ProjectSort(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// projectKeys(table, camera, params, keys, at): the keys-only pass
+napi_value ProjectKeys(napi_env env, napi_callback_info info) {
+  ARGS(5);
+  rdx_splat_table t;
+  rdx_camera c;
+  rdx_params p;
+  read_table(env, argv[0], &t);
+  if (!read_camera(env, argv[1], &c)) return nullptr;
+  read_params(env, argv[2], &p);
+  check(env, rdx_project_keys(thread_context(), nullptr, &t, &c, &p, (uint32_t)num(env, argv[4], 0),
+                              (uint32_t*)device_ptr(env, argv[3])));
+  return nullptr;
+}
+
+// projectRay(table, camera, params, method, channels[], keys, at): 3dgut (1), htgs (2), raygs (3)
+napi_value ProjectRay(napi_env env, napi_callback_info info) {
+  ARGS(7);
+  rdx_splat_table t;
+  rdx_camera c;
+  rdx_params p;
+  read_table(env, argv[0], &t);
+  if (!read_camera(env, argv[1], &c)) return nullptr;
+  read_params(env, argv[2], &p);
+  uint32_t* channels[3] = {nullptr, nullptr, nullptr};
+  for (uint32_t k = 0; k < 3; k++) {
+    napi_value w;
+    bool has = false;
+    napi_has_element(env, argv[4], k, &has);
+    if (has && napi_get_element(env, argv[4], k, &w) == napi_ok) channels[k] = (uint32_t*)device_ptr(env, w);
+  }
+  check(env, rdx_project_ray(thread_context(), nullptr, &t, &c, &p, (int)num(env, argv[3], 0), (uint32_t)num(env, argv[6], 0),
+                             channels, (uint32_t*)device_ptr(env, argv[5])));
+  return nullptr;
+}
+
 napi_value Version(napi_env env, napi_callback_info) {
   napi_value s;
   napi_create_string_utf8(env, rdx_version(), NAPI_AUTO_LENGTH, &s);
@@ -522,6 +557,8 @@ napi_value Init(napi_env env, napi_value exports) {
       {"sortSplats", SortSplats},
       {"projectEwa", ProjectEwa},
       {"projectSort", ProjectSort},
+      {"projectKeys", ProjectKeys},
+      {"projectRay", ProjectRay},
   };
   for (const auto& f : fns) {
     napi_value v;

@@ -161,9 +161,53 @@ async function gpuChecks() {
   });
 }
 
+async function gpuChecksKeysAndRays() {
+  await check('SplatProjector.projectKeys and projectRay: the keys of projectEwa with cullKeys off', () => {
+    const random = rng(5);
+    const n = 4096;
+    const planes = new Float32Array(14 * n);
+    for (let i = 0; i < n; i += 1) {
+      planes[i] = (random() - 0.5) * 4;
+      planes[n + i] = (random() - 0.5) * 3;
+      planes[2 * n + i] = -2 - random() * 6;
+      for (let k = 3; k < 6; k += 1) planes[k * n + i] = Math.log(0.01 + random() * 0.1);
+      planes[9 * n + i] = 1;
+      planes[10 * n + i] = i % 7 === 0 ? 0.0005 : 0.2 + random() * 0.8; // some below minAlpha
+      for (let k = 11; k < 14; k += 1) planes[k * n + i] = random();
+    }
+    const f = 1 / Math.tan(Math.PI / 6);
+    const near = 0.1;
+    const far = 100;
+    const projection = [f / 2, 0, 0, 0, 0, f, 0, 0, 0, 0, -far / (far - near), -1, 0, 0, (-far * near) / (far - near), 0];
+    const view = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1];
+    const camera = { head: view, eyes: [{ view, projection }], size: [128, 64], ndcZMin: 0 };
+    const table = { layout: 'f32', count: n, planes: DeviceBuffer.fromArray(planes) };
+    const params = { sortBits: 32, cullKeys: false };
+    const projector = new SplatProjector();
+    const ewaKeys = new DeviceBuffer(4 * n);
+    projector.project(table, camera, params, { words: [new DeviceBuffer(16 * n)], keys: ewaKeys });
+    const keys = new DeviceBuffer(4 * n);
+    projector.projectKeys(table, camera, params, { keys });
+    const rayKeys = new DeviceBuffer(4 * n);
+    const channels = [new DeviceBuffer(16 * n), new DeviceBuffer(16 * n)];
+    projector.projectRay(table, camera, params, '3dgut', { channels, keys: rayKeys });
+    const a = ewaKeys.read(0, n);
+    const b = keys.read(0, n);
+    const c = rayKeys.read(0, n);
+    const invalid = a.filter((k) => k === 0xffffffff).length;
+    if (invalid < n / 8) return `only ${invalid} invalid keys`;
+    const w = channels[0].read(0, 4 * n);
+    if (!w.some((x) => x !== 0)) return 'no 3dgut words';
+    return firstMismatch(b, a, n) || firstMismatch(c, a, n);
+  });
+}
+
 (async () => {
   await cpuChecks();
-  if (process.argv[2] === 'gpu') await gpuChecks();
+  if (process.argv[2] === 'gpu') {
+    await gpuChecks();
+    await gpuChecksKeysAndRays();
+  }
   console.log(failures.length ? `${failures.length} failed` : 'all passed');
   process.exit(failures.length ? 1 : 0);
 })();
