@@ -304,6 +304,17 @@ int rdx_project_sort(rdx_context* ctx, void* stream, const rdx_splat_table* tabl
                      uint32_t* out_indices_dev, uint32_t* out_count_dev, uint32_t* out_draw_args_dev,
                      const rdx_float_out* side);
 
+/* ---- File decode onto the device ------------------------------------------
+ * RDX chunks store each channel's elements byte-transposed, then gzipped
+ * (src/core/rdx.ts:8-17, Compression.compressTransposed, compress.ts:192-195).
+ * After the host gunzips a payload, this undoes the transpose on the device
+ * (the second step of Compression.decompressTransposed, compress.ts:197-204,
+ * transposeBytes 6-19): planes_dev holds element_bytes planes of `count`
+ * bytes (plane k = byte k of every element); out_dev gets `count` elements
+ * of element_bytes bytes (a channel buffer at the chunk's start element).
+ * Stream-ordered; element_bytes 1 .. 64. */
+int rdx_untranspose_bytes(void* stream, const uint8_t* planes_dev, uint32_t count, uint32_t element_bytes, void* out_dev);
+
 /* ---- Batched views: many (view, collection) segments in one sequence -----
  * No reference counterpart: the reference re-projects and re-sorts per
  * render and per object (SplatProjector.ts:127-137); here every view of a

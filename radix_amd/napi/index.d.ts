@@ -111,4 +111,7 @@ export class SplatProjector {
     out: { channels: DeviceBuffer[]; keys?: DeviceBuffer; at?: number }): void;
 }
 
+export function untransposeBytes(planes: DeviceBuffer, count: number, elementBytes: number, out: DeviceBuffer,
+  outOffset?: number): void;
+
 export function version(): string;

@@ -176,7 +176,15 @@ class SplatProjector {
   }
 }
 
+// An RDX payload, gunzipped into `planes` (a DeviceBuffer), back to `count`
+// elements of `elementBytes` in `out` from byte `outOffset` (the second step
+// of Compression.decompressTransposed, src/core/compress.ts:197-204)
+function untransposeBytes(planes, count, elementBytes, out, outOffset = 0) {
+  native.untransposeBytes(planes.handle, count, elementBytes, out.handle, outOffset);
+}
+
 module.exports = {
+  untransposeBytes,
   alignCapacity, SortKeys, sortSplatKeys, DeviceBuffer, RadixSort, SplatSorter, SplatProjector,
   version: native.version,
 };

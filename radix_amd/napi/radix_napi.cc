@@ -530,6 +530,15 @@ napi_value ProjectRay(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// untransposeBytes(planes, count, elementBytes, out, outOffsetBytes): an RDX payload's planes back to elements
+napi_value UntransposeBytes(napi_env env, napi_callback_info info) {
+  ARGS(5);
+  char* out = (char*)device_ptr(env, argv[3]);
+  check(env, rdx_untranspose_bytes(nullptr, (const uint8_t*)device_ptr(env, argv[0]), (uint32_t)num(env, argv[1], 0),
+                                   (uint32_t)num(env, argv[2], 0), out ? out + (size_t)num(env, argv[4], 0) : nullptr));
+  return nullptr;
+}
+
 napi_value Version(napi_env env, napi_callback_info) {
   napi_value s;
   napi_create_string_utf8(env, rdx_version(), NAPI_AUTO_LENGTH, &s);
@@ -559,6 +568,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"projectSort", ProjectSort},
       {"projectKeys", ProjectKeys},
       {"projectRay", ProjectRay},
+      {"untransposeBytes", UntransposeBytes},
   };
   for (const auto& f : fns) {
     napi_value v;
