@@ -315,6 +315,21 @@ int rdx_project_sort(rdx_context* ctx, void* stream, const rdx_splat_table* tabl
  * Stream-ordered; element_bytes 1 .. 64. */
 int rdx_untranspose_bytes(void* stream, const uint8_t* planes_dev, uint32_t count, uint32_t element_bytes, void* out_dev);
 
+/* SPZ (Niantic's format in its gzip container, src/core/spz.ts:140-175,
+ * 260-349) decoded on the device into f32 planes: body_dev is the unzipped
+ * file after its 16-byte header (columns of centres, alphas, colours,
+ * scales, rotations, SH); planes_dev gets 14 + 3K planes of plane_stride
+ * floats (centre 3, log-scale 3, quaternion 4, opacity, rgb 3, SH 3K:
+ * rdx_splat_table's f32 layout) with the values readSpz computes: centres
+ * int24 x 2^-fractional_bits (version 1: half floats), opacity a / 255,
+ * colour (b / 255 - 0.5) SH_C0 / 0.15 + 0.5, log-scale b / 16 - 10,
+ * rotations smallest-three (versions 3 and 4) or xyz with w >= 0 (1, 2),
+ * SH (b - 128) / 128 coefficient-major RGB. Versions 1-4 and SH degrees 0-3
+ * ("Unsupported SPZ version v or SH degree d" otherwise; degree 4 is
+ * RDX_E_UNSUPPORTED); no coordinate-system conversion. */
+int rdx_spz_decode(void* stream, const uint8_t* body_dev, size_t body_bytes, uint32_t count, int version, int sh_degree,
+                   int fractional_bits, float* planes_dev, size_t plane_stride);
+
 /* ---- Batched views: many (view, collection) segments in one sequence -----
  * No reference counterpart: the reference re-projects and re-sorts per
  * render and per object (SplatProjector.ts:127-137); here every view of a

@@ -197,6 +197,8 @@ _SIGNATURES = [
      [C.c_void_p, C.c_void_p, C.POINTER(SplatTable), C.POINTER(Camera), C.POINTER(Params), C.c_int, C.c_uint32,
       C.c_void_p, C.c_void_p]),
     ("rdx_untranspose_bytes", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
+    ("rdx_spz_decode", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_size_t]),
     ("rdx_draw_create", C.c_int, [C.c_void_p, C.POINTER(DrawSettings), C.POINTER(C.c_void_p)]),
     ("rdx_draw_destroy", C.c_int, [C.c_void_p]),
     ("rdx_draw_splats", C.c_int,

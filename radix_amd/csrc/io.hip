@@ -84,3 +84,113 @@ int rdx_untranspose_bytes(void* stream, const uint8_t* planes_dev, uint32_t coun
 }
 
 }  // extern "C"
+
+// ---- SPZ (Niantic's gzip container, src/core/spz.ts) ------------------------------
+
+namespace rdx {
+
+constexpr double SPZ_COLOR_SCALE = 0.28209479177387814 / 0.15;  // SH_C0 / 0.15 (spz.ts:50, ply.ts:15)
+
+// Column offsets of the unzipped body after the 16-byte header (spz.ts:152-160)
+struct SpzCols {
+  size_t centers, alphas, colors, scales, rotations, sh;
+};
+
+// One splat per thread into the f32 planes (centre 3, log-scale 3, quat 4,
+// opacity, rgb 3, SH 3K; plane stride `st`), as readSpz computes the values
+// (spz.ts:297-347) before encoding them into a layout: centres as int24 x
+// 2^-fractionalBits (v1: half floats), opacity a / 255, colour
+// (b / 255 - 0.5) SH_C0 / 0.15 + 0.5, log-scale b / 16 - 10 (the scale
+// exp(b / 16 - 10)), rotations as smallest-three (v3+) or x, y, z in
+// [-1, 1] with w = sqrt(1 - |xyz|^2) (v2, v1), SH (b - 128) / 128.
+static __global__ __launch_bounds__(256) void k_spz_decode(const uint8_t* __restrict__ body, SpzCols c, uint32_t count,
+                                                           int version, int sh_values, float fixed,
+                                                           float* __restrict__ planes, size_t st) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= count) return;
+  float* out = planes + i;
+  if (version == 1) {
+    const uint8_t* p = body + c.centers + (size_t)i * 6;
+#pragma unroll
+    for (int k = 0; k < 3; k++) out[k * st] = half_to_float(p[2 * k] | (p[2 * k + 1] << 8));
+  } else {
+    const uint8_t* p = body + c.centers + (size_t)i * 9;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const int32_t v = (int32_t)(((uint32_t)p[3 * k] | ((uint32_t)p[3 * k + 1] << 8) | ((uint32_t)p[3 * k + 2] << 16)) << 8) >> 8;
+      out[k * st] = (float)v * fixed;
+    }
+  }
+  const uint8_t* sc = body + c.scales + (size_t)i * 3;
+#pragma unroll
+  for (int k = 0; k < 3; k++) out[(3 + k) * st] = (float)sc[k] / 16.0f - 10.0f;
+  // The rotation and colour maps in double, rounded once: w = sqrt(1 - |xyz|^2)
+  // is ill-conditioned near w = 0, where a float evaluation strays by many ulps
+  // from readSpz's double one. One splat's few FP64 operations are free next to
+  // its bytes.
+  double q[4];
+  if (version >= 3) {
+    const uint8_t* p = body + c.rotations + (size_t)i * 4;
+    uint32_t rest = (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+    const uint32_t largest = rest >> 30;
+    double sum = 0.0;
+#pragma unroll
+    for (int k = 3; k >= 0; k--) {
+      if ((uint32_t)k != largest) {
+        const double v = 0.70710678118654752 * ((double)(rest & 511u) / 511.0) * ((rest >> 9) & 1u ? -1.0 : 1.0);
+        rest >>= 10;
+        q[k] = v;
+        sum += v * v;
+      }
+    }
+    q[largest] = sqrt(fmax(1.0 - sum, 0.0));
+  } else {
+    const uint8_t* p = body + c.rotations + (size_t)i * 3;
+    q[0] = (double)p[0] / 127.5 - 1.0;
+    q[1] = (double)p[1] / 127.5 - 1.0;
+    q[2] = (double)p[2] / 127.5 - 1.0;
+    q[3] = sqrt(fmax(0.0, 1.0 - q[0] * q[0] - q[1] * q[1] - q[2] * q[2]));
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) out[(6 + k) * st] = (float)q[k];
+  out[10 * st] = (float)body[c.alphas + i] / 255.0f;
+  const uint8_t* col = body + c.colors + (size_t)i * 3;
+#pragma unroll
+  for (int k = 0; k < 3; k++) out[(11 + k) * st] = (float)(((double)col[k] / 255.0 - 0.5) * SPZ_COLOR_SCALE + 0.5);
+  const uint8_t* sh = body + c.sh + (size_t)i * sh_values;
+  for (int v = 0; v < sh_values; v++) out[(14 + v) * st] = ((float)sh[v] - 128.0f) / 128.0f;
+}
+
+}  // namespace rdx
+
+extern "C" {
+
+int rdx_spz_decode(void* stream, const uint8_t* body_dev, size_t body_bytes, uint32_t count, int version, int sh_degree,
+                   int fractional_bits, float* planes_dev, size_t plane_stride) {
+  if (version < 1 || version > 4 || sh_degree < 0 || sh_degree > 3)
+    return fail(version >= 1 && version <= 4 && sh_degree == 4 ? RDX_E_UNSUPPORTED : RDX_E_INVALID,
+                "Unsupported SPZ version %d or SH degree %d", version, sh_degree);
+  if (fractional_bits < 0 || fractional_bits > 30) return fail(RDX_E_INVALID, "SPZ fractional bits %d", fractional_bits);
+  if (count == 0) return RDX_OK;
+  if (!body_dev || !planes_dev) return fail(RDX_E_INVALID, "rdx_spz_decode: NULL body or planes");
+  if (plane_stride < count) return fail(RDX_E_INVALID, "rdx_spz_decode: plane stride %zu below %u", plane_stride, count);
+  static const int SH_VECTORS[4] = {0, 3, 8, 15};
+  const int sh_values = SH_VECTORS[sh_degree] * 3;
+  SpzCols c;
+  c.centers = 0;
+  const size_t n = count;
+  c.alphas = c.centers + n * (version == 1 ? 6 : 9);
+  c.colors = c.alphas + n;
+  c.scales = c.colors + n * 3;
+  c.rotations = c.scales + n * 3;
+  c.sh = c.rotations + n * (version >= 3 ? 4 : 3);
+  const size_t need = c.sh + n * sh_values;
+  if (body_bytes < need) return fail(RDX_E_INVALID, "SPZ body holds %zu bytes, %zu needed for %u splats", body_bytes, need, count);
+  const float fixed = 1.0f / (float)(1u << fractional_bits);
+  hipLaunchKernelGGL(k_spz_decode, dim3((count + 255) / 256), dim3(256), 0, (hipStream_t)stream, body_dev, c, count, version,
+                     sh_values, fixed, planes_dev, plane_stride);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? RDX_OK : hip_fail(e, "rdx_spz_decode");
+}
+
+}  // extern "C"
