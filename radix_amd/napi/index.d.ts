@@ -114,4 +114,12 @@ export class SplatProjector {
 export function untransposeBytes(planes: DeviceBuffer, count: number, elementBytes: number, out: DeviceBuffer,
   outOffset?: number): void;
 
+export interface SpzMeta {
+  format: 'spz'; version: number; count: number; shDegree: number; fractionalBits: number; antialiased: boolean;
+}
+export function readSpz(bytes: Uint8Array): {
+  table: { layout: 'f32'; count: number; planes: DeviceBuffer; planeStride: number; shCodec: 'f32' | 'none'; shDegree: number };
+  meta: SpzMeta;
+};
+
 export function version(): string;

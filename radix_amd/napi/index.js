@@ -183,8 +183,46 @@ function untransposeBytes(planes, count, elementBytes, out, outOffset = 0) {
   native.untransposeBytes(planes.handle, count, elementBytes, out.handle, outOffset);
 }
 
+const SPZ_MAGIC = 0x5053474e;
+const SH_VECTORS = [0, 3, 8, 15, 24];
+
+// An SPZ file (Niantic's gzip container) decoded onto the device as an f32
+// table for SplatProjector (SpzFile.read, src/core/spz.ts:140-170, 260-349):
+// Node's zlib gunzips, the header is read here, one upload puts the columns
+// in device memory and the decode kernel writes the planes (centre 3,
+// log-scale 3, quaternion 4, opacity, rgb 3, SH 3K; stride `count`).
+// Returns { table, meta }; table.planes is the caller's to dispose.
+function readSpz(bytes) {
+  const data = Buffer.from(bytes.buffer, bytes.byteOffset, bytes.byteLength);
+  if (data.length >= 4 && data.readUInt32LE(0) === SPZ_MAGIC) {
+    throw new Error("SPZ 4's NGSP (zstd) container is not supported here");
+  }
+  const raw = require('zlib').gunzipSync(data);
+  if (raw.length < 16 || raw.readUInt32LE(0) !== SPZ_MAGIC) throw new Error('Invalid SPZ file');
+  const version = raw.readUInt32LE(4);
+  const count = raw.readUInt32LE(8);
+  const [shDegree, fractionalBits, flags] = [raw[12], raw[13], raw[14]];
+  if (version < 1 || version > 4 || !(shDegree in SH_VECTORS) || shDegree > 3) {
+    throw new Error(`Unsupported SPZ version ${version} or SH degree ${shDegree}`);
+  }
+  const body = raw.subarray(16);
+  const n = Math.max(count, 1);
+  const planes = new DeviceBuffer(4 * n * (14 + 3 * SH_VECTORS[shDegree]));
+  const staged = DeviceBuffer.fromArray(body.length ? body : new Uint8Array(1));
+  try {
+    native.spzDecode(staged.handle, body.length, count, version, shDegree, fractionalBits, planes.handle, n);
+  } catch (e) {
+    planes.dispose();
+    throw e;
+  } finally {
+    staged.dispose();
+  }
+  const table = { layout: 'f32', count, planes, planeStride: n, shCodec: shDegree > 0 ? 'f32' : 'none', shDegree };
+  return { table, meta: { format: 'spz', version, count, shDegree, fractionalBits, antialiased: (flags & 1) !== 0 } };
+}
+
 module.exports = {
-  untransposeBytes,
+  untransposeBytes, readSpz,
   alignCapacity, SortKeys, sortSplatKeys, DeviceBuffer, RadixSort, SplatSorter, SplatProjector,
   version: native.version,
 };

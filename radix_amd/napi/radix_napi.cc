@@ -539,6 +539,16 @@ napi_value UntransposeBytes(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// spzDecode(body, bodyBytes, count, version, shDegree, fractionalBits, planes, planeStride):
+// an SPZ body (the gunzipped file after its 16-byte header) to f32 planes
+napi_value SpzDecode(napi_env env, napi_callback_info info) {
+  ARGS(8);
+  check(env, rdx_spz_decode(nullptr, (const uint8_t*)device_ptr(env, argv[0]), (size_t)num(env, argv[1], 0),
+                            (uint32_t)num(env, argv[2], 0), (int)num(env, argv[3], 0), (int)num(env, argv[4], 0),
+                            (int)num(env, argv[5], 0), (float*)device_ptr(env, argv[6]), (size_t)num(env, argv[7], 0)));
+  return nullptr;
+}
+
 napi_value Version(napi_env env, napi_callback_info) {
   napi_value s;
   napi_create_string_utf8(env, rdx_version(), NAPI_AUTO_LENGTH, &s);
@@ -569,6 +579,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"projectKeys", ProjectKeys},
       {"projectRay", ProjectRay},
       {"untransposeBytes", UntransposeBytes},
+      {"spzDecode", SpzDecode},
   };
   for (const auto& f : fns) {
     napi_value v;

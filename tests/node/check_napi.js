@@ -5,7 +5,8 @@
 // that make no GPU call; `gpu` adds the GPU sorts and projection.
 const path = require('path');
 const radix = require(path.join(__dirname, '..', '..', 'radix_amd', 'napi', 'index.js'));
-const { SortKeys, sortSplatKeys, DeviceBuffer, RadixSort, SplatSorter, SplatProjector, alignCapacity } = radix;
+const { SortKeys, sortSplatKeys, DeviceBuffer, RadixSort, SplatSorter, SplatProjector, alignCapacity, readSpz } = radix;
+const zlib = require('zlib');
 
 const failures = [];
 async function check(name, fn) {
@@ -202,11 +203,99 @@ async function gpuChecksKeysAndRays() {
   });
 }
 
+// readSpz's per-splat values in doubles (src/core/spz.ts:297-347, 571-586),
+// plane by plane, over an unzipped body
+function spzPlanes(body, n, version, shDegree, fractionalBits) {
+  const K = [0, 3, 8, 15][shDegree];
+  const out = Array.from({ length: 14 + 3 * K }, () => new Float64Array(n));
+  const alphas = n * (version === 1 ? 6 : 9);
+  const colors = alphas + n;
+  const scales = colors + 3 * n;
+  const rotations = scales + 3 * n;
+  const sh = rotations + n * (version >= 3 ? 4 : 3);
+  const dv = new DataView(body.buffer, body.byteOffset, body.byteLength);
+  for (let i = 0; i < n; i += 1) {
+    for (let k = 0; k < 3; k += 1) {
+      const at = i * 9 + 3 * k;
+      out[k][i] = (((body[at] | (body[at + 1] << 8) | (body[at + 2] << 16)) << 8) >> 8) / 2 ** fractionalBits;
+      out[3 + k][i] = body[scales + 3 * i + k] / 16 - 10;
+      out[11 + k][i] = (body[colors + 3 * i + k] / 255 - 0.5) * (0.28209479177387814 / 0.15) + 0.5;
+    }
+    const q = [0, 0, 0, 0];
+    if (version >= 3) {
+      let rest = dv.getUint32(rotations + 4 * i, true);
+      const largest = rest >>> 30;
+      let sum = 0;
+      for (let k = 3; k >= 0; k -= 1) {
+        if (k !== largest) {
+          q[k] = Math.SQRT1_2 * ((rest & 511) / 511) * ((rest >>> 9) & 1 ? -1 : 1);
+          rest >>>= 10;
+          sum += q[k] * q[k];
+        }
+      }
+      q[largest] = Math.sqrt(Math.max(1 - sum, 0));
+    } else {
+      for (let k = 0; k < 3; k += 1) q[k] = body[rotations + 3 * i + k] / 127.5 - 1;
+      q[3] = Math.sqrt(Math.max(0, 1 - q[0] * q[0] - q[1] * q[1] - q[2] * q[2]));
+    }
+    for (let k = 0; k < 4; k += 1) out[6 + k][i] = q[k];
+    out[10][i] = body[alphas + i] / 255;
+    for (let v = 0; v < 3 * K; v += 1) out[14 + v][i] = (body[sh + 3 * K * i + v] - 128) / 128;
+  }
+  return out;
+}
+
+async function gpuChecksSpz() {
+  await check('readSpz: planes equal readSpz\'s values (versions 2, 3; SH 0, 1, 3), its errors', () => {
+    const random = rng(13);
+    for (const [version, shDegree, n] of [[3, 1, 1000], [2, 3, 257], [3, 0, 1]]) {
+      const K = [0, 3, 8, 15][shDegree];
+      const bodyBytes = n * (9 + 1 + 3 + 3 + (version >= 3 ? 4 : 3) + 3 * K);
+      const raw = new Uint8Array(16 + bodyBytes);
+      const dv = new DataView(raw.buffer);
+      dv.setUint32(0, 0x5053474e, true);
+      dv.setUint32(4, version, true);
+      dv.setUint32(8, n, true);
+      raw[12] = shDegree;
+      raw[13] = 12;
+      raw[14] = 1;
+      for (let b = 16; b < raw.length; b += 1) raw[b] = Math.floor(random() * 256);
+      const { table, meta } = readSpz(zlib.gzipSync(raw));
+      const expected = spzPlanes(raw.subarray(16), n, version, shDegree, 12);
+      const got = table.planes.read(0, n * expected.length, Float32Array);
+      table.planes.dispose();
+      if (meta.count !== n || meta.version !== version || !meta.antialiased) return `meta ${JSON.stringify(meta)}`;
+      for (let p = 0; p < expected.length; p += 1) {
+        for (let i = 0; i < n; i += 1) {
+          const e = Math.fround(expected[p][i]);
+          const g = got[p * n + i];
+          // one float ulp: the kernel rounds its double maps once
+          if (Math.abs(g - e) > Math.abs(e) * 2 ** -23 + 1e-45) return `v${version} plane ${p} splat ${i}: ${g} vs ${e}`;
+        }
+      }
+    }
+    const errors = [
+      [Buffer.from('NGSP0000000000000000'), 'NGSP'],
+      [zlib.gzipSync(Buffer.alloc(32)), 'Invalid SPZ file'],
+    ];
+    for (const [bytes, text] of errors) {
+      try {
+        readSpz(bytes);
+        return `no error for ${text}`;
+      } catch (e) {
+        if (!e.message.includes(text)) return e.message;
+      }
+    }
+    return undefined;
+  });
+}
+
 (async () => {
   await cpuChecks();
   if (process.argv[2] === 'gpu') {
     await gpuChecks();
     await gpuChecksKeysAndRays();
+    await gpuChecksSpz();
   }
   console.log(failures.length ? `${failures.length} failed` : 'all passed');
   process.exit(failures.length ? 1 : 0);
