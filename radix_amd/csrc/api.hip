@@ -375,7 +375,7 @@ uint32_t rdx_key_compute(float distance, int bits) {
 int rdx_compute_keys(void* stream, const float* d, uint32_t n, int bits, uint32_t* keys) {
   if (int rc = check_bits(bits, "SortKeys.compute")) return rc;
   if (n == 0) return RDX_OK;
-  const uint32_t grid = std::min<uint32_t>((n + 255) / 256, 4096u);
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(((uint64_t)n + 255) / 256, 4096u);
   hipLaunchKernelGGL(k_compute_keys, dim3(grid), dim3(256), 0, (hipStream_t)stream, d, n, bits, keys);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? RDX_OK : hip_fail(e, "rdx_compute_keys");
@@ -642,7 +642,7 @@ int rdx_context_autotune(rdx_context* ctx, void* stream, const uint32_t* keys_de
       return rc;
     }
     // below the invalid key: every key is sorted
-    hipLaunchKernelGGL(k_random_keys, dim3(std::min<uint32_t>((n + 255) / 256, 4096u)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_random_keys, dim3((uint32_t)std::min<uint64_t>(((uint64_t)n + 255) / 256, 4096u)), dim3(256), 0, s,
                        (uint32_t*)keys.ptr, n, key_invalid(bits) >> 1, 0x2545f491u);
     keys_dev = (const uint32_t*)keys.ptr;
   }

@@ -69,7 +69,7 @@ int rdx_untranspose_bytes(void* stream, const uint8_t* planes_dev, uint32_t coun
   const bool aligned = count % 4 == 0 && ((uintptr_t)planes_dev & 3) == 0 &&
                        ((uintptr_t)out_dev & (element_bytes >= 16 ? 15 : element_bytes - 1)) == 0;
   const uint32_t quads = count / 4;
-  const dim3 g4((quads + 255) / 256), g1((count + 255) / 256);
+  const dim3 g4((uint32_t)(((uint64_t)quads + 255) / 256)), g1((uint32_t)(((uint64_t)count + 255) / 256));
   if (aligned && element_bytes == 16) {
     hipLaunchKernelGGL(k_untranspose4<16>, g4, dim3(256), 0, s, (const uint32_t*)planes_dev, quads, (uint32_t*)out_dev);
   } else if (aligned && element_bytes == 8) {
@@ -187,7 +187,7 @@ int rdx_spz_decode(void* stream, const uint8_t* body_dev, size_t body_bytes, uin
   const size_t need = c.sh + n * sh_values;
   if (body_bytes < need) return fail(RDX_E_INVALID, "SPZ body holds %zu bytes, %zu needed for %u splats", body_bytes, need, count);
   const float fixed = 1.0f / (float)(1u << fractional_bits);
-  hipLaunchKernelGGL(k_spz_decode, dim3((count + 255) / 256), dim3(256), 0, (hipStream_t)stream, body_dev, c, count, version,
+  hipLaunchKernelGGL(k_spz_decode, dim3((uint32_t)(((uint64_t)count + 255) / 256)), dim3(256), 0, (hipStream_t)stream, body_dev, c, count, version,
                      sh_values, fixed, planes_dev, plane_stride);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? RDX_OK : hip_fail(e, "rdx_spz_decode");
