@@ -134,8 +134,10 @@ template <int RB, int BLOCK, int KPT>
 static void launch_kpass(PassKernel k, uint32_t grid, hipStream_t s, const PassArgs& a) {
   if (k == PassKernel::count)
     hipLaunchKernelGGL((k_count<RB, BLOCK, KPT>), dim3(grid), dim3(BLOCK), 0, s, a);
+  else if (a.vals_in)
+    hipLaunchKernelGGL((k_scatter<RB, BLOCK, KPT, true>), dim3(grid), dim3(BLOCK), 0, s, a);
   else
-    hipLaunchKernelGGL((k_scatter<RB, BLOCK, KPT>), dim3(grid), dim3(BLOCK), 0, s, a);
+    hipLaunchKernelGGL((k_scatter<RB, BLOCK, KPT, false>), dim3(grid), dim3(BLOCK), 0, s, a);
 }
 
 // 256-thread blocks for digits of up to 8 bits (a thread per digit), 512 for all.

@@ -347,7 +347,10 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(PassArgs a, int bins) {
 // One digit pass's write-out. Tile t covers input keys [t * TILE, (t + 1) *
 // TILE) of its segment: wave w holds [w * 64 * KPT, ...), as KPT rounds of 64
 // consecutive keys, so (round, lane) order is input order.
-template <int RB, int BLOCK, int KPT>
+// VALS: the pass reads values (a.vals_in); otherwise a key's value is its
+// input index. A compile-time choice: a branch around the value loads would
+// hide the count of loads in flight from the ranking's waits.
+template <int RB, int BLOCK, int KPT, bool VALS>
 __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
   constexpr int BINS = 1 << RB;
   constexpr uint32_t DMASK = BINS - 1;
@@ -388,22 +391,43 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
   const uint32_t local_tile = tile - g.tile0;
   const uint32_t tile_base = local_tile * (uint32_t)TILE;
 
-  // Every load of the tile at once: keys and values (below the capacity),
-  // the digit totals and the tile's scanned counts.
-  uint32_t keys[KPT], vals[KPT], rank[KPT];
-  const uint32_t* keys_in = a.keys_in + g.off;
-  const uint32_t* vals_in = a.vals_in ? a.vals_in + g.off : nullptr;
-  const uint32_t wbase = tile_base + (uint32_t)(w * WTILE) + (uint32_t)lane;
-#pragma unroll
-  for (int j = 0; j < KPT; j++) {
-    const uint32_t i = wbase + (uint32_t)(j * RDX_WAVE);
-    // the pass's last reads of its input: nontemporal
-    keys[j] = i < g.cap ? __builtin_nontemporal_load(keys_in + i) : 0u;
-    vals[j] = vals_in ? (i < g.cap ? __builtin_nontemporal_load(vals_in + i) : 0u) : i;
-  }
+  // Every load of the tile at once: first the digit totals and the tile's
+  // scanned counts, then the keys, then the values (below the capacity).
+  // Vector loads complete in issue order, so the digit-start scan waits only
+  // for the first two and round j of the ranking only for keys[j]: the
+  // ranking runs while the rest of the tile arrives.
   const uint32_t* tot = a.tot + (size_t)g.s * TOT_STRIDE;
   const uint32_t c_tot = tid < BINS ? tot[tid] : 0u;
   const uint32_t c_pre = tid < BINS ? a.cnt[(size_t)tile * a.cstride + tid] : 0u;
+  uint32_t keys[KPT], vals[KPT], rank[KPT];
+  const uint32_t* keys_in = a.keys_in + g.off;
+  const uint32_t* vals_in = VALS ? a.vals_in + g.off : nullptr;
+  const uint32_t wbase = tile_base + (uint32_t)(w * WTILE) + (uint32_t)lane;
+  // The loads take no branch, or the compiler could not count the loads in
+  // flight and would wait for all of them before the first use: a lane past
+  // the capacity reads the last element again (an empty segment: a word of
+  // the totals) and drops it. The pass's last reads of its input:
+  // nontemporal.
+  const uint32_t last = g.cap ? g.cap - 1u : 0u;
+  const uint32_t* ksrc = g.cap ? keys_in : tot;
+#pragma unroll
+  for (int j = 0; j < KPT; j++) {
+    const uint32_t i = wbase + (uint32_t)(j * RDX_WAVE);
+    const uint32_t k = __builtin_nontemporal_load(ksrc + min(i, last));
+    keys[j] = i < g.cap ? k : 0u;
+  }
+  if constexpr (VALS) {
+    const uint32_t* vsrc = g.cap ? vals_in : tot;
+    // (a lane past the capacity is never kept: its value is not read)
+#pragma unroll
+    for (int j = 0; j < KPT; j++) {
+      const uint32_t i = wbase + (uint32_t)(j * RDX_WAVE);
+      vals[j] = __builtin_nontemporal_load(vsrc + min(i, last));
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < KPT; j++) vals[j] = wbase + (uint32_t)(j * RDX_WAVE);
+  }
   const uint32_t n = seg_count(a, g);
 
   // Digit starts of the segment: exclusive scan of the digit totals; with
@@ -428,12 +452,6 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
 
   // Rank.
   uint32_t dig[KPT];
-#pragma unroll
-  for (int j = 0; j < KPT; j++) {
-    const uint32_t i = wbase + (uint32_t)(j * RDX_WAVE);
-    rank[j] = i < n && !(a.drop && keys[j] == a.invalid) ? 0u : 0xffffffffu;  // 0xffffffff: not kept
-    dig[j] = digit_of(a, keys[j], kbase, DMASK);
-  }
   __syncthreads();  // s_wave, s_match zeroed
   // Rank each round of 64 keys in input order. Up to PEEL digits of the
   // round (those of its first remaining lanes) are ranked by one ballot each.
@@ -445,8 +463,11 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
   const uint64_t lane_bit = 1ull << lane;
 #pragma unroll
   for (int j = 0; j < KPT; j++) {
-    const bool keep = rank[j] == 0u;
-    const uint32_t d = dig[j];
+    const uint32_t i = wbase + (uint32_t)(j * RDX_WAVE);
+    const bool keep = i < n && !(a.drop && keys[j] == a.invalid);
+    rank[j] = keep ? 0u : 0xffffffffu;  // 0xffffffff: not kept
+    const uint32_t d = digit_of(a, keys[j], kbase, DMASK);
+    dig[j] = d;
     bool done = !keep;
     uint64_t rem = a.peel ? ballot(keep) : 0ull;
 #pragma unroll
