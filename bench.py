@@ -247,6 +247,44 @@ def bench_keys(steps: int, proj, table, rig) -> dict:
             "bytes_per_splat": 48, "hbm_frac": round(bytes_ / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
+def bench_loaders(steps: int, n: int = 4194304) -> dict:
+    """The loaders' device kernels on synthetic bytes (any bytes are valid
+    input): the RDX untranspose of an extSplats chunk (two 16-byte channels:
+    32 B in, 32 B out per splat) and the SPZ decode of a version-3 SH3 body
+    (65 B in, the 59 f32 planes out: 236 B), HIP events on the launch stream."""
+    import ctypes as C
+
+    import torch
+
+    from radix_amd import _native as N
+    from radix_amd.sort import _stream
+
+    lib = N.lib()
+    g = torch.Generator(device="cuda").manual_seed(5)
+    planes = [torch.randint(0, 256, (16 * n,), dtype=torch.uint8, device="cuda", generator=g) for _ in range(2)]
+    outs = [torch.empty(16 * n, dtype=torch.uint8, device="cuda") for _ in range(2)]
+
+    def untranspose():
+        for p, o in zip(planes, outs):
+            N.check(lib.rdx_untranspose_bytes(_stream(), C.c_void_p(p.data_ptr()), n, 16, C.c_void_p(o.data_ptr())))
+
+    rdx_ms = time_events(untranspose, steps)
+    del planes, outs
+    body = torch.randint(0, 256, (65 * n,), dtype=torch.uint8, device="cuda", generator=g)
+    out = torch.empty((59, n), dtype=torch.float32, device="cuda")
+
+    def spz():
+        N.check(lib.rdx_spz_decode(_stream(), C.c_void_p(body.data_ptr()), body.numel(), n, 3, 3, 12,
+                                   C.c_void_p(out.data_ptr()), n))
+
+    spz_ms = time_events(spz, steps)
+    frac = lambda b, ms: round(b * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    return {"workload": "4,194,304 splats of synthetic bytes: RDX untranspose of an extSplats chunk (extA, extB), "
+                        "SPZ v3 SH3 body decode to f32 planes",
+            "rdx_untranspose": {"ms": round(rdx_ms, 4), "bytes_per_splat": 64, "hbm_frac": frac(64, rdx_ms)},
+            "spz_decode": {"ms": round(spz_ms, 4), "bytes_per_splat": 301, "hbm_frac": frac(301, spz_ms)}}
+
+
 def bench_ray(steps: int, proj, table, rig) -> dict:
     """The ray methods' projection words over the plain_4m table: the f32
     SH3 splat in (236 B), a key and 8 words out (36 B; raygs writes 7 of them
@@ -455,6 +493,9 @@ def main() -> int:
         out["ray_4m"] = bench_ray(max(3, args.steps // 5), proj, table, rig)
         log("keys-only pass")
         out["keys_pass_4m"] = bench_keys(max(3, args.steps // 5), proj, table, rig)
+        torch.cuda.empty_cache()
+        log("loaders")
+        out["loaders_4m"] = bench_loaders(max(3, args.steps // 5))
         torch.cuda.empty_cache()
         log("batched workload")
         out["batched_vs_plain"] = bench_batched(max(3, args.steps // 5))
