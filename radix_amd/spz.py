@@ -44,7 +44,8 @@ def read_spz(source, device="cuda", stream=None, object_to_world=None):
     n, deg = head["count"], head["sh_degree"]
     if deg > 3:
         raise ValueError(f"Unsupported SPZ version {head['version']} or SH degree {deg}")
-    dev_body = torch.frombuffer(bytearray(body), dtype=torch.uint8).to(device)
+    # (an empty file still gets a one-byte buffer: frombuffer refuses none)
+    dev_body = torch.frombuffer(bytearray(body) or bytearray(1), dtype=torch.uint8).to(device)
     planes = torch.zeros((14 + 3 * SH_VECTORS[deg], max(n, 1)), dtype=torch.float32, device=device)
     N.check(N.lib().rdx_spz_decode(_stream(stream), C.c_void_p(dev_body.data_ptr()), len(body), n, head["version"], deg,
                                    head["fractional_bits"], C.c_void_p(planes.data_ptr()), planes.shape[1]))

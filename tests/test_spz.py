@@ -106,6 +106,19 @@ def test_device_planes_equal_the_oracle(cuda, version, degree, n):
 
 
 @pytest.mark.gpu
+def test_empty_file(cuda):
+    import gzip
+    import struct
+
+    from radix_amd import spz
+
+    f = gzip.compress(struct.pack("<IIIBBBB", 0x5053474E, 3, 0, 2, 12, 1, 0))
+    table = spz.read_spz(f)
+    assert table.count == 0 and table.meta["sh_degree"] == 2
+    assert table.tensors["planes"].shape[0] == 14 + 3 * 8
+
+
+@pytest.mark.gpu
 def test_spz_table_projects_like_the_oracle(cuda):
     import torch
 
