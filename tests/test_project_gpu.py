@@ -264,6 +264,33 @@ def test_stereo_per_eye_segmented_sort(cuda):
         assert np.all(near[bad])
 
 
+def test_stereo_16m_full_size_order(cuda):
+    # The quantized stereo workload at its full size (16,777,216 packedSplats
+    # + packedSh SH3, two 2064x2208 eyes, per-eye keys): each eye's indices,
+    # count and draw arguments are the oracle's SortKeys.sort of the keys the
+    # GPU wrote. Size-independent: the keys' own parity is held at 70,000
+    # splats above.
+    spec = W.stereo_16m_spec()
+    planes = W.generate(spec)
+    words, chans = W.pack(planes, 3)
+    del planes
+    table = DeviceTable.from_packed(words, spec.count, (0.0, 1.0, -10.0, 2.0), chans, 3)
+    del words, chans
+    n = spec.count
+    out = Projector().project_sort(table, W.stereo_camera(), Params(sort_bits=32, key_per_eye=True))
+    torch.cuda.synchronize()
+    keys = out["keys"].cpu().numpy().view(np.uint32).reshape(2, n)
+    idx = out["indices"].cpu().numpy().view(np.uint32).reshape(2, n)
+    counts = out["count"].cpu().numpy()
+    args = out["args"].cpu().numpy().reshape(2, 5)
+    for e in range(2):
+        ordering, visible = O.sort(keys[e], 32)
+        assert visible > n // 2
+        assert counts[e] == visible
+        assert np.array_equal(idx[e, :visible], ordering[:visible])
+        assert list(args[e]) == [6144, -(-visible // 1024), 0, 0, 0]
+
+
 def test_plain_small_against_golden_fixture(cuda):
     # The committed vectors (tests/golden/plain_small.npz): keys bit-exact
     # away from cull thresholds, and then the order and visible count exactly.
