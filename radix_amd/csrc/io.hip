@@ -96,32 +96,29 @@ struct SpzCols {
   size_t centers, alphas, colors, scales, rotations, sh;
 };
 
-// One splat per thread into the f32 planes (centre 3, log-scale 3, quat 4,
-// opacity, rgb 3, SH 3K; plane stride `st`), as readSpz computes the values
+// The values of one splat into the f32 planes (centre 3, log-scale 3, quat 4,
+// opacity, rgb 3, SH 3K; plane stride `st`), as readSpz computes them
 // (spz.ts:297-347) before encoding them into a layout: centres as int24 x
 // 2^-fractionalBits (v1: half floats), opacity a / 255, colour
 // (b / 255 - 0.5) SH_C0 / 0.15 + 0.5, log-scale b / 16 - 10 (the scale
 // exp(b / 16 - 10)), rotations as smallest-three (v3+) or x, y, z in
-// [-1, 1] with w = sqrt(1 - |xyz|^2) (v2, v1), SH (b - 128) / 128.
-static __global__ __launch_bounds__(256) void k_spz_decode(const uint8_t* __restrict__ body, SpzCols c, uint32_t count,
-                                                           int version, int sh_values, float fixed,
-                                                           float* __restrict__ planes, size_t st) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= count) return;
-  float* out = planes + i;
+// [-1, 1] with w = sqrt(1 - |xyz|^2) (v2, v1), SH (b - 128) / 128. The
+// byte pointers are the splat's own in each column, in global memory or in
+// an LDS stage.
+__device__ __forceinline__ void spz_splat(const uint8_t* cen, const uint8_t* alpha, const uint8_t* col,
+                                          const uint8_t* sc, const uint8_t* rot, const uint8_t* sh, int version,
+                                          int sh_values, float fixed, float* __restrict__ out, size_t st) {
   if (version == 1) {
-    const uint8_t* p = body + c.centers + (size_t)i * 6;
 #pragma unroll
-    for (int k = 0; k < 3; k++) out[k * st] = half_to_float(p[2 * k] | (p[2 * k + 1] << 8));
+    for (int k = 0; k < 3; k++) out[k * st] = half_to_float(cen[2 * k] | (cen[2 * k + 1] << 8));
   } else {
-    const uint8_t* p = body + c.centers + (size_t)i * 9;
+    const uint8_t* p = cen;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       const int32_t v = (int32_t)(((uint32_t)p[3 * k] | ((uint32_t)p[3 * k + 1] << 8) | ((uint32_t)p[3 * k + 2] << 16)) << 8) >> 8;
       out[k * st] = (float)v * fixed;
     }
   }
-  const uint8_t* sc = body + c.scales + (size_t)i * 3;
 #pragma unroll
   for (int k = 0; k < 3; k++) out[(3 + k) * st] = (float)sc[k] / 16.0f - 10.0f;
   // The rotation and colour maps in double, rounded once: w = sqrt(1 - |xyz|^2)
@@ -130,7 +127,7 @@ static __global__ __launch_bounds__(256) void k_spz_decode(const uint8_t* __rest
   // its bytes.
   double q[4];
   if (version >= 3) {
-    const uint8_t* p = body + c.rotations + (size_t)i * 4;
+    const uint8_t* p = rot;
     uint32_t rest = (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
     const uint32_t largest = rest >> 30;
     double sum = 0.0;
@@ -145,7 +142,7 @@ static __global__ __launch_bounds__(256) void k_spz_decode(const uint8_t* __rest
     }
     q[largest] = sqrt(fmax(1.0 - sum, 0.0));
   } else {
-    const uint8_t* p = body + c.rotations + (size_t)i * 3;
+    const uint8_t* p = rot;
     q[0] = (double)p[0] / 127.5 - 1.0;
     q[1] = (double)p[1] / 127.5 - 1.0;
     q[2] = (double)p[2] / 127.5 - 1.0;
@@ -153,12 +150,79 @@ static __global__ __launch_bounds__(256) void k_spz_decode(const uint8_t* __rest
   }
 #pragma unroll
   for (int k = 0; k < 4; k++) out[(6 + k) * st] = (float)q[k];
-  out[10 * st] = (float)body[c.alphas + i] / 255.0f;
-  const uint8_t* col = body + c.colors + (size_t)i * 3;
+  out[10 * st] = (float)alpha[0] / 255.0f;
 #pragma unroll
   for (int k = 0; k < 3; k++) out[(11 + k) * st] = (float)(((double)col[k] / 255.0 - 0.5) * SPZ_COLOR_SCALE + 0.5);
-  const uint8_t* sh = body + c.sh + (size_t)i * sh_values;
   for (int v = 0; v < sh_values; v++) out[(14 + v) * st] = ((float)sh[v] - 128.0f) / 128.0f;
+}
+
+// The byte path, for a body at any alignment: each thread reads its own
+// bytes from global memory.
+static __global__ __launch_bounds__(256) void k_spz_decode(const uint8_t* __restrict__ body, SpzCols c, uint32_t count,
+                                                           int version, int sh_values, float fixed,
+                                                           float* __restrict__ planes, size_t st) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= count) return;
+  const size_t s = i;
+  spz_splat(body + c.centers + s * (version == 1 ? 6 : 9), body + c.alphas + s, body + c.colors + s * 3,
+            body + c.scales + s * 3, body + c.rotations + s * (version >= 3 ? 4 : 3), body + c.sh + s * sh_values,
+            version, sh_values, fixed, planes + i, st);
+}
+
+// The staged path, for a 4-byte-aligned body. A wave's byte loads in the byte
+// path span up to 45 cache lines each (45 SH bytes per splat). Here block b
+// first copies its 256 splats' span of every column into LDS, one dword per
+// lane over consecutive addresses, and each thread then reads its bytes
+// there.
+constexpr uint32_t SPZ_STAGE_WORDS = 64u * 65u + 6u * 2u;  // 256 splats x 65 bytes (9+1+3+3+4+45) + a head and tail word per column
+
+// Copies body[s, s + len) into the stage from word `w` on (the aligned words
+// that cover it; only bytes below body_bytes are read) and advances `w`;
+// returns the stage byte offset of body[s].
+__device__ __forceinline__ uint32_t spz_stage(const uint8_t* __restrict__ body, size_t body_bytes, size_t s,
+                                              uint32_t len, uint32_t* stage, uint32_t& w) {
+  const size_t a = s & ~(size_t)3;
+  const uint32_t head = (uint32_t)(s - a);
+  const uint32_t off = 4u * w + head;
+  if (len == 0) return off;
+  const uint32_t words = (head + len + 3u) / 4u;
+  for (uint32_t q = threadIdx.x; q < words; q += 256u) {
+    const size_t at = a + 4u * (size_t)q;
+    uint32_t v;
+    if (at + 4u <= body_bytes) {
+      v = *reinterpret_cast<const uint32_t*>(body + at);
+    } else {  // the body's last, partial word
+      v = 0u;
+      for (uint32_t k = 0; k < 4u; k++)
+        if (at + k < body_bytes) v |= (uint32_t)body[at + k] << (8u * k);
+    }
+    stage[w + q] = v;
+  }
+  w += words;
+  return off;
+}
+
+static __global__ __launch_bounds__(256) void k_spz_decode_staged(const uint8_t* __restrict__ body, size_t body_bytes,
+                                                                  SpzCols c, uint32_t count, int version, int sh_values,
+                                                                  float fixed, float* __restrict__ planes, size_t st) {
+  __shared__ uint32_t stage[SPZ_STAGE_WORDS];
+  const uint32_t b0 = blockIdx.x * 256u;
+  const uint32_t nb = min(256u, count - b0);
+  const uint32_t bc = version == 1 ? 6u : 9u, br = version >= 3 ? 4u : 3u, bs = (uint32_t)sh_values;
+  const size_t s0 = b0;
+  uint32_t w = 0;
+  const uint32_t o_cen = spz_stage(body, body_bytes, c.centers + s0 * bc, nb * bc, stage, w);
+  const uint32_t o_alpha = spz_stage(body, body_bytes, c.alphas + s0, nb, stage, w);
+  const uint32_t o_col = spz_stage(body, body_bytes, c.colors + s0 * 3u, nb * 3u, stage, w);
+  const uint32_t o_sc = spz_stage(body, body_bytes, c.scales + s0 * 3u, nb * 3u, stage, w);
+  const uint32_t o_rot = spz_stage(body, body_bytes, c.rotations + s0 * br, nb * br, stage, w);
+  const uint32_t o_sh = spz_stage(body, body_bytes, c.sh + s0 * bs, nb * bs, stage, w);
+  __syncthreads();
+  const uint32_t t = threadIdx.x;
+  if (t >= nb) return;
+  const uint8_t* sb = reinterpret_cast<const uint8_t*>(stage);
+  spz_splat(sb + o_cen + t * bc, sb + o_alpha + t, sb + o_col + 3u * t, sb + o_sc + 3u * t, sb + o_rot + t * br,
+            sb + o_sh + t * bs, version, sh_values, fixed, planes + b0 + t, st);
 }
 
 }  // namespace rdx
@@ -187,8 +251,13 @@ int rdx_spz_decode(void* stream, const uint8_t* body_dev, size_t body_bytes, uin
   const size_t need = c.sh + n * sh_values;
   if (body_bytes < need) return fail(RDX_E_INVALID, "SPZ body holds %zu bytes, %zu needed for %u splats", body_bytes, need, count);
   const float fixed = 1.0f / (float)(1u << fractional_bits);
-  hipLaunchKernelGGL(k_spz_decode, dim3((uint32_t)(((uint64_t)count + 255) / 256)), dim3(256), 0, (hipStream_t)stream, body_dev, c, count, version,
-                     sh_values, fixed, planes_dev, plane_stride);
+  const dim3 grid((uint32_t)(((uint64_t)count + 255) / 256));
+  if (((uintptr_t)body_dev & 3u) == 0)
+    hipLaunchKernelGGL(k_spz_decode_staged, grid, dim3(256), 0, (hipStream_t)stream, body_dev, body_bytes, c, count,
+                       version, sh_values, fixed, planes_dev, plane_stride);
+  else
+    hipLaunchKernelGGL(k_spz_decode, grid, dim3(256), 0, (hipStream_t)stream, body_dev, c, count, version, sh_values,
+                       fixed, planes_dev, plane_stride);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? RDX_OK : hip_fail(e, "rdx_spz_decode");
 }

@@ -106,6 +106,36 @@ def test_device_planes_equal_the_oracle(cuda, version, degree, n):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("version,degree,n", [(3, 3, 1001), (1, 2, 301)])  # 1001 x 65 B: a partial last word
+def test_misaligned_body_takes_the_byte_path(cuda, version, degree, n):
+    # A body at an odd address is decoded by the byte path; a 4-byte-aligned
+    # one by the LDS-staged path. Both must write the same planes.
+    import ctypes as C
+    import gzip
+
+    import torch
+
+    from radix_amd import _native as N
+
+    centers, log_scales, quats, opacity, rgb, sh = scene(n, 31 + version)
+    raw = gzip.decompress(encode_spz(centers, log_scales, quats, opacity, rgb, sh, degree, version=version))
+    body = raw[16:]
+    k = 3 * S.SH_VECTORS[degree]
+    outs = []
+    for pad in (0, 1):
+        dev = torch.zeros(len(body) + 4, dtype=torch.uint8, device="cuda")
+        dev[pad:pad + len(body)] = torch.frombuffer(bytearray(body), dtype=torch.uint8).to("cuda")
+        planes = torch.full((14 + k, n), float("nan"), dtype=torch.float32, device="cuda")
+        N.check(N.lib().rdx_spz_decode(None, C.c_void_p(dev.data_ptr() + pad), len(body), n, version, degree, 12,
+                                       C.c_void_p(planes.data_ptr()), n))
+        torch.cuda.synchronize()
+        outs.append(planes.cpu().numpy())
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+    ref = S.decode(body, n, version, degree, 12).astype(np.float32)
+    assert np.array_equal(outs[0][:6], ref[:6]) and np.array_equal(outs[0][14:], ref[14:])
+
+
+@pytest.mark.gpu
 def test_empty_file(cuda):
     import gzip
     import struct
