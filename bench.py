@@ -219,7 +219,8 @@ def bench_draw(steps: int, frame, words, idx, cnt, width: int, height: int, para
                         "(ewa fragments, premultiplied over, no early stop)",
             "ms": round(ms, 4), "Gsplats_per_s": round(st["drawn"] / ms / 1e6, 3), "entries": st["entries"],
             "frame_ms": round(ms_frame, 4), "frame_fps": round(1e3 / ms_frame, 1),
-            "stages_ms": {k: round(v["ms_avg"], 4) for k, v in stages.items() if k.startswith("draw")}}
+            "stages_ms": {(k if k.startswith("draw") else "entries_" + k): round(v["ms_avg"], 4)
+                          for k, v in stages.items()}}
 
 
 def bench_keys(steps: int, proj, table, rig) -> dict:

@@ -192,17 +192,18 @@ __global__ __launch_bounds__(BLOCK) void k_count(PassArgs a) {
   const uint32_t* keys = a.keys_in + g.off;
   const bool rule = a.rows > BINS;
 
+  // a tile past the segment's keys writes nothing: the scan stops before it
+  // (a sort sized by a capacity, its count on the device, launches them all,
+  // and those past the count load nothing)
+  const uint32_t n = seg_count(a, g);
+  if (base >= n && base > 0) return;  // uniform per block; a segment's first tile always writes
   const uint32_t wbase = base + (uint32_t)(w * WTILE) + (uint32_t)lane;
   uint32_t k[KPT];
 #pragma unroll
   for (int j = 0; j < KPT; j++) {
     const uint32_t i = wbase + (uint32_t)(j * RDX_WAVE);
-    k[j] = i < g.cap ? keys[i] : 0u;
+    k[j] = i < n ? keys[i] : 0u;
   }
-  const uint32_t n = seg_count(a, g);
-  // a tile past the segment's keys writes nothing: the scan stops before it
-  // (a sort sized by a capacity, its count on the device, launches them all)
-  if (base >= n && base > 0) return;  // uniform per block; a segment's first tile always writes
   __syncthreads();  // rows zeroed
   uint32_t* row = s_rows + (w * SUBS + (lane & (SUBS - 1))) * ROW;
   uint32_t hidden = 0, kmin = 0xffffffffu, kmax = 0u;
@@ -374,7 +375,19 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
   const int tid = threadIdx.x;
   const int w = tid / RDX_WAVE;
   const int lane = tid & (RDX_WAVE - 1);
-  const uint32_t tile = xcd_tile(blockIdx.x, a.tiles);
+  // One segment: the tiles that hold keys are mapped over all XCDs. Its count
+  // can be far below the capacity the grid covers (a count on the device,
+  // the kept keys after a dropping pass), and a map over the whole grid would
+  // give all of them to the first XCDs. The tiles past them keep their own
+  // index and return below. A bijection of [0, tiles) either way.
+  uint32_t tile;
+  if (!a.seg && !a.seg_of_tile) {
+    const uint32_t n0 = seg_count(a, seg_info(a, 0u));
+    const uint32_t used = max(1u, (uint32_t)(((uint64_t)n0 + TILE - 1) / TILE));
+    tile = blockIdx.x < used ? xcd_tile(blockIdx.x, used) : blockIdx.x;
+  } else {
+    tile = xcd_tile(blockIdx.x, a.tiles);
+  }
   for (int d = tid; d < WAVES * BINS; d += BLOCK) {
     (&s_wave[0][0])[d] = 0u;
     (&s_match[0][0])[d] = 0ull;
@@ -390,9 +403,14 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
   }
   const uint32_t local_tile = tile - g.tile0;
   const uint32_t tile_base = local_tile * (uint32_t)TILE;
+  // A tile past the segment's count (a sort sized by a capacity, its count
+  // on the device) has nothing to write; a segment's first tile still writes
+  // its count and draw arguments below.
+  const uint32_t n = seg_count(a, g);
+  if (tile_base >= n && local_tile != 0) return;  // uniform per block
 
   // Every load of the tile at once: first the digit totals and the tile's
-  // scanned counts, then the keys, then the values (below the capacity).
+  // scanned counts, then the keys, then the values (below the count).
   // Vector loads complete in issue order, so the digit-start scan waits only
   // for the first two and round j of the ranking only for keys[j]: the
   // ranking runs while the rest of the tile arrives.
@@ -405,20 +423,20 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
   const uint32_t wbase = tile_base + (uint32_t)(w * WTILE) + (uint32_t)lane;
   // The loads take no branch, or the compiler could not count the loads in
   // flight and would wait for all of them before the first use: a lane past
-  // the capacity reads the last element again (an empty segment: a word of
-  // the totals) and drops it. The pass's last reads of its input:
+  // the segment's count reads its last key again (an empty segment: a word
+  // of the totals) and drops it. The pass's last reads of its input:
   // nontemporal.
-  const uint32_t last = g.cap ? g.cap - 1u : 0u;
-  const uint32_t* ksrc = g.cap ? keys_in : tot;
+  const uint32_t last = n ? n - 1u : 0u;
+  const uint32_t* ksrc = n ? keys_in : tot;
 #pragma unroll
   for (int j = 0; j < KPT; j++) {
     const uint32_t i = wbase + (uint32_t)(j * RDX_WAVE);
     const uint32_t k = __builtin_nontemporal_load(ksrc + min(i, last));
-    keys[j] = i < g.cap ? k : 0u;
+    keys[j] = i < n ? k : 0u;
   }
   if constexpr (VALS) {
-    const uint32_t* vsrc = g.cap ? vals_in : tot;
-    // (a lane past the capacity is never kept: its value is not read)
+    const uint32_t* vsrc = n ? vals_in : tot;
+    // (a lane past the count is never kept: its value is not read)
 #pragma unroll
     for (int j = 0; j < KPT; j++) {
       const uint32_t i = wbase + (uint32_t)(j * RDX_WAVE);
@@ -428,7 +446,6 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
 #pragma unroll
     for (int j = 0; j < KPT; j++) vals[j] = wbase + (uint32_t)(j * RDX_WAVE);
   }
-  const uint32_t n = seg_count(a, g);
 
   // Digit starts of the segment: exclusive scan of the digit totals; with
   // the tile's prefix from the scanned counts, each digit's global start.
