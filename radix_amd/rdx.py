@@ -11,6 +11,7 @@ becomes a splat table in HBM without a CPU pass over its elements.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import json
 import os
@@ -113,11 +114,13 @@ def decode_chunk(header: dict, buf, into: RdxDeviceData, stream=None, pool: Opti
             raise ValueError(f"Chunk at {start}: {name}.{label} holds {len(raw)} bytes, expected {count * eb}")
         if count == 0:
             continue
-        src = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(dev, non_blocking=False)
-        staged.append(src)
-        _, out = into.bundles[name]["channels"][label]
-        N.check(N.lib().rdx_untranspose_bytes(_stream(stream), C.c_void_p(src.data_ptr()), count, eb,
-                                              C.c_void_p(out.data_ptr() + start * eb)))
+        # the upload and the transpose on one stream (the caller's, if given)
+        with torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext():
+            src = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(dev, non_blocking=False)
+            staged.append(src)
+            _, out = into.bundles[name]["channels"][label]
+            N.check(N.lib().rdx_untranspose_bytes(_stream(stream), C.c_void_p(src.data_ptr()), count, eb,
+                                                  C.c_void_p(out.data_ptr() + start * eb)))
     if staged:
         torch.cuda.synchronize(dev)
 

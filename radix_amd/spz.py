@@ -4,6 +4,7 @@ src/core/spz.ts): the host unzips the file and reads its 16-byte header
 puts the columns in HBM, and rdx_spz_decode writes the f32 splat table."""
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import gzip
 import os
@@ -44,11 +45,13 @@ def read_spz(source, device="cuda", stream=None, object_to_world=None):
     n, deg = head["count"], head["sh_degree"]
     if deg > 3:
         raise ValueError(f"Unsupported SPZ version {head['version']} or SH degree {deg}")
-    # (an empty file still gets a one-byte buffer: frombuffer refuses none)
-    dev_body = torch.frombuffer(bytearray(body) or bytearray(1), dtype=torch.uint8).to(device)
-    planes = torch.zeros((14 + 3 * SH_VECTORS[deg], max(n, 1)), dtype=torch.float32, device=device)
-    N.check(N.lib().rdx_spz_decode(_stream(stream), C.c_void_p(dev_body.data_ptr()), len(body), n, head["version"], deg,
-                                   head["fractional_bits"], C.c_void_p(planes.data_ptr()), planes.shape[1]))
+    # the upload and the decode on one stream (the caller's, if given)
+    with torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext():
+        # (an empty file still gets a one-byte buffer: frombuffer refuses none)
+        dev_body = torch.frombuffer(bytearray(body) or bytearray(1), dtype=torch.uint8).to(device)
+        planes = torch.zeros((14 + 3 * SH_VECTORS[deg], max(n, 1)), dtype=torch.float32, device=device)
+        N.check(N.lib().rdx_spz_decode(_stream(stream), C.c_void_p(dev_body.data_ptr()), len(body), n, head["version"],
+                                       deg, head["fractional_bits"], C.c_void_p(planes.data_ptr()), planes.shape[1]))
     torch.cuda.synchronize()
     t = DeviceTable.from_f32_planes(planes, n, deg, device=device, object_to_world=object_to_world)
     t.meta = {"format": "spz", **head}

@@ -132,3 +132,18 @@ def test_packed_file_with_encoding_meta(cuda):
     for e in range(2):
         assert torch.equal(x["words"][e], y["words"][e])
     assert torch.equal(x["keys"], y["keys"])
+
+
+@pytest.mark.gpu
+def test_decode_on_a_caller_stream(cuda):
+    # Uploads and transposes run on the caller's stream: the channels equal
+    # the sources (a stream of default priority).
+    import torch
+
+    count = 9001
+    a, b, _ = ext_source(count, 41)
+    f = encode_rdx({"extSplats": {"channels": {"extA": "uint32x4", "extB": "uint32x4"}, "data": {"extA": a, "extB": b}}},
+                   count, chunk_size=4096)
+    data = rdx.read_rdx(f, stream=torch.cuda.Stream())
+    assert np.array_equal(data.words("extSplats", "extA").cpu().numpy().view(np.uint32), a)
+    assert np.array_equal(data.words("extSplats", "extB").cpu().numpy().view(np.uint32), b)

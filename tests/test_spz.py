@@ -136,6 +136,20 @@ def test_misaligned_body_takes_the_byte_path(cuda, version, degree, n):
 
 
 @pytest.mark.gpu
+def test_decode_on_a_caller_stream(cuda):
+    # The upload and the decode run on the caller's stream (default priority).
+    import torch
+
+    from radix_amd import spz
+
+    n = 3000
+    f = encode_spz(*scene(n, 51), 3, version=3)
+    a = spz.read_spz(f).tensors["planes"]
+    b = spz.read_spz(f, stream=torch.cuda.Stream()).tensors["planes"]
+    assert torch.equal(a, b)
+
+
+@pytest.mark.gpu
 def test_empty_file(cuda):
     import gzip
     import struct
