@@ -11,6 +11,7 @@ import ctypes as C
 from typing import Optional
 
 from . import _native as N
+from .sort import on_stream
 
 
 class Drawer:
@@ -29,6 +30,7 @@ class Drawer:
         N.check(N.lib().rdx_draw_create(self.context.handle, C.byref(s), C.byref(h)))
         self._h = h
 
+    @on_stream
     def draw(self, words, order, count, width: int, height: int, params, out=None, stream=None):
         """words: (n, 4) int32 device tensor of one eye's ewa words; order:
         int32 device tensor of splat indices, back to front; count: int32

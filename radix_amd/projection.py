@@ -18,7 +18,7 @@ import numpy as np
 
 from . import _native as N
 from .camera import CameraRig
-from .sort import Context, _ptr, _stream, default_context
+from .sort import Context, _ptr, _stream, default_context, on_stream
 
 SH_COEFFS = [0, 3, 8, 15]
 
@@ -157,6 +157,7 @@ class Projector:
             setattr(f, k, v.data_ptr())
         return out, f
 
+    @on_stream
     def project(self, table: DeviceTable, rig: CameraRig, params: Params, side: bool = False, stream=None):
         """Words (per eye, (n, 4) int32), keys ((n,) or (eyes, n) int32) and optional float outputs."""
         import torch
@@ -180,6 +181,7 @@ class Projector:
         out["keys"] = keys.view(2, -1)[:, :n] if per_eye else keys[:n]
         return out
 
+    @on_stream
     def project_keys(self, table: DeviceTable, rig: CameraRig, params: Params, keys=None, at: int = 0, stream=None):
         """The keys-only pass (SplatProjection.writeKeys, src/render/SplatProjection.ts:165-172): the
         guarded head-distance key of each splat into keys[at:at + n] ((n,) int32 when keys is None)."""
@@ -196,6 +198,7 @@ class Projector:
 
     METHODS = {"3dgut": 1, "htgs": 2, "raygs": 3}
 
+    @on_stream
     def project_ray(self, table: DeviceTable, rig: CameraRig, params: Params, method: str, keys: bool = True,
                     stream=None):
         """The 3dgut, htgs or raygs words (SplatProjection.write, src/render/SplatProjection.ts:141-151):
@@ -218,6 +221,7 @@ class Projector:
         words = torch.cat(chans, dim=1)[:n, : (4 + 3 * eyes) if method == "raygs" else (6 + 2 * eyes)]
         return {"channels": [c[:n] for c in chans], "words": words, "keys": kd[:n] if keys else None}
 
+    @on_stream
     def project_sort(self, table: DeviceTable, rig: CameraRig, params: Params, words=None, keys=None, indices=None,
                      count=None, args=None, side: bool = False, stream=None):
         """The fused path: words, keys, the visible indices back to front, count and draw arguments.

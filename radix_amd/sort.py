@@ -27,6 +27,28 @@ def align_capacity(count: int) -> int:
     return max(1, -(-count // ROW_ELEMENTS)) * ROW_ELEMENTS
 
 
+def on_stream(fn):
+    """Runs a method with a `stream` argument under that stream, when one is
+    given: its allocations, fills and launches are then ordered on the
+    caller's stream rather than split between it and torch's current one."""
+    import functools
+    import inspect
+
+    sig = inspect.signature(fn)
+
+    @functools.wraps(fn)
+    def run(*args, **kwargs):
+        stream = sig.bind(*args, **kwargs).arguments.get("stream")
+        if stream is None:
+            return fn(*args, **kwargs)
+        import torch
+
+        with torch.cuda.stream(stream):
+            return fn(*args, **kwargs)
+
+    return run
+
+
 class Context:
     """A native context: the device scratch of sorts on one stream."""
 
@@ -52,6 +74,7 @@ class Context:
         N.check(self._lib.rdx_context_get_tuning(self._h, C.byref(t)))
         return (t.digit_bits, t.keys_per_thread, t.block_threads)
 
+    @on_stream
     def autotune(self, count: int, bits: int = 32, keys=None, reps: int = 5, stream=None,
                  cache: Optional[str] = None) -> Tuple[Tuple[int, int, int], float]:
         """Time every supported tuning on `count` keys (`keys`: a device
@@ -158,6 +181,7 @@ class SortKeys:
         return int(N.lib().rdx_key_compute(C.c_float(distance), bits))
 
     @staticmethod
+    @on_stream
     def compute_device(distances, bits: int, out=None, stream=None):
         """Keys of a float32 device tensor of distances (GpuSortKeys.compute)."""
         import torch
@@ -225,6 +249,7 @@ class RadixSort:
     def capacity(self) -> int:
         return min(self.keys.numel(), self.values.numel())
 
+    @on_stream
     def sort(self, count: int, bits: int = 32, stream=None) -> None:
         if count <= 0:
             return
@@ -233,6 +258,7 @@ class RadixSort:
         N.check(N.lib().rdx_sort_pairs(self.context.handle, _stream(stream), _ptr(self.keys), _ptr(self.values),
                                        count, bits))
 
+    @on_stream
     def sort_indirect(self, count_buffer, bits: int = 32, stream=None) -> None:
         N.check(N.lib().rdx_sort_pairs_indirect(self.context.handle, _stream(stream), _ptr(self.keys),
                                                 _ptr(self.values), _ptr(count_buffer), self.capacity, bits))
@@ -254,6 +280,7 @@ class SplatSorter:
         self.count = torch.zeros(1, dtype=torch.int32, device=self.device)
         self.indirect = torch.zeros(5, dtype=torch.int32, device=self.device)
 
+    @on_stream
     def sort(self, keys, count: int, bits: int, stream=None) -> None:
         import torch
 

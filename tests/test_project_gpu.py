@@ -264,6 +264,23 @@ def test_stereo_per_eye_segmented_sort(cuda):
         assert np.all(near[bad])
 
 
+def test_project_sort_on_a_caller_stream(cuda):
+    # Outputs allocated, filled and written on the caller's stream (default
+    # priority) equal those of torch's current stream.
+    spec = W.plain_4m_spec(seed=29, count=300000)
+    table = DeviceTable.from_f32_planes(W.generate(spec), spec.count, 3)
+    rig = W.plain_4m_camera(seed=29)
+    a = Projector().project_sort(table, rig, Params(sort_bits=32))
+    side = torch.cuda.Stream()
+    b = Projector().project_sort(table, rig, Params(sort_bits=32), stream=side)
+    side.synchronize()
+    torch.cuda.synchronize()
+    n = int(a["count"].item())
+    assert n == int(b["count"].item()) and n > spec.count // 2
+    assert torch.equal(a["indices"][:n], b["indices"][:n]) and torch.equal(a["keys"], b["keys"])
+    assert torch.equal(a["words"][0], b["words"][0])
+
+
 def test_stereo_16m_full_size_order(cuda):
     # The quantized stereo workload at its full size (16,777,216 packedSplats
     # + packedSh SH3, two 2064x2208 eyes, per-eye keys): each eye's indices,
