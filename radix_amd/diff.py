@@ -15,6 +15,7 @@ import numpy as np
 
 from . import _native as N
 from . import workloads as W
+from .sort import on_stream
 
 PARAMS = 59
 
@@ -62,6 +63,7 @@ class Rasterizer:
         N.check(N.lib().rdx_raster_create(self.context.handle, C.byref(s), C.byref(h)))
         self._h = h
 
+    @on_stream
     def forward(self, params, rig, prm, stream=None):
         """params: (59, n) float32 device tensor. Returns (image (3, H, W), alpha (H, W))."""
         import torch
@@ -81,6 +83,7 @@ class Rasterizer:
                                            C.c_void_p(alpha.data_ptr())))
         return image, alpha
 
+    @on_stream
     def backward(self, grad_image, stream=None):
         import torch
 
