@@ -46,6 +46,35 @@ static __global__ __launch_bounds__(256) void k_untranspose4(const uint32_t* __r
   }
 }
 
+// 16-byte elements (uint32x4 channels: extSplats, extSh, packedSplats),
+// staged: a thread's four elements are 64 contiguous output bytes, so direct
+// stores from neighbouring lanes land 64 bytes apart. The block's 1,024
+// elements (16 KB) go through LDS and leave as consecutive 16-byte stores.
+static __global__ __launch_bounds__(256) void k_untranspose16_staged(const uint32_t* __restrict__ planes, uint32_t quads,
+                                                                     uint4* __restrict__ out) {
+  __shared__ uint4 s[1024];
+  const uint32_t q0 = blockIdx.x * 256u;
+  const uint32_t nq = min(256u, quads - q0);  // the block's quads (the last block: fewer)
+  const uint32_t t = threadIdx.x;
+  if (t < nq) {
+    uint32_t p[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) p[k] = __builtin_nontemporal_load(planes + (size_t)k * quads + q0 + t);
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      uint32_t v[4];
+#pragma unroll
+      for (int w = 0; w < 4; w++)
+        v[w] = ((p[4 * w] >> (8 * e)) & 0xffu) | (((p[4 * w + 1] >> (8 * e)) & 0xffu) << 8) |
+               (((p[4 * w + 2] >> (8 * e)) & 0xffu) << 16) | (((p[4 * w + 3] >> (8 * e)) & 0xffu) << 24);
+      s[4 * t + e] = make_uint4(v[0], v[1], v[2], v[3]);
+    }
+  }
+  __syncthreads();
+  uint4* dst = out + (size_t)q0 * 4u;
+  for (uint32_t v = t; v < 4u * nq; v += 256u) dst[v] = s[v];
+}
+
 // Any element size and count: a thread per element, byte by byte.
 static __global__ __launch_bounds__(256) void k_untranspose_bytes(const uint8_t* __restrict__ planes, uint32_t count,
                                                                   uint32_t eb, uint8_t* __restrict__ out) {
@@ -71,7 +100,7 @@ int rdx_untranspose_bytes(void* stream, const uint8_t* planes_dev, uint32_t coun
   const uint32_t quads = count / 4;
   const dim3 g4((uint32_t)(((uint64_t)quads + 255) / 256)), g1((uint32_t)(((uint64_t)count + 255) / 256));
   if (aligned && element_bytes == 16) {
-    hipLaunchKernelGGL(k_untranspose4<16>, g4, dim3(256), 0, s, (const uint32_t*)planes_dev, quads, (uint32_t*)out_dev);
+    hipLaunchKernelGGL(k_untranspose16_staged, g4, dim3(256), 0, s, (const uint32_t*)planes_dev, quads, (uint4*)out_dev);
   } else if (aligned && element_bytes == 8) {
     hipLaunchKernelGGL(k_untranspose4<8>, g4, dim3(256), 0, s, (const uint32_t*)planes_dev, quads, (uint32_t*)out_dev);
   } else if (aligned && element_bytes == 4) {

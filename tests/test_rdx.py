@@ -46,7 +46,7 @@ def test_containers_and_errors():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("count,eb", [(4096, 16), (4097, 16), (1000, 8), (999, 4), (777, 3), (5, 1), (65536, 16), (3, 16)])
+@pytest.mark.parametrize("count,eb", [(4096, 16), (4097, 16), (1200, 16), (1000, 8), (999, 4), (777, 3), (5, 1), (65536, 16), (3, 16)])
 def test_untranspose_kernel(cuda, count, eb):
     import ctypes as C
 
