@@ -388,7 +388,7 @@ def main() -> int:
                          "the FETCH_SIZE / WRITE_SIZE counters to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU design is described, not built)", file=sys.stderr)
         return 2
 

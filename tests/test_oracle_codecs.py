@@ -90,7 +90,7 @@ def test_quat_oct_budgets(axis_bits, angle_bits, limit):
 
 
 def test_packed_codec_within_ranges():
-    # check-core.ts:442-470, with BASELINE's encoding [-10, 2] for log-scales
+    # check-core.ts:442-470, with the quantized stereo workload's encoding [-10, 2] for log-scales
     enc = d3([0.0, 1.0, -10.0, 2.0])
     r = Rng(23)
     w = np.zeros(4, dtype=np.uint32)
