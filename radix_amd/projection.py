@@ -2,7 +2,7 @@
 SplatProjection.write (src/render/SplatProjection.ts:141-153) and the
 per-frame GPU path of SplatRenderer.update (src/render/SplatRenderer.ts:412-456).
 
-Tables live on the device as torch tensors: f32 planes (a structure of arrays,
+Tables live on the device as torch tensors: f32 planes (a structure-of-arrays
 layout), or the reference's extSplats / packedSplats words with their SH
 bundles. Outputs keep the reference layouts: per eye a uint32x4 word array
 (channel w<e> of the projected bundle), one uint32 key per splat, and the
