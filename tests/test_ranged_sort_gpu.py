@@ -55,6 +55,18 @@ def test_distance_keys_three_passes(cuda, n):
     sort_and_check(keys)
 
 
+@pytest.mark.parametrize("frac", [0.97, 0.9995])
+def test_mostly_culled_keys(cuda, frac):
+    # A view that culls most splats: after the dropping pass the kept keys
+    # fill only the first tiles, which the later passes' scatter spreads over
+    # every XCD (and the tiles past them return). Three and four passes.
+    rng = Rng(37)
+    n = 3_000_017
+    for spread in (38.0, 1e6):
+        keys = with_invalid(O.keys_compute(2.0 + spread * rng.random(n), 32), rng, frac)
+        sort_and_check(keys)
+
+
 @pytest.mark.parametrize("reach", [SPAN - 1, SPAN])
 def test_range_boundary(cuda, reach):
     # the smallest kept key 0x3F800123 (base 0x3F800000); the largest at
