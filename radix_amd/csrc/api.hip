@@ -188,7 +188,8 @@ int SortScratch::reserve(uint32_t n, uint32_t segments, uint32_t tiles) {
   if ((rc = cnt.ensure((size_t)count_stride(MAX_BINS + 1) * tiles * 4))) return rc;
   if ((rc = tot.ensure((size_t)TOT_STRIDE * std::max<uint32_t>(segments, 1) * 4))) return rc;
   if ((rc = nvalid.ensure((size_t)std::max<uint32_t>(segments, 1) * 4))) return rc;
-  if ((rc = mm.ensure((size_t)2 * (tiles + std::max<uint32_t>(segments, 1)) * 4))) return rc;
+  // ranged sorts: tile and segment ranges, then the gated pass's done counts
+  if ((rc = mm.ensure(((size_t)2 * tiles + (size_t)3 * std::max<uint32_t>(segments, 1)) * 4))) return rc;
   return RDX_OK;
 }
 
@@ -248,6 +249,7 @@ int run_sort(Context* ctx, SortScratch& sc, hipStream_t s, const SortPlan& p, co
   const uint32_t* vin = vals_in;
   uint32_t* tile_mm = (uint32_t*)sc.mm.ptr;
   uint32_t* seg_mm = tile_mm + 2 * (size_t)L.tiles;
+  uint32_t* done = seg_mm + 2 * (size_t)std::max<uint32_t>(L.segments, 1);
   for (int pass = 0; pass < p.passes; pass++) {
     const bool last = pass == p.passes - 1;
     PassArgs a;
@@ -287,6 +289,7 @@ int run_sort(Context* ctx, SortScratch& sc, hipStream_t s, const SortPlan& p, co
       a.seg_mm = seg_mm;
       a.gate = pass == 3;
       a.alt = pass == 2;
+      a.done = done;
       if (pass < 2) {
         a.keys_out = (uint32_t*)sc.keys[pass].ptr;
         a.vals_out = (uint32_t*)sc.vals[pass].ptr;
@@ -311,7 +314,7 @@ int run_sort(Context* ctx, SortScratch& sc, hipStream_t s, const SortPlan& p, co
       Stage st(ctx, s, name);
       if (!launch_pass_kernel(p, PassKernel::count, L.tiles, s, a)) return fail(RDX_E_INVALID, "unsupported sort tuning");
     }
-    {
+    if (!a.gate) {  // the gated pass's count blocks scan (gated_scan)
       snprintf(name, sizeof name, "sort_scan%d", pass);
       Stage st(ctx, s, name);
       // column groups per segment, and the range reduction in a ranged sort's first pass

@@ -54,6 +54,8 @@ constexpr uint32_t PEEL_MIN = 8;
 constexpr int RANGED_RB = 9;
 constexpr int RANGED_BITS = 3 * RANGED_RB;
 constexpr uint32_t RANGED_LOW = (1u << RANGED_RB) - 1u;
+// The fourth (gated) pass's digits: bits RANGED_BITS..31 of key - base.
+constexpr int GATED_BINS = 1 << (32 - RANGED_BITS);
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs
 // on XCD b % 8), each with its own L2. The scatter maps them so that each
@@ -101,6 +103,10 @@ struct PassArgs {
   int alt;                   // segments that reach beyond write to keys_alt / vals_alt instead
   uint32_t* keys_alt;
   uint32_t* vals_alt;
+  // gated pass: per segment, the count blocks done so far; the last one
+  // scans the segment's GATED_BINS digit columns (no separate scan launch).
+  // Zeroed by the first ranged pass's scan.
+  uint32_t* done;
 };
 
 struct SegInfo {
@@ -156,6 +162,46 @@ __device__ __forceinline__ bool hidden_rule(uint32_t k, int bits) {
 }
 
 // ---- count ---------------------------------------------------------------------
+
+// The gated pass's scan of one segment, by the segment's last count block:
+// its digits are bits RANGED_BITS..31 of key - base, so only the first
+// GATED_BINS columns hold counts (the count wrote zeros in the rest, whose
+// exclusive prefixes are zeros too). Thread (c, d) sums column d over the
+// c-th run of tiles; the runs' sums are scanned per column; each thread
+// writes its run's prefixes in place. The totals go to tot[s] (zeros past
+// GATED_BINS).
+template <int BLOCK>
+__device__ __forceinline__ void gated_scan(const PassArgs& a, const SegInfo& g, uint32_t used, uint32_t* lds) {
+  constexpr int CH = BLOCK / GATED_BINS;
+  static_assert(CH * GATED_BINS == BLOCK && CH >= 1, "whole runs of columns");
+  const int tid = threadIdx.x;
+  const int d = tid % GATED_BINS, c = tid / GATED_BINS;
+  const uint32_t per = (used + CH - 1) / CH;
+  const uint32_t t0 = min(used, (uint32_t)c * per), t1 = min(used, t0 + per);
+  uint32_t* col = a.cnt + (size_t)g.tile0 * a.cstride + d;
+  uint32_t sum = 0;
+  for (uint32_t t = t0; t < t1; t++) sum += col[(size_t)t * a.cstride];
+  __syncthreads();  // lds (the count rows) is free
+  lds[c * GATED_BINS + d] = sum;
+  __syncthreads();
+  if (c == 0) {
+    uint32_t run = 0;
+    for (int k = 0; k < CH; k++) {
+      const uint32_t x = lds[k * GATED_BINS + d];
+      lds[k * GATED_BINS + d] = run;
+      run += x;
+    }
+    a.tot[(size_t)g.s * TOT_STRIDE + d] = run;
+  }
+  for (int r = GATED_BINS + tid; r < MAX_BINS; r += BLOCK) a.tot[(size_t)g.s * TOT_STRIDE + r] = 0u;
+  __syncthreads();
+  uint32_t run = lds[c * GATED_BINS + d];
+  for (uint32_t t = t0; t < t1; t++) {
+    const uint32_t x = col[(size_t)t * a.cstride];
+    col[(size_t)t * a.cstride] = run;
+    run += x;
+  }
+}
 
 // Tile counts of one pass's digit. Wave w holds keys [w * 64 * KPT, ...) of
 // the tile and counts them with LDS atomics into SUBS private rows per wave
@@ -246,6 +292,22 @@ __global__ __launch_bounds__(BLOCK) void k_count(PassArgs a) {
     for (int v = 0; v < WAVES; v++) c += s_rule[v];
     a.cnt[(size_t)tile * a.cstride + BINS] = c;
   }
+  if (a.gate) {
+    // The gated pass has no scan launch: the segment's last count block to
+    // finish scans its digit columns. Every block's row is released before
+    // its done count (agent scope, across the XCDs' L2s); the last block
+    // acquires them all. No block waits on another.
+    __shared__ uint32_t s_last;
+    const uint32_t used = max(1u, (uint32_t)(((uint64_t)n + TILE - 1) / TILE));
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) s_last = atomicAdd(&a.done[g.s], 1u) == used - 1u ? 1u : 0u;
+    __syncthreads();
+    if (s_last) {
+      __threadfence();
+      gated_scan<BLOCK>(a, g, used, s_rows);
+    }
+  }
 }
 
 // ---- scan ----------------------------------------------------------------------
@@ -303,6 +365,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(PassArgs a, int bins) {
       }
       a.seg_mm[2 * s] = mn;
       a.seg_mm[2 * s + 1] = mx;
+      if (a.done) a.done[s] = 0u;  // the gated pass's count blocks count from zero
     }
     return;
   }
