@@ -484,3 +484,118 @@ void orc_project_table_ext(uint32_t count, const uint32_t* ext_a, const uint32_t
     emit(&o, v, p, count, i, out);
   }
 }
+
+/* --- The keys-only pass (SplatProjection.writeKeys, SplatProjection.ts:165-172) ---
+ * The "keys" program (SplatProjector.ts:152-194) has no wanted form, so a
+ * stored-splat generator stays in the splat form: transformSplat
+ * (splatTransform.tsl.ts:154-165) keeps the scales, times cbrt(|det M|), for
+ * a rotation times a uniform scale (isUniformScale, :46-62: G = M^T M, every
+ * entry of G - s2 I within 1e-6 s2, s2 = trace(G) / 3, and det M > 0) and
+ * otherwise takes fromCov of M Sigma M^T (splats.tsl.ts:189-198), whose
+ * scales are the square roots of its eigenvalues. inSortGuard's extent is
+ * then the largest head-space scale (projection.tsl.ts:145). */
+static double det3_cm(const double* m) {
+  return AT(m, 0, 0) * (AT(m, 1, 1) * AT(m, 2, 2) - AT(m, 1, 2) * AT(m, 2, 1)) -
+         AT(m, 0, 1) * (AT(m, 1, 0) * AT(m, 2, 2) - AT(m, 1, 2) * AT(m, 2, 0)) +
+         AT(m, 0, 2) * (AT(m, 1, 0) * AT(m, 2, 1) - AT(m, 1, 1) * AT(m, 2, 0));
+}
+
+static int is_uniform_scale(const double* m) {
+  double g[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) g[i][j] = AT(m, 0, i) * AT(m, 0, j) + AT(m, 1, i) * AT(m, 1, j) + AT(m, 2, i) * AT(m, 2, j);
+  double s2 = (g[0][0] + g[1][1] + g[2][2]) / 3.0;
+  if (!(s2 > 0)) return 0;
+  for (int i = 0; i < 3; i++)
+    for (int j = i; j < 3; j++)
+      if (fabs(g[i][j] - (i == j ? s2 : 0.0)) > 1e-6 * s2) return 0;
+  return det3_cm(m) > 0;
+}
+
+/* Largest eigenvalue of a symmetric 3x3 (xx, yy, zz, xy, xz, yz) by cyclic Jacobi rotations */
+static double sym3_max_eigen(const double* c) {
+  double a[3][3] = {{c[0], c[3], c[4]}, {c[3], c[1], c[5]}, {c[4], c[5], c[2]}};
+  for (int sweep = 0; sweep < 50; sweep++) {
+    double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+    double scale = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+    if (off <= 1e-30 * scale || off == 0.0) break;
+    for (int p = 0; p < 2; p++)
+      for (int q = p + 1; q < 3; q++) {
+        if (a[p][q] == 0.0) continue;
+        double theta = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
+        for (int k = 0; k < 3; k++) {
+          double akp = a[k][p], akq = a[k][q];
+          a[k][p] = cs * akp - sn * akq;
+          a[k][q] = sn * akp + cs * akq;
+        }
+        for (int k = 0; k < 3; k++) {
+          double apk = a[p][k], aqk = a[q][k];
+          a[p][k] = cs * apk - sn * aqk;
+          a[q][k] = sn * apk + cs * aqk;
+        }
+      }
+  }
+  return fmax(a[0][0], fmax(a[1][1], a[2][2]));
+}
+
+void orc_project_keys_table_f32(const orc_table_f32* t, const orc_view* v, const orc_params* p, uint32_t* keys,
+                                double* margin) {
+  const uint32_t n = t->count;
+  const size_t st = t->stride ? t->stride : n;
+  const double* M = v->object_to_head;
+  const int uniform = is_uniform_scale(M);
+  const double uscale = cbrt(fabs(det3_cm(M)));
+  for (uint32_t i = 0; i < n; i++) {
+    double c0[3], center[3], scales[3], quat[4], cov[6];
+    for (int k = 0; k < 3; k++) {
+      c0[k] = t->center[k * st + i];
+      scales[k] = exp((double)t->log_scale[k * st + i]);
+    }
+    for (int k = 0; k < 4; k++) quat[k] = t->quat[k * st + i];
+    xform_point(M, c0, center);
+    /* prepareSplat (projection.tsl.ts:42-64), splat form: the scales grow by inflate */
+    double opacity = t->opacity[i], inflate = 1.0;
+    if (opacity > 1.0) {
+      opacity = fmin(opacity * 4.0 - 3.0, 5.0);
+      if (p->lod_inflate) {
+        inflate = pow(exp((opacity * opacity - 1.0) / M_E), 1.0 / 3.0);
+        opacity = 1.0;
+      }
+    }
+    double extent;
+    if (uniform) {
+      extent = fmax(scales[0], fmax(scales[1], scales[2])) * uscale;
+    } else {
+      double c6[6];
+      orc_cov_from_scale_quat(scales, quat, c6);
+      cov_xform(M, c6, cov);
+      extent = sqrt(fmax(sym3_max_eigen(cov), 0.0));
+    }
+    extent *= inflate;
+    int degenerate = scales[0] == 0 && scales[1] == 0 && scales[2] == 0;
+    int invisible = opacity <= 0.0 || opacity < p->min_alpha || degenerate;
+    double m = rel_margin(opacity, p->min_alpha);
+    int guarded = 1;
+    if (p->sort_guard > 0) {
+      /* inSortGuard (projection.tsl.ts:144-155) with the head frame: eye 0's lens */
+      const double* P = v->eye[0].projection;
+      double hclip[4];
+      clip_of(P, center, hclip);
+      double radius = extent * p->max_std_dev;
+      double w = fmax(hclip[3], 1e-6);
+      double rx = AT(P, 0, 0) * radius / w, ry = AT(P, 1, 1) * radius / w;
+      int inside = fabs(hclip[0] / w) - rx <= p->sort_guard + 1.0 && fabs(hclip[1] / w) - ry <= p->sort_guard + 1.0;
+      int near = -center[2] <= radius;
+      guarded = center[2] < radius && (inside || near);
+      m = fmin(m, rel_margin(fabs(hclip[0] / w) - rx, p->sort_guard + 1.0));
+      m = fmin(m, rel_margin(fabs(hclip[1] / w) - ry, p->sort_guard + 1.0));
+      m = fmin(m, rel_margin(-center[2], radius));
+      m = fmin(m, rel_margin(center[2], radius));
+    }
+    double dist = sqrt(center[0] * center[0] + center[1] * center[1] + center[2] * center[2]);
+    keys[i] = !invisible && guarded ? orc_key_compute(dist, p->sort_bits) : orc_key_invalid(p->sort_bits);
+    if (margin) margin[i] = m;
+  }
+}

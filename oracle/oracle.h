@@ -152,6 +152,11 @@ typedef struct orc_batch_out {
 } orc_batch_out;
 
 void orc_project_table_f32(const orc_table_f32* t, const orc_view* v, const orc_params* p, orc_batch_out* out);
+/* The keys-only pass (SplatProjection.writeKeys) over an f32 table: the
+ * splat-form sortGuard; margin[i]: relative distance to the nearest cull
+ * threshold (may be NULL). */
+void orc_project_keys_table_f32(const orc_table_f32* t, const orc_view* v, const orc_params* p, uint32_t* keys,
+                                double* margin);
 
 /* Decode then project a packed + packedSh table or an ext + extSh table
  * (uint32x4 arrays; sh_channels: one uint32x4 array per texel). */

@@ -118,6 +118,8 @@ def lib() -> C.CDLL:
             "orc_params_default": (None, [C.POINTER(OrcParams)]),
             "orc_project_ewa": (None, [C.POINTER(OrcSplat), C.POINTER(OrcView), C.POINTER(OrcParams),
                                        C.POINTER(OrcOut)]),
+            "orc_project_keys_table_f32": (None, [C.POINTER(OrcTableF32), C.POINTER(OrcView), C.POINTER(OrcParams),
+                                                   C.c_void_p, C.c_void_p]),
             "orc_project_table_f32": (None, [C.POINTER(OrcTableF32), C.POINTER(OrcView), C.POINTER(OrcParams),
                                              C.POINTER(OrcBatchOut)]),
             "orc_project_table_packed": (None, [C.c_uint32, C.c_void_p, _dp, C.c_int, C.c_int, C.c_void_p,
@@ -237,6 +239,19 @@ def project_f32(planes: np.ndarray, sh_degree: int, view: OrcView, prm: OrcParam
                     base + 14 * st if sh_degree > 0 else None, sh_degree)
     return _run_batch(lambda o: lib().orc_project_table_f32(C.byref(t), C.byref(view), C.byref(prm), C.byref(o)),
                       n, eyes, per_eye)
+
+
+def project_keys_f32(planes: np.ndarray, view: OrcView, prm: OrcParams):
+    """The keys-only pass over an f32-planes table: keys (n,) and margins (n,)."""
+    n = planes.shape[1]
+    planes = np.ascontiguousarray(planes, dtype=np.float32)
+    base = planes.ctypes.data
+    st = n * 4
+    t = OrcTableF32(n, n, base, base + 3 * st, base + 6 * st, base + 10 * st, base + 11 * st, None, 0)
+    keys = np.zeros(n, dtype=np.uint32)
+    margin = np.zeros(n, dtype=np.float64)
+    lib().orc_project_keys_table_f32(C.byref(t), C.byref(view), C.byref(prm), keys.ctypes.data, margin.ctypes.data)
+    return {"keys": keys, "margin": margin}
 
 
 class OrcRaySplat(C.Structure):

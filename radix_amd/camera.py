@@ -45,6 +45,18 @@ def perspective(fov_deg: float, aspect: float, near: float, far: float, webgpu: 
     return np.array([[x, 0, a, 0], [0, y, b, 0], [0, 0, c, d], [0, 0, -1, 0]], dtype=np.float64)
 
 
+def orthographic(left: float, right: float, top: float, bottom: float, near: float, far: float,
+                 webgpu: bool = True) -> np.ndarray:
+    """OrthographicCamera.updateProjectionMatrix + Matrix4.makeOrthographic (row-major)."""
+    w, h, p = 1.0 / (right - left), 1.0 / (top - bottom), 1.0 / (far - near)
+    x, y = (right + left) * w, (top + bottom) * h
+    if webgpu:
+        z, z_inv = near * p, -p
+    else:
+        z, z_inv = (far + near) * p, -2.0 * p
+    return np.array([[2 * w, 0, 0, -x], [0, 2 * h, 0, -y], [0, 0, z_inv, -z], [0, 0, 0, 1]], dtype=np.float64)
+
+
 def look_at_world(eye, target, up=(0.0, 1.0, 0.0)) -> np.ndarray:
     """A camera's world matrix looking from `eye` at `target` (Matrix4.lookAt)."""
     eye = np.asarray(eye, dtype=np.float64)
@@ -112,6 +124,15 @@ def mono(world: np.ndarray, fov_deg: float, width: int, height: int, near: float
     proj = perspective(fov_deg, width / height, near, far, webgpu)
     return CameraRig(head=view, eyes=[View(view.copy(), proj)], width=width, height=height,
                      ndc_z_min=0.0 if webgpu else -1.0)
+
+
+def mono_ortho(world: np.ndarray, half_height: float, width: int, height: int, near: float = 0.01,
+               far: float = 1000.0) -> CameraRig:
+    """A plain orthographic camera (frustum [-h a, h a] x [-h, h], a = width / height)."""
+    view = np.linalg.inv(world)
+    a = width / height
+    proj = orthographic(-half_height * a, half_height * a, half_height, -half_height, near, far, True)
+    return CameraRig(head=view, eyes=[View(view.copy(), proj, ortho=True)], width=width, height=height, ndc_z_min=0.0)
 
 
 def stereo(head_world: np.ndarray, ipd: float, fov_deg: float, width: int, height: int, near: float = 0.01,

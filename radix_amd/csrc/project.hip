@@ -135,6 +135,21 @@ int make_uniforms(const rdx_splat_table* t, const rdx_camera* cam, const rdx_par
   u->cull_keys = prm->cull_keys ? 1 : 0;
   u->key_per_eye = prm->key_per_eye ? 1 : 0;
   for (int k = 0; k < 4; k++) u->enc[k] = t->packed_encoding[k];
+  // isUniformScale (splatTransform.tsl.ts:46-62): G = M^T M within 1e-6 s2 of
+  // s2 I (s2 = trace(G) / 3) and det M > 0; the scale is cbrt |det M|
+  {
+    double g[3][3];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) g[i][j] = cm(A, 0, i) * cm(A, 0, j) + cm(A, 1, i) * cm(A, 1, j) + cm(A, 2, i) * cm(A, 2, j);
+    const double s2 = (g[0][0] + g[1][1] + g[2][2]) / 3.0;
+    bool uniform = s2 > 0.0;
+    for (int i = 0; i < 3 && uniform; i++)
+      for (int j = i; j < 3; j++)
+        if (fabs(g[i][j] - (i == j ? s2 : 0.0)) > 1e-6 * s2) uniform = false;
+    const double dA = det3(u->H);
+    u->head_uniform = uniform && dA > 0.0 ? 1 : 0;
+    u->head_scale = (float)cbrt(fabs(dA));
+  }
   return RDX_OK;
 }
 

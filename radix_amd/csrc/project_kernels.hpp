@@ -49,6 +49,10 @@ struct ProjU {
   float focal_distance, aperture_tan, sort_guard;
   int lod_inflate, sh_degree, sort_bits, cull_keys, key_per_eye;
   float enc[4];       // packed encoding: rgbMin, rgbMax, lnScaleMin, lnScaleMax
+  // object -> head is a rotation times a uniform scale (isUniformScale,
+  // splatTransform.tsl.ts:46-62), and that scale (cbrt |det|)
+  int head_uniform;
+  float head_scale;
 };
 
 struct TableP {
@@ -579,24 +583,10 @@ __device__ __forceinline__ void prepare(const Splat& sp, const U& u, float& op, 
   invisible = op <= 0.0f || op < u.min_alpha || (s.x == 0.0f && s.y == 0.0f && s.z == 0.0f);
 }
 
-// sortGuard (projection.tsl.ts:144-155): a CPU-sorted splat keeps its key
-// only if its head-space bound (max std devs along the head's axes) reaches
-// the guard-widened screen of eye 0, or the head is inside it.
-__device__ __forceinline__ bool sort_guarded(const Splat& sp, float3 s, const double* hc, const ProjU& u) {
-  float Rg[9];
-  quat_matrix(sp.q, Rg);
-  // head-space cov diagonal: (H3 R S)(H3 R S)^T
-  float diag[3];
-  for (int r = 0; r < 3; r++) {
-    float acc = 0.0f;
-    for (int k = 0; k < 3; k++) {
-      const float hr = (float)u.H[3 * r] * Rg[k] + (float)u.H[3 * r + 1] * Rg[3 + k] + (float)u.H[3 * r + 2] * Rg[6 + k];
-      const float sk = k == 0 ? s.x : (k == 1 ? s.y : s.z);
-      acc += hr * hr * sk * sk;
-    }
-    diag[r] = acc;
-  }
-  const float radius = sqrtf(fmaxf(fmaxf(diag[0], diag[1]), diag[2])) * u.max_std_dev;
+// inSortGuard (projection.tsl.ts:144-155): a CPU-sorted splat keeps its key
+// only if its head-space bound (`radius` = extent x maxStdDev) reaches the
+// guard-widened screen of eye 0, or the head is inside it.
+__device__ __forceinline__ bool in_sort_guard(float radius, const double* hc, const ProjU& u) {
   const float* P0 = u.eye[0].Pf;
   const float hx0 = (float)hc[0], hy0 = (float)hc[1], hz0 = (float)hc[2];
   const float hw = P0[12] * hx0 + P0[13] * hy0 + P0[14] * hz0 + P0[15];
@@ -607,6 +597,62 @@ __device__ __forceinline__ bool sort_guarded(const Splat& sp, float3 s, const do
   const bool inside = fabsf(hx / w) - rx <= u.sort_guard + 1.0f && fabsf(hy / w) - ry <= u.sort_guard + 1.0f;
   const bool near = -hz0 <= radius;
   return hz0 < radius && (inside || near);
+}
+
+// The covariance form's extent (the ewa method's input): the square root of
+// the largest diagonal entry of the head-space covariance (H R S)(H R S)^T.
+__device__ __forceinline__ float cov_extent(const Splat& sp, float3 s, const ProjU& u) {
+  float Rg[9];
+  quat_matrix(sp.q, Rg);
+  float diag[3];
+  for (int r = 0; r < 3; r++) {
+    float acc = 0.0f;
+    for (int k = 0; k < 3; k++) {
+      const float hr = (float)u.H[3 * r] * Rg[k] + (float)u.H[3 * r + 1] * Rg[3 + k] + (float)u.H[3 * r + 2] * Rg[6 + k];
+      const float sk = k == 0 ? s.x : (k == 1 ? s.y : s.z);
+      acc += hr * hr * sk * sk;
+    }
+    diag[r] = acc;
+  }
+  return sqrtf(fmaxf(fmaxf(diag[0], diag[1]), diag[2]));
+}
+
+// The splat form's extent (the keys pass: a stored splat stays a splat,
+// SplatProjector.ts:152-194): its largest head-space scale. A uniform
+// transform scales the scales (transformSplat, splatTransform.tsl.ts:154-165);
+// any other goes through the covariance (fromCov, splats.tsl.ts:189-198),
+// whose scales are the square roots of the eigenvalues of M M^T, M = H R S:
+// the largest by the trigonometric solution of the characteristic cubic
+// (Smith, "Eigenvalues of a symmetric 3x3 matrix", CACM 1961), in double.
+__device__ __forceinline__ float splat_extent(const Splat& sp, float3 s, const ProjU& u) {
+  if (u.head_uniform) return fmaxf(fmaxf(s.x, s.y), s.z) * u.head_scale;
+  float Rg[9];
+  quat_matrix(sp.q, Rg);
+  double M[9];
+  for (int r = 0; r < 3; r++)
+    for (int k = 0; k < 3; k++) {
+      const double sk = k == 0 ? s.x : (k == 1 ? s.y : s.z);
+      M[3 * r + k] = (u.H[3 * r] * Rg[k] + u.H[3 * r + 1] * Rg[3 + k] + u.H[3 * r + 2] * Rg[6 + k]) * sk;
+    }
+  double A[6];  // xx, yy, zz, xy, xz, yz of M M^T
+  const int ij[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+  for (int e = 0; e < 6; e++) {
+    const int a = ij[e][0], b = ij[e][1];
+    A[e] = M[3 * a] * M[3 * b] + M[3 * a + 1] * M[3 * b + 1] + M[3 * a + 2] * M[3 * b + 2];
+  }
+  const double p1 = A[3] * A[3] + A[4] * A[4] + A[5] * A[5];
+  const double q = (A[0] + A[1] + A[2]) / 3.0;
+  const double d0 = A[0] - q, d1 = A[1] - q, d2 = A[2] - q;
+  const double p2 = d0 * d0 + d1 * d1 + d2 * d2 + 2.0 * p1;
+  double lmax = q;
+  if (p2 > 0.0) {
+    const double p = sqrt(p2 / 6.0), ip = 1.0 / p;
+    const double b0 = d0 * ip, b1 = d1 * ip, b2 = d2 * ip, b3 = A[3] * ip, b4 = A[4] * ip, b5 = A[5] * ip;
+    const double det = b0 * (b1 * b2 - b5 * b5) - b3 * (b3 * b2 - b5 * b4) + b4 * (b3 * b5 - b1 * b4);
+    const double r = fmin(fmax(det * 0.5, -1.0), 1.0);
+    lmax = q + 2.0 * p * cos(acos(r) / 3.0);
+  }
+  return (float)sqrt(fmax(lmax, 0.0));
 }
 
 // The projection kernel. DEG: the f32 table's SH degree (compile time), or
@@ -704,7 +750,7 @@ __global__ __launch_bounds__(256) void k_project(TableP t, ProjU u, OutP o) {
     }
 
     // sortGuard for the CPU sort's keys (projection.tsl.ts:144-155)
-    const bool guarded = u.cull_keys || !(u.sort_guard > 0.0f) || sort_guarded(sp, s, hc, u);
+    const bool guarded = u.cull_keys || !(u.sort_guard > 0.0f) || in_sort_guard(cov_extent(sp, s, u) * u.max_std_dev, hc, u);
 
     // Keys (SplatProjection.ts:143-151)
     const bool keyed = u.cull_keys ? visible : (!invisible && guarded);
@@ -749,8 +795,9 @@ __global__ __launch_bounds__(256) void k_project(TableP t, ProjU u, OutP o) {
 // The keys-only pass (SplatProjection.writeKeys, SplatProjection.ts:165-172,
 // the "keys" pass of SplatProjector.ts:184-189): the CPU sort's keys, from
 // the splat's centre, scales, rotation and opacity alone (no SH, no EWA):
-// key = cullInvisible || !guarded ? invalid : compute(|head centre|). The
-// head-space centre and distance are computed as k_project computes them.
+// key = cullInvisible || !guarded ? invalid : compute(|head centre|), with
+// the splat form's sortGuard extent (splat_extent). The head-space centre
+// and distance are computed as k_project computes them.
 template <int LAYOUT>
 __global__ __launch_bounds__(256) void k_project_keys(TableP t, ProjU u, uint32_t* keys, uint32_t at) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
@@ -774,7 +821,7 @@ __global__ __launch_bounds__(256) void k_project_keys(TableP t, ProjU u, uint32_
     for (int r = 0; r < 3; r++) hc[r] = fma(u.H[3 * r], dx, fma(u.H[3 * r + 1], dy, u.H[3 * r + 2] * dz));
     head_dist = sqrt(fma(hc[0], hc[0], fma(hc[1], hc[1], hc[2] * hc[2])));
   }
-  const bool guarded = !(u.sort_guard > 0.0f) || sort_guarded(sp, s, hc, u);
+  const bool guarded = !(u.sort_guard > 0.0f) || in_sort_guard(splat_extent(sp, s, u) * u.max_std_dev, hc, u);
   keys[at + i] = !invisible && guarded ? sort_key(fbits((float)head_dist), u.sort_bits) : key_invalid(u.sort_bits);
 }
 
