@@ -150,6 +150,19 @@ int rdx_sort_pairs_indirect(rdx_context* ctx, void* stream, uint32_t* keys_dev, 
 int rdx_sort_splats(rdx_context* ctx, void* stream, const uint32_t* keys_dev, uint32_t n, int bits,
                     uint32_t* out_indices_dev, uint32_t* out_count_dev, uint32_t* out_draw_args_dev);
 
+/* The splat sorter without allocation. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). It does
+ * what rdx_sort_splats does under the default tuning, with its scratch in a
+ * static device array of about 17 MB per GPU, held until the process ends,
+ * instead of a context's buffers: at most RDX_STATIC_SORT_MAX_KEYS keys
+ * (more fail with RDX_E_CAPACITY). Nothing is allocated or freed. Calls on
+ * one GPU share its array: calls that overlap on one GPU (two streams, two
+ * threads) race and must be ordered by the caller. */
+#define RDX_STATIC_SORT_MAX_KEYS 1048576u
+int rdx_sort_splats_static(void* stream, const uint32_t* keys_dev, uint32_t n, int bits, uint32_t* out_indices_dev,
+                           uint32_t* out_count_dev, uint32_t* out_draw_args_dev);
+
 /* ---- Projection (ewa) ---------------------------------------------------- */
 
 #define RDX_LAYOUT_F32_SOA 0  /* float planes: center, log-scale, quat, opacity, rgb, SH */

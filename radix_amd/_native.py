@@ -126,6 +126,7 @@ class DrawSettings(C.Structure):
 
 
 BATCH_GRAPH = 1
+STATIC_SORT_MAX_KEYS = 1048576
 BATCH_MAX_VIEWS = 4096
 
 
@@ -168,6 +169,8 @@ _SIGNATURES = [
      [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int]),
     ("rdx_sort_splats", C.c_int,
      [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rdx_sort_splats_static", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("rdx_params_default", None, [C.POINTER(Params)]),
     ("rdx_project_ewa", C.c_int,
      [C.c_void_p, C.c_void_p, C.POINTER(SplatTable), C.POINTER(Camera), C.POINTER(Params), C.c_uint32,

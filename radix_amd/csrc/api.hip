@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/radix_hip.h"
@@ -595,6 +596,91 @@ int rdx_sort_splats(rdx_context* ctx, void* stream, const uint32_t* keys, uint32
   L.tiles = tiles_of(p, n);
   if (int rc = c->sort.reserve(n, 1, L.tiles)) return rc;
   return run_sort(c, c->sort, s, p, L, keys, nullptr, nullptr, out_indices, n, nullptr, true, key_invalid(bits), false,
+                  out_count, out_args);
+}
+
+// ---- the splat sorter without allocation ----------------------------------------
+
+// Scratch of rdx_sort_splats_static, one copy per GPU for the life of the
+// process: the passes' two (key, value) buffer pairs of up to
+// RDX_STATIC_SORT_MAX_KEYS words, the tile counts, digit totals, kept count
+// and the ranged sort's tile and segment ranges and done count, each region
+// at a 256-byte boundary. Host code never names `scratch` as an address: its
+// device address comes from hipGetSymbolAddress, per device.
+namespace {
+constexpr uint32_t kStaticTile = 512 * 16;  // the default tuning's tile
+constexpr uint32_t kStaticTiles = (RDX_STATIC_SORT_MAX_KEYS + kStaticTile - 1) / kStaticTile;
+constexpr size_t round64(size_t w) { return (w + 63) / 64 * 64; }
+constexpr size_t kStaticPair = round64(RDX_STATIC_SORT_MAX_KEYS);
+constexpr size_t kStaticCnt = round64((size_t)count_stride(MAX_BINS + 1) * kStaticTiles);
+constexpr size_t kStaticTot = round64(TOT_STRIDE);
+constexpr size_t kStaticNvalid = 64;
+constexpr size_t kStaticMm = round64(2 * (size_t)kStaticTiles + 3);
+constexpr size_t kStaticWords = 4 * kStaticPair + kStaticCnt + kStaticTot + kStaticNvalid + kStaticMm;
+std::mutex g_static_mu;
+constexpr int kMaxDevices = 64;
+void* g_static_ptr[kMaxDevices];
+}  // namespace
+
+static __device__ __align__(256) uint32_t scratch[kStaticWords];
+
+// The current device's copy of `scratch` (resolved on its first use there).
+static int static_scratch(uint32_t** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "rdx_sort_splats_static: hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return fail(RDX_E_UNSUPPORTED, "rdx_sort_splats_static: device %d", dev);
+  std::lock_guard<std::mutex> lock(g_static_mu);
+  if (!g_static_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return fail(RDX_E_HIP, "rdx_sort_splats_static: hipGetSymbolAddress: %s", hipGetErrorString(e));
+    }
+    g_static_ptr[dev] = p;
+  }
+  *out = (uint32_t*)g_static_ptr[dev];
+  return RDX_OK;
+}
+
+static void view(Buffer& b, uint32_t*& at, size_t words) {
+  b.ptr = at;
+  b.bytes = words * 4;
+  at += words;
+}
+
+int rdx_sort_splats_static(void* stream, const uint32_t* keys, uint32_t n, int bits, uint32_t* out_indices,
+                           uint32_t* out_count, uint32_t* out_args) {
+  if (int rc = check_bits(bits, "WgpuRadixSort")) return rc;
+  if (n > RDX_STATIC_SORT_MAX_KEYS)
+    return fail(RDX_E_CAPACITY, "rdx_sort_splats_static: %u keys exceed its static scratch of %u keys", n,
+                (unsigned)RDX_STATIC_SORT_MAX_KEYS);
+  if (n && (!keys || !out_indices)) return fail(RDX_E_INVALID, "rdx_sort_splats_static: NULL keys or indices");
+  hipStream_t s = (hipStream_t)stream;
+  if (n == 0) {
+    hipLaunchKernelGGL(k_empty_outputs, dim3(1), dim3(64), 0, s, out_count, out_args);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? RDX_OK : hip_fail(e, "rdx_sort_splats_static");
+  }
+  uint32_t* base = nullptr;
+  if (int rc = static_scratch(&base)) return rc;
+  const rdx_sort_tuning t = {8, 16, 512};
+  const SortPlan p = compact_plan(t, bits);
+  SegLayout L;
+  L.tiles = tiles_of(p, n);
+  // views into the static array (never released: SortScratch has no destructor)
+  SortScratch sc;
+  uint32_t* at = base;
+  for (int i = 0; i < 2; i++) {
+    view(sc.keys[i], at, kStaticPair);
+    view(sc.vals[i], at, kStaticPair);
+  }
+  view(sc.cnt, at, kStaticCnt);
+  view(sc.tot, at, kStaticTot);
+  view(sc.nvalid, at, kStaticNvalid);
+  view(sc.mm, at, kStaticMm);
+  return run_sort(nullptr, sc, s, p, L, keys, nullptr, nullptr, out_indices, n, nullptr, true, key_invalid(bits), false,
                   out_count, out_args);
 }
 
