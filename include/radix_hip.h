@@ -104,6 +104,11 @@ int rdx_device_upload(void* stream, void* dst_dev, const void* src_host, size_t 
 int rdx_device_download(void* stream, void* dst_host, const void* src_dev, size_t bytes);
 int rdx_device_fill_u32(void* stream, uint32_t* dst_dev, uint32_t value, size_t count);
 int rdx_stream_synchronize(void* stream);
+/* A HIP stream of its own for a binding's in-flight call (default priority,
+ * not synchronizing with the null stream): the queue a WebGPU device gives
+ * each submission (src/wgpu/WgpuCompute.ts:64-73), one per concurrent call. */
+int rdx_stream_create(void** out);
+int rdx_stream_destroy(void* stream);
 
 /* ---- The CPU-sort API, backed by the GPU --------------------------------- */
 
@@ -115,7 +120,9 @@ int rdx_stream_synchronize(void* stream);
  * 0xffff; 24: key >> 11 == 0x1fff; 32: key >> 16 == 0xffff).
  * ordering_len < count fails with the reference's message
  * "SortKeys.sort: ordering holds <len> of <count> indices" (sort.ts:70-72).
- * Synchronous: returns when `ordering` holds the result. Also the body of
+ * Synchronous: returns when `ordering` holds the result; the copies and
+ * kernels run on a stream the context owns, so sorts through contexts on
+ * different threads overlap on the device. Also the body of
  * sortSplatKeys (src/worker/splatSort.ts:17-30) and the worker's sortKeys
  * handler (src/worker/coreWorker.ts:63-65). */
 int rdx_sort_keys(rdx_context* ctx, const uint32_t* keys, uint32_t count, int bits,
@@ -402,21 +409,25 @@ typedef struct rdx_raster rdx_raster;
 typedef struct rdx_raster_settings {
   int sh_degree;                  /* SH bands used, 0..3 */
   float t_stop;                   /* transmittance stop (0: 1e-4) */
-  uint32_t max_entries_per_splat; /* (splat, tile) entry capacity per splat (0: 16) */
+  uint32_t max_entries_per_splat; /* limit on (splat, tile) entries per splat (0: none); the entry
+                                     buffers are sized to each forward's count */
 } rdx_raster_settings;
 int rdx_raster_create(rdx_context* ctx, const rdx_raster_settings* settings, rdx_raster** out);
 int rdx_raster_destroy(rdx_raster* raster);
 /* The forward on a mono camera (eye 0; renderSize = camera->size) with the
  * params' maxStdDev, minAlpha, blurAmount and clipXY. Keeps what the
- * backward needs; params_dev must stay unchanged until then. */
+ * backward needs; params_dev must stay unchanged until then. Synchronizes
+ * the stream once, after binning, to read the frame's (splat, tile) entry
+ * count and size the entry buffers to it; a count above the settings' limit
+ * fails with RDX_E_CAPACITY (nothing is drawn). */
 int rdx_raster_forward(rdx_raster* raster, void* stream, const float* params_dev, uint32_t n, const rdx_camera* camera,
                        const rdx_params* params, float* image_dev, float* alpha_dev);
 /* The adjoint of the last forward: dL/dimage (3 planes) -> dL/dparams (59
  * planes of n, overwritten). Transmittance is recomputed from each pixel's
  * final one, back to front over its contributors. */
 int rdx_raster_backward(rdx_raster* raster, void* stream, const float* grad_image_dev, float* grad_params_dev);
-/* The last forward's drawn splats, (splat, tile) entries, entry capacity and
- * whether they overflowed it (then nothing was drawn). Synchronizes. */
+/* The last forward's drawn splats, (splat, tile) entries and entry capacity;
+ * overflowed is 0 (a count past the limit fails the forward). Synchronizes. */
 int rdx_raster_stats(rdx_raster* raster, void* stream, uint32_t* visible, uint32_t* entries, uint32_t* capacity,
                      int* overflowed);
 
@@ -437,7 +448,8 @@ typedef struct rdx_draw rdx_draw;
 typedef struct rdx_draw_settings {
   float t_stop;                   /* 0: every fragment (the reference's blend); t > 0: stop a pixel at transmittance t */
   int linear_output;              /* 1: colours sRGB-decoded first (an output storing linear colour, splatDraw.ts:176) */
-  uint32_t max_entries_per_splat; /* (splat, tile) entry capacity per drawn splat (0: 16) */
+  uint32_t max_entries_per_splat; /* limit on (splat, tile) entries per draw position (0: none); the
+                                     entry buffers are sized to each draw's count */
 } rdx_draw_settings;
 int rdx_draw_create(rdx_context* ctx, const rdx_draw_settings* settings, rdx_draw** out);
 int rdx_draw_destroy(rdx_draw* draw);
@@ -447,12 +459,16 @@ int rdx_draw_destroy(rdx_draw* draw);
  * to capacity) are drawn; indices >= n are skipped (splatDraw.ts:121-122).
  * params: max_std_dev, min_alpha, falloff. rgba_dev: 4 planes of width x
  * height floats, premultiplied r, g, b and alpha over a transparent target,
- * row 0 at the top (framebuffer rows, pixel centres at + 0.5). */
+ * row 0 at the top (framebuffer rows, pixel centres at + 0.5). Synchronizes
+ * the stream once, after binning, to size the entry buffers to the frame's
+ * (splat, tile) entries; a count above the settings' limit fails with
+ * RDX_E_CAPACITY (nothing is drawn). */
 int rdx_draw_splats(rdx_draw* draw, void* stream, const uint32_t* words_dev, uint32_t n, const uint32_t* order_dev,
                     const uint32_t* count_dev, uint32_t capacity, uint32_t width, uint32_t height,
                     const rdx_params* params, float* rgba_dev);
-/* The last draw's drawn positions, (splat, tile) entries, entry capacity
- * and whether they overflowed it (then nothing was drawn). Synchronizes. */
+/* The last draw's drawn positions, (splat, tile) entries and entry
+ * capacity; overflowed is 0 (a count past the limit fails the draw).
+ * Synchronizes. */
 int rdx_draw_stats(rdx_draw* draw, void* stream, uint32_t* drawn, uint32_t* entries, uint32_t* capacity,
                    int* overflowed);
 

@@ -161,6 +161,8 @@ _SIGNATURES = [
     ("rdx_device_download", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     ("rdx_device_fill_u32", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_size_t]),
     ("rdx_stream_synchronize", C.c_int, [C.c_void_p]),
+    ("rdx_stream_create", C.c_int, [C.POINTER(C.c_void_p)]),
+    ("rdx_stream_destroy", C.c_int, [C.c_void_p]),
     ("rdx_context_profile", C.c_int, [C.c_void_p, C.c_int]),
     ("rdx_context_stage_times", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int]),
     ("rdx_sort_keys", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_size_t, _u32p]),

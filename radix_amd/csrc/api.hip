@@ -473,6 +473,21 @@ int rdx_device_fill_u32(void* stream, uint32_t* dst, uint32_t value, size_t coun
   return e == hipSuccess ? RDX_OK : hip_fail(e, "rdx_device_fill_u32");
 }
 
+int rdx_stream_create(void** out) {
+  if (!out) return fail(RDX_E_INVALID, "rdx_stream_create: out is NULL");
+  hipStream_t s = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e != hipSuccess) return hip_fail(e, "rdx_stream_create");
+  *out = s;
+  return RDX_OK;
+}
+
+int rdx_stream_destroy(void* stream) {
+  if (!stream) return RDX_OK;
+  hipError_t e = hipStreamDestroy((hipStream_t)stream);
+  return e == hipSuccess ? RDX_OK : hip_fail(e, "rdx_stream_destroy");
+}
+
 int rdx_stream_synchronize(void* stream) {
   hipError_t e = hipStreamSynchronize((hipStream_t)stream);
   return e == hipSuccess ? RDX_OK : hip_fail(e, "rdx_stream_synchronize");
@@ -538,7 +553,14 @@ int rdx_sort_keys(rdx_context* ctx, const uint32_t* keys, uint32_t count, int bi
   if ((rc = c->sort.reserve(count, 1, L.tiles))) return rc;
   if ((rc = c->io_keys.ensure((size_t)count * 4))) return rc;
   if ((rc = c->io_vals.ensure((size_t)count * 4))) return rc;
-  hipStream_t s = nullptr;
+  if (!c->host_stream) {
+    hipError_t se = hipStreamCreateWithFlags(&c->host_stream, hipStreamNonBlocking);
+    if (se != hipSuccess) {
+      c->host_stream = nullptr;
+      return hip_fail(se, "SortKeys.sort: stream");
+    }
+  }
+  hipStream_t s = c->host_stream;
   uint32_t* dkeys = (uint32_t*)c->io_keys.ptr;
   uint32_t* dord = (uint32_t*)c->io_vals.ptr;
   hipError_t e = hipMemcpyAsync(dkeys, keys, (size_t)count * 4, hipMemcpyHostToDevice, s);

@@ -61,7 +61,12 @@ struct Context {
   SortScratch sort;
   Buffer io_keys, io_vals;  // host-array sorts: device copies
   Buffer proj;              // projection scratch (keys of the fused path)
+  // The stream of host-array sorts (rdx_sort_keys), made on first use: the
+  // sorts of contexts on different threads overlap on the device.
+  hipStream_t host_stream = nullptr;
   void release() {
+    if (host_stream) (void)hipStreamDestroy(host_stream);
+    host_stream = nullptr;
     sort.release();
     io_keys.release();
     io_vals.release();

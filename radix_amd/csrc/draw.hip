@@ -36,27 +36,34 @@ enum { D_DRAWN = 0, D_ENTRIES = 1, D_OVERFLOW = 2, D_ITEMS = 3, D_WORDS = 4 };
 static SortPlan draw_entry_plan() { return make_plan(rdx_sort_tuning{8, 16, 512}, 16); }
 
 static int reserve(Draw* d, uint32_t capacity, uint32_t tiles) {
-  const uint32_t per = d->set.max_entries_per_splat ? d->set.max_entries_per_splat : 16u;
   const size_t nn = std::max<uint32_t>(capacity, 1);
-  const uint32_t cap = (uint32_t)std::min<uint64_t>((uint64_t)nn * per, 0x7fffffffull);
   int rc;
   if ((rc = d->pre.ensure(nn * 3 * sizeof(float4))) || (rc = d->touch.ensure(nn * 4)) ||
       (rc = d->rect.ensure(nn * sizeof(uint4))) || (rc = d->offsets.ensure(nn * 4)) ||
-      (rc = d->bsum.ensure((nn / DSCAN_TILE + 2) * 4)) || (rc = d->counts.ensure(D_WORDS * 4)))
+      (rc = d->bsum.ensure((nn / DSCAN_TILE + 2) * 4)) || (rc = d->counts.ensure(D_WORDS * 4)) ||
+      (rc = d->range.ensure((size_t)tiles * sizeof(uint2))) || (rc = d->item_start.ensure((size_t)tiles * 4)))
     return rc;
-  if ((rc = d->tile_key.ensure((size_t)cap * 4)) || (rc = d->emit_idx.ensure((size_t)cap * 4)) ||
-      (rc = d->emit_splat.ensure((size_t)cap * 4)) || (rc = d->range.ensure((size_t)tiles * sizeof(uint2))))
-    return rc;
+  d->capacity = (uint32_t)nn;
+  return RDX_OK;
+}
+
+// The entry buffers and the render's work items for `need` entries over
+// `tiles` tiles: grown (an eighth to spare), never shrunk.
+static int reserve_entries(Draw* d, uint32_t need, uint32_t tiles) {
   // a tile of e entries is ceil(e / DRAW_CHUNK) >= 1 items: at most tiles + cap / DRAW_CHUNK
+  if (d->cap && need <= d->cap && tiles + (d->cap + DRAW_CHUNK - 1) / DRAW_CHUNK <= d->max_items) return RDX_OK;
+  const uint32_t cap = (uint32_t)std::min<uint64_t>((uint64_t)std::max(need, d->cap) + need / 8 + 1024, 0x7fffffffull);
+  int rc;
+  if ((rc = d->tile_key.ensure((size_t)cap * 4)) || (rc = d->emit_idx.ensure((size_t)cap * 4)) ||
+      (rc = d->emit_splat.ensure((size_t)cap * 4)))
+    return rc;
   const uint32_t max_items = tiles + (cap + DRAW_CHUNK - 1) / DRAW_CHUNK;
-  if ((rc = d->item_start.ensure((size_t)tiles * 4)) || (rc = d->item_tile.ensure((size_t)max_items * 4)) ||
-      (rc = d->item_chunk.ensure((size_t)max_items * 4)) ||
+  if ((rc = d->item_tile.ensure((size_t)max_items * 4)) || (rc = d->item_chunk.ensure((size_t)max_items * 4)) ||
       (rc = d->partial.ensure((size_t)max_items * DIFF_BLOCK * sizeof(float4))))
     return rc;
-  d->max_items = max_items;
   const SortPlan pe = draw_entry_plan();
   if ((rc = d->sort_entries.reserve(cap, 1, tiles_of(pe, cap)))) return rc;
-  d->capacity = (uint32_t)nn;
+  d->max_items = max_items;
   d->cap = cap;
   return RDX_OK;
 }
@@ -137,7 +144,25 @@ int rdx_draw_splats(rdx_draw* draw, void* stream, const uint32_t* words_dev, uin
     hipLaunchKernelGGL(k_diff_scan_partial, dim3(blocks), dim3(DSCAN_BLOCK), 0, s, nullptr,
                        (const uint32_t*)d->touch.ptr, counts + D_DRAWN, (uint32_t*)d->bsum.ptr);
     hipLaunchKernelGGL(k_diff_scan_blocks, dim3(1), dim3(1024), 0, s, (uint32_t*)d->bsum.ptr, blocks, counts + D_ENTRIES,
-                       d->cap, counts + D_OVERFLOW);
+                       0xffffffffu, counts + D_OVERFLOW);
+  }
+  {
+    // The frame's entry count, read back once (the stream waits for the
+    // scan): the entry buffers are sized to it, so no frame is cut short; a
+    // per-position limit in the settings turns a larger count into an error.
+    uint32_t need = 0;
+    hipError_t e = hipMemcpyAsync(&need, counts + D_ENTRIES, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "rdx_draw_splats: entry count");
+    const uint64_t limit = std::min<uint64_t>(
+        d->set.max_entries_per_splat ? (uint64_t)d->set.max_entries_per_splat * capacity : 0x7fffffffull, 0x7fffffffull);
+    if (need > limit)
+      return fail(RDX_E_CAPACITY, "rdx_draw_splats: %u (splat, tile) entries exceed the limit of %llu", need,
+                  (unsigned long long)limit);
+    if (int rc = reserve_entries(d, need, tiles)) return rc;
+  }
+  {
+    Stage st(c, s, "draw_bin");
     hipLaunchKernelGGL(k_diff_scan_final, dim3(blocks), dim3(DSCAN_BLOCK), 0, s, nullptr, (const uint32_t*)d->touch.ptr,
                        counts + D_DRAWN, (const uint32_t*)d->bsum.ptr, (uint32_t*)d->offsets.ptr);
     hipLaunchKernelGGL(k_diff_emit, dim3(blocks_for(d->capacity)), dim3(256), 0, s, nullptr, counts + D_DRAWN,

@@ -95,9 +95,6 @@ static SortPlan entry_plan() { return make_plan(rdx_sort_tuning{8, 8, 256}, 16);
 
 static int reserve(Raster* r, uint32_t n, const DiffU& u) {
   const SortPlan p = depth_plan();
-  const uint32_t per = r->set.max_entries_per_splat ? r->set.max_entries_per_splat : 16u;
-  const uint64_t cap64 = std::min<uint64_t>((uint64_t)std::max<uint32_t>(n, 1) * per, 0x7fffffffull);
-  const uint32_t cap = (uint32_t)cap64;
   const size_t px = (size_t)u.width * u.height, tiles = (size_t)u.tiles_x * u.tiles_y;
   const size_t nn = std::max<uint32_t>(n, 1);
   int rc;
@@ -106,17 +103,42 @@ static int reserve(Raster* r, uint32_t n, const DiffU& u) {
       (rc = r->order.ensure(nn * 4)) || (rc = r->offsets.ensure(nn * 4)) || (rc = r->counts.ensure(C_WORDS * 4)) ||
       (rc = r->bsum.ensure((nn / DSCAN_TILE + 2) * 4)))
     return rc;
-  if ((rc = r->tile_key.ensure((size_t)cap * 4)) || (rc = r->emit_idx.ensure((size_t)cap * 4)) ||
-      (rc = r->emit_splat.ensure((size_t)cap * 4)) || (rc = r->gentry.ensure((size_t)cap * DIFF_PIX_GRADS * 4)))
-    return rc;
   if ((rc = r->range.ensure(tiles * sizeof(uint2))) || (rc = r->t_final.ensure(px * 4)) || (rc = r->n_contrib.ensure(px * 4)) ||
       (rc = r->gpix.ensure(nn * DIFF_PIX_GRADS * 4)))
     return rc;
   if ((rc = r->sort_splats.reserve(n, 1, tiles_of(p, n)))) return rc;
+  return RDX_OK;
+}
+
+// The (splat, tile) entry buffers for `need` entries: grown (with an eighth
+// to spare, so a frame's count wandering up does not reallocate every time),
+// never shrunk.
+static int reserve_entries(Raster* r, uint32_t need) {
+  if (need <= r->cap && r->cap) return RDX_OK;
+  const uint32_t cap = (uint32_t)std::min<uint64_t>((uint64_t)need + need / 8 + 1024, 0x7fffffffull);
+  int rc;
+  if ((rc = r->tile_key.ensure((size_t)cap * 4)) || (rc = r->emit_idx.ensure((size_t)cap * 4)) ||
+      (rc = r->emit_splat.ensure((size_t)cap * 4)) || (rc = r->gentry.ensure((size_t)cap * DIFF_PIX_GRADS * 4)))
+    return rc;
   const SortPlan pe = entry_plan();
   if ((rc = r->sort_entries.reserve(cap, 1, tiles_of(pe, cap)))) return rc;
   r->cap = cap;
   return RDX_OK;
+}
+
+// The frame's entry count, read back once (the stream waits for the scan):
+// the entry buffers are sized to it, so no frame is ever cut short. A
+// per-splat limit in the settings turns a larger count into an error.
+static int size_entries(Raster* r, hipStream_t s, const uint32_t* total_dev, uint32_t n) {
+  uint32_t need = 0;
+  hipError_t e = hipMemcpyAsync(&need, total_dev, 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "rdx_raster_forward: entry count");
+  const uint64_t limit = r->set.max_entries_per_splat ? (uint64_t)r->set.max_entries_per_splat * n : 0x7fffffffull;
+  if (need > std::min<uint64_t>(limit, 0x7fffffffull))
+    return fail(RDX_E_CAPACITY, "rdx_raster_forward: %u (splat, tile) entries exceed the limit of %llu", need,
+                (unsigned long long)std::min<uint64_t>(limit, 0x7fffffffull));
+  return reserve_entries(r, need);
 }
 
 static uint32_t grid_for(size_t items, uint32_t block = 256, uint32_t max_grid = 1u << 20) {
@@ -189,7 +211,11 @@ int rdx_raster_forward(rdx_raster* raster, void* stream, const float* params_dev
     hipLaunchKernelGGL(k_diff_scan_partial, dim3(blocks), dim3(DSCAN_BLOCK), 0, s, (const uint32_t*)r->order.ptr,
                        (const uint32_t*)r->touch.ptr, counts + C_VISIBLE, (uint32_t*)r->bsum.ptr);
     hipLaunchKernelGGL(k_diff_scan_blocks, dim3(1), dim3(1024), 0, s, (uint32_t*)r->bsum.ptr, blocks, counts + C_ENTRIES,
-                       r->cap, counts + C_OVERFLOW);
+                       0xffffffffu, counts + C_OVERFLOW);
+  }
+  if (int rc = size_entries(r, s, counts + C_ENTRIES, n)) return rc;
+  {
+    Stage st(c, s, "diff_scan");
     hipLaunchKernelGGL(k_diff_scan_final, dim3(blocks), dim3(DSCAN_BLOCK), 0, s, (const uint32_t*)r->order.ptr,
                        (const uint32_t*)r->touch.ptr, counts + C_VISIBLE, (const uint32_t*)r->bsum.ptr,
                        (uint32_t*)r->offsets.ptr);

@@ -54,10 +54,16 @@ def workload(count: int = 262144, seed: int = 4):
 class Rasterizer:
     """rdx_raster_*: the tile compositor and its adjoint on the device."""
 
-    def __init__(self, sh_degree: int = 3, t_stop: float = 1e-4, entries_per_splat: int = 16, context=None):
+    def __init__(self, sh_degree: int = 3, t_stop: float = 1e-4, entries_per_splat: int = 0, context=None):
+        """entries_per_splat: a limit on (splat, tile) entries per splat (0:
+        none; the entry buffers follow each forward's count); a forward past
+        it raises RadixError instead of drawing."""
         from .sort import default_context
 
         self.context = context or default_context()
+        # One forward state lives on the device: each forward bumps the
+        # generation, and a backward of an older one is refused.
+        self.generation = 0
         s = N.RasterSettings(sh_degree, t_stop, entries_per_splat)
         h = C.c_void_p()
         N.check(N.lib().rdx_raster_create(self.context.handle, C.byref(s), C.byref(h)))
@@ -78,17 +84,22 @@ class Rasterizer:
         image = torch.empty((3, H, W), dtype=torch.float32, device=params.device)
         alpha = torch.empty((H, W), dtype=torch.float32, device=params.device)
         self._params = params  # the backward reads the forward's parameters
+        self.generation += 1
         N.check(N.lib().rdx_raster_forward(self._h, _stream(stream), C.c_void_p(params.data_ptr()), n,
                                            C.byref(rig.desc()), C.byref(prm.desc()), C.c_void_p(image.data_ptr()),
                                            C.c_void_p(alpha.data_ptr())))
         return image, alpha
 
     @on_stream
-    def backward(self, grad_image, stream=None):
+    def backward(self, grad_image, generation=None, stream=None):
+        """dL/dparams (59, n) of the last forward (of `generation`, if given:
+        a later forward on this Rasterizer raises instead of differentiating
+        the wrong image)."""
         import torch
 
         from .sort import _stream
 
+        check_generation(self, generation)
         g = grad_image.contiguous().float()
         out = torch.empty_like(self._params)
         N.check(N.lib().rdx_raster_backward(self._h, _stream(stream), C.c_void_p(g.data_ptr()),
@@ -115,20 +126,36 @@ class Rasterizer:
             pass
 
 
+def check_generation(rasterizer, generation) -> None:
+    if generation is not None and generation != rasterizer.generation:
+        raise RuntimeError(
+            f"Rasterizer: the backward of forward {generation} was asked after forward {rasterizer.generation} "
+            "replaced its state; composite each image that is differentiated together with its own Rasterizer")
+
+
 def composite(params, rig, prm, rasterizer: "Rasterizer"):
     """The image of `params` (59, n) as a differentiable torch function: a
-    torch.autograd.Function whose backward is the hand-written adjoint."""
+    torch.autograd.Function whose backward is the hand-written adjoint.
+
+    The rasterizer keeps one forward's state: composite two images with one
+    Rasterizer before backward and the first backward raises (its forward's
+    generation is no longer current). The parameters are saved for backward,
+    so an in-place change to them before backward raises too (autograd's
+    version counter)."""
     import torch
 
     class _Composite(torch.autograd.Function):
         @staticmethod
         def forward(ctx, p):
             image, _ = rasterizer.forward(p, rig, prm)
+            ctx.generation = rasterizer.generation
+            ctx.save_for_backward(p)
             return image
 
         @staticmethod
         def backward(ctx, grad_image):
-            return rasterizer.backward(grad_image)
+            ctx.saved_tensors  # raises if the parameters were changed in place
+            return rasterizer.backward(grad_image, generation=ctx.generation)
 
     return _Composite.apply(params)
 

@@ -17,11 +17,13 @@ from .sort import on_stream
 class Drawer:
     """rdx_draw_*: composite sorted splats into an RGBA image on the device."""
 
-    def __init__(self, t_stop: float = 0.0, linear_output: bool = False, entries_per_splat: int = 16, context=None):
+    def __init__(self, t_stop: float = 0.0, linear_output: bool = False, entries_per_splat: int = 0, context=None):
         """t_stop 0 composites every fragment (the reference's blend); t_stop
         > 0 stops a pixel once its transmittance would fall below it.
         linear_output sRGB-decodes the colours first (an output that stores
-        linear values, splatDraw.ts:176)."""
+        linear values, splatDraw.ts:176). entries_per_splat: a limit on
+        (splat, tile) entries per draw position (0: none; the entry buffers
+        follow each draw's count); a draw past it raises RadixError."""
         from .sort import default_context
 
         self.context = context or default_context()

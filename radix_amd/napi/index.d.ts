@@ -1,6 +1,18 @@
 // Types of the Node binding (index.js): the reference's sort and projection
-// interfaces over libradix_hip.so.
+// interfaces over libradix_hip.so, the compositor and its adjoint, the draw
+// and the batched views.
 export type SortKeyBits = 16 | 24 | 32;
+
+/** A HIP stream with its own scratch: calls given it are ordered on it and overlap with other streams. */
+export class GpuStream {
+  constructor();
+  synchronize(): void;
+  done(): Promise<void>;
+}
+
+export interface OnStream {
+  stream?: GpuStream;
+}
 
 export function alignCapacity(count: number): number;
 
@@ -27,10 +39,11 @@ export function sortSplatKeys(keys: Uint32Array, options: { bits: SortKeyBits; o
 export class DeviceBuffer {
   readonly bytes: number;
   constructor(bytes: number);
-  static fromArray(array: ArrayBufferView): DeviceBuffer;
-  upload(array: ArrayBufferView, byteOffset?: number): void;
-  read(start?: number, count?: number): Uint32Array;
-  fill(value: number, count?: number): void;
+  static fromArray(array: ArrayBufferView, options?: OnStream): DeviceBuffer;
+  upload(array: ArrayBufferView, byteOffset?: number, options?: OnStream): void;
+  read<T extends Uint32Array | Float32Array = Uint32Array>(start?: number, count?: number,
+    Type?: { new (n: number): T }, options?: OnStream): T;
+  fill(value: number, count?: number, options?: OnStream): void;
   dispose(): void;
 }
 
@@ -39,8 +52,8 @@ export class RadixSort {
   readonly values: DeviceBuffer;
   constructor(args: { keys: DeviceBuffer; values: DeviceBuffer });
   readonly capacity: number;
-  sort(count: number, options?: { bits?: SortKeyBits }): void;
-  sortIndirect(countBuffer: DeviceBuffer, options?: { bits?: SortKeyBits }): void;
+  sort(count: number, options?: { bits?: SortKeyBits } & OnStream): void;
+  sortIndirect(countBuffer: DeviceBuffer, options?: { bits?: SortKeyBits } & OnStream): void;
   dispose(): void;
 }
 
@@ -50,8 +63,8 @@ export class SplatSorter {
   readonly count: DeviceBuffer;
   readonly indirect: DeviceBuffer;
   constructor(capacity?: number);
-  sort(keys: DeviceBuffer, count: number, bits: SortKeyBits): void;
-  readCount(): Promise<number>;
+  sort(keys: DeviceBuffer, count: number, bits: SortKeyBits, options?: OnStream): void;
+  readCount(options?: OnStream): Promise<number>;
   dispose(): void;
 }
 
@@ -102,17 +115,49 @@ export interface SplatProjectionParams {
 
 export class SplatProjector {
   project(table: SplatTable, camera: SplatCameraViews, params: SplatProjectionParams,
-    out: { words: DeviceBuffer[]; keys?: DeviceBuffer; at?: number }): void;
+    out: { words: DeviceBuffer[]; keys?: DeviceBuffer; at?: number } & OnStream): void;
   projectSort(table: SplatTable, camera: SplatCameraViews, params: SplatProjectionParams,
-    out: { words?: DeviceBuffer[]; keys?: DeviceBuffer; indices: DeviceBuffer; count?: DeviceBuffer; indirect?: DeviceBuffer }): void;
+    out: { words?: DeviceBuffer[]; keys?: DeviceBuffer; indices: DeviceBuffer; count?: DeviceBuffer; indirect?: DeviceBuffer } & OnStream): void;
   projectKeys(table: SplatTable, camera: SplatCameraViews, params: SplatProjectionParams,
-    out: { keys: DeviceBuffer; at?: number }): void;
+    out: { keys: DeviceBuffer; at?: number } & OnStream): void;
   projectRay(table: SplatTable, camera: SplatCameraViews, params: SplatProjectionParams, method: '3dgut' | 'htgs' | 'raygs',
-    out: { channels: DeviceBuffer[]; keys?: DeviceBuffer; at?: number }): void;
+    out: { channels: DeviceBuffer[]; keys?: DeviceBuffer; at?: number } & OnStream): void;
+}
+
+/** The differentiable compositor: 59 parameter planes of `count` floats in, 3 image planes out; the adjoint back. */
+export class Rasterizer {
+  readonly generation: number;
+  constructor(settings?: { shDegree?: number; tStop?: number; maxEntriesPerSplat?: number } & OnStream);
+  /** Returns the forward's generation (backward of an older one throws). */
+  forward(params: DeviceBuffer, count: number, camera: SplatCameraViews, renderParams: SplatProjectionParams,
+    out: { image: DeviceBuffer; alpha?: DeviceBuffer } & OnStream): number;
+  backward(gradImage: DeviceBuffer, gradParams: DeviceBuffer, options?: { generation?: number } & OnStream): void;
+  stats(options?: OnStream): { visible: number; entries: number; capacity: number };
+  dispose(): void;
+}
+
+/** The splat draw for native hosts: ewa words in a back-to-front order into premultiplied RGBA planes. */
+export class SplatDrawer {
+  constructor(settings?: { tStop?: number; linearOutput?: boolean; maxEntriesPerSplat?: number } & OnStream);
+  draw(words: DeviceBuffer, n: number, order: DeviceBuffer, count: DeviceBuffer, capacity: number, width: number,
+    height: number, params: SplatProjectionParams, rgba: DeviceBuffer, options?: OnStream): void;
+  stats(options?: OnStream): { drawn: number; entries: number; capacity: number };
+  dispose(): void;
+}
+
+/** Many (view, collection) segments projected and sorted in one sequence. */
+export class SplatBatch {
+  readonly total: number;
+  readonly offsets: number[];
+  constructor(tables: SplatTable[], views: { table: number; camera: SplatCameraViews; words?: DeviceBuffer }[],
+    params: SplatProjectionParams, options?: { keys?: DeviceBuffer } & OnStream);
+  setCameras(cameras: SplatCameraViews[]): void;
+  run(out: { indices: DeviceBuffer; counts?: DeviceBuffer; drawArgs?: DeviceBuffer; graph?: boolean } & OnStream): void;
+  dispose(): void;
 }
 
 export function untransposeBytes(planes: DeviceBuffer, count: number, elementBytes: number, out: DeviceBuffer,
-  outOffset?: number): void;
+  outOffset?: number, options?: OnStream): void;
 
 export interface SpzMeta {
   format: 'spz'; version: number; count: number; shDegree: number; fractionalBits: number; antialiased: boolean;

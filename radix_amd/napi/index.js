@@ -3,8 +3,13 @@
 // (src/core/sort.ts SortKeys, src/worker/splatSort.ts sortSplatKeys,
 // src/wgpu/WgpuRadixSort.ts, src/wgpu/WgpuSplatSorter.ts and the
 // projection writer of src/render/SplatProjection.ts) over the N-API
-// binding of libradix_hip.so. Plain ES2018 so it runs on Node 12 as well
-// as under TypeScript (types in index.d.ts).
+// binding of libradix_hip.so, plus the grown forms with no reference
+// counterpart: the differentiable compositor and its adjoint (Rasterizer),
+// the draw for native hosts (SplatDrawer) and many views in one sequence
+// (SplatBatch). GPU calls take an optional { stream } (a GpuStream): calls on
+// different streams are in flight together, each with its own scratch; a
+// stream's done() resolves when its work has finished. Plain ES2018 so it
+// runs on Node 12 as well as under TypeScript (types in index.d.ts).
 const path = require('path');
 
 const native = require(path.join(__dirname, 'radix_napi.node'));
@@ -50,6 +55,27 @@ function sortSplatKeys(keys, { bits, ordering } = {}) {
   return native.sortKeysAsync(keys, count, bits, reuse).then((visible) => ({ ordering: reuse, visible, keys }));
 }
 
+// A HIP stream with its own scratch: the calls given it run in its order,
+// overlapping with calls on other streams (one per in-flight call)
+class GpuStream {
+  constructor() {
+    this.handle = native.streamCreate();
+  }
+
+  // Waits (blocking this thread) for the stream's work
+  synchronize() {
+    native.streamSynchronize(this.handle);
+  }
+
+  // A Promise resolved when the stream's work has finished (waited on the libuv pool)
+  done() {
+    return native.streamSynchronizeAsync(this.handle);
+  }
+}
+
+const streamOf = (options) => (options && options.stream ? options.stream.handle : undefined);
+const handleOf = (b) => (b && b.handle !== undefined ? b.handle : b === undefined ? null : b);
+
 // Device memory (the storage of a WgpuBuffer, src/wgpu/WgpuBuffer.ts)
 class DeviceBuffer {
   constructor(bytes) {
@@ -57,25 +83,26 @@ class DeviceBuffer {
     this.handle = native.deviceAlloc(bytes);
   }
 
-  static fromArray(array) {
+  static fromArray(array, options) {
     const b = new DeviceBuffer(array.byteLength);
-    b.upload(array);
+    b.upload(array, 0, options);
     return b;
   }
 
-  upload(array, byteOffset = 0) {
-    native.deviceUpload(this.handle, array, byteOffset);
+  // Waits for the copy (and, with a stream, for that stream's earlier work)
+  upload(array, byteOffset = 0, options) {
+    native.deviceUpload(this.handle, array, byteOffset, streamOf(options));
   }
 
-  // The first `count` uint32 from element `start`
-  read(start = 0, count = this.bytes / 4 - start, Type = Uint32Array) {
+  // The first `count` uint32 from element `start`, after the stream's earlier work
+  read(start = 0, count = this.bytes / 4 - start, Type = Uint32Array, options) {
     const out = new Type(count);
-    native.deviceDownload(this.handle, start * out.BYTES_PER_ELEMENT, out);
+    native.deviceDownload(this.handle, start * out.BYTES_PER_ELEMENT, out, streamOf(options));
     return out;
   }
 
-  fill(value, count = this.bytes / 4) {
-    native.deviceFillU32(this.handle, value, count);
+  fill(value, count = this.bytes / 4, options) {
+    native.deviceFillU32(this.handle, value, count, streamOf(options));
   }
 
   dispose() {
@@ -91,16 +118,20 @@ class RadixSort {
   }
 
   get capacity() {
-    return Math.min(this.keys.bytes, this.values.bytes) / 4;
+    return Math.floor(Math.min(this.keys.bytes, this.values.bytes) / 4);
   }
 
-  sort(count, { bits = 32 } = {}) {
+  sort(count, { bits = 32, stream } = {}) {
     if (count <= 0) return;
-    native.sortPairs(this.keys.handle, this.values.handle, count, bits);
+    if (count > this.capacity) {
+      throw new Error(`WgpuRadixSort: ${count} keys past the buffers' capacity of ${this.capacity}`);
+    }
+    native.sortPairs(this.keys.handle, this.values.handle, count, bits, streamOf({ stream }));
   }
 
-  sortIndirect(countBuffer, { bits = 32 } = {}) {
-    native.sortPairsIndirect(this.keys.handle, this.values.handle, countBuffer.handle, this.capacity, bits);
+  sortIndirect(countBuffer, { bits = 32, stream } = {}) {
+    native.sortPairsIndirect(this.keys.handle, this.values.handle, countBuffer.handle, this.capacity, bits,
+      streamOf({ stream }));
   }
 
   dispose() {}
@@ -115,16 +146,20 @@ class SplatSorter {
     this.indirect = new DeviceBuffer(20);
   }
 
-  sort(keys, count, bits) {
+  sort(keys, count, bits, options) {
+    if (count * 4 > keys.bytes) {
+      throw new Error(`WgpuSplatSorter: ${count} keys past the buffer's ${Math.floor(keys.bytes / 4)}`);
+    }
     if (this.values.bytes < count * 4) {
       this.values.dispose();
       this.values = new DeviceBuffer(count * 4);
     }
-    native.sortSplats(keys.handle, count, bits, this.values.handle, this.count.handle, this.indirect.handle);
+    native.sortSplats(keys.handle, count, bits, this.values.handle, this.count.handle, this.indirect.handle,
+      streamOf(options));
   }
 
-  readCount() {
-    return Promise.resolve(this.count.read(0, 1)[0]);
+  readCount(options) {
+    return Promise.resolve(this.count.read(0, 1, Uint32Array, options)[0]);
   }
 
   dispose() {
@@ -140,11 +175,10 @@ function tableDesc(table) {
   const d = Object.assign({}, table);
   d.layout = typeof table.layout === 'string' ? LAYOUTS[table.layout] : table.layout || 0;
   d.shCodec = typeof table.shCodec === 'string' ? SH_CODECS[table.shCodec] : table.shCodec || 0;
-  const unwrap = (b) => (b && b.handle !== undefined ? b.handle : b);
   ['planes', 'extA', 'extB', 'packed'].forEach((k) => {
-    if (d[k]) d[k] = unwrap(d[k]);
+    if (d[k]) d[k] = handleOf(d[k]);
   });
-  if (d.shChannels) d.shChannels = d.shChannels.map(unwrap);
+  if (d.shChannels) d.shChannels = d.shChannels.map(handleOf);
   return d;
 }
 
@@ -153,34 +187,129 @@ function tableDesc(table) {
 // takes it ({head, eyes: [{view, projection, ortho}], size, ndcZMin},
 // matrices as Matrix4.elements), params as SplatParams' fields
 class SplatProjector {
-  project(table, camera, params, { words, keys, at = 0 }) {
-    native.projectEwa(tableDesc(table), camera, params || {}, words.map((w) => w.handle), keys ? keys.handle : null, at);
+  project(table, camera, params, { words, keys, at = 0, stream }) {
+    native.projectEwa(tableDesc(table), camera, params || {}, words.map(handleOf), handleOf(keys), at,
+      streamOf({ stream }));
   }
 
-  projectSort(table, camera, params, { words, keys, indices, count, indirect }) {
-    native.projectSort(tableDesc(table), camera, params || {}, (words || []).map((w) => w.handle),
-      keys ? keys.handle : null, indices.handle, count ? count.handle : null, indirect ? indirect.handle : null);
+  projectSort(table, camera, params, { words, keys, indices, count, indirect, stream }) {
+    native.projectSort(tableDesc(table), camera, params || {}, (words || []).map(handleOf), handleOf(keys),
+      handleOf(indices), handleOf(count), handleOf(indirect), streamOf({ stream }));
   }
 
   // The keys-only pass (SplatProjection.writeKeys, src/render/SplatProjection.ts:165-172)
-  projectKeys(table, camera, params, { keys, at = 0 }) {
-    native.projectKeys(tableDesc(table), camera, params || {}, keys.handle, at);
+  projectKeys(table, camera, params, { keys, at = 0, stream }) {
+    native.projectKeys(tableDesc(table), camera, params || {}, handleOf(keys), at, streamOf({ stream }));
   }
 
   // The words of "3dgut", "htgs" or "raygs" into the projected bundle's
   // channels w0, w1 (and w2 with two eyes) (SplatProjection.ts:46-53, 141-151)
-  projectRay(table, camera, params, method, { channels, keys, at = 0 }) {
+  projectRay(table, camera, params, method, { channels, keys, at = 0, stream }) {
     const code = { '3dgut': 1, htgs: 2, raygs: 3 }[method];
     if (code === undefined) throw new Error(`SplatProjection: unknown method ${method}`);
-    native.projectRay(tableDesc(table), camera, params || {}, code, channels.map((c) => c.handle), keys ? keys.handle : null, at);
+    native.projectRay(tableDesc(table), camera, params || {}, code, channels.map(handleOf), handleOf(keys), at,
+      streamOf({ stream }));
+  }
+}
+
+// The differentiable compositor (no reference counterpart): the ewa splats
+// of 59 parameter planes (centre 3, log-scale 3, quaternion 4, opacity
+// logit, base colour 3, SH 45) composited front to back over 16x16 tiles
+// with the draw's fragment alpha and premultiplied "over"
+// (src/render/splatDraw.ts:62-174, projection.tsl.ts:444-455), and its
+// hand-written adjoint. Images are 3 planes of width x height floats, row 0
+// at the bottom. One forward's state at a time: backward() differentiates
+// the last forward (forward() returns its generation; a backward given an
+// older one throws).
+class Rasterizer {
+  constructor({ shDegree = 3, tStop = 1e-4, maxEntriesPerSplat = 0, stream } = {}) {
+    this.handle = native.rasterCreate({ shDegree, tStop, maxEntriesPerSplat }, streamOf({ stream }));
+    this.generation = 0;
+  }
+
+  forward(params, count, camera, renderParams, { image, alpha, stream } = {}) {
+    native.rasterForward(this.handle, handleOf(params), count, camera, renderParams || {}, handleOf(image),
+      handleOf(alpha), streamOf({ stream }));
+    this.generation += 1;
+    return this.generation;
+  }
+
+  backward(gradImage, gradParams, { generation, stream } = {}) {
+    if (generation !== undefined && generation !== this.generation) {
+      throw new Error(`Rasterizer: the backward of forward ${generation} was asked after forward ${this.generation} replaced its state`);
+    }
+    native.rasterBackward(this.handle, handleOf(gradImage), handleOf(gradParams), streamOf({ stream }));
+  }
+
+  stats(options) {
+    return native.rasterStats(this.handle, streamOf(options));
+  }
+
+  dispose() {
+    native.rasterDestroy(this.handle);
+  }
+}
+
+// The splat draw for native hosts (splatDraw.ts:98-181 without a
+// rasteriser): one eye's ewa words in a back-to-front order composited into
+// premultiplied RGBA (4 planes of width x height floats, row 0 at the top)
+class SplatDrawer {
+  constructor({ tStop = 0, linearOutput = false, maxEntriesPerSplat = 0, stream } = {}) {
+    this.handle = native.drawCreate({ tStop, linearOutput, maxEntriesPerSplat }, streamOf({ stream }));
+  }
+
+  draw(words, n, order, count, capacity, width, height, params, rgba, options) {
+    native.drawSplats(this.handle, handleOf(words), n, handleOf(order), handleOf(count), capacity, width, height,
+      params || {}, handleOf(rgba), streamOf(options));
+  }
+
+  stats(options) {
+    return native.drawStats(this.handle, streamOf(options));
+  }
+
+  dispose() {
+    native.drawDestroy(this.handle);
+  }
+}
+
+// Many views of many collections projected and segment-sorted in one
+// sequence (no reference counterpart: the reference projects and sorts per
+// object and render, SplatProjector.ts:127-137). views: [{table, camera,
+// words?}] (table: an index into tables; one eye per camera). View v's
+// keys and indices are at offsets[v] (SplatCollection's offsets,
+// SplatCollection.ts:54-65).
+class SplatBatch {
+  constructor(tables, views, params, { keys, stream } = {}) {
+    this.tables = tables;
+    this.views = views;
+    this.handle = native.batchCreate(tables.map(tableDesc), views.map((v) => ({
+      table: v.table, camera: v.camera, words: handleOf(v.words),
+    })), params || {}, handleOf(keys), streamOf({ stream }));
+    const info = native.batchInfo(this.handle);
+    this.total = info.total;
+    this.offsets = info.offsets;
+  }
+
+  setCameras(cameras) {
+    native.batchSetCameras(this.handle, cameras);
+  }
+
+  // indices: total uint32; counts: a uint32 per view; drawArgs: 5 per view.
+  // graph: capture the sequence into a HIP graph once and replay it.
+  run({ indices, counts, drawArgs, graph = true, stream }) {
+    native.batchRun(this.handle, handleOf(indices), handleOf(counts), handleOf(drawArgs), graph, streamOf({ stream }));
+  }
+
+  dispose() {
+    native.batchDestroy(this.handle);
   }
 }
 
 // An RDX payload, gunzipped into `planes` (a DeviceBuffer), back to `count`
 // elements of `elementBytes` in `out` from byte `outOffset` (the second step
 // of Compression.decompressTransposed, src/core/compress.ts:197-204)
-function untransposeBytes(planes, count, elementBytes, out, outOffset = 0) {
-  native.untransposeBytes(planes.handle, count, elementBytes, out.handle, outOffset);
+function untransposeBytes(planes, count, elementBytes, out, outOffset = 0, options) {
+  native.untransposeBytes(handleOf(planes), count, elementBytes, handleOf(out), outOffset, streamOf(options));
 }
 
 const SPZ_MAGIC = 0x5053474e;
@@ -223,6 +352,7 @@ function readSpz(bytes) {
 
 module.exports = {
   untransposeBytes, readSpz,
-  alignCapacity, SortKeys, sortSplatKeys, DeviceBuffer, RadixSort, SplatSorter, SplatProjector,
+  alignCapacity, SortKeys, sortSplatKeys, GpuStream, DeviceBuffer, RadixSort, SplatSorter, SplatProjector,
+  Rasterizer, SplatDrawer, SplatBatch,
   version: native.version,
 };

@@ -5,7 +5,10 @@
 // that make no GPU call; `gpu` adds the GPU sorts and projection.
 const path = require('path');
 const radix = require(path.join(__dirname, '..', '..', 'radix_amd', 'napi', 'index.js'));
-const { SortKeys, sortSplatKeys, DeviceBuffer, RadixSort, SplatSorter, SplatProjector, alignCapacity, readSpz } = radix;
+const { SortKeys, sortSplatKeys, DeviceBuffer, RadixSort, SplatSorter, SplatProjector, alignCapacity, readSpz, GpuStream,
+  Rasterizer, SplatDrawer, SplatBatch, untransposeBytes } = radix;
+const native = require(path.join(__dirname, '..', '..', 'radix_amd', 'napi', 'radix_napi.node'));
+const fs = require('fs');
 const zlib = require('zlib');
 
 const failures = [];
@@ -61,6 +64,59 @@ async function cpuChecks() {
     } catch (e) {
       return e.message === 'SortKeys.sort: ordering holds 3 of 10 indices' ? undefined : e.message;
     }
+  });
+}
+
+// The error a call throws, or a failure message when it does not throw that one
+function throwsWith(fn, text) {
+  try {
+    fn();
+  } catch (e) {
+    return e.message.includes(text) ? undefined : `threw "${e.message}", expected "${text}"`;
+  }
+  return `no error (expected "${text}")`;
+}
+
+// Argument checks that throw before any GPU call
+async function cpuBoundsChecks() {
+  await check('bounds: counts past the arrays and buffers throw before any device work', () => {
+    return throwsWith(() => native.sortKeysAsync(new Uint32Array(4), 10, 24, new Uint32Array(8)), 'count above the keys given')
+      || throwsWith(() => new RadixSort({ keys: { bytes: 16, handle: null }, values: { bytes: 64, handle: null } }).sort(10),
+        "past the buffers' capacity of 4")
+      || throwsWith(() => SplatSorter.prototype.sort.call({ values: { bytes: 0 } }, { bytes: 8 }, 10, 32),
+        "past the buffer's 2")
+      || throwsWith(() => native.sortPairs({}, {}, 10, 32), 'WgpuRadixSort.sort: keys is not a device buffer')
+      || throwsWith(() => native.sortPairs(null, null, 0, 32, 5), 'the last argument is not a stream')
+      || throwsWith(() => native.sortSplats(undefined, 10, 32), 'WgpuSplatSorter.sort: keys is required')
+      || throwsWith(() => native.untransposeBytes({}, 4, 16, {}, 0), 'untransposeBytes: planes is not a device buffer')
+      || throwsWith(() => native.projectSort({ layout: 0, count: 10 }, { eyes: [] }, {}, [], null, null),
+        'table.planes is required')
+      || throwsWith(() => native.rasterForward({}, null, 1), 'Rasterizer.forward: not a rasterizer');
+  });
+}
+
+// Bounds checks against real device buffers: each call throws, the device untouched
+async function gpuBoundsChecks() {
+  await check('bounds: short device buffers throw (sorts, projection, loaders, compositor, draw)', () => {
+    const small = new DeviceBuffer(40);
+    const big = new DeviceBuffer(4096);
+    const planes = new DeviceBuffer(14 * 4 * 8);
+    const eye = { view: [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1], projection: [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, -1, -1, 0, 0, -0.1, 0] };
+    const camera = { eyes: [eye], size: [16, 16], ndcZMin: 0 };
+    const rast = new Rasterizer();
+    const drawer = new SplatDrawer();
+    const out = throwsWith(() => native.sortPairs(small.handle, big.handle, 100, 32), 'keys holds 40 bytes, 400 needed')
+      || throwsWith(() => native.sortSplats(big.handle, 100, 32, small.handle, null, null), 'values holds 40 bytes, 400 needed')
+      || throwsWith(() => native.projectSort({ layout: 0, count: 16, planes: planes.handle }, camera, {}, [], null, big.handle),
+        'table.planes holds 448 bytes, 896 needed')
+      || throwsWith(() => untransposeBytes(big, 100, 16, big, 3000), 'out holds 4096 bytes, 4600 needed')
+      || throwsWith(() => native.spzDecode(big.handle, 100, 10, 3, 0, 12, small.handle, 10), 'planes holds 40 bytes, 560 needed')
+      || throwsWith(() => rast.forward(small, 100, camera, {}, { image: big }), 'params holds 40 bytes, 23600 needed')
+      || throwsWith(() => drawer.draw(big, 10, big, big, 10, 16, 16, {}, small), 'rgba holds 40 bytes, 4096 needed');
+    [small, big, planes].forEach((b) => b.dispose());
+    rast.dispose();
+    drawer.dispose();
+    return out;
   });
 }
 
@@ -290,12 +346,170 @@ async function gpuChecksSpz() {
   });
 }
 
+// A random f32 table in front of a camera at the origin (renderUtils.ts:49-62 ranges)
+function randomPlanes(seed, n) {
+  const random = rng(seed);
+  const planes = new Float32Array(14 * n);
+  for (let i = 0; i < n; i += 1) {
+    planes[i] = (random() - 0.5) * 4;
+    planes[n + i] = (random() - 0.5) * 3;
+    planes[2 * n + i] = -2 - random() * 6;
+    for (let k = 3; k < 6; k += 1) planes[k * n + i] = Math.log(0.01 + random() * 0.1);
+    planes[9 * n + i] = 1;
+    planes[10 * n + i] = 0.2 + random() * 0.8;
+    for (let k = 11; k < 14; k += 1) planes[k * n + i] = random();
+  }
+  return planes;
+}
+
+function cameraAt(x, width = 128, height = 64) {
+  const f = 1 / Math.tan(Math.PI / 6);
+  const near = 0.1;
+  const far = 100;
+  const projection = [f / (width / height), 0, 0, 0, 0, f, 0, 0, 0, 0, -far / (far - near), -1, 0, 0, (-far * near) / (far - near), 0];
+  const view = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, -x, 0, 0, 1];
+  return { head: view, eyes: [{ view, projection }], size: [width, height], ndcZMin: 0 };
+}
+
+async function gpuChecksStreams() {
+  await check('GpuStream: two projectSort calls in flight on their own streams, each the order of SortKeys.sort', async () => {
+    const runs = [[31, 6000, 0.0], [32, 9000, 0.4]].map(([seed, n, x]) => {
+      const stream = new GpuStream();
+      const table = { layout: 'f32', count: n, planes: DeviceBuffer.fromArray(randomPlanes(seed, n), { stream }) };
+      const out = { keys: new DeviceBuffer(4 * n), indices: new DeviceBuffer(4 * n), count: new DeviceBuffer(4), stream };
+      new SplatProjector().projectSort(table, cameraAt(x), { sortBits: 32 }, out);
+      return { n, stream, out, table };
+    });
+    await Promise.all(runs.map((r) => r.stream.done()));
+    for (const { n, stream, out, table } of runs) {
+      const keys = out.keys.read(0, n, Uint32Array, { stream });
+      const { ordering, visible } = SortKeys.sort({ keys, count: n, bits: 32 });
+      const got = out.count.read(0, 1, Uint32Array, { stream })[0];
+      if (got !== visible || visible < n / 2) return `count ${got}, visible ${visible}`;
+      const m = firstMismatch(out.indices.read(0, visible, Uint32Array, { stream }), ordering, visible);
+      [out.keys, out.indices, out.count, table.planes].forEach((b) => b.dispose());
+      if (m) return m;
+    }
+    return undefined;
+  });
+  await check('sortSplatKeys: three promises in flight, each the order of SortKeys.sort', async () => {
+    const random = rng(33);
+    const sets = [20000, 50000, 3000].map((n) => new Uint32Array(n).map(() => (random() < 0.1 ? 0xffffff : Math.floor(random() * 0xfeffff))));
+    const results = await Promise.all(sets.map((keys) => sortSplatKeys(keys.slice(), { bits: 24 })));
+    for (let k = 0; k < sets.length; k += 1) {
+      const here = SortKeys.sort({ keys: sets[k], count: sets[k].length, bits: 24 });
+      if (results[k].visible !== here.visible) return `set ${k}: visible ${results[k].visible} vs ${here.visible}`;
+      const m = firstMismatch(results[k].ordering, here.ordering, sets[k].length);
+      if (m) return `set ${k}: ${m}`;
+    }
+    return undefined;
+  });
+}
+
+async function gpuChecksBatch() {
+  await check('SplatBatch: each view segment is the order of SortKeys.sort of its keys (graph and plain runs)', () => {
+    const sizes = [3000, 5000];
+    const tables = sizes.map((n, k) => ({ layout: 'f32', count: n, planes: DeviceBuffer.fromArray(randomPlanes(40 + k, n)) }));
+    const views = [[0, 0.0], [1, 0.3], [0, -0.5]].map(([table, x]) => ({ table, camera: cameraAt(x) }));
+    const total = views.reduce((s, v) => s + sizes[v.table], 0);
+    const keys = new DeviceBuffer(4 * total);
+    const batch = new SplatBatch(tables, views, { sortBits: 32 }, { keys });
+    if (batch.total !== total) return `total ${batch.total} vs ${total}`;
+    const indices = new DeviceBuffer(4 * total);
+    const counts = new DeviceBuffer(4 * views.length);
+    const drawArgs = new DeviceBuffer(20 * views.length);
+    for (const graph of [true, false, true]) {
+      batch.run({ indices, counts, drawArgs, graph });
+      const k = keys.read(0, total);
+      const idx = indices.read(0, total);
+      const c = counts.read(0, views.length);
+      const args = drawArgs.read(0, 5 * views.length);
+      for (let v = 0; v < views.length; v += 1) {
+        const off = batch.offsets[v];
+        const n = sizes[views[v].table];
+        const { ordering, visible } = SortKeys.sort({ keys: k.subarray(off, off + n), count: n, bits: 32 });
+        if (c[v] !== visible || visible < n / 2) return `view ${v}: count ${c[v]}, visible ${visible}`;
+        if (args[5 * v + 1] !== Math.ceil(visible / 1024)) return `view ${v}: draw arguments ${args.subarray(5 * v, 5 * v + 5)}`;
+        const m = firstMismatch(idx.subarray(off, off + visible), ordering, visible);
+        if (m) return `view ${v} (graph ${graph}): ${m}`;
+      }
+    }
+    batch.dispose();
+    [keys, indices, counts, drawArgs].concat(tables.map((t) => t.planes)).forEach((b) => b.dispose());
+    return undefined;
+  });
+}
+
+function readF32(file) {
+  const b = fs.readFileSync(file);
+  return new Float32Array(new Uint8Array(b).buffer);
+}
+
+function readU32(file) {
+  const b = fs.readFileSync(file);
+  return new Uint32Array(new Uint8Array(b).buffer);
+}
+
+// The compositor's forward and adjoint, and the draw, on fixtures written by
+// tests/test_napi.py (which compares what this writes with the FP64 oracle)
+async function gpuFixture(dir) {
+  await check('Rasterizer: forward and adjoint of the L2 loss on the fixture scene (on a stream)', () => {
+    const spec = JSON.parse(fs.readFileSync(path.join(dir, 'scene.json'), 'utf8'));
+    const theta = readF32(path.join(dir, 'theta.f32'));
+    const target = readF32(path.join(dir, 'target.f32'));
+    const { n, camera } = spec;
+    const [width, height] = camera.size;
+    const px = width * height;
+    const stream = new GpuStream();
+    const params = DeviceBuffer.fromArray(theta, { stream });
+    const image = new DeviceBuffer(12 * px);
+    const rast = new Rasterizer({ shDegree: 3, stream });
+    const generation = rast.forward(params, n, camera, {}, { image, stream });
+    const img = image.read(0, 3 * px, Float32Array, { stream });
+    // L = mean((image - target)^2): dL/dimage = 2 (image - target) / (3 width height)
+    const g = new Float32Array(3 * px);
+    for (let i = 0; i < g.length; i += 1) g[i] = (2 * (img[i] - target[i])) / g.length;
+    const gradImage = DeviceBuffer.fromArray(g, { stream });
+    const gradParams = new DeviceBuffer(59 * 4 * n);
+    rast.backward(gradImage, gradParams, { generation, stream });
+    fs.writeFileSync(path.join(dir, 'image.f32'), Buffer.from(img.buffer));
+    fs.writeFileSync(path.join(dir, 'grad.f32'), Buffer.from(gradParams.read(0, 59 * n, Float32Array, { stream }).buffer));
+    const stats = rast.stats({ stream });
+    if (!(stats.entries > 0 && stats.capacity >= stats.entries)) return `stats ${JSON.stringify(stats)}`;
+    rast.forward(params, n, camera, {}, { image, stream });
+    const stale = throwsWith(() => rast.backward(gradImage, gradParams, { generation, stream }), 'replaced its state');
+    rast.dispose();
+    [params, image, gradImage, gradParams].forEach((b) => b.dispose());
+    return stale;
+  });
+  await check('SplatDrawer: the fixture frame drawn', () => {
+    const spec = JSON.parse(fs.readFileSync(path.join(dir, 'draw.json'), 'utf8'));
+    const words = DeviceBuffer.fromArray(readU32(path.join(dir, 'words.u32')));
+    const order = DeviceBuffer.fromArray(readU32(path.join(dir, 'order.u32')));
+    const count = DeviceBuffer.fromArray(new Uint32Array([spec.count]));
+    const rgba = new DeviceBuffer(16 * spec.width * spec.height);
+    const drawer = new SplatDrawer();
+    drawer.draw(words, spec.n, order, count, spec.capacity, spec.width, spec.height, {}, rgba);
+    fs.writeFileSync(path.join(dir, 'rgba.f32'), Buffer.from(rgba.read(0, 4 * spec.width * spec.height, Float32Array).buffer));
+    const stats = drawer.stats();
+    drawer.dispose();
+    [words, order, count, rgba].forEach((b) => b.dispose());
+    return stats.drawn === spec.count ? undefined : `drawn ${stats.drawn} of ${spec.count}`;
+  });
+}
+
 (async () => {
   await cpuChecks();
+  await cpuBoundsChecks();
   if (process.argv[2] === 'gpu') {
     await gpuChecks();
     await gpuChecksKeysAndRays();
     await gpuChecksSpz();
+    await gpuBoundsChecks();
+    await gpuChecksStreams();
+    await gpuChecksBatch();
+  } else if (process.argv[2] === 'fixture') {
+    await gpuFixture(process.argv[3]);
   }
   console.log(failures.length ? `${failures.length} failed` : 'all passed');
   process.exit(failures.length ? 1 : 0);

@@ -12,6 +12,7 @@ import torch
 from oracle import oracle as O
 from radix_amd import DeviceTable, Params, Projector
 from radix_amd import workloads as W
+from radix_amd import _native as N
 from radix_amd.draw import Drawer, draw_projected
 from tests.helpers import Rng
 
@@ -154,12 +155,18 @@ def test_dense_tile_split_into_chunks(cuda):
     assert np.abs(early - img.cpu().numpy()).max() <= 1.01 * stop + 1e-5
 
 
-def test_entry_overflow_draws_nothing(cuda):
+def test_entry_limit_raises_and_no_limit_sizes_to_the_frame(cuda):
     Wd, Ht, n = 128, 128, 200
     words = crafted(n, Wd, Ht, 7)
     order = np.arange(n, dtype=np.uint32)
     d = Drawer(entries_per_splat=1)
-    img = d.draw(dev_i32(words), dev_i32(order), dev_i32([n]), Wd, Ht, Params())
+    with pytest.raises(N.RadixError, match=r"entries exceed the limit of 200") as err:
+        d.draw(dev_i32(words), dev_i32(order), dev_i32([n]), Wd, Ht, Params())
+    assert err.value.code == N.RDX_E_CAPACITY
+    free = Drawer()
+    img = free.draw(dev_i32(words), dev_i32(order), dev_i32([n]), Wd, Ht, Params())
     torch.cuda.synchronize()
-    assert d.stats()["overflowed"]
-    assert float(img.abs().max()) == 0.0
+    st = free.stats()
+    assert st["entries"] > n and st["capacity"] >= st["entries"] and not st["overflowed"]
+    ref = O.draw(words, order, n, Wd, Ht)
+    compare(img.cpu().numpy(), ref, frac=0.01)

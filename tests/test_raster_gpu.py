@@ -10,6 +10,7 @@ import torch
 
 from oracle import oracle as O
 from radix_amd import Params
+from radix_amd import _native as N
 from radix_amd import camera as cam
 from radix_amd import diff as D
 from tests.test_oracle_raster import small_scene
@@ -82,16 +83,21 @@ def test_sh_degree_and_untouched_splats(cuda):
     assert rel < 1e-3
 
 
-def test_entry_overflow_is_reported_not_drawn(cuda):
+def test_entry_limit_raises_and_no_limit_sizes_to_the_frame(cuda):
+    # large splats: many more (splat, tile) entries than splats
     theta, r, target = small_scene(n=100, seed=50, log_scale=-1.5)
     rast = D.Rasterizer(sh_degree=3, entries_per_splat=1)
-    img, alpha = rast.forward(D.params_tensor(theta), scene_rig(), Params())
-    st = rast.stats()
-    assert st["overflowed"]
-    assert float(alpha.abs().max()) == 0.0
-    rast2 = D.Rasterizer(sh_degree=3, entries_per_splat=64)
-    rast2.forward(D.params_tensor(theta), scene_rig(), Params())
-    assert not rast2.stats()["overflowed"]
+    with pytest.raises(N.RadixError, match=r"entries exceed the limit of 100") as err:
+        rast.forward(D.params_tensor(theta), scene_rig(), Params())
+    assert err.value.code == N.RDX_E_CAPACITY
+    free = D.Rasterizer(sh_degree=3)
+    img, alpha = free.forward(D.params_tensor(theta), scene_rig(), Params())
+    torch.cuda.synchronize()
+    st = free.stats()
+    assert st["entries"] > 100 and st["capacity"] >= st["entries"] and not st["overflowed"]
+    ref, ref_alpha, _ = O.diff_render(theta, r)
+    d = np.abs(img.cpu().numpy().astype(np.float64) - ref)
+    assert np.mean(d > 1e-4) < 0.005 and np.median(d) < 1e-6
 
 
 def test_full_size_directional_derivatives(cuda):
