@@ -48,20 +48,48 @@ def stage_times(ctx) -> dict:
     return out
 
 
-def cpu_baseline(keys_u32, n: int, reps: int = 15, warm: int = 3) -> dict:
+def host_cpu() -> dict:
+    """The host the CPU baselines ran on: its model, logical CPUs and those this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"model": model, "cpu_count": os.cpu_count(), "usable_cpus": usable}
+
+
+def distances_of(keys):
+    """Distances whose 32-bit keys are `keys`: the float bits of ~key, infinity for the invalid key."""
+    import numpy as np
+
+    k = np.ascontiguousarray(keys, dtype=np.uint32)
+    dist = (~k).view(np.float32).astype(np.float64)
+    dist[k == 0xFFFFFFFF] = np.inf
+    return dist
+
+
+def cpu_baseline(keys_u32, n: int, planes, rig, reps: int = 15, warm: int = 3) -> dict:
     """The oracle's restatement of src/core's SortKeys.compute + SortKeys.sort
-    (32 bits) on the host, one thread, median of `reps` after `warm` warm-ups
-    (scripts/bench-core.ts:261-311)."""
+    (32 bits) on the host, one thread (a sort is single-threaded in the
+    reference), median of `reps` after `warm` warm-ups
+    (scripts/bench-core.ts:261-311); and the "+SH" figure: the same plus
+    Sh.evaluate of every splat at SH3 (src/core/sh.ts:362-373) from the
+    head, timed over the same 4M table."""
     import numpy as np
 
     from oracle import oracle as O
 
     lib = O.lib()
     keys = np.ascontiguousarray(keys_u32[:n], dtype=np.uint32)
-    # distances that give these keys back: the float bits of ~key, and
-    # infinity for the invalid key
-    dist = (~keys).view(np.float32).astype(np.float64)
-    dist[keys == 0xFFFFFFFF] = np.inf
+    dist = distances_of(keys)
     out_keys = np.empty(n, dtype=np.uint32)
     ordering = np.empty(n, dtype=np.uint32)
     times = []
@@ -73,10 +101,64 @@ def cpu_baseline(keys_u32, n: int, reps: int = 15, warm: int = 3) -> dict:
         if r >= warm:
             times.append(dt)
     med = statistics.median(times)
+    # Sh.evaluate per splat: centres and the 45 SH planes of the table, colours accumulated in a copy
+    p = np.ascontiguousarray(planes[:, :n], dtype=np.float32)
+    rgb = np.ascontiguousarray(p[11:14])
+    origin = np.linalg.inv(rig.head)[:3, 3].astype(np.float64)
+    sh_times = []
+    for r in range(4):
+        t0 = time.perf_counter()
+        lib.orc_sh_apply_table(n, n, p[0:3].ctypes.data, p[14:59].ctypes.data, 3, O.dptr(np.ascontiguousarray(origin)),
+                               rgb.ctypes.data)
+        sh_times.append(time.perf_counter() - t0)
+    sh = statistics.median(sh_times[1:])
     return {"value": n / med / 1e9, "unit": "Gsplats/s", "cores": 1, "kind": "port",
+            "host": host_cpu(),
             "sample": f"oracle/sort.c restating src/core SortKeys.compute + SortKeys.sort (32-bit, 2x16-bit LSD) "
                       f"over the {n:,} keys of one step, median of {reps} after {warm} warm-ups, 1 thread; "
-                      f"median {med * 1e3:.1f} ms"}
+                      f"median {med * 1e3:.1f} ms",
+            "plus_sh": {"value": n / (med + sh) / 1e9, "unit": "Gsplats/s", "cores": 1, "kind": "port",
+                        "sample": f"the same plus oracle/codec.c's Sh.evaluate (sh.ts:362-373) of all {n:,} splats "
+                                  f"at SH3 from the head, median {sh * 1e3:.1f} ms of 3 after a warm-up"}}
+
+
+def cpu_baseline_views(keys_u32, segments, reps: int = 3) -> dict:
+    """The reference's CPU sort path for many views: SortKeys.compute + sort
+    of each view's keys on a pool of min(32, usable CPUs) threads, one view
+    per task (SplatView.sort on the worker pool, SplatRenderer.ts:106-116;
+    the oracle's C restatement releases the GIL under ctypes)."""
+    import concurrent.futures as cf
+
+    import numpy as np
+
+    from oracle import oracle as O
+
+    lib = O.lib()
+    host = host_cpu()
+    threads = max(1, min(32, host["usable_cpus"] or 1))
+    work = []
+    for off, n in segments:
+        k = np.ascontiguousarray(keys_u32[off:off + n], dtype=np.uint32)
+        work.append((distances_of(k), np.empty(n, dtype=np.uint32), np.empty(n, dtype=np.uint32)))
+
+    def one(w):
+        dist, out_keys, ordering = w
+        n = dist.size
+        lib.orc_keys_compute(dist.ctypes.data, n, 32, out_keys.ctypes.data)
+        lib.orc_sort(out_keys.ctypes.data, n, 32, ordering.ctypes.data)
+
+    total = sum(n for _, n in segments)
+    times = []
+    with cf.ThreadPoolExecutor(threads) as pool:
+        for r in range(reps + 1):
+            t0 = time.perf_counter()
+            list(pool.map(one, work))
+            if r:
+                times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    return {"value": total / med / 1e9, "unit": "Gsplat-views/s", "cores": threads, "kind": "port", "host": host,
+            "sample": f"oracle/sort.c's SortKeys.compute + sort of each of the {len(segments)} views' keys "
+                      f"({total:,} in all), one view per task on {threads} threads, median {med * 1e3:.1f} ms of {reps}"}
 
 
 def oracle_projection_rate(planes, rig, count: int) -> dict:
@@ -109,7 +191,7 @@ def time_events(fn, steps: int, warmup: int = 2) -> float:
     return e0.elapsed_time(e1) / steps
 
 
-def bench_batched(steps: int) -> dict:
+def bench_batched(steps: int, with_cpu: bool = True) -> dict:
     """Batched views (8 collections, 32 views at 1280x720): one graph replay of
     the batch against 32 plain rdx_project_sort calls, both as splat-views/s
     and as a fraction of HBM peak over sum_c 236 n_c + sum_v 80 n_v bytes."""
@@ -123,7 +205,10 @@ def bench_batched(steps: int) -> dict:
     views = list(zip(wl.view_table, wl.cameras))
     prm = Params(sort_bits=32)
     ctx = Context()
-    b = Batch(tables, views, prm, context=ctx)
+    # the batch's sort tuning from the autotuner (every supported tuning timed on this batch)
+    b = Batch(tables, views, prm, tuning="auto", tune_reps=3)
+    tuning = {"digit_bits": b.tuning[0], "keys_per_thread": b.tuning[1], "block_threads": b.tuning[2],
+              "source": b.tuning_source, "tuned_ms": round(b.tuning_ms, 4)}
     splat_views = b.total
     bytes_ = 236 * sum(sp.count for sp in wl.specs) + 80 * splat_views
     ms_b = time_events(lambda: b.run(graph=True), steps)
@@ -143,16 +228,22 @@ def bench_batched(steps: int) -> dict:
 
     ms_p = time_events(plain_all, steps)
     plain_bytes = sum((236 + 80) * tables[t].count for t, _ in views)
+    cpu = None
+    if with_cpu:
+        torch.cuda.synchronize()
+        keys_h = b.keys.cpu().numpy().view("uint32")
+        cpu = cpu_baseline_views(keys_h, [b.segment(v) for v in range(len(views))])
     b.close()
     return {"workload": "8 collections (log-uniform 1e4..6e6 splats, SH3 f32), 32 views 1280x720 on radius 4-12 orbits",
             "collections": [sp.count for sp in wl.specs], "splat_views": splat_views,
             "batched": {"ms": round(ms_b, 4), "Gsplat_views_per_s": round(splat_views / ms_b / 1e6, 3),
                         "hbm_frac": round(bytes_ / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                        "bytes": bytes_, "form": "one HIP graph: 1 projection launch + 4 x (count, scan, scatter)"},
+                        "bytes": bytes_, "tuning": tuning,
+                        "form": "one HIP graph: 1 projection launch + the segmented sort's passes"},
             "plain": {"ms": round(ms_p, 4), "Gsplat_views_per_s": round(splat_views / ms_p / 1e6, 3),
                       "hbm_frac_of_batched_bytes": round(bytes_ / (ms_p * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                       "bytes_read_and_written": plain_bytes, "form": "32 rdx_project_sort calls"},
-            "speedup": round(ms_p / ms_b, 3)}
+            "speedup": round(ms_p / ms_b, 3), "cpu_baseline": cpu}
 
 
 def bench_stereo(steps: int, count: int = 16777216) -> dict:
@@ -499,7 +590,7 @@ def main() -> int:
         out["loaders_4m"] = bench_loaders(max(3, args.steps // 5))
         torch.cuda.empty_cache()
         log("batched workload")
-        out["batched_vs_plain"] = bench_batched(max(3, args.steps // 5))
+        out["batched_vs_plain"] = bench_batched(max(3, args.steps // 5), with_cpu=not args.no_cpu)
         torch.cuda.empty_cache()
         log("stereo 16M workload")
         out["stereo_16m"] = bench_stereo(max(3, args.steps // 5))
@@ -510,7 +601,7 @@ def main() -> int:
     if not args.no_cpu:
         log("cpu baseline")
         keys_h = keys.cpu().numpy().view(np.uint32)
-        out["cpu_baseline"] = cpu_baseline(keys_h, n)
+        out["cpu_baseline"] = cpu_baseline(keys_h, n, planes, rig)
         out["cpu_oracle_projection"] = oracle_projection_rate(planes, rig, 262144)
     print(json.dumps(out), flush=True)
     return 0

@@ -79,10 +79,20 @@ int rdx_context_get_tuning(const rdx_context* ctx, rdx_sort_tuning* out);
  * `count` of them, max 64, in the order of rdx_sort_tuning_candidate) and
  * keeps the fastest on the context. keys_dev: the keys to sort (NULL:
  * uniform random keys of `bits` bits made on the device). Writes the chosen
- * tuning and its time per sort in milliseconds. Synchronous. */
+ * tuning and its time per sort in milliseconds, and stores them in the
+ * tuning cache (form "sort", one segment). Synchronous. */
 int rdx_sort_tuning_candidate(int index, rdx_sort_tuning* out);  /* RDX_E_INVALID past the last */
 int rdx_context_autotune(rdx_context* ctx, void* stream, const uint32_t* keys_dev, uint32_t n, int bits, int reps,
                          rdx_sort_tuning* out, double* ms_out);
+/* The tuning cache: the autotuners' results per (gfx arch, form "sort" or
+ * "batch", key bits, ceil(log2) of the keys, ceil(log2) of the segments),
+ * kept for the process and, with a cache file (this call, or the
+ * environment's RDX_TUNING_CACHE), read from it and appended to it, one
+ * line per result: arch form bits n_bucket segment_bucket digit_bits
+ * keys_per_thread block_threads ms. NULL or "" keeps the cache in memory. */
+int rdx_tuning_cache_file(const char* path);
+/* 1 and the cached tuning (and its time) when the cache holds one, 0 when not. */
+int rdx_tuning_cache_lookup(const char* form, uint64_t n, uint32_t segments, int bits, rdx_sort_tuning* out, double* ms);
 
 /* Stage timing: with profiling on, each kernel a call on this context
  * launches is bracketed by HIP events on that call's stream. No reference
@@ -391,6 +401,19 @@ const uint32_t* rdx_batch_keys(const rdx_batch* batch);
 #define RDX_BATCH_GRAPH 1
 int rdx_batch_run(rdx_batch* batch, void* stream, uint32_t* out_indices_dev, uint32_t* out_counts_dev,
                   uint32_t* out_draw_args_dev, int flags);
+/* The autotuner of a batch (no reference counterpart: the reference fixes
+ * 8-bit digits and its keys per thread, WgpuRadixSort.ts:13-15, 80-95):
+ * for every supported tuning (rdx_sort_tuning_candidate) it creates the
+ * batch these arguments describe on the context, times `reps` graph replays
+ * of its projection and segmented sort on `stream`, and keeps the fastest
+ * tuning on the context, so a batch created next uses it. The result goes
+ * into the tuning cache under (gfx arch, "batch", key bits, ceil(log2) of
+ * the total keys, ceil(log2) of the views); with RDX_TUNE_CACHED a cached
+ * tuning is taken without timing (*cached_out = 1). Synchronous. */
+#define RDX_TUNE_CACHED 1
+int rdx_batch_autotune(rdx_context* ctx, void* stream, const rdx_splat_table* tables, int num_tables,
+                       const rdx_batch_view* views, int num_views, const rdx_params* params, int reps, int flags,
+                       rdx_sort_tuning* out, double* ms_out, int* cached_out);
 
 /* ---- Differentiable compositor ------------------------------------------------
  * No reference counterpart: the reference draws the sorted splats with

@@ -312,6 +312,24 @@ void orc_sh_evaluate(const double* c, const double* dir, int degree, double* rgb
     for (int ch = 0; ch < 3; ch++) rgb[ch] += c[3 * k + ch] * basis[k];
 }
 
+/* Sh.evaluate over a table (the CPU baseline's "+SH" column): for splat i,
+ * rgb_i += evaluate(coefficients of i, normalize(centre_i - origin)); f32
+ * planes of `stride` (centres 3, SH coefficient-major 3K), rgb 3 planes. */
+void orc_sh_apply_table(size_t n, size_t stride, const float* center, const float* sh, int degree, const double* origin,
+                        float* rgb) {
+  const int K = SH_COEFFS[degree < 0 ? 0 : (degree > 3 ? 3 : degree)];
+  double c[45], out[3], d[3];
+  for (size_t i = 0; i < n; i++) {
+    for (int k = 0; k < 3; k++) d[k] = (double)center[k * stride + i] - origin[k];
+    double len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (len > 0)
+      for (int k = 0; k < 3; k++) d[k] /= len;
+    for (int v = 0; v < 3 * K; v++) c[v] = sh[v * stride + i];
+    orc_sh_evaluate(c, d, degree, out);
+    for (int k = 0; k < 3; k++) rgb[k * stride + i] += (float)out[k];
+  }
+}
+
 /* math.ts:29-45: Sigma = M M^T, M = R(q) diag(scales) (q normalised; zero: identity) */
 void orc_cov_from_scale_quat(const double* s, const double* q, double* cov) {
   double len = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);

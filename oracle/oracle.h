@@ -54,6 +54,8 @@ void orc_ext_decode(const uint32_t* a4, const uint32_t* b4, double* center, doub
 /* ---- src/core/sh.ts ------------------------------------------------------ */
 void orc_sh_basis(const double* dir, double* out24);                       /* sh.ts:376-407 */
 void orc_sh_evaluate(const double* coeffs, const double* dir, int degree, double* rgb); /* sh.ts:362-373 */
+void orc_sh_apply_table(size_t n, size_t stride, const float* center, const float* sh, int degree, const double* origin,
+                        float* rgb);
 uint32_t orc_sh_scale_encode(double max);                                  /* sh.ts:71-80 */
 double orc_sh_scale_decode(uint32_t code);                                 /* sh.ts:82-84 */
 int orc_sh_words(int codec, int degree); /* codec 2 ext, 3 packed (sh.ts:114-174) */

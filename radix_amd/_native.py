@@ -126,6 +126,7 @@ class DrawSettings(C.Structure):
 
 
 BATCH_GRAPH = 1
+TUNE_CACHED = 1
 STATIC_SORT_MAX_KEYS = 1048576
 BATCH_MAX_VIEWS = 4096
 
@@ -184,6 +185,12 @@ _SIGNATURES = [
      [C.c_void_p, C.POINTER(SplatTable), C.c_int, C.POINTER(BatchView), C.c_int, C.POINTER(Params), C.c_void_p,
       C.POINTER(C.c_void_p)]),
     ("rdx_batch_destroy", C.c_int, [C.c_void_p]),
+    ("rdx_batch_autotune", C.c_int,
+     [C.c_void_p, C.c_void_p, C.POINTER(SplatTable), C.c_int, C.POINTER(BatchView), C.c_int, C.POINTER(Params), C.c_int,
+      C.c_int, C.POINTER(SortTuning), C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    ("rdx_tuning_cache_file", C.c_int, [C.c_char_p]),
+    ("rdx_tuning_cache_lookup", C.c_int,
+     [C.c_char_p, C.c_uint64, C.c_uint32, C.c_int, C.POINTER(SortTuning), C.POINTER(C.c_double)]),
     ("rdx_batch_set_cameras", C.c_int, [C.c_void_p, C.POINTER(Camera), C.c_int]),
     ("rdx_batch_total", C.c_uint64, [C.c_void_p]),
     ("rdx_batch_segment_offset", C.c_uint64, [C.c_void_p, C.c_int]),

@@ -23,10 +23,19 @@ class Batch:
     args[5 v : 5 v + 5]; keys[off_v : off_v + n_v] hold its sort keys."""
 
     def __init__(self, tables: Sequence[DeviceTable], views: Sequence[Tuple[int, CameraRig]], params: Params,
-                 words: bool = True, context: Optional[Context] = None, device="cuda"):
+                 words: bool = True, context: Optional[Context] = None, device="cuda", tuning=None,
+                 tune_reps: int = 5, stream=None):
+        """tuning: None (the context's), a (digit bits, keys per thread,
+        threads) tuple, or "auto": rdx_batch_autotune picks it by timing this
+        batch under every supported tuning (a cached result for the same
+        arch, key bits and sizes is taken without timing). The choice is in
+        self.tuning, its time per run in self.tuning_ms and "autotuned",
+        "cache", "given" or "context" in self.tuning_source."""
         import torch
 
-        self.context = context or default_context()
+        if tuning is not None and tuning != "auto":
+            context = Context(tuple(tuning))
+        self.context = context or (Context() if tuning == "auto" else default_context())
         self.tables = list(tables)
         self.views = [(int(t), rig) for t, rig in views]
         self.params = params
@@ -45,6 +54,18 @@ class Batch:
             bv[v].table = t
             bv[v].camera = rig.desc()
         self._views_desc = bv
+        self.tuning_ms = None
+        self.tuning_source = "given" if tuning not in (None, "auto") else "context"
+        if tuning == "auto":
+            t = N.SortTuning()
+            ms = C.c_double()
+            cached = C.c_int()
+            N.check(N.lib().rdx_batch_autotune(self.context.handle, _stream(stream), self._tdesc, len(self.tables), bv,
+                                               len(self.views), C.byref(params.desc()), tune_reps, N.TUNE_CACHED,
+                                               C.byref(t), C.byref(ms), C.byref(cached)))
+            self.tuning_ms = ms.value
+            self.tuning_source = "cache" if cached.value else "autotuned"
+        self.tuning = self.context.tuning()
         h = C.c_void_p()
         N.check(N.lib().rdx_batch_create(self.context.handle, self._tdesc, len(self.tables), bv, len(self.views),
                                           C.byref(params.desc()), C.c_void_p(self.keys.data_ptr()), C.byref(h)))

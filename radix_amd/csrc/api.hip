@@ -7,9 +7,11 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/radix_hip.h"
 #include "api_internal.hpp"
@@ -706,6 +708,124 @@ int rdx_sort_splats_static(void* stream, const uint32_t* keys, uint32_t n, int b
                   out_count, out_args);
 }
 
+}  // extern "C"
+
+// ---- the tuning cache --------------------------------------------------------------
+
+namespace rdx {
+namespace {
+struct TuneEntry {
+  TuneKey key;
+  rdx_sort_tuning tuning;
+  double ms;
+};
+std::mutex g_tune_mu;
+std::vector<TuneEntry> g_tune;
+std::string g_tune_file;
+bool g_tune_loaded = false;
+
+bool same_key(const TuneKey& a, const TuneKey& b) {
+  return a.arch == b.arch && a.form == b.form && a.bits == b.bits && a.n_bucket == b.n_bucket &&
+         a.seg_bucket == b.seg_bucket;
+}
+
+int ceil_log2(uint64_t x) {
+  int b = 0;
+  while (b < 63 && (1ull << b) < x) b++;
+  return b;
+}
+
+// The file's lines: arch form bits n_bucket seg_bucket digit_bits keys_per_thread block_threads ms
+void load_file_locked() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  if (g_tune_file.empty()) {
+    const char* env = getenv("RDX_TUNING_CACHE");
+    if (env && *env) g_tune_file = env;
+  }
+  if (g_tune_file.empty()) return;
+  FILE* f = fopen(g_tune_file.c_str(), "r");
+  if (!f) return;
+  char arch[128], form[32];
+  TuneEntry e;
+  while (fscanf(f, "%127s %31s %d %d %d %d %d %d %lf", arch, form, &e.key.bits, &e.key.n_bucket, &e.key.seg_bucket,
+                &e.tuning.digit_bits, &e.tuning.keys_per_thread, &e.tuning.block_threads, &e.ms) == 9) {
+    e.key.arch = arch;
+    e.key.form = form;
+    if (!tuning_supported(e.tuning)) continue;
+    bool found = false;
+    for (auto& x : g_tune)
+      if (same_key(x.key, e.key)) x = e, found = true;
+    if (!found) g_tune.push_back(e);
+  }
+  fclose(f);
+}
+}  // namespace
+
+TuneKey tune_key(const char* form, uint64_t n, uint32_t segments, int bits) {
+  TuneKey k;
+  k.form = form;
+  k.bits = bits;
+  k.n_bucket = ceil_log2(std::max<uint64_t>(n, 1));
+  k.seg_bucket = ceil_log2(std::max<uint32_t>(segments, 1));
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+    std::string a = prop.gcnArchName;
+    k.arch = a.substr(0, a.find(':'));  // gfx950, without the feature suffixes
+  } else {
+    (void)hipGetLastError();
+    k.arch = "unknown";
+  }
+  return k;
+}
+
+bool tuning_cache_get(const TuneKey& k, rdx_sort_tuning* t, double* ms) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  load_file_locked();
+  for (const auto& e : g_tune)
+    if (same_key(e.key, k)) {
+      if (t) *t = e.tuning;
+      if (ms) *ms = e.ms;
+      return true;
+    }
+  return false;
+}
+
+void tuning_cache_put(const TuneKey& k, const rdx_sort_tuning& t, double ms) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  load_file_locked();
+  bool found = false;
+  for (auto& e : g_tune)
+    if (same_key(e.key, k)) e.tuning = t, e.ms = ms, found = true;
+  if (!found) g_tune.push_back(TuneEntry{k, t, ms});
+  if (g_tune_file.empty()) return;
+  FILE* f = fopen(g_tune_file.c_str(), "a");
+  if (!f) return;
+  fprintf(f, "%s %s %d %d %d %d %d %d %.6f\n", k.arch.c_str(), k.form.c_str(), k.bits, k.n_bucket, k.seg_bucket,
+          t.digit_bits, t.keys_per_thread, t.block_threads, ms);
+  fclose(f);
+}
+
+}  // namespace rdx
+
+extern "C" {
+
+int rdx_tuning_cache_file(const char* path) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tune_file = path ? path : "";
+  g_tune_loaded = false;
+  load_file_locked();
+  return RDX_OK;
+}
+
+int rdx_tuning_cache_lookup(const char* form, uint64_t n, uint32_t segments, int bits, rdx_sort_tuning* out,
+                            double* ms) {
+  if (!form || (strcmp(form, "sort") && strcmp(form, "batch")))
+    return fail(RDX_E_INVALID, "rdx_tuning_cache_lookup: form \"sort\" or \"batch\"");
+  return tuning_cache_get(tune_key(form, n, segments, bits), out, ms) ? 1 : 0;
+}
+
 static const rdx_sort_tuning k_candidates[] = {
     {8, 16, 256}, {8, 8, 256}, {8, 12, 256}, {8, 20, 256}, {8, 8, 512}, {8, 12, 512}, {8, 16, 512},
     {6, 16, 256}, {6, 8, 256}, {6, 12, 256}, {6, 20, 256}, {6, 8, 512}, {6, 12, 512}, {6, 16, 512},
@@ -793,6 +913,7 @@ int rdx_context_autotune(rdx_context* ctx, void* stream, const uint32_t* keys_de
   counts.release();
   if (rc) return rc;
   if (e != hipSuccess) return hip_fail(e, "rdx_context_autotune");
+  tuning_cache_put(tune_key("sort", n, 1, bits), best_t, best);
   if (out) *out = best_t;
   if (ms_out) *ms_out = best;
   return RDX_OK;

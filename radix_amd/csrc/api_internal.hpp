@@ -100,6 +100,19 @@ struct Stage {
   ~Stage();
 };
 
+// The tuning cache: a sort tuning and its time per (gfx arch, form, key bits,
+// ceil(log2) of the keys, ceil(log2) of the segments), kept for the process
+// and, when a cache file is set (rdx_tuning_cache_file or the environment's
+// RDX_TUNING_CACHE), read from and appended to that file.
+struct TuneKey {
+  std::string arch, form;
+  int bits;
+  int n_bucket, seg_bucket;
+};
+TuneKey tune_key(const char* form, uint64_t n, uint32_t segments, int bits);
+bool tuning_cache_get(const TuneKey& k, rdx_sort_tuning* t, double* ms);
+void tuning_cache_put(const TuneKey& k, const rdx_sort_tuning& t, double ms);
+
 int fail(int code, const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what);
 int check_bits(int bits, const char* who);

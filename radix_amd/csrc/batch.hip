@@ -382,4 +382,86 @@ int rdx_batch_run(rdx_batch* batch, void* stream, uint32_t* out_indices, uint32_
   return RDX_OK;
 }
 
+// Candidates in the order of rdx_sort_tuning_candidate; the batch sequence
+// (projection + segmented sort) timed per candidate as graph replays.
+int rdx_batch_autotune(rdx_context* ctx, void* stream, const rdx_splat_table* tables, int num_tables,
+                       const rdx_batch_view* views, int num_views, const rdx_params* params, int reps, int flags,
+                       rdx_sort_tuning* out, double* ms_out, int* cached_out) {
+  if (!ctx || !tables || !views || !params) return fail(RDX_E_INVALID, "rdx_batch_autotune: NULL argument");
+  if (int rc = check_bits(params->sort_bits, "WgpuRadixSort")) return rc;
+  if (num_views < 1 || num_views > RDX_BATCH_MAX_VIEWS || num_tables < 1)
+    return fail(RDX_E_INVALID, "rdx_batch_autotune: %d tables, %d views", num_tables, num_views);
+  Context* c = reinterpret_cast<Context*>(ctx);
+  uint64_t total = 0;
+  for (int v = 0; v < num_views; v++) {
+    if (views[v].table >= (uint32_t)num_tables)
+      return fail(RDX_E_INVALID, "batch view %d: table %u of %d", v, views[v].table, num_tables);
+    total += tables[views[v].table].count;
+  }
+  const TuneKey key = tune_key("batch", total, (uint32_t)num_views, params->sort_bits);
+  if (cached_out) *cached_out = 0;
+  rdx_sort_tuning best_t = c->tuning;
+  double best = 1e30;
+  if ((flags & RDX_TUNE_CACHED) && tuning_cache_get(key, &best_t, &best)) {
+    c->tuning = best_t;
+    if (out) *out = best_t;
+    if (ms_out) *ms_out = best;
+    if (cached_out) *cached_out = 1;
+    return RDX_OK;
+  }
+  reps = std::max(1, std::min(reps, 100));
+  hipStream_t s = (hipStream_t)stream;
+  const rdx_sort_tuning saved = c->tuning;
+  const bool prof = c->profiling;
+  c->profiling = false;
+  Buffer idx, counts, args;
+  int rc = RDX_OK;
+  if ((rc = idx.ensure((size_t)std::max<uint64_t>(total, 1) * 4)) || (rc = counts.ensure((size_t)num_views * 4)) ||
+      (rc = args.ensure((size_t)num_views * 20))) {
+    idx.release();
+    counts.release();
+    args.release();
+    c->profiling = prof;
+    return rc;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  rdx_sort_tuning cand;
+  for (int k = 0; rc == RDX_OK && e == hipSuccess && rdx_sort_tuning_candidate(k, &cand) == RDX_OK; k++) {
+    c->tuning = cand;
+    rdx_batch* b = nullptr;
+    if ((rc = rdx_batch_create(ctx, tables, num_tables, views, num_views, params, nullptr, &b))) break;
+    uint32_t* ip = (uint32_t*)idx.ptr;
+    uint32_t* cp = (uint32_t*)counts.ptr;
+    uint32_t* ap = (uint32_t*)args.ptr;
+    rc = rdx_batch_run(b, stream, ip, cp, ap, RDX_BATCH_GRAPH);  // capture, and one replay to warm
+    if (rc == RDX_OK) e = hipEventRecord(e0, s);
+    for (int r = 0; r < reps && rc == RDX_OK && e == hipSuccess; r++) rc = rdx_batch_run(b, stream, ip, cp, ap, RDX_BATCH_GRAPH);
+    if (rc == RDX_OK && e == hipSuccess) e = hipEventRecord(e1, s);
+    if (rc == RDX_OK && e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (rc == RDX_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (rc == RDX_OK && e == hipSuccess && ms / reps < best) {
+      best = ms / reps;
+      best_t = cand;
+    }
+    const int drc = rdx_batch_destroy(b);
+    if (rc == RDX_OK) rc = drc;
+  }
+  c->profiling = prof;
+  c->tuning = rc == RDX_OK && e == hipSuccess ? best_t : saved;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  idx.release();
+  counts.release();
+  args.release();
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_fail(e, "rdx_batch_autotune");
+  tuning_cache_put(key, best_t, best);
+  if (out) *out = best_t;
+  if (ms_out) *ms_out = best;
+  return RDX_OK;
+}
+
 }  // extern "C"

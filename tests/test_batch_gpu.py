@@ -174,3 +174,50 @@ def test_batch_segments_sort_like_the_oracle(cuda):
         ordering, visible = O.sort(keys[off:off + n], 32)
         assert int(b.counts[v]) == visible
         assert np.array_equal(idx[off:off + visible], ordering[:visible])
+
+
+@pytest.mark.parametrize("tuning", [(6, 16, 256), (4, 8, 512), (8, 12, 256), (8, 20, 256)])
+def test_batch_under_other_tunings_equals_plain(cuda, tuning):
+    # every tuning the autotuner may pick gives the plain path's order
+    wl, tables = small_batch(seed=9, collections=3, views=6)
+    prm = Params(sort_bits=32)
+    b = Batch(tables, list(zip(wl.view_table, wl.cameras)), prm, tuning=tuning)
+    assert b.tuning == tuning and b.tuning_source == "given"
+    b.run(graph=True)
+    check_views(b, tables, prm)
+
+
+def test_batched_workload_at_its_own_shape_autotuned(cuda):
+    # 8 collections (log-uniform 1e4..6e6 splats, SH3), 32 views at 1280x720:
+    # the tuning from the autotuner (then from its cache), graph replays; every
+    # segment's indices, count and draw arguments equal SortKeys.sort of its keys
+    wl = W.batched()
+    tables = []
+    for spec in wl.specs:
+        print(f"generating {spec.count} splats", flush=True)
+        tables.append(DeviceTable.from_f32_planes(W.generate(spec), spec.count, 3))
+    prm = Params(sort_bits=32)
+    views = list(zip(wl.view_table, wl.cameras))
+    b = Batch(tables, views, prm, words=False, tuning="auto", tune_reps=3)
+    print(f"tuning {b.tuning} ({b.tuning_source}, {b.tuning_ms:.3f} ms per run)", flush=True)
+    assert b.tuning_source == "autotuned" and b.tuning_ms > 0
+    again = Batch(tables, views, prm, words=False, tuning="auto")
+    assert again.tuning_source == "cache" and again.tuning == b.tuning
+    del again
+    assert b.total == sum(tables[t].count for t in wl.view_table) > 10_000_000
+    for r in range(2):
+        b.indices.fill_(-1)
+        b.counts.zero_()
+        b.run(graph=True)
+        torch.cuda.synchronize()
+        keys = b.keys.cpu().numpy().view(np.uint32)
+        idx = b.indices.cpu().numpy().view(np.uint32)
+        counts = b.counts.cpu().numpy()
+        args = b.args.cpu().numpy().view(np.uint32).reshape(-1, 5)
+        for v in range(len(views)):
+            off, n = b.segment(v)
+            ordering, visible = O.sort(keys[off:off + n], 32)
+            assert counts[v] == visible, (r, v)
+            assert list(args[v]) == [6144, -(-visible // 1024), 0, 0, 0], (r, v)
+            assert np.array_equal(idx[off:off + visible], ordering[:visible]), (r, v)
+        print(f"replay {r}: {len(views)} segments, {int(counts.sum())} visible of {b.total}", flush=True)

@@ -208,3 +208,21 @@ def test_cov_from_scale_quat_is_r_s2_rt():
         S = R @ np.diag(s ** 2) @ R.T
         exp = [S[0, 0], S[1, 1], S[2, 2], S[0, 1], S[0, 2], S[1, 2]]
         assert np.allclose(cov, exp, rtol=1e-12, atol=1e-15 * np.max(s) ** 2)
+
+
+def test_sh_apply_table_is_sh_evaluate_per_splat():
+    # the CPU baseline's table-wide Sh.evaluate: base colour plus evaluate() along normalize(centre - origin)
+    r = Rng(61)
+    n = 50
+    centre = (r.random(3 * n).reshape(3, n) - 0.5).astype(np.float32) * 4
+    sh = (r.random(45 * n).reshape(45, n) - 0.5).astype(np.float32)
+    rgb = r.random(3 * n).reshape(3, n).astype(np.float32)
+    origin = d3([0.1, -0.2, 0.3])
+    got = np.ascontiguousarray(rgb.copy())
+    L.orc_sh_apply_table(n, n, np.ascontiguousarray(centre).ctypes.data, np.ascontiguousarray(sh).ctypes.data, 3,
+                         O.dptr(origin), got.ctypes.data)
+    out = d3(np.zeros(3))
+    for i in range(n):
+        v = centre[:, i].astype(np.float64) - origin
+        L.orc_sh_evaluate(O.dptr(d3(sh[:, i])), O.dptr(d3(v / np.linalg.norm(v))), 3, O.dptr(out))
+        assert np.allclose(got[:, i], (rgb[:, i] + out).astype(np.float32), rtol=0, atol=1e-6)
