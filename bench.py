@@ -473,6 +473,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--count", type=int, default=4194304)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tuning", default=None,
+                    help="the headline step's sort tuning as digit_bits,keys_per_thread,block_threads (default: the library's)")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and stereo workloads")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 512 MiB five times (the reference kernel that scales "
@@ -506,7 +508,7 @@ def main() -> int:
     rig = W.plain_4m_camera()
     table = DeviceTable.from_f32_planes(planes, spec.count, 3)
     params = Params(sort_bits=32)
-    ctx = Context()
+    ctx = Context(tuple(int(x) for x in args.tuning.split(",")) if args.tuning else None)
     proj = Projector(ctx)
     n = spec.count
     words = [torch.empty((n, 4), dtype=torch.int32, device=dev)]
@@ -567,7 +569,7 @@ def main() -> int:
         "data": "synthetic (seeded 64-cluster cloud, random orbit camera)",
         "config": {"workload": "plain_4m: 4,194,304 splats, SH3 f32 planes (236 B/splat), 1920x1080 camera on a "
                                "radius-8 orbit, 60 deg, 32-bit keys, cullKeys on; step = rdx_project_sort",
-                   "splats": n, "visible": visible, "sort_bits": 32},
+                   "splats": n, "visible": visible, "sort_bits": 32, "sort_tuning": list(ctx.tuning())},
         "hbm_frac_pipeline": round(pipeline_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "pipeline_bytes_per_splat": B_IN_SH3 + B_OUT + B_SORT,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -429,11 +429,9 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
   __shared__ uint16_t s_wave[WAVES][BINS];  // per-wave digit counts, then exclusive offsets (< TILE)
   __shared__ uint32_t s_local[BINS];        // start of each digit within the tile
   __shared__ uint32_t s_out[BINS];          // global start - local start, for the write-out
-  __shared__ uint64_t s_stage[TILE];        // ranking: lane masks; then the staged keys and values
+  __shared__ uint64_t s_stage[TILE];        // ranking: lane masks; then the staged (key, value) pairs
   __shared__ uint32_t s_scan[WAVES];
   uint64_t (*s_match)[BINS] = reinterpret_cast<uint64_t (*)[BINS]>(s_stage);  // per-wave lane masks of a round
-  uint32_t* s_keys = reinterpret_cast<uint32_t*>(s_stage);
-  uint32_t* s_vals = s_keys + TILE;
 
   const int tid = threadIdx.x;
   const int w = tid / RDX_WAVE;
@@ -614,8 +612,7 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
     if (rank[j] != 0xffffffffu) {
       const uint32_t d = dig[j];
       const uint32_t p = s_local[d] + s_wave[w][d] + rank[j];
-      s_keys[p] = keys[j];
-      s_vals[p] = vals[j];
+      s_stage[p] = (uint64_t)keys[j] | ((uint64_t)vals[j] << 32);  // one 8-byte LDS write per key
     }
   }
   __syncthreads();
@@ -626,8 +623,9 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
 #pragma unroll
   for (int r = 0; r < KPT; r++) {
     const uint32_t i = (uint32_t)(r * BLOCK + tid);
-    ok[r] = i < tile_kept ? s_keys[i] : 0u;
-    ov[r] = i < tile_kept ? s_vals[i] : 0u;
+    const uint64_t kv = i < tile_kept ? s_stage[i] : 0ull;
+    ok[r] = (uint32_t)kv;
+    ov[r] = (uint32_t)(kv >> 32);
   }
 #pragma unroll
   for (int r = 0; r < KPT; r++) od[r] = s_out[digit_of(a, ok[r], kbase, DMASK)] + (uint32_t)(r * BLOCK + tid);
