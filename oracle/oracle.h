@@ -221,6 +221,8 @@ typedef struct OrcRaster {
 typedef struct OrcDiffFrozen OrcDiffFrozen;
 int orc_diff_render(const double* theta, int n, const OrcRaster* r, double* image, double* alpha);
 double orc_diff_loss(const double* theta, int n, const OrcRaster* r, const double* target);
+/* 1 (the default): pixels walk 16x16 tile lists of the splats that can reach them; 0: every splat (the same result). */
+void orc_raster_binning(int on);
 double orc_diff_grad(const double* theta, int n, const OrcRaster* r, const double* target, double* grad);
 /* the loss with every pixel's contributor list fixed at theta (smooth in theta) */
 OrcDiffFrozen* orc_diff_freeze(const double* theta, int n, const OrcRaster* r, const double* target);

@@ -109,6 +109,7 @@ def lib() -> C.CDLL:
             "orc_ext_decode": (None, [_u32p, _u32p, _dp, _dp, _dp, _dp, _dp]),
             "orc_sh_basis": (None, [_dp, _dp]),
             "orc_sh_evaluate": (None, [_dp, _dp, C.c_int, _dp]),
+            "orc_raster_binning": (None, [C.c_int]),
             "orc_sh_apply_table": (None, [C.c_size_t, C.c_size_t, C.c_void_p, C.c_void_p, C.c_int, _dp, C.c_void_p]),
             "orc_sh_scale_encode": (C.c_uint32, [_d]),
             "orc_sh_scale_decode": (_d, [C.c_uint32]),

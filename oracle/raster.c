@@ -178,6 +178,80 @@ static DSplat* forward_all(const double* theta, int n, const OrcRaster* r, int* 
   return sp;
 }
 
+/* Binning (an evaluation shortcut, not a change of the maths): a splat's
+   fragments are discarded where z^2 > cut^2, and z^2 = d^T Q d with Q the
+   inverse of the blurred 2D covariance S = [[ap, b], [b, dp]], so they lie
+   within |dx| <= cut sqrt(ap), |dy| <= cut sqrt(dp) of its centre. Each
+   16x16 tile lists, in draw order, the splats whose rectangle (one pixel
+   wider) meets it; a pixel walks its tile's list, which holds every splat
+   that can reach it in the same order as the full list. */
+#define BIN_TILE 16
+static int g_binning = 1;
+
+void orc_raster_binning(int on) { g_binning = on != 0; }
+
+typedef struct {
+  int tiles_x, tiles_y;
+  int* start; /* tiles + 1 */
+  int* list;  /* splat indices, per tile in draw order */
+} Bins;
+
+static int splat_rect(const DSplat* s, int W, int H, int* x0, int* x1, int* y0, int* y1) {
+  const double rx = s->cut * sqrt(fmax(s->ap, 0.0)) + 1.0, ry = s->cut * sqrt(fmax(s->dp, 0.0)) + 1.0;
+  if (!(rx < 1e9) || !(ry < 1e9)) {
+    *x0 = 0, *x1 = W - 1, *y0 = 0, *y1 = H - 1;
+    return 1;
+  }
+  /* pixel x's centre x + 0.5 within [mu - rx, mu + rx] */
+  const double fx0 = ceil(s->mu2[0] - rx - 0.5), fx1 = floor(s->mu2[0] + rx - 0.5);
+  const double fy0 = ceil(s->mu2[1] - ry - 0.5), fy1 = floor(s->mu2[1] + ry - 0.5);
+  if (fx1 < 0 || fy1 < 0 || fx0 > W - 1 || fy0 > H - 1 || fx0 > fx1 || fy0 > fy1) return 0;
+  *x0 = fx0 < 0 ? 0 : (int)fx0;
+  *x1 = fx1 > W - 1 ? W - 1 : (int)fx1;
+  *y0 = fy0 < 0 ? 0 : (int)fy0;
+  *y1 = fy1 > H - 1 ? H - 1 : (int)fy1;
+  return 1;
+}
+
+static void bin_splats(const DSplat* sp, const int* order, int m, int W, int H, Bins* b) {
+  b->tiles_x = (W + BIN_TILE - 1) / BIN_TILE;
+  b->tiles_y = (H + BIN_TILE - 1) / BIN_TILE;
+  const int tiles = b->tiles_x * b->tiles_y;
+  b->start = (int*)calloc((size_t)tiles + 1, sizeof(int));
+  for (int pass = 0; pass < 2; pass++) {
+    int* fill = NULL;
+    if (pass == 1) {
+      for (int t = 0; t < tiles; t++) b->start[t + 1] += b->start[t];
+      b->list = (int*)malloc(sizeof(int) * (size_t)(b->start[tiles] > 0 ? b->start[tiles] : 1));
+      fill = (int*)malloc(sizeof(int) * (size_t)tiles);
+      memcpy(fill, b->start, sizeof(int) * (size_t)tiles);
+    }
+    for (int j = 0; j < m; j++) {
+      int x0, x1, y0, y1;
+      const int i = order[j];
+      if (g_binning) {
+        if (!splat_rect(&sp[i], W, H, &x0, &x1, &y0, &y1)) continue;
+      } else {
+        x0 = 0, x1 = W - 1, y0 = 0, y1 = H - 1;
+      }
+      for (int ty = y0 / BIN_TILE; ty <= y1 / BIN_TILE; ty++)
+        for (int tx = x0 / BIN_TILE; tx <= x1 / BIN_TILE; tx++) {
+          const int t = ty * b->tiles_x + tx;
+          if (pass == 0)
+            b->start[t + 1]++;
+          else
+            b->list[fill[t]++] = i;
+        }
+    }
+    free(fill);
+  }
+}
+
+static void free_bins(Bins* b) {
+  free(b->start);
+  free(b->list);
+}
+
 /* alpha of splat s at pixel p, or -1 where discarded */
 static double pixel_alpha(const DSplat* s, const OrcRaster* r, double px, double py, double* dx, double* dy, double* g) {
   *dx = px - s->mu2[0];
@@ -195,11 +269,14 @@ int orc_diff_render(const double* theta, int n, const OrcRaster* r, double* imag
   int m;
   DSplat* sp = forward_all(theta, n, r, order, &m);
   const int W = r->width, H = r->height;
+  Bins bins;
+  bin_splats(sp, order, m, W, H, &bins);
   for (int y = 0; y < H; y++)
     for (int x = 0; x < W; x++) {
       double T = 1.0, C[3] = {0, 0, 0};
-      for (int j = 0; j < m; j++) {
-        const DSplat* s = &sp[order[j]];
+      const int t = (y / BIN_TILE) * bins.tiles_x + x / BIN_TILE;
+      for (int q = bins.start[t]; q < bins.start[t + 1]; q++) {
+        const DSplat* s = &sp[bins.list[q]];
         double dx, dy, g;
         const double a = pixel_alpha(s, r, x + 0.5, y + 0.5, &dx, &dy, &g);
         if (a < 0.0) continue;
@@ -211,6 +288,7 @@ int orc_diff_render(const double* theta, int n, const OrcRaster* r, double* imag
       for (int c = 0; c < 3; c++) image[(size_t)c * W * H + (size_t)y * W + x] = C[c];
       if (alpha_out) alpha_out[(size_t)y * W + x] = 1.0 - T;
     }
+  free_bins(&bins);
   free(sp);
   free(order);
   return m;
@@ -359,10 +437,13 @@ static double loss_pass(const double* theta, int n, const OrcRaster* r, const do
     rec_cap = (size_t)W * H * 4 + 16;
     record->list = (int*)malloc(sizeof(int) * rec_cap);
   }
+  Bins bins;
+  bin_splats(sp, order, frozen ? 0 : m, W, H, &bins);
   double loss = 0.0;
   for (int y = 0; y < H; y++)
     for (int x = 0; x < W; x++) {
       const size_t p = (size_t)y * W + x;
+      const int tile = (y / BIN_TILE) * bins.tiles_x + x / BIN_TILE;
       double T = 1.0, C[3] = {0, 0, 0};
       int cnt = 0;
       if (frozen) {
@@ -384,8 +465,8 @@ static double loss_pass(const double* theta, int n, const OrcRaster* r, const do
           T *= 1.0 - a;
         }
       } else {
-        for (int j = 0; j < m; j++) {
-          const int i = order[j];
+        for (int q = bins.start[tile]; q < bins.start[tile + 1]; q++) {
+          const int i = bins.list[q];
           const DSplat* s = &sp[i];
           double dx, dy, g;
           const double a = pixel_alpha(s, r, x + 0.5, y + 0.5, &dx, &dy, &g);
@@ -457,6 +538,7 @@ static double loss_pass(const double* theta, int n, const OrcRaster* r, const do
     }
     free(acc);
   }
+  free_bins(&bins);
   free(pl);
   free(pa);
   free(pT);

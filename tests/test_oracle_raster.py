@@ -134,3 +134,21 @@ def test_gradient_matches_unfrozen_differences_away_from_decisions():
         if abs(fd - g[i, k]) <= 1e-3 * max(abs(fd), abs(g[i, k]), 1e-6 * scale):
             agree += 1
     assert agree >= 0.9 * trials, agree
+
+
+def test_binning_changes_nothing():
+    # the tile lists hold every splat that can reach a pixel, in draw order:
+    # image, loss and gradient equal the walk over every splat bit for bit
+    theta, r, target = small_scene(n=300, seed=71, log_scale=-2.0)
+    L = O.lib()
+    try:
+        L.orc_raster_binning(0)
+        img0, a0, d0 = O.diff_render(theta, r)
+        loss0, g0 = O.diff_grad(theta, r, target)
+    finally:
+        L.orc_raster_binning(1)
+    img1, a1, d1 = O.diff_render(theta, r)
+    loss1, g1 = O.diff_grad(theta, r, target)
+    assert d0 == d1 and np.array_equal(img0, img1) and np.array_equal(a0, a1)
+    assert loss0 == loss1 and np.array_equal(g0, g1)
+    assert np.count_nonzero(img1) > 1000

@@ -152,3 +152,26 @@ def test_full_size_directional_derivatives(cuda):
     an = float((g * v).sum().item())
     print("opacity direction: fd", fd, "analytic", an, flush=True)
     assert an == pytest.approx(fd, rel=0.1)
+
+
+def test_full_size_gradient_matches_the_fp64_adjoint(cuda):
+    # 262,144 splats at 512 x 512: the GPU adjoint of the L2 loss against the
+    # FP64 host adjoint (oracle/raster.c, binned) of the same loss, per kind
+    # of parameter: over 512 random parameters of each kind and over all of them
+    theta, target_theta, rig = D.workload()
+    r = O.raster(rig.width, rig.height, rig.eyes[0].view, rig.eyes[0].projection, sh_degree=3)
+    target, _, _ = O.diff_render(target_theta, r)
+    loss_ref, g_ref = O.diff_grad(theta, r, target)
+    loss, g, _, rast = gpu_loss_grad(theta, rig, target)
+    print(f"loss {loss:.6e} vs FP64 {loss_ref:.6e}; stats {rast.stats()}", flush=True)
+    assert loss == pytest.approx(loss_ref, rel=1e-4)
+    rng = np.random.default_rng(512)
+    for name, sl in KINDS.items():
+        a, b = g[:, sl], g_ref[:, sl]
+        flat = np.nonzero(b.reshape(-1))[0]
+        pick = rng.choice(flat, size=512, replace=False)
+        ra, rb = a.reshape(-1)[pick], b.reshape(-1)[pick]
+        rel512 = np.linalg.norm(ra - rb) / np.linalg.norm(rb)
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        print(f"{name}: 512 random parameters {rel512:.2e}, all {rel:.2e}", flush=True)
+        assert rel512 < 1e-3 and rel < 1e-3, name
