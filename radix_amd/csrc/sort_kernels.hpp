@@ -163,6 +163,22 @@ __device__ __forceinline__ bool hidden_rule(uint32_t k, int bits) {
 
 // ---- count ---------------------------------------------------------------------
 
+// Words that one block of a kernel writes and another block of the same
+// kernel reads (the gated pass's scan below) go through the device's
+// coherence point, not an XCD's L2: the writer stores them with agent-scope
+// atomic stores (written through) and waits for its stores to complete
+// before it takes a ticket; the reader loads them with agent-scope atomic
+// loads (which miss the L2) once it holds the last ticket. An agent-scope
+// fence in their place would write back the whole L2 of the writer's XCD in
+// every block.
+__device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void wait_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 // The gated pass's scan of one segment, by the segment's last count block:
 // its digits are bits RANGED_BITS..31 of key - base, so only the first
 // GATED_BINS columns hold counts (the count wrote zeros in the rest, whose
@@ -179,8 +195,15 @@ __device__ __forceinline__ void gated_scan(const PassArgs& a, const SegInfo& g, 
   const uint32_t per = (used + CH - 1) / CH;
   const uint32_t t0 = min(used, (uint32_t)c * per), t1 = min(used, t0 + per);
   uint32_t* col = a.cnt + (size_t)g.tile0 * a.cstride + d;
+  constexpr uint32_t CHUNK = 16;  // loads in flight per thread
   uint32_t sum = 0;
-  for (uint32_t t = t0; t < t1; t++) sum += col[(size_t)t * a.cstride];
+  for (uint32_t t = t0; t < t1; t += CHUNK) {
+    uint32_t v[CHUNK];
+#pragma unroll
+    for (uint32_t k = 0; k < CHUNK; k++) v[k] = t + k < t1 ? ld_agent(col + (size_t)(t + k) * a.cstride) : 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < CHUNK; k++) sum += v[k];
+  }
   __syncthreads();  // lds (the count rows) is free
   lds[c * GATED_BINS + d] = sum;
   __syncthreads();
@@ -196,10 +219,15 @@ __device__ __forceinline__ void gated_scan(const PassArgs& a, const SegInfo& g, 
   for (int r = GATED_BINS + tid; r < MAX_BINS; r += BLOCK) a.tot[(size_t)g.s * TOT_STRIDE + r] = 0u;
   __syncthreads();
   uint32_t run = lds[c * GATED_BINS + d];
-  for (uint32_t t = t0; t < t1; t++) {
-    const uint32_t x = col[(size_t)t * a.cstride];
-    col[(size_t)t * a.cstride] = run;
-    run += x;
+  for (uint32_t t = t0; t < t1; t += CHUNK) {
+    uint32_t v[CHUNK];
+#pragma unroll
+    for (uint32_t k = 0; k < CHUNK; k++) v[k] = t + k < t1 ? ld_agent(col + (size_t)(t + k) * a.cstride) : 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < CHUNK; k++) {
+      if (t + k < t1) col[(size_t)(t + k) * a.cstride] = run;
+      run += v[k];
+    }
   }
 }
 
@@ -285,7 +313,10 @@ __global__ __launch_bounds__(BLOCK) void k_count(PassArgs a) {
     uint32_t c = 0;
 #pragma unroll
     for (int r = 0; r < WAVES * SUBS; r++) c += s_rows[r * ROW + d];
-    a.cnt[(size_t)tile * a.cstride + d] = c;
+    if (a.gate)
+      st_agent(a.cnt + (size_t)tile * a.cstride + d, c);  // read by the segment's last block
+    else
+      a.cnt[(size_t)tile * a.cstride + d] = c;
   }
   if (rule && tid == 0) {
     uint32_t c = 0;
@@ -294,19 +325,16 @@ __global__ __launch_bounds__(BLOCK) void k_count(PassArgs a) {
   }
   if (a.gate) {
     // The gated pass has no scan launch: the segment's last count block to
-    // finish scans its digit columns. Every block's row is released before
-    // its done count (agent scope, across the XCDs' L2s); the last block
-    // acquires them all. No block waits on another.
+    // finish scans its digit columns. Every block's row is at the coherence
+    // point before it counts itself done; the last block reads them all
+    // from there. No block waits on another.
     __shared__ uint32_t s_last;
     const uint32_t used = max(1u, (uint32_t)(((uint64_t)n + TILE - 1) / TILE));
-    __threadfence();
+    wait_stores();
     __syncthreads();
     if (tid == 0) s_last = atomicAdd(&a.done[g.s], 1u) == used - 1u ? 1u : 0u;
     __syncthreads();
-    if (s_last) {
-      __threadfence();
-      gated_scan<BLOCK>(a, g, used, s_rows);
-    }
+    if (s_last) gated_scan<BLOCK>(a, g, used, s_rows);
   }
 }
 
