@@ -56,7 +56,9 @@ __global__ __launch_bounds__(256) void k_project_batch(BatchDesc D, BatchU u, ui
     s_group = g;
   }
   __syncthreads();
-  const BatchGroup G = D.groups[s_group];
+  // uniform across the block: the group's and its views' fields load through
+  // the scalar cache into SGPRs instead of once per lane
+  const BatchGroup G = D.groups[__builtin_amdgcn_readfirstlane(s_group)];
   const TableP& t = D.tables[G.table];
   const int deg = DEG >= 0 ? DEG : D.table_sh_degree[G.table];
   const uint32_t nv = G.num_views;
@@ -71,7 +73,8 @@ __global__ __launch_bounds__(256) void k_project_batch(BatchDesc D, BatchU u, ui
   if (i < t.count) {
     Splat sp;
     float3 coef[15];
-    load_splat<LAYOUT, SHC, DEG>(t, rs, enc, i, deg, sp, coef);
+    float band[3];
+    load_splat<LAYOUT, SHC, DEG>(t, rs, enc, i, deg, sp, coef, band);
     float op, std_dev;
     float3 s;
     bool invisible;
@@ -89,8 +92,10 @@ __global__ __launch_bounds__(256) void k_project_batch(BatchDesc D, BatchU u, ui
           quat_matrix(sp.q, R);
           have_r = true;
         }
-        const float3 rgb = sh_colour(sp, V.h, deg, coef);
-        const EwaResult r = ewa_eye(V.eye, C, true, R, s, op, std_dev, rgb, u);
+        const float3 rgb = sh_colour(sp, V.h, deg, coef, SHC == SHC_PACKED ? band : nullptr);
+        float Q[9];
+        eye_axes(V.eye, R, Q);
+        const EwaResult r = ewa_eye(V.eye, C, true, R, Q, s, op, std_dev, encode_ext_rgb(rgb.x, rgb.y, rgb.z), u);
         if (r.ok) {
           words = r.words;
           visible = true;

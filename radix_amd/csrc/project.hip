@@ -114,6 +114,8 @@ int make_uniforms(const rdx_splat_table* t, const rdx_camera* cam, const rdx_par
     E.fy = (float)(cam->size[1] * 0.5 * cm(cam->eye[e].projection, 1, 1));
     E.ortho = cam->eye[e].ortho ? 1 : 0;
   }
+  // eyes that differ by a translation have the same float L: one Q = L R serves both
+  u->shared_axes = cam->eyes == 2 && memcmp(u->eye[0].Lf, u->eye[1].Lf, sizeof u->eye[0].Lf) == 0 ? 1 : 0;
   u->size[0] = (float)cam->size[0];
   u->size[1] = (float)cam->size[1];
   u->ndc_z_min = (float)cam->ndc_z_min;

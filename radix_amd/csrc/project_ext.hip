@@ -7,6 +7,8 @@ template <int SHC, int EYES>
 static void pick(hipStream_t s, dim3 grid, const TableP& t, const ProjU& u, const OutP& o, bool side) {
   if (side)
     hipLaunchKernelGGL((k_project<LAYOUT_EXT, SHC, -1, EYES, true>), grid, dim3(256), 0, s, t, u, o);
+  else if (u.sh_degree == 3)  // the common degree at a compile-time constant: no degree branches
+    hipLaunchKernelGGL((k_project<LAYOUT_EXT, SHC, 3, EYES, false>), grid, dim3(256), 0, s, t, u, o);
   else
     hipLaunchKernelGGL((k_project<LAYOUT_EXT, SHC, -1, EYES, false>), grid, dim3(256), 0, s, t, u, o);
 }

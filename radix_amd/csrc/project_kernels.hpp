@@ -53,6 +53,7 @@ struct ProjU {
   // splatTransform.tsl.ts:46-62), and that scale (cbrt |det|)
   int head_uniform;
   float head_scale;
+  int shared_axes;  // eye 1's Lf equals eye 0's (a rig whose eyes differ by a translation)
 };
 
 struct TableP {
@@ -200,35 +201,66 @@ __device__ __forceinline__ uint32_t encode_opacity8(float o) {
 
 // ---- SH (src/gpu/sh.tsl.ts:60-139; basis constants of src/core/sh.ts:381-405) ----
 
-// Bands 1..deg of coefficients coef[k] (float3) along unit d.
-__device__ __forceinline__ float3 sh_eval(float3 d, int deg, const float3* coef) {
+// The real SH basis of bands 1..deg along unit d, w[k] for coefficient k.
+__device__ __forceinline__ void sh_basis(float3 d, int deg, float w[15]) {
   const float x = d.x, y = d.y, z = d.z;
-  float3 s = make_float3(0.0f, 0.0f, 0.0f);
-  auto acc = [&](int k, float w) {
-    s.x = fmaf(coef[k].x, w, s.x);
-    s.y = fmaf(coef[k].y, w, s.y);
-    s.z = fmaf(coef[k].z, w, s.z);
-  };
   if (deg >= 1) {
-    acc(0, -0.4886025f * y);
-    acc(1, 0.4886025f * z);
-    acc(2, -0.4886025f * x);
+    w[0] = -0.4886025f * y;
+    w[1] = 0.4886025f * z;
+    w[2] = -0.4886025f * x;
     if (deg >= 2) {
       const float xx = x * x, yy = y * y, zz = z * z;
-      acc(3, 1.0925484f * x * y);
-      acc(4, -1.0925484f * y * z);
-      acc(5, 0.3153916f * (2.0f * zz - xx - yy));
-      acc(6, -1.0925484f * x * z);
-      acc(7, 0.5462742f * (xx - yy));
+      w[3] = 1.0925484f * x * y;
+      w[4] = -1.0925484f * y * z;
+      w[5] = 0.3153916f * (2.0f * zz - xx - yy);
+      w[6] = -1.0925484f * x * z;
+      w[7] = 0.5462742f * (xx - yy);
       if (deg >= 3) {
-        acc(8, -0.5900436f * y * (3.0f * xx - yy));
-        acc(9, 2.8906114f * x * y * z);
-        acc(10, -0.4570458f * y * (4.0f * zz - xx - yy));
-        acc(11, 0.3731763f * z * (2.0f * zz - 3.0f * xx - 3.0f * yy));
-        acc(12, -0.4570458f * x * (4.0f * zz - xx - yy));
-        acc(13, 1.4453057f * z * (xx - yy));
-        acc(14, -0.5900436f * x * (xx - 3.0f * yy));
+        w[8] = -0.5900436f * y * (3.0f * xx - yy);
+        w[9] = 2.8906114f * x * y * z;
+        w[10] = -0.4570458f * y * (4.0f * zz - xx - yy);
+        w[11] = 0.3731763f * z * (2.0f * zz - 3.0f * xx - 3.0f * yy);
+        w[12] = -0.4570458f * x * (4.0f * zz - xx - yy);
+        w[13] = 1.4453057f * z * (xx - yy);
+        w[14] = -0.5900436f * x * (xx - 3.0f * yy);
       }
+    }
+  }
+}
+
+// Bands 1..deg of coefficients coef[k] (float3) along unit d. With `band`
+// (packed SH: coef[k] holds the band's integer codes), each band's sum is
+// scaled by its band[b] once instead of every coefficient.
+__device__ __forceinline__ float3 sh_eval(float3 d, int deg, const float3* coef, const float* band = nullptr) {
+  float w[15];
+  sh_basis(d, deg, w);
+  const int K = deg >= 3 ? 15 : (deg == 2 ? 8 : (deg == 1 ? 3 : 0));
+  float3 s = make_float3(0.0f, 0.0f, 0.0f);
+  if (!band) {
+#pragma unroll
+    for (int k = 0; k < 15; k++) {
+      if (k < K) {
+        s.x = fmaf(coef[k].x, w[k], s.x);
+        s.y = fmaf(coef[k].y, w[k], s.y);
+        s.z = fmaf(coef[k].z, w[k], s.z);
+      }
+    }
+    return s;
+  }
+#pragma unroll
+  for (int b = 0; b < 3; b++) {
+    const int k0 = b == 0 ? 0 : (b == 1 ? 3 : 8), k1 = b == 0 ? 3 : (b == 1 ? 8 : 15);
+    if (k0 < K) {
+      float3 t = make_float3(0.0f, 0.0f, 0.0f);
+#pragma unroll
+      for (int k = k0; k < k1; k++) {
+        t.x = fmaf(coef[k].x, w[k], t.x);
+        t.y = fmaf(coef[k].y, w[k], t.y);
+        t.z = fmaf(coef[k].z, w[k], t.z);
+      }
+      s.x = fmaf(t.x, band[b], s.x);
+      s.y = fmaf(t.y, band[b], s.y);
+      s.z = fmaf(t.z, band[b], s.z);
     }
   }
   return s;
@@ -302,7 +334,7 @@ __device__ __forceinline__ void st_stream(uint4* p, uint4 w) {
 // DEG >= 0: a compile-time degree (f32 planes); DEG < 0: the runtime `deg`.
 template <int LAYOUT, int SHC, int DEG>
 __device__ __forceinline__ void load_splat(const TableP& t, const F32Rsrc& rs, const float* enc, uint32_t i, int deg,
-                                           Splat& sp, float3* coef) {
+                                           Splat& sp, float3* coef, float* band) {
   if constexpr (LAYOUT == LAYOUT_F32) {
     constexpr int K = DEG >= 3 ? 15 : (DEG == 2 ? 8 : (DEG == 1 ? 3 : 0));
     const uint32_t v = i * 4u, s = rs.st4;
@@ -370,30 +402,34 @@ __device__ __forceinline__ void load_splat(const TableP& t, const F32Rsrc& rs, c
           w[4 * x + 2] = have ? tex[x].z : 0u;
           w[4 * x + 3] = have ? tex[x].w : 0u;
         }
-        const float sc[3] = {sh_scale(sh_byte(w, 0)), deg >= 2 ? sh_scale(sh_byte(w, 10)) : 0.0f,
-                             deg >= 3 ? sh_scale(sh_byte(w, 26)) : 0.0f};
+        // the codes here, each band's scale applied to its sum (sh_eval)
+        band[0] = sh_scale(sh_byte(w, 0));
+        band[1] = deg >= 2 ? sh_scale(sh_byte(w, 10)) : 0.0f;
+        band[2] = deg >= 3 ? sh_scale(sh_byte(w, 26)) : 0.0f;
 #pragma unroll
         for (int k = 0; k < 15; k++) {
-          const int band = k < 3 ? 0 : (k < 8 ? 1 : 2);
-          const int start = band == 0 ? 0 : (band == 1 ? 10 : 26);
-          const int first = band == 0 ? 0 : (band == 1 ? 3 : 8);
+          const int b = k < 3 ? 0 : (k < 8 ? 1 : 2);
+          const int start = b == 0 ? 0 : (b == 1 ? 10 : 26);
+          const int first = b == 0 ? 0 : (b == 1 ? 3 : 8);
           const int at = start + 1 + (k - first) * 3;
-          coef[k] = make_float3((float)(int8_t)sh_byte(w, at) * sc[band], (float)(int8_t)sh_byte(w, at + 1) * sc[band],
-                                (float)(int8_t)sh_byte(w, at + 2) * sc[band]);
+          coef[k] = make_float3((float)(int8_t)sh_byte(w, at), (float)(int8_t)sh_byte(w, at + 1),
+                                (float)(int8_t)sh_byte(w, at + 2));
         }
       }
     }
   }
 }
 
-// base + SH(normalize(c - origin)), clamped at 0 (sh.tsl.ts:255-266)
-__device__ __forceinline__ float3 sh_colour(const Splat& sp, const double* origin, int deg, const float3* coef) {
+// base + SH(normalize(c - origin)), clamped at 0 (sh.tsl.ts:255-266); band:
+// the packed bands' scales (NULL for the other codecs)
+__device__ __forceinline__ float3 sh_colour(const Splat& sp, const double* origin, int deg, const float3* coef,
+                                            const float* band = nullptr) {
   if (deg <= 0) return sp.rgb;
   const float dx = (float)((double)sp.c.x - origin[0]);
   const float dy = (float)((double)sp.c.y - origin[1]);
   const float dz = (float)((double)sp.c.z - origin[2]);
   const float rl = rsqrtf(dx * dx + dy * dy + dz * dz);
-  const float3 s = sh_eval(make_float3(dx * rl, dy * rl, dz * rl), deg, coef);
+  const float3 s = sh_eval(make_float3(dx * rl, dy * rl, dz * rl), deg, coef, band);
   return make_float3(fmaxf(sp.rgb.x + s.x, 0.0f), fmaxf(sp.rgb.y + s.y, 0.0f), fmaxf(sp.rgb.z + s.z, 0.0f));
 }
 
@@ -454,23 +490,28 @@ struct EwaResult {
   uint4 words;
 };
 
-// The ewa method for one eye (projections.tsl.ts:95-148 with boundFromCov,
-// projection.tsl.ts:215-240, and the word codecs): `pass` is the eye's
-// centre culls, R the rotation, s the (inflated) scales. U supplies the
-// params (min_alpha, blur, ...).
-template <typename U>
-__device__ __forceinline__ EwaResult ewa_eye(const EyeU& E, const EyeCentre& C, bool pass, const float* R, float3 s,
-                                             float op, float std_dev, float3 rgb, const U& u) {
-  EwaResult r;
-  const float x = (float)C.ce[0], y = (float)C.ce[1], z = (float)C.ce[2];
-  r.z = z;
-  // Q = L R, J rows (projections.tsl.ts:104-117)
-  float Q[9];
+// Q = L R, the splat's axes in eye space (projections.tsl.ts:104-110)
+__device__ __forceinline__ void eye_axes(const EyeU& E, const float* R, float Q[9]) {
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
     for (int j = 0; j < 3; j++)
       Q[3 * i + j] = fmaf(E.Lf[3 * i], R[j], fmaf(E.Lf[3 * i + 1], R[3 + j], E.Lf[3 * i + 2] * R[6 + j]));
+}
+
+// The ewa method for one eye (projections.tsl.ts:95-148 with boundFromCov,
+// projection.tsl.ts:215-240, and the word codecs): `pass` is the eye's
+// centre culls, R the rotation, Q = eye_axes(E, R), s the (inflated)
+// scales, rgb_word the colour's word (encode_ext_rgb, the same for every eye
+// of one SH origin). U supplies the params (min_alpha, blur, ...).
+template <typename U>
+__device__ __forceinline__ EwaResult ewa_eye(const EyeU& E, const EyeCentre& C, bool pass, const float* R,
+                                             const float* Q, float3 s, float op, float std_dev, uint32_t rgb_word,
+                                             const U& u) {
+  EwaResult r;
+  const float x = (float)C.ce[0], y = (float)C.ce[1], z = (float)C.ce[2];
+  r.z = z;
+  // J rows (projections.tsl.ts:111-117)
   float j00, j02, j11, j12, kdet, l0, l1, l2;
   if (E.ortho) {
     j00 = E.fx;
@@ -559,7 +600,7 @@ __device__ __forceinline__ EwaResult ewa_eye(const EyeU& E, const EyeCentre& C, 
   if (r.ok) {
     const float stored = r.opc > 1.0f ? (r.opc + 3.0f) * 0.25f : r.opc;
     r.words = make_uint4(encode_fixed16(r.ndc.x) | (encode_fixed16(r.ndc.y) << 16),
-                         encode_depth20(z) | (encode_angle12(r.angle) << 20), encode_ext_rgb(rgb.x, rgb.y, rgb.z),
+                         encode_depth20(z) | (encode_angle12(r.angle) << 20), rgb_word,
                          encode_opacity8(stored) | (encode_log_scale12(r.sc1) << 8) | (encode_log_scale12(r.sc2) << 20));
   }
   return r;
@@ -674,7 +715,8 @@ __global__ __launch_bounds__(256) void k_project(TableP t, ProjU u, OutP o) {
   if (i < n) {
     Splat sp;
     float3 coef[15];
-    load_splat<LAYOUT, SHC, DEG>(t, rs, u.enc, i, deg, sp, coef);
+    float band[3];
+    load_splat<LAYOUT, SHC, DEG>(t, rs, u.enc, i, deg, sp, coef, band);
     float op, std_dev;
     float3 s;
     bool invisible;
@@ -688,10 +730,12 @@ __global__ __launch_bounds__(256) void k_project(TableP t, ProjU u, OutP o) {
       C[e].pass = C[e].pass && !invisible;
       any |= C[e].pass;
     }
-    // The key distance: |head-space centre| (per eye with key_per_eye)
+    // The key distance: |head-space centre| (per eye with key_per_eye); the
+    // head-space centre also places the sortGuard test
     double head_dist = C[0].dist;
     double hc[3] = {C[0].ce[0], C[0].ce[1], C[0].ce[2]};
-    if (!u.head_is_eye0) {
+    const bool need_head = !per_eye || SIDE || (!u.cull_keys && u.sort_guard > 0.0f);
+    if (!u.head_is_eye0 && need_head) {
       const double dx = (double)sp.c.x - u.h[0], dy = (double)sp.c.y - u.h[1], dz = (double)sp.c.z - u.h[2];
       for (int r = 0; r < 3; r++) hc[r] = fma(u.H[3 * r], dx, fma(u.H[3 * r + 1], dy, u.H[3 * r + 2] * dz));
       head_dist = sqrt(fma(hc[0], hc[0], fma(hc[1], hc[1], hc[2] * hc[2])));
@@ -709,11 +753,22 @@ __global__ __launch_bounds__(256) void k_project(TableP t, ProjU u, OutP o) {
     float R[9];
     if (any || SIDE) {
       quat_matrix(sp.q, R);
-      rgb = sh_colour(sp, u.h, deg, coef);
+      rgb = sh_colour(sp, u.h, deg, coef, SHC == SHC_PACKED ? band : nullptr);
+      const uint32_t rgb_word = encode_ext_rgb(rgb.x, rgb.y, rgb.z);
+      // the eyes of a rig usually share their rotation (shared_axes): then
+      // one Q serves both
+      float Q0[9];
+      eye_axes(u.eye[0], R, Q0);
 #pragma unroll
       for (int e = 0; e < EYES; e++) {
         if (!C[e].pass && !SIDE) continue;
-        const EwaResult r = ewa_eye(u.eye[e], C[e], C[e].pass, R, s, op, std_dev, rgb, u);
+        float Qe[9];
+        const float* Q = Q0;
+        if (e > 0 && !u.shared_axes) {
+          eye_axes(u.eye[e], R, Qe);
+          Q = Qe;
+        }
+        const EwaResult r = ewa_eye(u.eye[e], C[e], C[e].pass, R, Q, s, op, std_dev, rgb_word, u);
         if (r.ok) {
           words[e] = r.words;
           vis[e] = true;
@@ -806,7 +861,8 @@ __global__ __launch_bounds__(256) void k_project_keys(TableP t, ProjU u, uint32_
   if constexpr (LAYOUT == LAYOUT_F32) rs = f32_rsrc(t, 0);
   Splat sp;
   float3 coef[1];
-  load_splat<LAYOUT, SHC_NONE, LAYOUT == LAYOUT_F32 ? 0 : -1>(t, rs, u.enc, i, 0, sp, coef);
+  float band[3];
+  load_splat<LAYOUT, SHC_NONE, LAYOUT == LAYOUT_F32 ? 0 : -1>(t, rs, u.enc, i, 0, sp, coef, band);
   float op, std_dev;
   float3 s;
   bool invisible;

@@ -431,12 +431,13 @@ __global__ __launch_bounds__(256) void k_project_ray(TableP t, RayU r, RayOut o)
   if constexpr (LAYOUT == LAYOUT_F32) rs = f32_rsrc(t, deg);
   Splat sp;
   float3 coef[15];
-  load_splat<LAYOUT, SHC, DEG>(t, rs, u.enc, i, deg, sp, coef);
+  float band[3];
+  load_splat<LAYOUT, SHC, DEG>(t, rs, u.enc, i, deg, sp, coef, band);
   float op, std_dev;
   float3 s;
   bool invisible;
   prepare(sp, u, op, s, std_dev, invisible);
-  const float3 rgb = sh_colour(sp, u.h, deg, coef);
+  const float3 rgb = sh_colour(sp, u.h, deg, coef, SHC == SHC_PACKED ? band : nullptr);
 
   // head-space centre (double) and its distance, as the ewa kernel takes them
   double hc[3], head_dist;
