@@ -191,8 +191,7 @@ int SortScratch::reserve(uint32_t n, uint32_t segments, uint32_t tiles) {
   if ((rc = cnt.ensure((size_t)count_stride(MAX_BINS + 1) * tiles * 4))) return rc;
   if ((rc = tot.ensure((size_t)TOT_STRIDE * std::max<uint32_t>(segments, 1) * 4))) return rc;
   if ((rc = nvalid.ensure((size_t)std::max<uint32_t>(segments, 1) * 4))) return rc;
-  // ranged sorts: tile and segment ranges, then the gated pass's done counts
-  if ((rc = mm.ensure(((size_t)2 * tiles + (size_t)3 * std::max<uint32_t>(segments, 1)) * 4))) return rc;
+  if ((rc = mm.ensure(ranged_words(tiles, segments) * 4))) return rc;
   return RDX_OK;
 }
 
@@ -250,9 +249,14 @@ int run_sort(Context* ctx, SortScratch& sc, hipStream_t s, const SortPlan& p, co
   const int bins = 1 << p.rb;
   const uint32_t* kin = keys_in;
   const uint32_t* vin = vals_in;
+  // the ranged sort's words (ranged_words): tile and segment ranges, the
+  // gated pass's done tickets, tile rows and digit totals
+  const size_t nseg = std::max<uint32_t>(L.segments, 1);
   uint32_t* tile_mm = (uint32_t*)sc.mm.ptr;
   uint32_t* seg_mm = tile_mm + 2 * (size_t)L.tiles;
-  uint32_t* done = seg_mm + 2 * (size_t)std::max<uint32_t>(L.segments, 1);
+  uint32_t* done = seg_mm + 2 * nseg;
+  uint32_t* cnt_g = done + nseg;
+  uint32_t* tot_g = cnt_g + (size_t)GATED_BINS * L.tiles;
   for (int pass = 0; pass < p.passes; pass++) {
     const bool last = pass == p.passes - 1;
     PassArgs a;
@@ -292,6 +296,9 @@ int run_sort(Context* ctx, SortScratch& sc, hipStream_t s, const SortPlan& p, co
       a.seg_mm = seg_mm;
       a.gate = pass == 3;
       a.alt = pass == 2;
+      a.zero_g = pass == 1;
+      a.cnt_g = cnt_g;
+      a.tot_g = tot_g;
       a.done = done;
       if (pass < 2) {
         a.keys_out = (uint32_t*)sc.keys[pass].ptr;
@@ -312,12 +319,13 @@ int run_sort(Context* ctx, SortScratch& sc, hipStream_t s, const SortPlan& p, co
       a.draw_args = pass == 0 ? draw_args : nullptr;
     }
     char name[32];
-    {
+    // the gated pass is a scatter alone: the third pass counted and scanned its digits
+    if (!a.gate) {
       snprintf(name, sizeof name, "sort_count%d", pass);
       Stage st(ctx, s, name);
       if (!launch_pass_kernel(p, PassKernel::count, L.tiles, s, a)) return fail(RDX_E_INVALID, "unsupported sort tuning");
     }
-    if (!a.gate) {  // the gated pass's count blocks scan (gated_scan)
+    if (!a.gate) {
       snprintf(name, sizeof name, "sort_scan%d", pass);
       Stage st(ctx, s, name);
       // column groups per segment, and the range reduction in a ranged sort's first pass
@@ -639,7 +647,7 @@ constexpr size_t kStaticPair = round64(RDX_STATIC_SORT_MAX_KEYS);
 constexpr size_t kStaticCnt = round64((size_t)count_stride(MAX_BINS + 1) * kStaticTiles);
 constexpr size_t kStaticTot = round64(TOT_STRIDE);
 constexpr size_t kStaticNvalid = 64;
-constexpr size_t kStaticMm = round64(2 * (size_t)kStaticTiles + 3);
+constexpr size_t kStaticMm = round64(ranged_words(kStaticTiles, 1));
 constexpr size_t kStaticWords = 4 * kStaticPair + kStaticCnt + kStaticTot + kStaticNvalid + kStaticMm;
 std::mutex g_static_mu;
 constexpr int kMaxDevices = 64;

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/radix_hip.h"
+#include "sort_kernels.hpp"
 
 namespace rdx {
 
@@ -30,6 +31,14 @@ struct StageRecord {
   hipEvent_t start, end;
 };
 
+// Words of a ranged sort's own scratch: [tiles][2] then [segments][2]
+// smallest and largest kept key, [segments] gated-pass tickets,
+// [tiles][GATED_BINS] gated-pass tile rows and [segments][GATED_BINS]
+// gated-pass digit totals.
+constexpr size_t ranged_words(size_t tiles, size_t segments) {
+  return 2 * tiles + 3 * (segments ? segments : 1) + (size_t)GATED_BINS * (tiles + (segments ? segments : 1));
+}
+
 // Device scratch of sorts: the passes' alternating (key, value) buffers
 // and the tile counts, digit totals and kept counts.
 struct SortScratch {
@@ -37,7 +46,7 @@ struct SortScratch {
   Buffer cnt;     // [tiles][count_stride(MAX_BINS + 1)]
   Buffer tot;     // [segments][TOT_STRIDE]
   Buffer nvalid;  // [segments]
-  Buffer mm;      // ranged sorts: [tiles][2] then [segments][2] smallest and largest kept key
+  Buffer mm;      // ranged sorts: ranged_words(tiles, segments)
   Buffer seg;     // segment table [segments][4] and tile map [tiles] (small segmented sorts)
   int reserve(uint32_t n, uint32_t segments, uint32_t tiles);
   void release() {

@@ -100,13 +100,20 @@ struct PassArgs {
   uint32_t* seg_mm;          // [segments][2] the same per segment (the first pass's scan)
   int gate;                  // run only for segments whose keys reach beyond RANGED_BITS above the base
   int peel;                  // rank a round's most present digits by ballots first (the top digit pass)
-  int alt;                   // segments that reach beyond write to keys_alt / vals_alt instead
+  int alt;                   // segments that reach beyond write to keys_alt / vals_alt instead, and
+                             // count their gated digits per output tile into cnt_g
   uint32_t* keys_alt;
   uint32_t* vals_alt;
-  // gated pass: per segment, the count blocks done so far; the last one
-  // scans the segment's GATED_BINS digit columns (no separate scan launch).
-  // Zeroed by the first ranged pass's scan.
-  uint32_t* done;
+  // The gated (fourth) pass has no count or scan launch of its own. For the
+  // segments that need it, the second ranged pass's count blocks clear their
+  // tile's row of cnt_g (zero_g), the third pass's scatter counts each key's
+  // gated digit into the row of the tile it lands in, and the segment's last
+  // scatter block to finish (its ticket in done, cleared by the first pass's
+  // scan) scans the rows in place and writes the digit totals to tot_g.
+  int zero_g;
+  uint32_t* cnt_g;           // [tiles][GATED_BINS]
+  uint32_t* tot_g;           // [segments][GATED_BINS]
+  uint32_t* done;            // [segments]
 };
 
 struct SegInfo {
@@ -163,29 +170,25 @@ __device__ __forceinline__ bool hidden_rule(uint32_t k, int bits) {
 
 // ---- count ---------------------------------------------------------------------
 
-// Words that one block of a kernel writes and another block of the same
-// kernel reads (the gated pass's scan below) go through the device's
-// coherence point, not an XCD's L2: the writer stores them with agent-scope
-// atomic stores (written through) and waits for its stores to complete
-// before it takes a ticket; the reader loads them with agent-scope atomic
-// loads (which miss the L2) once it holds the last ticket. An agent-scope
-// fence in their place would write back the whole L2 of the writer's XCD in
-// every block.
-__device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// Counts that the blocks of one kernel add up and one block of the same
+// kernel then reads (the gated pass's counts below) are device-scope atomic
+// adds, which are performed at the device's coherence point, not in an
+// XCD's L2. Each block waits for its adds to complete before it takes a
+// ticket; the block holding the last ticket reads the counts with
+// agent-scope atomic loads, which miss the L2. An agent-scope fence in
+// their place would write back the whole L2 of the writer's XCD in every
+// block.
 __device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void wait_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
-// The gated pass's scan of one segment, by the segment's last count block:
-// its digits are bits RANGED_BITS..31 of key - base, so only the first
-// GATED_BINS columns hold counts (the count wrote zeros in the rest, whose
-// exclusive prefixes are zeros too). Thread (c, d) sums column d over the
-// c-th run of tiles; the runs' sums are scanned per column; each thread
-// writes its run's prefixes in place. The totals go to tot[s] (zeros past
-// GATED_BINS).
+// The gated pass's scan of one segment, by the segment's last third-pass
+// scatter block: thread (c, d) sums column d of cnt_g over the c-th run of
+// the segment's tiles; the runs' sums are scanned per column; each thread
+// writes its run's exclusive prefixes in place, and the column totals go to
+// tot_g. The rows were counted with device-scope atomics, so they are read
+// at the coherence point.
 template <int BLOCK>
 __device__ __forceinline__ void gated_scan(const PassArgs& a, const SegInfo& g, uint32_t used, uint32_t* lds) {
   constexpr int CH = BLOCK / GATED_BINS;
@@ -194,17 +197,16 @@ __device__ __forceinline__ void gated_scan(const PassArgs& a, const SegInfo& g, 
   const int d = tid % GATED_BINS, c = tid / GATED_BINS;
   const uint32_t per = (used + CH - 1) / CH;
   const uint32_t t0 = min(used, (uint32_t)c * per), t1 = min(used, t0 + per);
-  uint32_t* col = a.cnt + (size_t)g.tile0 * a.cstride + d;
+  uint32_t* col = a.cnt_g + (size_t)g.tile0 * GATED_BINS + d;
   constexpr uint32_t CHUNK = 16;  // loads in flight per thread
   uint32_t sum = 0;
   for (uint32_t t = t0; t < t1; t += CHUNK) {
     uint32_t v[CHUNK];
 #pragma unroll
-    for (uint32_t k = 0; k < CHUNK; k++) v[k] = t + k < t1 ? ld_agent(col + (size_t)(t + k) * a.cstride) : 0u;
+    for (uint32_t k = 0; k < CHUNK; k++) v[k] = t + k < t1 ? ld_agent(col + (size_t)(t + k) * GATED_BINS) : 0u;
 #pragma unroll
     for (uint32_t k = 0; k < CHUNK; k++) sum += v[k];
   }
-  __syncthreads();  // lds (the count rows) is free
   lds[c * GATED_BINS + d] = sum;
   __syncthreads();
   if (c == 0) {
@@ -214,18 +216,17 @@ __device__ __forceinline__ void gated_scan(const PassArgs& a, const SegInfo& g, 
       lds[k * GATED_BINS + d] = run;
       run += x;
     }
-    a.tot[(size_t)g.s * TOT_STRIDE + d] = run;
+    a.tot_g[(size_t)g.s * GATED_BINS + d] = run;
   }
-  for (int r = GATED_BINS + tid; r < MAX_BINS; r += BLOCK) a.tot[(size_t)g.s * TOT_STRIDE + r] = 0u;
   __syncthreads();
   uint32_t run = lds[c * GATED_BINS + d];
   for (uint32_t t = t0; t < t1; t += CHUNK) {
     uint32_t v[CHUNK];
 #pragma unroll
-    for (uint32_t k = 0; k < CHUNK; k++) v[k] = t + k < t1 ? ld_agent(col + (size_t)(t + k) * a.cstride) : 0u;
+    for (uint32_t k = 0; k < CHUNK; k++) v[k] = t + k < t1 ? ld_agent(col + (size_t)(t + k) * GATED_BINS) : 0u;
 #pragma unroll
     for (uint32_t k = 0; k < CHUNK; k++) {
-      if (t + k < t1) col[(size_t)(t + k) * a.cstride] = run;
+      if (t + k < t1) col[(size_t)(t + k) * GATED_BINS] = run;
       run += v[k];
     }
   }
@@ -260,7 +261,8 @@ __global__ __launch_bounds__(BLOCK) void k_count(PassArgs a) {
     s_mm[1] = 0u;
   }
   const SegInfo g = seg_info(a, a.seg_of_tile ? a.seg_of_tile[tile] : 0u);
-  if (a.gate && !seg_wide(a, g.s)) return;  // uniform per block
+  // clear this tile's row of the gated pass's counts (its segment needs that pass)
+  if (a.zero_g && tid < GATED_BINS && seg_wide(a, g.s)) a.cnt_g[(size_t)tile * GATED_BINS + tid] = 0u;
   const uint32_t kbase = seg_base(a, g.s);
   const uint32_t base = (tile - g.tile0) * (uint32_t)TILE;
   const uint32_t* keys = a.keys_in + g.off;
@@ -313,28 +315,12 @@ __global__ __launch_bounds__(BLOCK) void k_count(PassArgs a) {
     uint32_t c = 0;
 #pragma unroll
     for (int r = 0; r < WAVES * SUBS; r++) c += s_rows[r * ROW + d];
-    if (a.gate)
-      st_agent(a.cnt + (size_t)tile * a.cstride + d, c);  // read by the segment's last block
-    else
-      a.cnt[(size_t)tile * a.cstride + d] = c;
+    a.cnt[(size_t)tile * a.cstride + d] = c;
   }
   if (rule && tid == 0) {
     uint32_t c = 0;
     for (int v = 0; v < WAVES; v++) c += s_rule[v];
     a.cnt[(size_t)tile * a.cstride + BINS] = c;
-  }
-  if (a.gate) {
-    // The gated pass has no scan launch: the segment's last count block to
-    // finish scans its digit columns. Every block's row is at the coherence
-    // point before it counts itself done; the last block reads them all
-    // from there. No block waits on another.
-    __shared__ uint32_t s_last;
-    const uint32_t used = max(1u, (uint32_t)(((uint64_t)n + TILE - 1) / TILE));
-    wait_stores();
-    __syncthreads();
-    if (tid == 0) s_last = atomicAdd(&a.done[g.s], 1u) == used - 1u ? 1u : 0u;
-    __syncthreads();
-    if (s_last) gated_scan<BLOCK>(a, g, used, s_rows);
   }
 }
 
@@ -364,7 +350,6 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(PassArgs a, int bins) {
     tile0 = a.seg[4 * s];
     ntiles = a.seg[4 * s + 3];
   }
-  if (a.gate && !seg_wide(a, s)) return;  // uniform per block
   {
     // the tiles that hold keys (at least the first): the count kernel wrote only theirs
     const uint32_t nk = seg_count(a, seg_info(a, s));
@@ -469,6 +454,9 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
   // the kept keys after a dropping pass), and a map over the whole grid would
   // give all of them to the first XCDs. The tiles past them keep their own
   // index and return below. A bijection of [0, tiles) either way.
+  // The gated pass: most sorts are done after three passes, and their blocks
+  // return before anything else (uniform per block).
+  if (a.gate && !a.seg_of_tile && !seg_wide(a, 0u)) return;
   uint32_t tile;
   if (!a.seg && !a.seg_of_tile) {
     const uint32_t n0 = seg_count(a, seg_info(a, 0u));
@@ -477,16 +465,17 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
   } else {
     tile = xcd_tile(blockIdx.x, a.tiles);
   }
+  const SegInfo g = seg_info(a, a.seg_of_tile ? a.seg_of_tile[tile] : 0u);
+  if (a.gate && !seg_wide(a, g.s)) return;  // (segments of a batch, each on its own)
   for (int d = tid; d < WAVES * BINS; d += BLOCK) {
     (&s_wave[0][0])[d] = 0u;
     (&s_match[0][0])[d] = 0ull;
   }
-  const SegInfo g = seg_info(a, a.seg_of_tile ? a.seg_of_tile[tile] : 0u);
-  if (a.gate && !seg_wide(a, g.s)) return;  // uniform per block
   const uint32_t kbase = seg_base(a, g.s);
   uint32_t* keys_out = a.keys_out;
   uint32_t* vals_out = a.vals_out;
-  if (a.alt && seg_wide(a, g.s)) {
+  const bool to_gated = a.alt && seg_wide(a, g.s);  // this segment goes on to the gated pass
+  if (to_gated) {
     keys_out = a.keys_alt;
     vals_out = a.vals_alt;
   }
@@ -503,9 +492,12 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
   // Vector loads complete in issue order, so the digit-start scan waits only
   // for the first two and round j of the ranking only for keys[j]: the
   // ranking runs while the rest of the tile arrives.
-  const uint32_t* tot = a.tot + (size_t)g.s * TOT_STRIDE;
-  const uint32_t c_tot = tid < BINS ? tot[tid] : 0u;
-  const uint32_t c_pre = tid < BINS ? a.cnt[(size_t)tile * a.cstride + tid] : 0u;
+  // (the gated pass: its digits' totals and scanned rows from the third pass)
+  const uint32_t* tot = a.gate ? a.tot_g + (size_t)g.s * GATED_BINS : a.tot + (size_t)g.s * TOT_STRIDE;
+  const uint32_t* pre = a.gate ? a.cnt_g + (size_t)tile * GATED_BINS : a.cnt + (size_t)tile * a.cstride;
+  const int rows = a.gate ? GATED_BINS : BINS;
+  const uint32_t c_tot = tid < rows ? tot[tid] : 0u;
+  const uint32_t c_pre = tid < rows ? pre[tid] : 0u;
   uint32_t keys[KPT], vals[KPT], rank[KPT];
   const uint32_t* keys_in = a.keys_in + g.off;
   const uint32_t* vals_in = VALS ? a.vals_in + g.off : nullptr;
@@ -664,6 +656,26 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
       if (keys_out) keys_out[od[r]] = ok[r];
       vals_out[od[r]] = ov[r];
     }
+  }
+  if (to_gated) {
+    // Count each key's gated digit into the row of the tile it lands in; the
+    // segment's last block to finish scans the rows for the gated pass.
+#pragma unroll
+    for (int r = 0; r < KPT; r++) {
+      const uint32_t i = (uint32_t)(r * BLOCK + tid);
+      if (i < tile_kept && od[r] < a.dst_cap) {
+        const uint32_t t = g.tile0 + (od[r] - g.off) / (uint32_t)TILE;
+        const uint32_t d = ((ok[r] & a.key_mask) - kbase) >> RANGED_BITS;
+        atomicAdd(&a.cnt_g[(size_t)t * GATED_BINS + d], 1u);
+      }
+    }
+    __shared__ uint32_t s_last;
+    const uint32_t used = (uint32_t)(((uint64_t)n + TILE - 1) / TILE);  // the blocks that reach here
+    wait_stores();
+    __syncthreads();  // (also: every read of s_stage is done)
+    if (tid == 0) s_last = atomicAdd(&a.done[g.s], 1u) == used - 1u ? 1u : 0u;
+    __syncthreads();
+    if (s_last) gated_scan<BLOCK>(a, g, used, reinterpret_cast<uint32_t*>(s_stage));
   }
 }
 
