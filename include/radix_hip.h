@@ -201,8 +201,8 @@ typedef struct rdx_splat_table {
   const float* quat;       /* 4 planes x, y, z, w (any length; normalised here) */
   const float* opacity;    /* 1 plane, 0..1 (LoD values above 1 allowed) */
   const float* rgb;        /* 3 planes: the base colour (SH DC) */
-  size_t plane_stride;     /* elements per plane (>= count); 0 = count; at most 23,860,929
-                              (45 SH planes within 4 GiB: 32-bit buffer offsets) */
+  size_t plane_stride;     /* elements per plane (>= count); 0 = count; at most 71,582,788
+                              (planes are read in groups of at most 15 within 4 GiB) */
   /* EXT: two uint32x4 arrays; PACKED: one, with its encoding */
   const uint32_t* ext_a;
   const uint32_t* ext_b;

@@ -98,7 +98,7 @@ static void launch_kind(hipStream_t s, uint32_t blocks, const BatchDesc& d, cons
 }
 
 static bool launch_batch_kind(hipStream_t s, const BatchKind& k, const BatchDesc& d, const BatchU& u, uint32_t* keys) {
-  if (k.layout == LAYOUT_F32) {
+  if (k.layout == LAYOUT_F32 && k.shc == SHC_F32) {
     switch (k.deg) {
       case 0: launch_kind<LAYOUT_F32, SHC_F32, 0>(s, k.blocks, d, u, keys); return true;
       case 1: launch_kind<LAYOUT_F32, SHC_F32, 1>(s, k.blocks, d, u, keys); return true;
@@ -106,6 +106,11 @@ static bool launch_batch_kind(hipStream_t s, const BatchKind& k, const BatchDesc
       case 3: launch_kind<LAYOUT_F32, SHC_F32, 3>(s, k.blocks, d, u, keys); return true;
     }
     return false;
+  }
+  if (k.layout == LAYOUT_F32 && k.shc == SHC_F32_WIDE) {
+    if (k.deg == 2) launch_kind<LAYOUT_F32, SHC_F32_WIDE, 2>(s, k.blocks, d, u, keys);
+    else launch_kind<LAYOUT_F32, SHC_F32_WIDE, 3>(s, k.blocks, d, u, keys);
+    return true;
   }
   if (k.layout == LAYOUT_EXT) {
     if (k.shc == SHC_PACKED) launch_kind<LAYOUT_EXT, SHC_PACKED, -1>(s, k.blocks, d, u, keys);
@@ -201,8 +206,10 @@ int rdx_batch_create(rdx_context* ctx, const rdx_splat_table* tables, int num_ta
   for (int v = 0; v < num_views; v++) {
     const rdx_splat_table& t = tables[views[v].table];
     const int layout = t.layout == RDX_LAYOUT_F32_SOA ? LAYOUT_F32 : t.layout == RDX_LAYOUT_EXT ? LAYOUT_EXT : LAYOUT_PACKED;
-    const int shc = layout == LAYOUT_F32 ? SHC_F32 : (t.sh_codec == RDX_SH_PACKED ? SHC_PACKED : SHC_EXT);
     const int deg = layout == LAYOUT_F32 ? b->table_deg[views[v].table] : -1;
+    const uint64_t stride = t.plane_stride ? t.plane_stride : t.count;
+    const int shc = layout == LAYOUT_F32 ? (f32_sh_wide(stride, deg) ? SHC_F32_WIDE : SHC_F32)
+                                         : (t.sh_codec == RDX_SH_PACKED ? SHC_PACKED : SHC_EXT);
     by_kind[std::make_tuple(layout, shc, deg)].push_back(v);
   }
   b->slot_of_view.assign(num_views, -1);

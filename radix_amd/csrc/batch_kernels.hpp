@@ -67,7 +67,7 @@ __global__ __launch_bounds__(256) void k_project_batch(BatchDesc D, BatchU u, ui
   const uint32_t invalid = key_invalid(u.sort_bits);
 
   F32Rsrc rs;
-  if constexpr (LAYOUT == LAYOUT_F32) rs = f32_rsrc(t, deg);
+  if constexpr (LAYOUT == LAYOUT_F32) rs = f32_rsrc<SHC>(t, deg);
 
   const uint32_t i = chunk * BLOCK + threadIdx.x;
   if (i < t.count) {

@@ -184,10 +184,10 @@ int check_table(const rdx_splat_table* t) {
       if (deg > 0 && (t->sh_codec != RDX_SH_F32 || !t->sh))
         return fail(RDX_E_INVALID, "f32 table: SH must be f32 planes");
       if (t->plane_stride && t->plane_stride < t->count) return fail(RDX_E_INVALID, "f32 table: plane stride below count");
-      // planes are read with 32-bit offsets: the SH group (45 planes at degree 3) within 4 GiB
-      if ((uint64_t)(t->plane_stride ? t->plane_stride : t->count) * 4u * 45u > 0xffffffffull)
-        return fail(RDX_E_UNSUPPORTED, "f32 table: %u splats per plane exceed the 4 GiB plane group limit",
-                    (unsigned)(t->plane_stride ? t->plane_stride : t->count));
+      // planes are read with 32-bit offsets within groups of at most 15 planes (4 GiB each)
+      if ((uint64_t)(t->plane_stride ? t->plane_stride : t->count) > F32_MAX_STRIDE)
+        return fail(RDX_E_UNSUPPORTED, "f32 table: %zu splats per plane exceed the limit of %llu",
+                    (size_t)(t->plane_stride ? t->plane_stride : t->count), (unsigned long long)F32_MAX_STRIDE);
       break;
     case RDX_LAYOUT_EXT:
       if (t->count && (!t->ext_a || !t->ext_b)) return fail(RDX_E_INVALID, "extSplats table: missing words");

@@ -24,6 +24,9 @@ namespace rdx {
 
 constexpr int LAYOUT_F32 = 0, LAYOUT_EXT = 1, LAYOUT_PACKED = 2;
 constexpr int SHC_NONE = 0, SHC_F32 = 1, SHC_EXT = 2, SHC_PACKED = 3;
+// f32 SH planes of a table too large for one 4 GiB window over them (read
+// in groups of SH_GROUP coefficients, f32_rsrc)
+constexpr int SHC_F32_WIDE = 4;
 constexpr uint32_t BUFFER_DWORD3 = 0x00020000u;  // raw buffer resource, gfx9 family
 
 struct EyeU {
@@ -282,9 +285,19 @@ struct Splat {
 };
 
 // Buffer resources of an f32 table's plane groups (one 32-bit lane offset
-// i * 4 serves every plane; a plane's start is a scalar offset).
+// i * 4 serves every plane; a plane's start is a scalar offset). A resource
+// spans at most 4 GiB: one resource serves all SH planes of a table up to
+// f32_sh_window splats per plane; larger tables (SHC_F32_WIDE) read them
+// through three resources of SH_GROUP coefficients (15 planes) each, up to
+// F32_MAX_STRIDE splats per plane.
+constexpr int SH_GROUP = 5;
+constexpr uint64_t F32_MAX_STRIDE = 0xffffffffull / (4ull * 3 * SH_GROUP);
+__host__ __device__ constexpr int sh_coefs(int deg) { return deg >= 3 ? 15 : (deg == 2 ? 8 : (deg == 1 ? 3 : 0)); }
+__host__ __device__ constexpr bool f32_sh_wide(uint64_t stride, int deg) {
+  return stride * 4ull * 3ull * (uint64_t)sh_coefs(deg) > 0xffffffffull;
+}
 struct F32Rsrc {
-  __amdgpu_buffer_rsrc_t c, l, q, o, g, h;
+  __amdgpu_buffer_rsrc_t c, l, q, o, g, h[3];
   uint32_t st4;  // bytes per plane
 };
 
@@ -294,7 +307,9 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const float* p, uin
                                            (int)BUFFER_DWORD3);
 }
 
+template <int SHC>
 __device__ __forceinline__ F32Rsrc f32_rsrc(const TableP& t, int deg) {
+  constexpr int SHG = SHC == SHC_F32_WIDE ? SH_GROUP : 15;
   F32Rsrc r;
   r.st4 = t.stride * 4u;
   r.c = plane_rsrc(t.center, 3, r.st4);
@@ -303,7 +318,12 @@ __device__ __forceinline__ F32Rsrc f32_rsrc(const TableP& t, int deg) {
   r.o = plane_rsrc(t.opacity, 1, r.st4);
   r.g = plane_rsrc(t.rgb, 3, r.st4);
   const int K = deg >= 3 ? 15 : (deg == 2 ? 8 : (deg == 1 ? 3 : 0));
-  r.h = plane_rsrc(t.sh ? t.sh : t.rgb, (uint32_t)(3 * K), r.st4);
+  const float* sh = t.sh ? t.sh : t.rgb;
+#pragma unroll
+  for (int g = 0; g < 3; g++) {
+    const int k0 = g * SHG, k1 = min(K, k0 + SHG);
+    r.h[g] = plane_rsrc(sh + (size_t)(3 * k0) * t.stride, k1 > k0 ? (uint32_t)(3 * (k1 - k0)) : 0u, r.st4);
+  }
   return r;
 }
 
@@ -344,8 +364,12 @@ __device__ __forceinline__ void load_splat(const TableP& t, const F32Rsrc& rs, c
     sp.op = ldp(rs.o, v, 0);
     sp.rgb = make_float3(ldp(rs.g, v, 0), ldp(rs.g, v, s), ldp(rs.g, v, 2 * s));
 #pragma unroll
-    for (int k = 0; k < K; k++)
-      coef[k] = make_float3(ldp(rs.h, v, (3 * k) * s), ldp(rs.h, v, (3 * k + 1) * s), ldp(rs.h, v, (3 * k + 2) * s));
+    for (int k = 0; k < K; k++) {
+      constexpr int SHG = SHC == SHC_F32_WIDE ? SH_GROUP : 15;
+      const __amdgpu_buffer_rsrc_t h = rs.h[k / SHG];
+      const uint32_t p = (uint32_t)(3 * (k % SHG));
+      coef[k] = make_float3(ldp(h, v, p * s), ldp(h, v, (p + 1) * s), ldp(h, v, (p + 2) * s));
+    }
     sp.s = make_float3(__expf(l0), __expf(l1), __expf(l2));
   } else {
     // uint32x4 words: issue the SH texels' loads with the splat's
@@ -706,7 +730,7 @@ __global__ __launch_bounds__(256) void k_project(TableP t, ProjU u, OutP o) {
   const bool per_eye = EYES == 2 && u.key_per_eye;
   const int deg = DEG >= 0 ? DEG : u.sh_degree;
   F32Rsrc rs;
-  if constexpr (LAYOUT == LAYOUT_F32) rs = f32_rsrc(t, deg);
+  if constexpr (LAYOUT == LAYOUT_F32) rs = f32_rsrc<SHC>(t, deg);
 
   // One splat per thread, no grid-stride loop: a loop lets the compiler hoist
   // the per-eye uniforms out of it and keep them all live in SGPRs, which
@@ -858,7 +882,7 @@ __global__ __launch_bounds__(256) void k_project_keys(TableP t, ProjU u, uint32_
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= t.count) return;
   F32Rsrc rs;
-  if constexpr (LAYOUT == LAYOUT_F32) rs = f32_rsrc(t, 0);
+  if constexpr (LAYOUT == LAYOUT_F32) rs = f32_rsrc<SHC_NONE>(t, 0);
   Splat sp;
   float3 coef[1];
   float band[3];

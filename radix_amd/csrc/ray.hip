@@ -77,8 +77,18 @@ static void launch_layout(hipStream_t st, dim3 grid, const rdx_splat_table* t, c
     switch (r.u.sh_degree) {
       case 0: hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_F32, SHC_F32, 0, EYES>), grid, dim3(256), 0, st, tp, r, o); break;
       case 1: hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_F32, SHC_F32, 1, EYES>), grid, dim3(256), 0, st, tp, r, o); break;
-      case 2: hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_F32, SHC_F32, 2, EYES>), grid, dim3(256), 0, st, tp, r, o); break;
-      default: hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_F32, SHC_F32, 3, EYES>), grid, dim3(256), 0, st, tp, r, o); break;
+      case 2:
+        if (f32_sh_wide(tp.stride, 2))
+          hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_F32, SHC_F32_WIDE, 2, EYES>), grid, dim3(256), 0, st, tp, r, o);
+        else
+          hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_F32, SHC_F32, 2, EYES>), grid, dim3(256), 0, st, tp, r, o);
+        break;
+      default:
+        if (f32_sh_wide(tp.stride, 3))
+          hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_F32, SHC_F32_WIDE, 3, EYES>), grid, dim3(256), 0, st, tp, r, o);
+        else
+          hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_F32, SHC_F32, 3, EYES>), grid, dim3(256), 0, st, tp, r, o);
+        break;
     }
   } else if (t->layout == RDX_LAYOUT_EXT) {
     if (shc == SHC_PACKED)

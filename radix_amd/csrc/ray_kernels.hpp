@@ -428,7 +428,7 @@ __global__ __launch_bounds__(256) void k_project_ray(TableP t, RayU r, RayOut o)
   if (i >= t.count) return;
   const int deg = DEG >= 0 ? DEG : u.sh_degree;
   F32Rsrc rs;
-  if constexpr (LAYOUT == LAYOUT_F32) rs = f32_rsrc(t, deg);
+  if constexpr (LAYOUT == LAYOUT_F32) rs = f32_rsrc<SHC>(t, deg);
   Splat sp;
   float3 coef[15];
   float band[3];
