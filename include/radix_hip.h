@@ -185,6 +185,8 @@ int rdx_sort_splats_static(void* stream, const uint32_t* keys_dev, uint32_t n, i
 #define RDX_LAYOUT_F32_SOA 0  /* float planes: center, log-scale, quat, opacity, rgb, SH */
 #define RDX_LAYOUT_EXT 1      /* extSplats: extA, extB uint32x4 (src/core/splats.ts:10-38) */
 #define RDX_LAYOUT_PACKED 2   /* packedSplats: packed uint32x4 (src/core/splats.ts:271-309) */
+#define RDX_LAYOUT_COV 3      /* covSplats: covA, covB uint32x4 in ext_a, ext_b (src/core/splats.ts:25-27,
+                                 219-266); the ewa method reads the covariance as stored */
 
 #define RDX_SH_NONE 0
 #define RDX_SH_F32 1     /* float planes 3k+c (k = coefficient past DC, c = channel) */

@@ -205,6 +205,57 @@ void orc_ext_decode(const uint32_t* a, const uint32_t* b, double* center, double
   orc_decode_quat_oct(b[3], 10, 12, quat);
 }
 
+/* --- splats.ts:219-266: the cov words (A as ext, no normal; B.x ext rgb,
+ * B.y half(ln xx) | half(ln yy) << 16, B.z half(ln zz) | half(c_xy) << 16,
+ * B.w half(c_xz) | half(c_yz) << 16, c = sign * ln|correlation|) --- */
+
+/* splats.ts:221-228: the correlation value / sqrt(a b) as sign(c) ln|c|
+ * (Math.max(-1, Math.min(1, NaN)) is NaN, which encodes as 0) */
+double orc_cov_encode_correlation(double value, double a, double b) {
+  double c = value / sqrt(a * b);
+  if (isnan(c)) return 0.0;
+  c = clampd(c, -1.0, 1.0);
+  if (c == 0.0) return 0.0;
+  return (c > 0 ? 1.0 : -1.0) * clampd(log(fabs(c)), -100.0, -1e-7);
+}
+
+/* splats.ts:230-232 */
+double orc_cov_decode_correlation(double e) { return e == 0.0 ? 0.0 : (e > 0 ? -1.0 : 1.0) * exp(-fabs(e)); }
+
+void orc_cov_encode(const double* center, const double* cov6, const double* rgb, double opacity, uint32_t* a,
+                    uint32_t* b) {
+  const double xx = cov6[0], yy = cov6[1], zz = cov6[2], xy = cov6[3], xz = cov6[4], yz = cov6[5];
+  for (int k = 0; k < 3; k++) {
+    float c = (float)center[k];
+    memcpy(&a[k], &c, 4);
+  }
+  a[3] = (uint32_t)clampd(js_round(opacity * 128.0), 0, 255) << 24; /* encoding.ts:140-143; normal none */
+  b[0] = orc_encode_ext_rgb(rgb[0], rgb[1], rgb[2]);
+  b[1] = orc_encode_half(log(xx)) | (orc_encode_half(log(yy)) << 16);
+  b[2] = orc_encode_half(log(zz)) | (orc_encode_half(orc_cov_encode_correlation(xy, xx, yy)) << 16);
+  b[3] = orc_encode_half(orc_cov_encode_correlation(xz, xx, zz)) |
+         (orc_encode_half(orc_cov_encode_correlation(yz, yy, zz)) << 16);
+}
+
+void orc_cov_decode(const uint32_t* a, const uint32_t* b, double* center, double* cov6, double* rgb,
+                    double* opacity) {
+  for (int k = 0; k < 3; k++) {
+    float c;
+    memcpy(&c, &a[k], 4);
+    center[k] = c;
+  }
+  *opacity = (a[3] >> 24) / 128.0;
+  orc_decode_ext_rgb(b[0], rgb);
+  const double xx = exp(orc_decode_half(b[1] & 0xffffu)), yy = exp(orc_decode_half(b[1] >> 16));
+  const double zz = exp(orc_decode_half(b[2] & 0xffffu));
+  cov6[0] = xx;
+  cov6[1] = yy;
+  cov6[2] = zz;
+  cov6[3] = orc_cov_decode_correlation(orc_decode_half(b[2] >> 16)) * sqrt(xx * yy);
+  cov6[4] = orc_cov_decode_correlation(orc_decode_half(b[3] & 0xffffu)) * sqrt(xx * zz);
+  cov6[5] = orc_cov_decode_correlation(orc_decode_half(b[3] >> 16)) * sqrt(yy * zz);
+}
+
 /* --- sh.ts: coefficients past DC per degree, packed band sizes --- */
 static const int SH_COEFFS[5] = {0, 3, 8, 15, 24};
 static const int PACKED_BAND_BYTES[4] = {10, 16, 22, 28};

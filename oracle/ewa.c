@@ -485,6 +485,27 @@ void orc_project_table_ext(uint32_t count, const uint32_t* ext_a, const uint32_t
   }
 }
 
+void orc_project_table_cov(uint32_t count, const uint32_t* cov_a, const uint32_t* cov_b, int sh_codec, int sh_degree,
+                           const uint32_t* const* sh_channels, const orc_view* v, const orc_params* p,
+                           orc_batch_out* out) {
+  for (uint32_t i = 0; i < count; i++) {
+    orc_splat s;
+    orc_cov_decode(&cov_a[4 * (size_t)i], &cov_b[4 * (size_t)i], s.center, s.cov, s.rgb, &s.opacity);
+    if (sh_codec && sh_degree > 0) {
+      uint32_t words[16];
+      int texels = (orc_sh_words(sh_codec, sh_degree) + 3) / 4;
+      for (int t = 0; t < texels; t++)
+        for (int k = 0; k < 4; k++) words[4 * t + k] = sh_channels[t][4 * (size_t)i + k];
+      double coeffs[45];
+      orc_sh_decode(sh_codec, words, sh_degree, coeffs);
+      apply_sh(s.center, coeffs, sh_degree, v, p, s.rgb);
+    }
+    orc_out o;
+    orc_project_ewa(&s, v, p, &o);
+    emit(&o, v, p, count, i, out);
+  }
+}
+
 /* --- The keys-only pass (SplatProjection.writeKeys, SplatProjection.ts:165-172) ---
  * The "keys" program (SplatProjector.ts:152-194) has no wanted form, so a
  * stored-splat generator stays in the splat form: transformSplat

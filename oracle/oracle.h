@@ -51,6 +51,15 @@ void orc_ext_encode(const double* center, const double* scales, const double* qu
 void orc_ext_decode(const uint32_t* a4, const uint32_t* b4, double* center, double* scales, double* quat,
                     double* rgb, double* opacity);
 
+/* splats.ts:219-266: cov words from the object-space covariance xx, yy, zz,
+ * xy, xz, yz (normal none) and back */
+void orc_cov_encode(const double* center, const double* cov6, const double* rgb, double opacity, uint32_t* a4,
+                    uint32_t* b4);
+void orc_cov_decode(const uint32_t* a4, const uint32_t* b4, double* center, double* cov6, double* rgb,
+                    double* opacity);
+double orc_cov_encode_correlation(double value, double a, double b); /* splats.ts:221-228 */
+double orc_cov_decode_correlation(double encoded);                   /* splats.ts:230-232 */
+
 /* ---- src/core/sh.ts ------------------------------------------------------ */
 void orc_sh_basis(const double* dir, double* out24);                       /* sh.ts:376-407 */
 void orc_sh_evaluate(const double* coeffs, const double* dir, int degree, double* rgb); /* sh.ts:362-373 */
@@ -166,6 +175,11 @@ void orc_project_table_packed(uint32_t count, const uint32_t* packed, const doub
                               const uint32_t* const* sh_channels, const orc_view* v, const orc_params* p,
                               orc_batch_out* out);
 void orc_project_table_ext(uint32_t count, const uint32_t* ext_a, const uint32_t* ext_b, int sh_codec, int sh_degree,
+                           const uint32_t* const* sh_channels, const orc_view* v, const orc_params* p,
+                           orc_batch_out* out);
+/* A covSplats table (covA, covB words) with ext or packed SH: the cov form
+ * the ewa method reads (GpuCovSplatReader.loadCovSplat, splatAccess.ts:107-109) */
+void orc_project_table_cov(uint32_t count, const uint32_t* cov_a, const uint32_t* cov_b, int sh_codec, int sh_degree,
                            const uint32_t* const* sh_channels, const orc_view* v, const orc_params* p,
                            orc_batch_out* out);
 

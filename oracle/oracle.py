@@ -107,6 +107,10 @@ def lib() -> C.CDLL:
             "orc_packed_decode": (None, [_u32p, _dp, _dp, _dp, _dp, _dp, _dp]),
             "orc_ext_encode": (None, [_dp, _dp, _dp, _dp, _d, _u32p, _u32p]),
             "orc_ext_decode": (None, [_u32p, _u32p, _dp, _dp, _dp, _dp, _dp]),
+            "orc_cov_encode": (None, [_dp, _dp, _dp, _d, _u32p, _u32p]),
+            "orc_cov_decode": (None, [_u32p, _u32p, _dp, _dp, _dp, _dp]),
+            "orc_cov_encode_correlation": (_d, [_d, _d, _d]),
+            "orc_cov_decode_correlation": (_d, [_d]),
             "orc_sh_basis": (None, [_dp, _dp]),
             "orc_sh_evaluate": (None, [_dp, _dp, C.c_int, _dp]),
             "orc_raster_binning": (None, [C.c_int]),
@@ -127,6 +131,8 @@ def lib() -> C.CDLL:
             "orc_project_table_packed": (None, [C.c_uint32, C.c_void_p, _dp, C.c_int, C.c_int, C.c_void_p,
                                                 C.POINTER(OrcView), C.POINTER(OrcParams), C.POINTER(OrcBatchOut)]),
             "orc_project_table_ext": (None, [C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                             C.POINTER(OrcView), C.POINTER(OrcParams), C.POINTER(OrcBatchOut)]),
+            "orc_project_table_cov": (None, [C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                              C.POINTER(OrcView), C.POINTER(OrcParams), C.POINTER(OrcBatchOut)]),
             "orc_view_from_camera": (None, [_dp, _dp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, _d, _d, _d,
                                             C.POINTER(OrcView)]),
@@ -320,6 +326,21 @@ def project_ext(ext_a, ext_b, sh_channels, sh_codec: int, sh_degree: int, view: 
     cp = (C.c_void_p * 4)(*[c.ctypes.data for c in chans] + [None] * (4 - len(chans)))
     codec = sh_codec if chans and sh_degree > 0 else 0
     return _run_batch(lambda o: lib().orc_project_table_ext(n, a.ctypes.data, b.ctypes.data, codec, sh_degree, cp,
+                                                            C.byref(view), C.byref(prm), C.byref(o)),
+                      n, eyes, per_eye)
+
+
+def project_cov(cov_a, cov_b, sh_channels, sh_codec: int, sh_degree: int, view: OrcView, prm: OrcParams):
+    """A covSplats table (covA, covB words) with ext (2) or packed (3) SH bundles."""
+    n = cov_a.shape[0]
+    eyes = view.eyes
+    per_eye = bool(prm.key_per_eye) and eyes == 2
+    a = np.ascontiguousarray(cov_a, dtype=np.uint32)
+    b = np.ascontiguousarray(cov_b, dtype=np.uint32)
+    chans = [np.ascontiguousarray(c, dtype=np.uint32) for c in (sh_channels or [])]
+    cp = (C.c_void_p * 4)(*[c.ctypes.data for c in chans] + [None] * (4 - len(chans)))
+    codec = sh_codec if chans and sh_degree > 0 else 0
+    return _run_batch(lambda o: lib().orc_project_table_cov(n, a.ctypes.data, b.ctypes.data, codec, sh_degree, cp,
                                                             C.byref(view), C.byref(prm), C.byref(o)),
                       n, eyes, per_eye)
 

@@ -23,6 +23,7 @@ RDX_E_UNSUPPORTED = -5
 LAYOUT_F32_SOA = 0
 LAYOUT_EXT = 1
 LAYOUT_PACKED = 2
+LAYOUT_COV = 3
 SH_NONE = 0
 SH_F32 = 1
 SH_EXT = 2

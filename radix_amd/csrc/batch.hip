@@ -117,6 +117,11 @@ static bool launch_batch_kind(hipStream_t s, const BatchKind& k, const BatchDesc
     else launch_kind<LAYOUT_EXT, SHC_EXT, -1>(s, k.blocks, d, u, keys);
     return true;
   }
+  if (k.layout == LAYOUT_COV) {
+    if (k.shc == SHC_PACKED) launch_kind<LAYOUT_COV, SHC_PACKED, -1>(s, k.blocks, d, u, keys);
+    else launch_kind<LAYOUT_COV, SHC_EXT, -1>(s, k.blocks, d, u, keys);
+    return true;
+  }
   if (k.shc == SHC_PACKED) launch_kind<LAYOUT_PACKED, SHC_PACKED, -1>(s, k.blocks, d, u, keys);
   else launch_kind<LAYOUT_PACKED, SHC_EXT, -1>(s, k.blocks, d, u, keys);
   return true;
@@ -205,7 +210,10 @@ int rdx_batch_create(rdx_context* ctx, const rdx_splat_table* tables, int num_ta
   std::map<std::tuple<int, int, int>, std::vector<int>> by_kind;  // kind -> views
   for (int v = 0; v < num_views; v++) {
     const rdx_splat_table& t = tables[views[v].table];
-    const int layout = t.layout == RDX_LAYOUT_F32_SOA ? LAYOUT_F32 : t.layout == RDX_LAYOUT_EXT ? LAYOUT_EXT : LAYOUT_PACKED;
+    const int layout = t.layout == RDX_LAYOUT_F32_SOA ? LAYOUT_F32
+                       : t.layout == RDX_LAYOUT_EXT   ? LAYOUT_EXT
+                       : t.layout == RDX_LAYOUT_COV   ? LAYOUT_COV
+                                                      : LAYOUT_PACKED;
     const int deg = layout == LAYOUT_F32 ? b->table_deg[views[v].table] : -1;
     const uint64_t stride = t.plane_stride ? t.plane_stride : t.count;
     const int shc = layout == LAYOUT_F32 ? (f32_sh_wide(stride, deg) ? SHC_F32_WIDE : SHC_F32)

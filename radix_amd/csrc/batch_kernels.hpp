@@ -78,7 +78,7 @@ __global__ __launch_bounds__(256) void k_project_batch(BatchDesc D, BatchU u, ui
     float op, std_dev;
     float3 s;
     bool invisible;
-    prepare(sp, u, op, s, std_dev, invisible);
+    prepare<LAYOUT == LAYOUT_COV>(sp, u, op, s, std_dev, invisible);
     float R[9];
     bool have_r = false;
     for (uint32_t v = 0; v < nv; v++) {
@@ -88,14 +88,19 @@ __global__ __launch_bounds__(256) void k_project_batch(BatchDesc D, BatchU u, ui
       EyeCentre C = eye_centre(V.eye, sp.c, u.ndc_z_min, u.clip_xy);
       C.pass = C.pass && !invisible;
       if (C.pass) {
-        if (!have_r) {
-          quat_matrix(sp.q, R);
-          have_r = true;
-        }
         const float3 rgb = sh_colour(sp, V.h, deg, coef, SHC == SHC_PACKED ? band : nullptr);
-        float Q[9];
-        eye_axes(V.eye, R, Q);
-        const EwaResult r = ewa_eye(V.eye, C, true, R, Q, s, op, std_dev, encode_ext_rgb(rgb.x, rgb.y, rgb.z), u);
+        EwaResult r;
+        if constexpr (LAYOUT == LAYOUT_COV) {
+          r = ewa_eye_cov(V.eye, C, true, sp.cv, op, std_dev, encode_ext_rgb(rgb.x, rgb.y, rgb.z), u);
+        } else {
+          if (!have_r) {
+            quat_matrix(sp.q, R);
+            have_r = true;
+          }
+          float Q[9];
+          eye_axes(V.eye, R, Q);
+          r = ewa_eye(V.eye, C, true, R, Q, s, op, std_dev, encode_ext_rgb(rgb.x, rgb.y, rgb.z), u);
+        }
         if (r.ok) {
           words = r.words;
           visible = true;

@@ -192,6 +192,9 @@ int check_table(const rdx_splat_table* t) {
     case RDX_LAYOUT_EXT:
       if (t->count && (!t->ext_a || !t->ext_b)) return fail(RDX_E_INVALID, "extSplats table: missing words");
       break;
+    case RDX_LAYOUT_COV:
+      if (t->count && (!t->ext_a || !t->ext_b)) return fail(RDX_E_INVALID, "covSplats table: missing words");
+      break;
     case RDX_LAYOUT_PACKED:
       if (t->count && !t->packed) return fail(RDX_E_INVALID, "packedSplats table: missing words");
       if (!(t->packed_encoding[1] != t->packed_encoding[0]) || !(t->packed_encoding[3] != t->packed_encoding[2]))
@@ -221,6 +224,7 @@ int launch_project(hipStream_t s, const rdx_splat_table* t, const ProjU& u, cons
   switch (t->layout) {
     case RDX_LAYOUT_F32_SOA: launch_project_f32(s, dim3(grid), tp, u, o, u.eyes, side); break;
     case RDX_LAYOUT_EXT: launch_project_ext(s, dim3(grid), shc, tp, u, o, u.eyes, side); break;
+    case RDX_LAYOUT_COV: launch_project_cov(s, dim3(grid), shc, tp, u, o, u.eyes, side); break;
     case RDX_LAYOUT_PACKED: launch_project_packed(s, dim3(grid), shc, tp, u, o, u.eyes, side); break;
   }
   hipError_t e = hipGetLastError();
@@ -306,6 +310,7 @@ int rdx_project_keys(rdx_context* ctx, void* stream, const rdx_splat_table* tabl
   switch (table->layout) {
     case RDX_LAYOUT_F32_SOA: hipLaunchKernelGGL(k_project_keys<LAYOUT_F32>, grid, dim3(256), 0, s, tp, u, keys, at); break;
     case RDX_LAYOUT_EXT: hipLaunchKernelGGL(k_project_keys<LAYOUT_EXT>, grid, dim3(256), 0, s, tp, u, keys, at); break;
+    case RDX_LAYOUT_COV: hipLaunchKernelGGL(k_project_keys<LAYOUT_COV>, grid, dim3(256), 0, s, tp, u, keys, at); break;
     case RDX_LAYOUT_PACKED: hipLaunchKernelGGL(k_project_keys<LAYOUT_PACKED>, grid, dim3(256), 0, s, tp, u, keys, at); break;
   }
   hipError_t e = hipGetLastError();

@@ -28,5 +28,7 @@ void launch_project_ext(hipStream_t s, dim3 grid, int shc, const TableP& t, cons
                         bool side);
 void launch_project_packed(hipStream_t s, dim3 grid, int shc, const TableP& t, const ProjU& u, const OutP& o, int eyes,
                            bool side);
+void launch_project_cov(hipStream_t s, dim3 grid, int shc, const TableP& t, const ProjU& u, const OutP& o, int eyes,
+                        bool side);
 
 }  // namespace rdx

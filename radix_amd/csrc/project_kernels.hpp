@@ -22,7 +22,7 @@
 
 namespace rdx {
 
-constexpr int LAYOUT_F32 = 0, LAYOUT_EXT = 1, LAYOUT_PACKED = 2;
+constexpr int LAYOUT_F32 = 0, LAYOUT_EXT = 1, LAYOUT_PACKED = 2, LAYOUT_COV = 3;
 constexpr int SHC_NONE = 0, SHC_F32 = 1, SHC_EXT = 2, SHC_PACKED = 3;
 // f32 SH planes of a table too large for one 4 GiB window over them (read
 // in groups of SH_GROUP coefficients, f32_rsrc)
@@ -282,7 +282,15 @@ struct Splat {
   float4 q;      // quaternion (any length)
   float3 rgb;    // base colour
   float op;
+  // covSplats: the object-space covariance xx, yy, zz, xy, xz, yz (the cov
+  // form), decoded in double: the 2D covariance's determinant of a thin
+  // splat cancels, and float entries would leave it off by the 2D
+  // condition number times 2^-24
+  double cv[6];
 };
+
+// splats.ts:230-232: a correlation stored as sign(c) ln|c|
+__device__ __forceinline__ double decode_correlation(double e) { return e == 0.0 ? 0.0 : (e > 0.0 ? -exp(-e) : exp(e)); }
 
 // Buffer resources of an f32 table's plane groups (one 32-bit lane offset
 // i * 4 serves every plane; a plane's start is a scalar offset). A resource
@@ -392,6 +400,22 @@ __device__ __forceinline__ void load_splat(const TableP& t, const F32Rsrc& rs, c
       sp.rgb = decode_ext_rgb(B.x);
       sp.s = make_float3(__expf(half_to_float(B.y)), __expf(half_to_float(B.y >> 16)), __expf(half_to_float(B.z)));
       sp.q = quat_oct_decode(B.w, 10, 12);
+    } else if constexpr (LAYOUT == LAYOUT_COV) {
+      // covSplats (splats.tsl.ts:105-121): A as ext, B the covariance
+      const uint4 A = ld_stream(t.ext_a + i), B = ld_stream(t.ext_b + i);
+      sp.c = make_float3(bitsf(A.x), bitsf(A.y), bitsf(A.z));
+      sp.op = (float)(A.w >> 24) * (1.0f / 128.0f);
+      sp.rgb = decode_ext_rgb(B.x);
+      const double xx = exp((double)half_to_float(B.y)), yy = exp((double)half_to_float(B.y >> 16)),
+                   zz = exp((double)half_to_float(B.z));
+      sp.cv[0] = xx;
+      sp.cv[1] = yy;
+      sp.cv[2] = zz;
+      sp.cv[3] = decode_correlation(half_to_float(B.z >> 16)) * sqrt(xx * yy);
+      sp.cv[4] = decode_correlation(half_to_float(B.w)) * sqrt(xx * zz);
+      sp.cv[5] = decode_correlation(half_to_float(B.w >> 16)) * sqrt(yy * zz);
+      sp.s = make_float3(0.0f, 0.0f, 0.0f);
+      sp.q = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
     } else {
       const uint4 W = ld_stream(t.packed + i);
       const float lo = enc[0], range = enc[1] - enc[0];
@@ -523,19 +547,18 @@ __device__ __forceinline__ void eye_axes(const EyeU& E, const float* R, float Q[
       Q[3 * i + j] = fmaf(E.Lf[3 * i], R[j], fmaf(E.Lf[3 * i + 1], R[3 + j], E.Lf[3 * i + 2] * R[6 + j]));
 }
 
-// The ewa method for one eye (projections.tsl.ts:95-148 with boundFromCov,
-// projection.tsl.ts:215-240, and the word codecs): `pass` is the eye's
-// centre culls, R the rotation, Q = eye_axes(E, R), s the (inflated)
-// scales, rgb_word the colour's word (encode_ext_rgb, the same for every eye
-// of one SH origin). U supplies the params (min_alpha, blur, ...).
-template <typename U>
-__device__ __forceinline__ EwaResult ewa_eye(const EyeU& E, const EyeCentre& C, bool pass, const float* R,
-                                             const float* Q, float3 s, float op, float std_dev, uint32_t rgb_word,
-                                             const U& u) {
-  EwaResult r;
+// An eye's 2D covariance before the blur (JΣJ^T, projections.tsl.ts:111-118)
+// and its determinant.
+struct Cov2 {
+  float a, b, d, det;
+};
+
+// The splat form: J rows (projections.tsl.ts:111-117), T = (J Q) diag(s),
+// cov2D = T T^T, and the determinant by Cauchy-Binet (no cancellation): R is
+// the rotation, Q = eye_axes(E, R), s the (inflated) scales.
+__device__ __forceinline__ Cov2 cov2d_splat(const EyeU& E, const EyeCentre& C, const float* R, const float* Q,
+                                            float3 s) {
   const float x = (float)C.ce[0], y = (float)C.ce[1], z = (float)C.ce[2];
-  r.z = z;
-  // J rows (projections.tsl.ts:111-117)
   float j00, j02, j11, j12, kdet, l0, l1, l2;
   if (E.ortho) {
     j00 = E.fx;
@@ -557,16 +580,75 @@ __device__ __forceinline__ EwaResult ewa_eye(const EyeU& E, const EyeCentre& C, 
     l1 = R[1] * C.d.x + R[4] * C.d.y + R[7] * C.d.z;
     l2 = R[2] * C.d.x + R[5] * C.d.y + R[8] * C.d.z;
   }
-  // T rows = (J Q) diag(s); cov2D = T T^T
   const float t00 = (j00 * Q[0] + j02 * Q[6]) * s.x, t01 = (j00 * Q[1] + j02 * Q[7]) * s.y,
               t02 = (j00 * Q[2] + j02 * Q[8]) * s.z;
   const float t10 = (j11 * Q[3] + j12 * Q[6]) * s.x, t11 = (j11 * Q[4] + j12 * Q[7]) * s.y,
               t12 = (j11 * Q[5] + j12 * Q[8]) * s.z;
-  const float a0 = t00 * t00 + t01 * t01 + t02 * t02;
-  const float d0 = t10 * t10 + t11 * t11 + t12 * t12;
-  const float b0 = t00 * t10 + t01 * t11 + t02 * t12;
+  Cov2 c;
+  c.a = t00 * t00 + t01 * t01 + t02 * t02;
+  c.d = t10 * t10 + t11 * t11 + t12 * t12;
+  c.b = t00 * t10 + t01 * t11 + t02 * t12;
   const float sxx = s.x * s.x, syy = s.y * s.y, szz = s.z * s.z;
-  const float det0 = kdet * kdet * E.det_l2 * (syy * szz * l0 * l0 + sxx * szz * l1 * l1 + sxx * syy * l2 * l2);
+  c.det = kdet * kdet * E.det_l2 * (syy * szz * l0 * l0 + sxx * szz * l1 * l1 + sxx * syy * l2 * l2);
+  return c;
+}
+
+// The cov form (covSplats): K = J L (L: object to eye), cov2D = K Σ K^T for
+// the object-space covariance cv (xx, yy, zz, xy, xz, yz, already
+// inflated) and its determinant a d - b^2, in double (the determinant of a
+// thin splat's 2D covariance cancels).
+__device__ __forceinline__ Cov2 cov2d_cov(const EyeU& E, const EyeCentre& C, const double* cv) {
+  const float x = (float)C.ce[0], y = (float)C.ce[1], z = (float)C.ce[2];
+  float j00, j02, j11, j12;
+  if (E.ortho) {
+    j00 = E.fx;
+    j02 = 0.0f;
+    j11 = E.fy;
+    j12 = 0.0f;
+  } else {
+    const float iz = __builtin_amdgcn_rcpf(z);
+    j00 = E.fx * iz;
+    j11 = E.fy * iz;
+    j02 = -j00 * x * iz;
+    j12 = -j11 * y * iz;
+  }
+  const float* L = E.Lf;
+  double k0[3], k1[3];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    k0[j] = (double)j00 * L[j] + (double)j02 * L[6 + j];
+    k1[j] = (double)j11 * L[3 + j] + (double)j12 * L[6 + j];
+  }
+  // Σ k0 and Σ k1
+  const double u0 = cv[0] * k0[0] + cv[3] * k0[1] + cv[4] * k0[2];
+  const double u1 = cv[3] * k0[0] + cv[1] * k0[1] + cv[5] * k0[2];
+  const double u2 = cv[4] * k0[0] + cv[5] * k0[1] + cv[2] * k0[2];
+  const double w0 = cv[0] * k1[0] + cv[3] * k1[1] + cv[4] * k1[2];
+  const double w1 = cv[3] * k1[0] + cv[1] * k1[1] + cv[5] * k1[2];
+  const double w2 = cv[4] * k1[0] + cv[5] * k1[1] + cv[2] * k1[2];
+  const double a = k0[0] * u0 + k0[1] * u1 + k0[2] * u2;
+  const double b = k1[0] * u0 + k1[1] * u1 + k1[2] * u2;
+  const double d = k1[0] * w0 + k1[1] * w1 + k1[2] * w2;
+  Cov2 c;
+  c.a = (float)a;
+  c.b = (float)b;
+  c.d = (float)d;
+  c.det = (float)(a * d - b * b);
+  return c;
+}
+
+// The ewa method for one eye after its 2D covariance (projections.tsl.ts:118-148
+// with boundFromCov, projection.tsl.ts:215-240, and the word codecs): `pass`
+// is the eye's centre culls, rgb_word the colour's word (encode_ext_rgb, the
+// same for every eye of one SH origin). U supplies the params (min_alpha,
+// blur, ...).
+template <typename U>
+__device__ __forceinline__ EwaResult ewa_words(const EyeU& E, const EyeCentre& C, bool pass, Cov2 c0, float op,
+                                               float std_dev, uint32_t rgb_word, const U& u) {
+  EwaResult r;
+  const float z = (float)C.ce[2];
+  r.z = z;
+  const float a0 = c0.a, b0 = c0.b, d0 = c0.d, det0 = c0.det;
   // blur and opacity compensation (projections.tsl.ts:119-129), depth of field (projection.tsl.ts:118-126)
   float blur = u.blur;
   if (u.focal_distance > 0.0f && u.aperture_tan > 0.0f && z < 0.0f) {
@@ -630,9 +712,26 @@ __device__ __forceinline__ EwaResult ewa_eye(const EyeU& E, const EyeCentre& C, 
   return r;
 }
 
-// prepareSplat (projection.tsl.ts:42-64) and cullInvisible (66-69)
+// The ewa method for one eye of a splat (scales s, rotation R, Q = eye_axes(E, R))
 template <typename U>
-__device__ __forceinline__ void prepare(const Splat& sp, const U& u, float& op, float3& s, float& std_dev,
+__device__ __forceinline__ EwaResult ewa_eye(const EyeU& E, const EyeCentre& C, bool pass, const float* R,
+                                             const float* Q, float3 s, float op, float std_dev, uint32_t rgb_word,
+                                             const U& u) {
+  return ewa_words(E, C, pass, cov2d_splat(E, C, R, Q, s), op, std_dev, rgb_word, u);
+}
+
+// The ewa method for one eye of a covariance (the cov form, cv inflated)
+template <typename U>
+__device__ __forceinline__ EwaResult ewa_eye_cov(const EyeU& E, const EyeCentre& C, bool pass, const double* cv, float op,
+                                                 float std_dev, uint32_t rgb_word, const U& u) {
+  return ewa_words(E, C, pass, cov2d_cov(E, C, cv), op, std_dev, rgb_word, u);
+}
+
+// prepareSplat (projection.tsl.ts:42-64) and cullInvisible (66-69). COVF:
+// the cov form (sp.cv grows; s is the inflation), else the splat form (s the
+// inflated scales).
+template <bool COVF, typename U>
+__device__ __forceinline__ void prepare(Splat& sp, const U& u, float& op, float3& s, float& std_dev,
                                         bool& invisible) {
   op = sp.op;
   float inflate = 1.0f;
@@ -643,9 +742,18 @@ __device__ __forceinline__ void prepare(const Splat& sp, const U& u, float& op, 
       op = 1.0f;
     }
   }
-  s = make_float3(sp.s.x * inflate, sp.s.y * inflate, sp.s.z * inflate);
   std_dev = op > 1.0f ? u.max_std_dev + (op - 1.0f) * 0.7f : u.max_std_dev;
-  invisible = op <= 0.0f || op < u.min_alpha || (s.x == 0.0f && s.y == 0.0f && s.z == 0.0f);
+  if constexpr (COVF) {
+    // the cov form grows by inflate^2 (projection.tsl.ts:54-56); s carries the factor
+    const double i2 = (double)inflate * inflate;
+#pragma unroll
+    for (int k = 0; k < 6; k++) sp.cv[k] *= i2;
+    s = make_float3(inflate, inflate, inflate);
+    invisible = op <= 0.0f || op < u.min_alpha || (sp.cv[0] == 0.0 && sp.cv[1] == 0.0 && sp.cv[2] == 0.0);
+  } else {
+    s = make_float3(sp.s.x * inflate, sp.s.y * inflate, sp.s.z * inflate);
+    invisible = op <= 0.0f || op < u.min_alpha || (s.x == 0.0f && s.y == 0.0f && s.z == 0.0f);
+  }
 }
 
 // inSortGuard (projection.tsl.ts:144-155): a CPU-sorted splat keeps its key
@@ -664,9 +772,30 @@ __device__ __forceinline__ bool in_sort_guard(float radius, const double* hc, co
   return hz0 < radius && (inside || near);
 }
 
+// The head-space covariance H Σ H^T of an object-space covariance cv (xx,
+// yy, zz, xy, xz, yz), in double: A as xx, yy, zz, xy, xz, yz.
+__device__ __forceinline__ void head_cov(const double* cv, const ProjU& u, double* A) {
+  const double S[9] = {cv[0], cv[3], cv[4], cv[3], cv[1], cv[5], cv[4], cv[5], cv[2]};
+  double HS[9];
+  for (int r = 0; r < 3; r++)
+    for (int k = 0; k < 3; k++) HS[3 * r + k] = u.H[3 * r] * S[k] + u.H[3 * r + 1] * S[3 + k] + u.H[3 * r + 2] * S[6 + k];
+  const int ij[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+  for (int e = 0; e < 6; e++) {
+    const int a = ij[e][0], b = ij[e][1];
+    A[e] = HS[3 * a] * u.H[3 * b] + HS[3 * a + 1] * u.H[3 * b + 1] + HS[3 * a + 2] * u.H[3 * b + 2];
+  }
+}
+
 // The covariance form's extent (the ewa method's input): the square root of
-// the largest diagonal entry of the head-space covariance (H R S)(H R S)^T.
+// the largest diagonal entry of the head-space covariance (H R S)(H R S)^T,
+// or H Σ H^T for a covSplats splat (COVF: sp.cv, inflated).
+template <bool COVF = false>
 __device__ __forceinline__ float cov_extent(const Splat& sp, float3 s, const ProjU& u) {
+  if constexpr (COVF) {
+    double A[6];
+    head_cov(sp.cv, u, A);
+    return (float)sqrt(fmax(fmax(fmax(A[0], A[1]), A[2]), 0.0));
+  }
   float Rg[9];
   quat_matrix(sp.q, Rg);
   float diag[3];
@@ -689,7 +818,17 @@ __device__ __forceinline__ float cov_extent(const Splat& sp, float3 s, const Pro
 // whose scales are the square roots of the eigenvalues of M M^T, M = H R S:
 // the largest by the trigonometric solution of the characteristic cubic
 // (Smith, "Eigenvalues of a symmetric 3x3 matrix", CACM 1961), in double.
+__device__ __forceinline__ double sym3_max_eigen(const double* A);
+
+template <bool COVF = false>
 __device__ __forceinline__ float splat_extent(const Splat& sp, float3 s, const ProjU& u) {
+  if constexpr (COVF) {
+    // a covSplats splat (sp.cv, inflated) in the splat form is fromCov of
+    // its head-space covariance: the largest scale is sqrt(lambda_max)
+    double A[6];
+    head_cov(sp.cv, u, A);
+    return (float)sqrt(fmax(sym3_max_eigen(A), 0.0));
+  }
   if (u.head_uniform) return fmaxf(fmaxf(s.x, s.y), s.z) * u.head_scale;
   float Rg[9];
   quat_matrix(sp.q, Rg);
@@ -705,6 +844,12 @@ __device__ __forceinline__ float splat_extent(const Splat& sp, float3 s, const P
     const int a = ij[e][0], b = ij[e][1];
     A[e] = M[3 * a] * M[3 * b] + M[3 * a + 1] * M[3 * b + 1] + M[3 * a + 2] * M[3 * b + 2];
   }
+  return (float)sqrt(fmax(sym3_max_eigen(A), 0.0));
+}
+
+// The largest eigenvalue of a symmetric 3x3 (xx, yy, zz, xy, xz, yz) by the
+// trigonometric solution of its characteristic cubic (Smith, CACM 1961).
+__device__ __forceinline__ double sym3_max_eigen(const double* A) {
   const double p1 = A[3] * A[3] + A[4] * A[4] + A[5] * A[5];
   const double q = (A[0] + A[1] + A[2]) / 3.0;
   const double d0 = A[0] - q, d1 = A[1] - q, d2 = A[2] - q;
@@ -717,7 +862,7 @@ __device__ __forceinline__ float splat_extent(const Splat& sp, float3 s, const P
     const double r = fmin(fmax(det * 0.5, -1.0), 1.0);
     lmax = q + 2.0 * p * cos(acos(r) / 3.0);
   }
-  return (float)sqrt(fmax(lmax, 0.0));
+  return lmax;
 }
 
 // The projection kernel. DEG: the f32 table's SH degree (compile time), or
@@ -744,7 +889,7 @@ __global__ __launch_bounds__(256) void k_project(TableP t, ProjU u, OutP o) {
     float op, std_dev;
     float3 s;
     bool invisible;
-    prepare(sp, u, op, s, std_dev, invisible);
+    prepare<LAYOUT == LAYOUT_COV>(sp, u, op, s, std_dev, invisible);
 
     EyeCentre C[EYES];
     bool any = false;
@@ -776,23 +921,28 @@ __global__ __launch_bounds__(256) void k_project(TableP t, ProjU u, OutP o) {
     float3 rgb = sp.rgb;
     float R[9];
     if (any || SIDE) {
-      quat_matrix(sp.q, R);
+      if constexpr (LAYOUT != LAYOUT_COV) quat_matrix(sp.q, R);
       rgb = sh_colour(sp, u.h, deg, coef, SHC == SHC_PACKED ? band : nullptr);
       const uint32_t rgb_word = encode_ext_rgb(rgb.x, rgb.y, rgb.z);
       // the eyes of a rig usually share their rotation (shared_axes): then
       // one Q serves both
       float Q0[9];
-      eye_axes(u.eye[0], R, Q0);
+      if constexpr (LAYOUT != LAYOUT_COV) eye_axes(u.eye[0], R, Q0);
 #pragma unroll
       for (int e = 0; e < EYES; e++) {
         if (!C[e].pass && !SIDE) continue;
-        float Qe[9];
-        const float* Q = Q0;
-        if (e > 0 && !u.shared_axes) {
-          eye_axes(u.eye[e], R, Qe);
-          Q = Qe;
+        EwaResult r;
+        if constexpr (LAYOUT == LAYOUT_COV) {
+          r = ewa_eye_cov(u.eye[e], C[e], C[e].pass, sp.cv, op, std_dev, rgb_word, u);
+        } else {
+          float Qe[9];
+          const float* Q = Q0;
+          if (e > 0 && !u.shared_axes) {
+            eye_axes(u.eye[e], R, Qe);
+            Q = Qe;
+          }
+          r = ewa_eye(u.eye[e], C[e], C[e].pass, R, Q, s, op, std_dev, rgb_word, u);
         }
-        const EwaResult r = ewa_eye(u.eye[e], C[e], C[e].pass, R, Q, s, op, std_dev, rgb_word, u);
         if (r.ok) {
           words[e] = r.words;
           vis[e] = true;
@@ -829,7 +979,7 @@ __global__ __launch_bounds__(256) void k_project(TableP t, ProjU u, OutP o) {
     }
 
     // sortGuard for the CPU sort's keys (projection.tsl.ts:144-155)
-    const bool guarded = u.cull_keys || !(u.sort_guard > 0.0f) || in_sort_guard(cov_extent(sp, s, u) * u.max_std_dev, hc, u);
+    const bool guarded = u.cull_keys || !(u.sort_guard > 0.0f) || in_sort_guard(cov_extent<LAYOUT == LAYOUT_COV>(sp, s, u) * u.max_std_dev, hc, u);
 
     // Keys (SplatProjection.ts:143-151)
     const bool keyed = u.cull_keys ? visible : (!invisible && guarded);
@@ -890,7 +1040,7 @@ __global__ __launch_bounds__(256) void k_project_keys(TableP t, ProjU u, uint32_
   float op, std_dev;
   float3 s;
   bool invisible;
-  prepare(sp, u, op, s, std_dev, invisible);
+  prepare<LAYOUT == LAYOUT_COV>(sp, u, op, s, std_dev, invisible);
   double hc[3], head_dist;
   if (u.head_is_eye0) {
     const EyeCentre C = eye_centre(u.eye[0], sp.c, u.ndc_z_min, u.clip_xy);
@@ -901,7 +1051,7 @@ __global__ __launch_bounds__(256) void k_project_keys(TableP t, ProjU u, uint32_
     for (int r = 0; r < 3; r++) hc[r] = fma(u.H[3 * r], dx, fma(u.H[3 * r + 1], dy, u.H[3 * r + 2] * dz));
     head_dist = sqrt(fma(hc[0], hc[0], fma(hc[1], hc[1], hc[2] * hc[2])));
   }
-  const bool guarded = !(u.sort_guard > 0.0f) || in_sort_guard(splat_extent(sp, s, u) * u.max_std_dev, hc, u);
+  const bool guarded = !(u.sort_guard > 0.0f) || in_sort_guard(splat_extent<LAYOUT == LAYOUT_COV>(sp, s, u) * u.max_std_dev, hc, u);
   keys[at + i] = !invisible && guarded ? sort_key(fbits((float)head_dist), u.sort_bits) : key_invalid(u.sort_bits);
 }
 

@@ -116,6 +116,8 @@ int rdx_project_ray(rdx_context* ctx, void* stream, const rdx_splat_table* table
     return fail(method == RDX_METHOD_EWA ? RDX_E_UNSUPPORTED : RDX_E_INVALID,
                 "rdx_project_ray: method %d is not 3dgut (1), htgs (2) or raygs (3) (ewa: rdx_project_ewa)", method);
   if (int rc = check_table(table)) return rc;
+  if (table->layout == RDX_LAYOUT_COV)
+    return fail(RDX_E_UNSUPPORTED, "rdx_project_ray: covSplats tables are read by the ewa method only");
   if (int rc = check_bits(params->sort_bits, "SplatProjection")) return rc;
   RayU r;
   memset(&r, 0, sizeof r);

@@ -436,7 +436,7 @@ __global__ __launch_bounds__(256) void k_project_ray(TableP t, RayU r, RayOut o)
   float op, std_dev;
   float3 s;
   bool invisible;
-  prepare(sp, u, op, s, std_dev, invisible);
+  prepare<false>(sp, u, op, s, std_dev, invisible);
   const float3 rgb = sh_colour(sp, u.h, deg, coef, SHC == SHC_PACKED ? band : nullptr);
 
   // head-space centre (double) and its distance, as the ewa kernel takes them

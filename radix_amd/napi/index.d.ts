@@ -69,12 +69,14 @@ export class SplatSorter {
 }
 
 export interface SplatTable {
-  layout: "f32" | "extSplats" | "packedSplats";
+  layout: "f32" | "extSplats" | "packedSplats" | "covSplats";
   count: number;
   planeStride?: number;
   planes?: DeviceBuffer;
   extA?: DeviceBuffer;
   extB?: DeviceBuffer;
+  covA?: DeviceBuffer;
+  covB?: DeviceBuffer;
   packed?: DeviceBuffer;
   rgbMin?: number;
   rgbMax?: number;

@@ -168,14 +168,14 @@ class SplatSorter {
 }
 SplatSorter.QUADS_PER_INSTANCE = 1024;
 
-const LAYOUTS = { f32: 0, extSplats: 1, packedSplats: 2 };
+const LAYOUTS = { f32: 0, extSplats: 1, packedSplats: 2, covSplats: 3 };
 const SH_CODECS = { none: 0, f32: 1, ext: 2, packed: 3 };
 
 function tableDesc(table) {
   const d = Object.assign({}, table);
   d.layout = typeof table.layout === 'string' ? LAYOUTS[table.layout] : table.layout || 0;
   d.shCodec = typeof table.shCodec === 'string' ? SH_CODECS[table.shCodec] : table.shCodec || 0;
-  ['planes', 'extA', 'extB', 'packed'].forEach((k) => {
+  ['planes', 'extA', 'extB', 'covA', 'covB', 'packed'].forEach((k) => {
     if (d[k]) d[k] = handleOf(d[k]);
   });
   if (d.shChannels) d.shChannels = d.shChannels.map(handleOf);

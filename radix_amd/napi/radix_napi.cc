@@ -624,6 +624,9 @@ void read_table(Call& c, napi_value t, rdx_splat_table* out) {
     out->ext_b = (const uint32_t*)c.buf(prop(env, t, "extB"), "table.extB", 16 * n, n == 0);
   } else if (out->layout == RDX_LAYOUT_PACKED) {
     out->packed = (const uint32_t*)c.buf(prop(env, t, "packed"), "table.packed", 16 * n, n == 0);
+  } else if (out->layout == RDX_LAYOUT_COV) {
+    out->ext_a = (const uint32_t*)c.buf(prop(env, t, "covA"), "table.covA", 16 * n, n == 0);
+    out->ext_b = (const uint32_t*)c.buf(prop(env, t, "covB"), "table.covB", 16 * n, n == 0);
   }
   out->packed_encoding[0] = (float)prop_num(env, t, "rgbMin", 0.0);
   out->packed_encoding[1] = (float)prop_num(env, t, "rgbMax", 1.0);

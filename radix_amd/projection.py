@@ -106,6 +106,15 @@ class DeviceTable:
             codec = sh_codec
         return cls(N.LAYOUT_EXT, count, tensors, sh_codec=codec, sh_degree=sh_degree, object_to_world=object_to_world)
 
+    @classmethod
+    def from_cov(cls, cov_a, cov_b, count: int, sh_channels=None, sh_codec: int = N.SH_EXT, sh_degree: int = 0,
+                 device="cuda", object_to_world=None):
+        """covSplats words (covA, covB uint32x4 per splat, src/core/splats.ts:25-27) with ext or packed SH."""
+        t = cls.from_ext(cov_a, cov_b, count, sh_channels, sh_codec=sh_codec, sh_degree=sh_degree, device=device,
+                         object_to_world=object_to_world)
+        t.layout = N.LAYOUT_COV
+        return t
+
     def desc(self) -> N.SplatTable:
         d = N.SplatTable()
         d.layout = self.layout
@@ -121,7 +130,7 @@ class DeviceTable:
             d.opacity = base + 10 * st
             d.rgb = base + 11 * st
             d.sh = base + 14 * st if self.sh_degree > 0 else None
-        elif self.layout == N.LAYOUT_EXT:
+        elif self.layout in (N.LAYOUT_EXT, N.LAYOUT_COV):  # covA, covB travel as ext_a, ext_b
             d.ext_a = t["ext_a"].data_ptr()
             d.ext_b = t["ext_b"].data_ptr()
         else:

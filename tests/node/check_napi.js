@@ -496,6 +496,26 @@ async function gpuFixture(dir) {
     [words, order, count, rgba].forEach((b) => b.dispose());
     return stats.drawn === spec.count ? undefined : `drawn ${stats.drawn} of ${spec.count}`;
   });
+  await check('SplatProjector: projectSort of the fixture covSplats table (packedSh bundles)', () => {
+    const spec = JSON.parse(fs.readFileSync(path.join(dir, 'cov.json'), 'utf8'));
+    const n = spec.n;
+    const covA = DeviceBuffer.fromArray(readU32(path.join(dir, 'covA.u32')));
+    const covB = DeviceBuffer.fromArray(readU32(path.join(dir, 'covB.u32')));
+    const shChannels = [0, 1, 2].map((t) => DeviceBuffer.fromArray(readU32(path.join(dir, `sh${t}.u32`))));
+    const table = { layout: 'covSplats', count: n, covA, covB, shCodec: 'packed', shDegree: 3, shChannels };
+    const words = [new DeviceBuffer(16 * n)];
+    const keys = new DeviceBuffer(4 * n);
+    const indices = new DeviceBuffer(4 * n);
+    const count = new DeviceBuffer(4);
+    const indirect = new DeviceBuffer(20);
+    new SplatProjector().projectSort(table, spec.camera, { sortBits: 32 }, { words, keys, indices, count, indirect });
+    const visible = count.read(0, 1, Uint32Array)[0];
+    fs.writeFileSync(path.join(dir, 'cov_keys.u32'), Buffer.from(keys.read(0, n, Uint32Array).buffer));
+    fs.writeFileSync(path.join(dir, 'cov_indices.u32'), Buffer.from(indices.read(0, visible, Uint32Array).buffer));
+    fs.writeFileSync(path.join(dir, 'cov_words.u32'), Buffer.from(words[0].read(0, 4 * n, Uint32Array).buffer));
+    [covA, covB, ...shChannels, ...words, keys, indices, count, indirect].forEach((b) => b.dispose());
+    return visible > 0 ? undefined : 'nothing visible';
+  });
 }
 
 (async () => {

@@ -144,6 +144,21 @@ def test_batch_mixed_layouts_and_many_views_of_one_table(cuda):
     check_views(b, tables, prm)
 
 
+def test_batch_with_cov_and_ext_tables(cuda):
+    # covSplats (the cov form) and extSplats tables in one batch, ext and packed SH bundles
+    from tests.test_project_gpu import cov_table, ext_table
+
+    ca, cb, cch = cov_table(6000, 44, 3, lod=True)
+    ea, eb, ech = ext_table(5000, 45, 2)
+    tables = [DeviceTable.from_cov(ca, cb, 6000, cch, sh_codec=3, sh_degree=3),
+              DeviceTable.from_ext(ea, eb, 5000, ech, sh_codec=2, sh_degree=3)]
+    views = [(0, W.orbit_camera(60 + v, 6.5, 320, 240)) for v in range(3)] + [(1, W.orbit_camera(70, 7.0, 400, 300))]
+    prm = Params(sort_bits=32)
+    b = Batch(tables, views, prm)
+    b.run(graph=False)
+    check_views(b, tables, prm)
+
+
 def test_batch_tiny_and_empty_segments(cuda):
     spec = W.plain_small_spec(seed=44)
     spec.count = 1

@@ -134,6 +134,53 @@ def test_ext_keys_pass(cuda, sh_codec):
         against_oracle(k, keys_oracle(decoded_planes("ext", (a, b)), rig, 32, guard))
 
 
+def cov_decoded_planes(a, b):
+    """f32 planes of a covSplats table's splats in the splat form: the
+    decoded covariance (orc_cov_decode) through fromCov (splats.tsl.ts:189-198)
+    as an eigendecomposition in double: scales sqrt(eigenvalues), rotation the
+    right-handed eigenbasis."""
+    from scipy.spatial.transform import Rotation
+
+    L = O.lib()
+    n = a.shape[0]
+    p = np.zeros((14, n), dtype=np.float32)
+    c, cov, rgb = (np.zeros(k) for k in (3, 6, 3))
+    op = C.c_double()
+    for i in range(n):
+        L.orc_cov_decode(O.uptr(np.ascontiguousarray(a[i])), O.uptr(np.ascontiguousarray(b[i])), O.dptr(c), O.dptr(cov),
+                         O.dptr(rgb), C.byref(op))
+        S = np.array([[cov[0], cov[3], cov[4]], [cov[3], cov[1], cov[5]], [cov[4], cov[5], cov[2]]])
+        w, V = np.linalg.eigh(S)
+        if np.linalg.det(V) < 0:
+            V[:, 0] = -V[:, 0]
+        p[0:3, i] = c
+        with np.errstate(divide="ignore"):
+            p[3:6, i] = 0.5 * np.log(np.maximum(w, 0.0))
+        p[6:10, i] = Rotation.from_matrix(V).as_quat()
+        p[10, i] = op.value
+        p[11:14, i] = rgb
+    return p
+
+
+def test_cov_keys_pass(cuda):
+    # a covSplats source in the keys pass (no wanted form) is read in the splat
+    # form (SplatProjector.ts:152-167): its extent is the largest scale of
+    # fromCov, sqrt of the head-space covariance's largest eigenvalue
+    from tests.test_project_gpu import cov_table
+
+    n = 8000
+    a, b, chans = cov_table(n, 16, 2, lod=True)
+    rig = W.orbit_camera(8, 7.0, 1024, 768)
+    table = DeviceTable.from_cov(a, b, n, chans, sh_codec=2, sh_degree=3)
+    planes = cov_decoded_planes(a, b)
+    for guard in (0.0, 0.7):
+        k, full = keys_both(table, rig, Params(sort_bits=32, cull_keys=False, sort_guard=guard))
+        if guard == 0:
+            assert np.array_equal(k, full)
+        excused = against_oracle(k, keys_oracle(planes, rig, 32, guard))
+        print(f"guard {guard}: {excused} excused", flush=True)
+
+
 @pytest.mark.parametrize("guard", [0.0, 0.5])
 def test_packed_stereo_head_keys_pass(cuda, guard):
     spec = W.stereo_16m_spec(count=65536, seed=3)

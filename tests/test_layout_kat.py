@@ -12,7 +12,8 @@ swapped or shifted field fails:
   codes, bytes little-endian into words).
 The oracle's encoders and decoders (oracle/codec.c) and the synthetic
 table packers the GPU tests use (radix_amd/csrc/synth.cpp) must reproduce
-them. The ext codec's round-trip budgets follow scripts/check-core.ts:341-400.
+them. The ext codec's round-trip budgets follow scripts/check-core.ts:341-400,
+the cov codec's (src/core/splats.ts:219-266) scripts/check-core.ts:402-427.
 """
 import ctypes as C
 import math
@@ -244,3 +245,86 @@ def test_ext_identity_and_zero_quaternions():
         q = d(np.zeros(4))
         L.orc_decode_quat_oct(int(b[3]), 10, 12, O.dptr(q))
         assert list(q) == [0, 0, 0, 1]
+
+
+# covSplats (src/core/splats.ts:25-27 and the codec at :219-266), A and B.x as
+# ext; rgb (0.25, -1, 0.5) as the ext answer, LoD opacity 1.25; covariance
+# diagonal e^-4, e^-2.5, e^0.75 and correlations c_xy = -e^-0.5, c_xz = e^-2,
+# c_yz = 0, i.e. xy = -e^-0.5 sqrt(xx yy) = -e^-3.75, xz = e^-2 sqrt(xx zz) =
+# e^-3.625, yz = 0. A correlation c is stored as sign(c) ln|c|:
+#   A = the centre's float bits | round(1.25 * 128) = 160 << 24 (normal none)
+#   B.x = 0x7A80FF40 (the ext answer's extRgb)
+#   B.y = half(-4) 0xc400 | half(-2.5) 0xc100 << 16
+#   B.z = half(0.75) 0x3a00 | half(-1 * -0.5 = 0.5) 0x3800 << 16
+#   B.w = half(+1 * -2 = -2) 0xc000 | half(0) 0 << 16
+COV_OPACITY = 1.25
+COV6 = [math.exp(-4), math.exp(-2.5), math.exp(0.75), -math.exp(-3.75), math.exp(-3.625), 0.0]
+COV_A = [0x3F800000, 0xC0000000, 0xC0800000, 0xA0000000]
+COV_B = [0x7A80FF40, 0xC100C400, 0x38003A00, 0x0000C000]
+
+
+def test_cov_words_known_answer():
+    a = np.zeros(4, dtype=np.uint32)
+    b = np.zeros(4, dtype=np.uint32)
+    L.orc_cov_encode(O.dptr(d(CENTER)), O.dptr(d(COV6)), O.dptr(d(EXT_RGB)), COV_OPACITY, O.uptr(a), O.uptr(b))
+    assert [hex(x) for x in a] == [hex(x) for x in COV_A]
+    assert [hex(x) for x in b] == [hex(x) for x in COV_B]
+    c, cov, rgb = (d(np.zeros(k)) for k in (3, 6, 3))
+    op = C.c_double()
+    L.orc_cov_decode(O.uptr(np.array(COV_A, dtype=np.uint32)), O.uptr(np.array(COV_B, dtype=np.uint32)), O.dptr(c),
+                     O.dptr(cov), O.dptr(rgb), C.byref(op))
+    assert list(c) == CENTER and op.value == 1.25
+    assert list(rgb) == [64 / 255, -1.0, 128 / 255]
+    # every field decoded from its own bits: ln of the diagonal, c = -sign(e) e^-|e| times sqrt(a b)
+    assert list(np.log(cov[:3])) == pytest.approx([-4.0, -2.5, 0.75], abs=1e-15)
+    assert cov[3] == pytest.approx(-math.exp(-0.5) * math.sqrt(cov[0] * cov[1]), rel=1e-15)
+    assert cov[4] == pytest.approx(math.exp(-2.0) * math.sqrt(cov[0] * cov[2]), rel=1e-15)
+    assert cov[5] == 0.0
+
+
+def test_cov_correlation_edges():
+    # splats.ts:221-232: |c| = 1 keeps -1e-7 (a half subnormal, 2 x 2^-24);
+    # the clamp to [-1, 1] comes first, so an infinite ratio is a full
+    # correlation and only a NaN (0 / 0) stores 0, as c = 0 does; |c| below
+    # e^-100 keeps -100
+    enc, dec = L.orc_cov_encode_correlation, L.orc_cov_decode_correlation
+    assert enc(1.0, 1.0, 1.0) == -1e-7 and enc(-2.0, 1.0, 4.0) == 1e-7
+    assert L.orc_encode_half(enc(1.0, 1.0, 1.0)) == 0x8002
+    assert dec(L.orc_decode_half(0x8002)) == pytest.approx(1.0 - 2.0 * 2 ** -24, rel=1e-15)
+    assert enc(0.0, 1.0, 1.0) == 0.0 and enc(0.0, 0.0, 0.0) == 0.0
+    assert enc(1.0, 0.0, 0.0) == -1e-7 and enc(-1.0, 0.0, 1.0) == 1e-7
+    assert enc(1e-50, 1.0, 1.0) == -100.0 and enc(-1e-50, 1.0, 1.0) == 100.0
+    assert L.orc_encode_half(-100.0) == 0xD640
+    assert dec(-100.0) == pytest.approx(math.exp(-100), rel=1e-15) and dec(100.0) == -dec(-100.0)
+    assert dec(0.0) == 0.0
+
+
+def test_cov_codec_budgets():
+    # check-core.ts:402-427: a random splat's covariance (covFromScaleQuat) through the words
+    r = Rng(79)
+    a = np.zeros(4, dtype=np.uint32)
+    b = np.zeros(4, dtype=np.uint32)
+    c, cov, rgb = (d(np.zeros(k)) for k in (3, 6, 3))
+    op = C.c_double()
+    ref = d(np.zeros(6))
+    for _ in range(2000):
+        u = r.random(14)
+        center = d((u[0:3] - 0.5) * 200)
+        scales = d(np.exp(u[3:6] * 9 - 6))
+        quat = d(u[6:10] * 2 - 1)
+        quat /= np.linalg.norm(quat)
+        colour = d(u[10:13] * 4 - 1)
+        opacity = float(u[13] * 255 / 128)
+        L.orc_cov_from_scale_quat(O.dptr(scales), O.dptr(quat), O.dptr(ref))
+        L.orc_cov_encode(O.dptr(center), O.dptr(ref), O.dptr(colour), opacity, O.uptr(a), O.uptr(b))
+        L.orc_cov_decode(O.uptr(a), O.uptr(b), O.dptr(c), O.dptr(cov), O.dptr(rgb), C.byref(op))
+        assert np.array_equal(c, center.astype(np.float32).astype(np.float64))
+        assert abs(op.value - opacity) <= 1 / 256 + 1e-12
+        assert np.max(np.abs(rgb - colour)) <= np.max(np.abs(colour)) / 255 + 1e-9
+        for k in range(3):
+            assert abs(math.log(cov[k]) - math.log(ref[k])) <= lnhalf_bound(ref[k])
+        for k, i, j in ((3, 0, 1), (4, 0, 2), (5, 1, 2)):
+            ab = math.sqrt(ref[i] * ref[j])
+            corr = abs(ref[k]) / ab
+            bound = ab * (2e-4 + corr * (lnhalf_bound(ref[i]) + lnhalf_bound(ref[j])) / 2)
+            assert abs(cov[k] - ref[k]) <= bound, (k, ref[k], cov[k])

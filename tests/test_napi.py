@@ -65,6 +65,20 @@ def test_napi_adjoint_and_draw_match_the_fp64_oracle(tmp_path):
     np.ascontiguousarray(ordering[:visible], dtype=np.uint32).tofile(tmp_path / "order.u32")
     (tmp_path / "draw.json").write_text(json.dumps({"n": spec.count, "count": int(visible), "capacity": int(visible),
                                                     "width": 320, "height": 240}))
+    # a covSplats table with packedSh bundles, projected and sorted by the binding
+    from radix_amd import DeviceTable, Params, Projector
+    from tests.test_project_gpu import cov_table
+
+    nc = 5000
+    ca, cb, cch = cov_table(nc, 46, 3, lod=True)
+    ca.tofile(tmp_path / "covA.u32")
+    cb.tofile(tmp_path / "covB.u32")
+    for t, ch in enumerate(cch):
+        ch.tofile(tmp_path / f"sh{t}.u32")
+    crig = W.orbit_camera(46, 6.0, 640, 480)
+    ccam = {"head": list(cam.cm(crig.head)), "size": [640, 480], "ndcZMin": 0,
+            "eyes": [{"view": list(cam.cm(crig.eyes[0].view)), "projection": list(cam.cm(crig.eyes[0].projection))}]}
+    (tmp_path / "cov.json").write_text(json.dumps({"n": nc, "camera": ccam}))
     node = shutil.which("node")
     if node is None:
         pytest.skip("node is not installed")
@@ -89,3 +103,16 @@ def test_napi_adjoint_and_draw_match_the_fp64_oracle(tmp_path):
     dref = O.draw(words, ordering[:visible], int(visible), 320, 240)
     dd = np.abs(rgba - dref)
     assert np.mean(dd.max(axis=0) > 1e-4) < 0.005 and np.median(dd) < 1e-6
+    # the covSplats table through the binding equals the Python host's call of the same kernels,
+    # and its order is SortKeys.sort of its keys
+    ctable = DeviceTable.from_cov(ca, cb, nc, cch, sh_codec=3, sh_degree=3)
+    out = Projector().project_sort(ctable, crig, Params(sort_bits=32))
+    import torch
+
+    torch.cuda.synchronize()
+    keys = np.fromfile(tmp_path / "cov_keys.u32", dtype=np.uint32)
+    assert np.array_equal(keys, out["keys"].cpu().numpy().view(np.uint32)[:nc])
+    assert np.array_equal(np.fromfile(tmp_path / "cov_words.u32", dtype=np.uint32).reshape(nc, 4),
+                          out["words"][0].cpu().numpy().view(np.uint32)[:nc])
+    ordering, visible = O.sort(keys, 32)
+    assert np.array_equal(np.fromfile(tmp_path / "cov_indices.u32", dtype=np.uint32), ordering[:visible])

@@ -140,6 +140,67 @@ def test_ext_layout_parity(cuda, sh_codec):
     compare(g, ref, 1)
 
 
+def cov_table(n, seed, sh_codec, lod=False):
+    """covSplats words (splats.ts:219-266) of a synthetic cloud's covariances
+    (covFromScaleQuat) with extSh (codec 2) or packedSh (codec 3) bundles,
+    built by the oracle's encoders (pinned by the known answers of
+    tests/test_layout_kat.py); `lod`: a fifth of the splats get LoD
+    opacities in (1, 255/128]."""
+    L = O.lib()
+    spec = W.plain_4m_spec(seed=seed, count=n)
+    p = W.generate(spec).astype(np.float64)
+    if lod:
+        p[10, ::5] = 1.0 + (255 / 128 - 1.0) * np.linspace(0.01, 1.0, p[10, ::5].size)
+    a = np.zeros((n, 4), dtype=np.uint32)
+    b = np.zeros((n, 4), dtype=np.uint32)
+    texels = 4 if sh_codec == 2 else 3
+    chans = [np.zeros((n, 4), dtype=np.uint32) for _ in range(texels)]
+    words = np.zeros(16, dtype=np.uint32)
+    cov = np.zeros(6)
+    for i in range(n):
+        s = np.ascontiguousarray(np.exp(p[3:6, i]))
+        q = np.ascontiguousarray(p[6:10, i])
+        L.orc_cov_from_scale_quat(O.dptr(s), O.dptr(q), O.dptr(cov))
+        L.orc_cov_encode(O.dptr(np.ascontiguousarray(p[0:3, i])), O.dptr(cov), O.dptr(np.ascontiguousarray(p[11:14, i])),
+                         float(p[10, i]), O.uptr(a[i]), O.uptr(b[i]))
+        words[:] = 0
+        L.orc_sh_encode(sh_codec, O.dptr(np.ascontiguousarray(p[14:59, i])), 3, O.uptr(words))
+        for t in range(texels):
+            chans[t][i] = words[4 * t:4 * t + 4]
+    return a, b, chans
+
+
+@pytest.mark.parametrize("sh_codec", [2, 3])
+def test_cov_layout_parity(cuda, sh_codec):
+    # the ewa method reads a covSplats splat's covariance as stored (GpuCovSplatReader.loadCovSplat,
+    # splatAccess.ts:107-109; decodeCov, splats.tsl.ts:105-121), LoD opacities included
+    n = 20000
+    a, b, chans = cov_table(n, 14, sh_codec, lod=True)
+    rig = W.orbit_camera(9, 7.0, 1024, 768)
+    table = DeviceTable.from_cov(a, b, n, chans, sh_codec=sh_codec, sh_degree=3)
+    g = run(table, rig, Params(sort_bits=32))
+    ref = O.project_cov(a, b, chans, sh_codec, 3, O.view_from_rig(rig), O.params(sort_bits=32))
+    rep = compare(g, ref, 1)
+    assert rep["eye0_compared"] > n // 4
+    check_order(table, rig, Params(sort_bits=32), keys_expected=g["keys"][:n])
+
+
+def test_cov_layout_transform_stereo_guard(cuda):
+    # a non-uniform object transform, two eyes, and the CPU sort's keys
+    # (cullKeys off, sortGuard on: the cov form's extent, projection.tsl.ts:144-155)
+    n = 12000
+    a, b, chans = cov_table(n, 15, 2)
+    obj = cam.translation([0.3, 0.2, -0.5])
+    obj[:3, :3] = np.array([[1.3, 0.2, 0.0], [0.0, 0.8, 0.1], [0.1, 0.0, 1.1]])
+    rig = cam.stereo(cam.orbit_world(cam.random_direction(23), 7.0), 0.064, 70.0, 900, 700)
+    prm = Params(sort_bits=32, cull_keys=False, sort_guard=0.1)
+    table = DeviceTable.from_cov(a, b, n, chans, sh_codec=2, sh_degree=3, object_to_world=obj)
+    g = run(table, rig, prm)
+    ref = O.project_cov(a, b, chans, 2, 3, O.view_from_rig(rig, object_to_world=obj), O.params(sort_bits=32, cull_keys=False,
+                                                                                           sort_guard=0.1))
+    compare(g, ref, 2)
+
+
 def test_object_transform_parity(cuda):
     spec = W.plain_4m_spec(seed=31, count=40000)
     planes = W.generate(spec)
