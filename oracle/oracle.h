@@ -275,6 +275,10 @@ typedef struct orc_ray_out {
   double margin;         /* the smallest over the head and eyes */
 } orc_ray_out;
 void orc_project_ray(int method, const orc_ray_splat* s, const orc_view* v, const orc_params* p, orc_ray_out* out);
+/* fromCov (splats.tsl.ts:189-198 with symEigens, math.tsl.ts:46-113, in
+ * double): scales and rotation (x, y, z, w; w >= 0) of a covariance
+ * xx, yy, zz, xy, xz, yz */
+void orc_from_cov(const double* cov6, double* scales, double* quat);
 /* A whole f32 table: words n x (6 + 2 eyes; raygs 4 + 3 eyes), keys, margins (each may be NULL but words) */
 void orc_project_ray_table_f32(int method, const orc_table_f32* t, const orc_view* v, const orc_params* p, uint32_t* words,
                                uint32_t* keys, double* margin, double* eye_margin);

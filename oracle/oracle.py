@@ -111,6 +111,7 @@ def lib() -> C.CDLL:
             "orc_cov_decode": (None, [_u32p, _u32p, _dp, _dp, _dp, _dp]),
             "orc_cov_encode_correlation": (_d, [_d, _d, _d]),
             "orc_cov_decode_correlation": (_d, [_d]),
+            "orc_from_cov": (None, [_dp, _dp, _dp]),
             "orc_sh_basis": (None, [_dp, _dp]),
             "orc_sh_evaluate": (None, [_dp, _dp, C.c_int, _dp]),
             "orc_raster_binning": (None, [C.c_int]),

@@ -10,9 +10,11 @@
  * inSortGuard 145-155, viewBasis 169-174, mipInflate 177-197,
  * boundFromCov 223-240, screenBound 243-248, clipBoundToScreen 250-275,
  * encodeLogScale 299-303, encodeOpacity 310, encodeCenter 317-333,
- * encodeBound 352-360), with the uniform object transform of
- * src/render/splatTransform.tsl.ts:142-152 and the codecs of
- * src/gpu/encoding.tsl.ts:50-134. Test infrastructure only.
+ * encodeBound 352-360), with the object transform of
+ * src/render/splatTransform.tsl.ts:46-62 and 142-165 (a rotation times a
+ * uniform scale turns the splat; any other goes through fromCov,
+ * src/gpu/splats.tsl.ts:189-198, with symEigens, src/gpu/math.tsl.ts:46-113)
+ * and the codecs of src/gpu/encoding.tsl.ts:50-134. Test infrastructure only.
  *
  * Matrices are column-major 4x4: element (row r, column c) at m[4c + r].
  */
@@ -348,6 +350,99 @@ static void basis_to_quat(const double* x, const double* y, const double* z, dou
   }
 }
 
+/* symEigens (math.tsl.ts:46-113) in double: Jacobi rotations on the
+ * largest off-diagonal entry (the first of (0,1), (0,2), (1,2) at a tie), at
+ * most 16, until it is within 1e-6 of the trace; eigenvalues descending by
+ * three conditional swaps, eigenvectors normalized, the third flipped to a
+ * right-handed basis. E's row j is eigenvector j. */
+static void sym_eigens(const double* c6, double* values, double vec[3][3]) {
+  double A[3][3] = {{c6[0], c6[3], c6[4]}, {c6[3], c6[1], c6[5]}, {c6[4], c6[5], c6[2]}};
+  double E[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  const double eps = 1e-6 * (fabs(c6[0]) + fabs(c6[1]) + fabs(c6[2]));
+  for (int it = 0; it < 16; it++) {
+    int p = 0, q = 1;
+    double mx = fabs(A[0][1]);
+    if (fabs(A[0][2]) > mx) {
+      mx = fabs(A[0][2]);
+      q = 2;
+    }
+    if (fabs(A[1][2]) > mx) {
+      mx = fabs(A[1][2]);
+      p = 1;
+      q = 2;
+    }
+    if (mx <= eps) break;
+    const double phi = 0.5 * atan2(2.0 * A[p][q], A[q][q] - A[p][p]);
+    const double c = cos(phi), s = sin(phi);
+    for (int k = 0; k < 3; k++) {
+      const double x = A[p][k], y = A[q][k];
+      A[p][k] = c * x - s * y;
+      A[q][k] = s * x + c * y;
+    }
+    for (int k = 0; k < 3; k++) {
+      const double x = A[k][p], y = A[k][q];
+      A[k][p] = c * x - s * y;
+      A[k][q] = s * x + c * y;
+    }
+    A[p][q] = A[q][p] = 0.0;
+    for (int k = 0; k < 3; k++) {
+      const double x = E[p][k], y = E[q][k];
+      E[p][k] = c * x - s * y;
+      E[q][k] = s * x + c * y;
+    }
+  }
+  double l[3] = {A[0][0], A[1][1], A[2][2]};
+  for (int j = 0; j < 3; j++) {
+    const double n = sqrt(E[j][0] * E[j][0] + E[j][1] * E[j][1] + E[j][2] * E[j][2]);
+    for (int k = 0; k < 3; k++) vec[j][k] = E[j][k] / n;
+  }
+  const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+  for (int k = 0; k < 3; k++) {
+    const int a = pairs[k][0], b = pairs[k][1];
+    if (l[a] < l[b]) {
+      double t = l[a];
+      l[a] = l[b];
+      l[b] = t;
+      for (int m = 0; m < 3; m++) {
+        t = vec[a][m];
+        vec[a][m] = vec[b][m];
+        vec[b][m] = t;
+      }
+    }
+  }
+  const double x[3] = {vec[0][1] * vec[1][2] - vec[0][2] * vec[1][1], vec[0][2] * vec[1][0] - vec[0][0] * vec[1][2],
+                       vec[0][0] * vec[1][1] - vec[0][1] * vec[1][0]};
+  if (x[0] * vec[2][0] + x[1] * vec[2][1] + x[2] * vec[2][2] < 0)
+    for (int m = 0; m < 3; m++) vec[2][m] = -vec[2][m];
+  for (int j = 0; j < 3; j++) values[j] = l[j];
+}
+
+/* fromCov (splats.tsl.ts:189-198): scales and rotation (w >= 0) of a covariance */
+void orc_from_cov(const double* c6, double* scales, double* quat) {
+  double values[3], vec[3][3];
+  sym_eigens(c6, values, vec);
+  for (int k = 0; k < 3; k++) scales[k] = sqrt(fmax(values[k], 0.0));
+  basis_to_quat(vec[0], vec[1], vec[2], quat);
+  qnorm(quat);
+  if (quat[3] < 0)
+    for (int k = 0; k < 4; k++) quat[k] = -quat[k];
+}
+
+/* isUniformScale (splatTransform.tsl.ts:46-62) of a column-major 4x4's
+ * upper-left 3x3: G = M^T M within 1e-6 s2 of s2 I (s2 = trace(G) / 3) and
+ * det M > 0 */
+static int is_uniform_scale(const double* M) {
+  double G[3][3];
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) G[a][b] = AT(M, 0, a) * AT(M, 0, b) + AT(M, 1, a) * AT(M, 1, b) + AT(M, 2, a) * AT(M, 2, b);
+  const double s2 = (G[0][0] + G[1][1] + G[2][2]) / 3.0;
+  if (!(s2 > 0)) return 0;
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++)
+      if (fabs(G[a][b] - (a == b ? s2 : 0.0)) > 1e-6 * s2) return 0;
+  return det3_of(M) > 0;
+}
+
 /* quadOffScreen (projections.tsl.ts:335-352) */
 static int quad_off_screen(const double* P, const double* mu, const double* a0, const double* a1, double* margin) {
   double c[4], A0[4], A1[4];
@@ -448,17 +543,34 @@ static int raygs_eye(const orc_eye* E, const double* c, const double* q, const d
 void orc_project_ray(int method, const orc_ray_splat* src, const orc_view* v, const orc_params* p, orc_ray_out* out) {
   memset(out, 0, sizeof *out);
   double margin = INFINITY;
-  /* transformUniform with a = object to head (splatTransform.tsl.ts:142-152) */
+  /* transformSplat with a = object to head (splatTransform.tsl.ts:154-165):
+   * transformUniform (142-152) for a rotation times a uniform scale, else
+   * fromCov of M Sigma M^T */
   const double* M = v->object_to_head;
-  double oscale = cbrt(fabs(det3_of(M)));
-  double tq[4];
-  quat_of_matrix(M, oscale, tq);
   double center[4], qn[4] = {src->quat[0], src->quat[1], src->quat[2], src->quat[3]};
   point_of(M, src->center, 1.0, center);
   qnorm(qn);
-  double quat[4];
-  qmul(tq, qn, quat);
-  double scales[3] = {src->scales[0] * oscale, src->scales[1] * oscale, src->scales[2] * oscale};
+  double quat[4], scales[3];
+  if (is_uniform_scale(M)) {
+    double oscale = cbrt(fabs(det3_of(M)));
+    double tq[4];
+    quat_of_matrix(M, oscale, tq);
+    qmul(tq, qn, quat);
+    for (int k = 0; k < 3; k++) scales[k] = src->scales[k] * oscale;
+  } else {
+    double cov[6], h[6];
+    orc_cov_from_scale_quat(src->scales, qn, cov);
+    const double S[3][3] = {{cov[0], cov[3], cov[4]}, {cov[3], cov[1], cov[5]}, {cov[4], cov[5], cov[2]}};
+    double MS[3][3];
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) MS[a][b] = AT(M, a, 0) * S[0][b] + AT(M, a, 1) * S[1][b] + AT(M, a, 2) * S[2][b];
+    const int ij[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+    for (int e = 0; e < 6; e++) {
+      const int a = ij[e][0], b = ij[e][1];
+      h[e] = MS[a][0] * AT(M, b, 0) + MS[a][1] * AT(M, b, 1) + MS[a][2] * AT(M, b, 2);
+    }
+    orc_from_cov(h, scales, quat);
+  }
 
   /* prepareSplat */
   double opacity = src->opacity, inflate = 1.0;

@@ -21,7 +21,7 @@ BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("RDX_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["api.hip", "project.hip", "project_f32.hip", "project_ext.hip", "project_packed.hip", "batch.hip", "raster.hip",
-               "draw.hip", "ray.hip", "io.hip"]
+               "draw.hip", "ray.hip", "ray_uniform.hip", "ray_nonuniform.hip", "io.hip"]
 SYNTH_SOURCES = ["synth.cpp"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

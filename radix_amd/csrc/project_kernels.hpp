@@ -731,10 +731,10 @@ __device__ __forceinline__ EwaResult ewa_eye_cov(const EyeU& E, const EyeCentre&
 // the cov form (sp.cv grows; s is the inflation), else the splat form (s the
 // inflated scales).
 template <bool COVF, typename U>
-__device__ __forceinline__ void prepare(Splat& sp, const U& u, float& op, float3& s, float& std_dev,
-                                        bool& invisible) {
+__device__ __forceinline__ void prepare(Splat& sp, const U& u, float& op, float3& s, float& std_dev, bool& invisible,
+                                        float& inflate) {
   op = sp.op;
-  float inflate = 1.0f;
+  inflate = 1.0f;
   if (op > 1.0f) {
     op = fminf(op * 4.0f - 3.0f, 5.0f);
     if (u.lod_inflate) {
@@ -754,6 +754,11 @@ __device__ __forceinline__ void prepare(Splat& sp, const U& u, float& op, float3
     s = make_float3(sp.s.x * inflate, sp.s.y * inflate, sp.s.z * inflate);
     invisible = op <= 0.0f || op < u.min_alpha || (s.x == 0.0f && s.y == 0.0f && s.z == 0.0f);
   }
+}
+template <bool COVF, typename U>
+__device__ __forceinline__ void prepare(Splat& sp, const U& u, float& op, float3& s, float& std_dev, bool& invisible) {
+  float inflate;
+  prepare<COVF>(sp, u, op, s, std_dev, invisible, inflate);
 }
 
 // inSortGuard (projection.tsl.ts:144-155): a CPU-sorted splat keeps its key

@@ -9,7 +9,7 @@
 #include "../../include/radix_hip.h"
 #include "api_internal.hpp"
 #include "project_host.hpp"
-#include "ray_kernels.hpp"
+#include "ray_launch.hpp"
 
 namespace rdx {
 
@@ -69,40 +69,6 @@ static double uniform_scale(const double* M) {
   return d > 0.0 ? std::cbrt(d) : 0.0;
 }
 
-template <int METHOD, int EYES>
-static void launch_layout(hipStream_t st, dim3 grid, const rdx_splat_table* t, const TableP& tp, const RayU& r,
-                          const RayOut& o) {
-  const int shc = t->sh_codec == RDX_SH_PACKED ? SHC_PACKED : SHC_EXT;
-  if (t->layout == RDX_LAYOUT_F32_SOA) {
-    switch (r.u.sh_degree) {
-      case 0: hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_F32, SHC_F32, 0, EYES>), grid, dim3(256), 0, st, tp, r, o); break;
-      case 1: hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_F32, SHC_F32, 1, EYES>), grid, dim3(256), 0, st, tp, r, o); break;
-      case 2:
-        if (f32_sh_wide(tp.stride, 2))
-          hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_F32, SHC_F32_WIDE, 2, EYES>), grid, dim3(256), 0, st, tp, r, o);
-        else
-          hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_F32, SHC_F32, 2, EYES>), grid, dim3(256), 0, st, tp, r, o);
-        break;
-      default:
-        if (f32_sh_wide(tp.stride, 3))
-          hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_F32, SHC_F32_WIDE, 3, EYES>), grid, dim3(256), 0, st, tp, r, o);
-        else
-          hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_F32, SHC_F32, 3, EYES>), grid, dim3(256), 0, st, tp, r, o);
-        break;
-    }
-  } else if (t->layout == RDX_LAYOUT_EXT) {
-    if (shc == SHC_PACKED)
-      hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_EXT, SHC_PACKED, -1, EYES>), grid, dim3(256), 0, st, tp, r, o);
-    else
-      hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_EXT, SHC_EXT, -1, EYES>), grid, dim3(256), 0, st, tp, r, o);
-  } else {
-    if (shc == SHC_PACKED)
-      hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_PACKED, SHC_PACKED, -1, EYES>), grid, dim3(256), 0, st, tp, r, o);
-    else
-      hipLaunchKernelGGL((k_project_ray<METHOD, LAYOUT_PACKED, SHC_EXT, -1, EYES>), grid, dim3(256), 0, st, tp, r, o);
-  }
-}
-
 }  // namespace rdx
 
 using namespace rdx;
@@ -116,8 +82,6 @@ int rdx_project_ray(rdx_context* ctx, void* stream, const rdx_splat_table* table
     return fail(method == RDX_METHOD_EWA ? RDX_E_UNSUPPORTED : RDX_E_INVALID,
                 "rdx_project_ray: method %d is not 3dgut (1), htgs (2) or raygs (3) (ewa: rdx_project_ewa)", method);
   if (int rc = check_table(table)) return rc;
-  if (table->layout == RDX_LAYOUT_COV)
-    return fail(RDX_E_UNSUPPORTED, "rdx_project_ray: covSplats tables are read by the ewa method only");
   if (int rc = check_bits(params->sort_bits, "SplatProjection")) return rc;
   RayU r;
   memset(&r, 0, sizeof r);
@@ -125,19 +89,38 @@ int rdx_project_ray(rdx_context* ctx, void* stream, const rdx_splat_table* table
   const int channels = camera->eyes == 2 ? 3 : 2;
   for (int c = 0; c < channels; c++)
     if (!channels_dev[c]) return fail(RDX_E_INVALID, "rdx_project_ray: channel w%d is NULL", c);
-  // the object transform as a uniform scale times a rotation
+  // The object transform: a uniform scale times a rotation turns and scales
+  // the splat (transformUniform, splatTransform.tsl.ts:142-152); any other
+  // goes through the covariance (fromCov of H Σ H^T, :154-165), after which
+  // each eye turns the head-space splat by its head -> eye rotation.
   const double s = uniform_scale(r.u.H);
-  if (!(s > 0.0))
-    return fail(RDX_E_UNSUPPORTED,
-                "rdx_project_ray: the object transform must be a rotation times a uniform scale (the ray methods read "
-                "scales and rotations)");
-  r.oscale = (float)s;
   double R[9];
-  for (int k = 0; k < 9; k++) R[k] = r.u.H[k] / s;
-  quat_of(R, r.hq);
-  for (int e = 0; e < camera->eyes; e++) {
-    for (int k = 0; k < 9; k++) R[k] = r.u.eye[e].L[k] / s;
-    quat_of(R, r.eq[e]);
+  if (s > 0.0) {
+    r.oscale = (float)s;
+    for (int k = 0; k < 9; k++) R[k] = r.u.H[k] / s;
+    quat_of(R, r.hq);
+    for (int e = 0; e < camera->eyes; e++) {
+      for (int k = 0; k < 9; k++) R[k] = r.u.eye[e].L[k] / s;
+      quat_of(R, r.eq[e]);
+    }
+  } else {
+    const double d = det3r(r.u.H);
+    if (!(std::fabs(d) > 0.0)) return fail(RDX_E_INVALID, "rdx_project_ray: the object transform is singular");
+    r.nonuniform = 1;
+    r.oscale = 1.0f;
+    // H^-1 (row-major) by cofactors; each eye's head -> eye rotation L_e H^-1
+    const double* H = r.u.H;
+    const double inv[9] = {(H[4] * H[8] - H[5] * H[7]) / d, (H[2] * H[7] - H[1] * H[8]) / d,
+                           (H[1] * H[5] - H[2] * H[4]) / d, (H[5] * H[6] - H[3] * H[8]) / d,
+                           (H[0] * H[8] - H[2] * H[6]) / d, (H[2] * H[3] - H[0] * H[5]) / d,
+                           (H[3] * H[7] - H[4] * H[6]) / d, (H[1] * H[6] - H[0] * H[7]) / d,
+                           (H[0] * H[4] - H[1] * H[3]) / d};
+    for (int e = 0; e < camera->eyes; e++) {
+      const double* L = r.u.eye[e].L;
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) R[3 * a + b] = L[3 * a] * inv[b] + L[3 * a + 1] * inv[3 + b] + L[3 * a + 2] * inv[6 + b];
+      quat_of(R, r.ehq[e]);
+    }
   }
   const uint32_t n = table->count;
   if (n == 0) return RDX_OK;
@@ -151,16 +134,10 @@ int rdx_project_ray(rdx_context* ctx, void* stream, const rdx_splat_table* table
   hipStream_t st = (hipStream_t)stream;
   Stage stage(reinterpret_cast<Context*>(ctx), st,
               method == RDX_METHOD_3DGUT ? "project_3dgut" : (method == RDX_METHOD_HTGS ? "project_htgs" : "project_raygs"));
-  if (method == RDX_METHOD_3DGUT) {
-    if (camera->eyes == 2) launch_layout<METHOD_3DGUT, 2>(st, grid, table, tp, r, o);
-    else launch_layout<METHOD_3DGUT, 1>(st, grid, table, tp, r, o);
-  } else if (method == RDX_METHOD_HTGS) {
-    if (camera->eyes == 2) launch_layout<METHOD_HTGS, 2>(st, grid, table, tp, r, o);
-    else launch_layout<METHOD_HTGS, 1>(st, grid, table, tp, r, o);
-  } else {
-    if (camera->eyes == 2) launch_layout<METHOD_RAYGS, 2>(st, grid, table, tp, r, o);
-    else launch_layout<METHOD_RAYGS, 1>(st, grid, table, tp, r, o);
-  }
+  if (r.nonuniform)
+    launch_ray_nonuniform(method, camera->eyes, st, grid, table, tp, r, o);
+  else
+    launch_ray_uniform(method, camera->eyes, st, grid, table, tp, r, o);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? RDX_OK : hip_fail(e, "rdx_project_ray");
 }

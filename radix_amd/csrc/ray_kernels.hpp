@@ -24,6 +24,11 @@ struct RayU {
   float hq[4];     // object -> head rotation (x, y, z, w)
   float eq[2][4];  // object -> eye rotations
   float oscale;    // the object transform's scale
+  // An object transform that is not a rotation times a uniform scale: the
+  // splat's head-space form is fromCov of H Σ H^T (u.H), and each eye turns
+  // it by its head -> eye rotation ehq.
+  int nonuniform;
+  float ehq[2][4];
 };
 
 // Word channels: ch[0], ch[1] uint32x4; with two eyes ch[2] uint32x2
@@ -330,6 +335,123 @@ __device__ __forceinline__ float4 basis_to_quat(float3 x, float3 y, float3 z) {
   return q;
 }
 
+// One Jacobi rotation of symEigens (math.tsl.ts:57-86) in the plane (P, Q):
+// rows then columns of A, then the rows of E (E's row j is eigenvector j).
+template <int P, int Q>
+__device__ __forceinline__ void jacobi_rotate(double (&A)[3][3], double (&E)[3][3]) {
+  const double phi = 0.5 * atan2(2.0 * A[P][Q], A[Q][Q] - A[P][P]);
+  double s, c;
+  sincos(phi, &s, &c);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double x = A[P][k], y = A[Q][k];
+    A[P][k] = c * x - s * y;
+    A[Q][k] = s * x + c * y;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double x = A[k][P], y = A[k][Q];
+    A[k][P] = c * x - s * y;
+    A[k][Q] = s * x + c * y;
+  }
+  A[P][Q] = 0.0;
+  A[Q][P] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double x = E[P][k], y = E[Q][k];
+    E[P][k] = c * x - s * y;
+    E[Q][k] = s * x + c * y;
+  }
+}
+
+// fromCov (splats.tsl.ts:189-198): the scales (square roots of the
+// eigenvalues, descending) and the rotation (basisToQuat of the right-handed
+// eigenbasis, w >= 0) of a covariance c6 = xx, yy, zz, xy, xz, yz, by
+// symEigens (math.tsl.ts:46-113): at most 16 Jacobi rotations, each on the
+// largest off-diagonal entry, until it is within 1e-6 of the trace. In
+// double: a thin splat's small eigenvalue from float sweeps carries the
+// large one's rounding (its relative error times the aspect ratio), and the
+// raygs frame's c^2 scales with its inverse.
+__device__ __forceinline__ void from_cov(const double* c6, float3& scales, float4& quat) {
+  double A[3][3] = {{c6[0], c6[3], c6[4]}, {c6[3], c6[1], c6[5]}, {c6[4], c6[5], c6[2]}};
+  double E[3][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+  const double eps = 1e-6 * (fabs(c6[0]) + fabs(c6[1]) + fabs(c6[2]));
+#pragma unroll 1
+  for (int it = 0; it < 16; it++) {
+    int pq = 0;
+    double mx = fabs(A[0][1]);
+    if (fabs(A[0][2]) > mx) {
+      mx = fabs(A[0][2]);
+      pq = 1;
+    }
+    if (fabs(A[1][2]) > mx) {
+      mx = fabs(A[1][2]);
+      pq = 2;
+    }
+    if (mx <= eps) break;
+    if (pq == 0) jacobi_rotate<0, 1>(A, E);
+    else if (pq == 1) jacobi_rotate<0, 2>(A, E);
+    else jacobi_rotate<1, 2>(A, E);
+  }
+  double l[3] = {A[0][0], A[1][1], A[2][2]};
+  double v[3][3];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    const double r = 1.0 / sqrt(E[j][0] * E[j][0] + E[j][1] * E[j][1] + E[j][2] * E[j][2]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) v[j][k] = E[j][k] * r;
+  }
+  const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int a = pairs[k][0], b = pairs[k][1];
+    if (l[a] < l[b]) {
+      const double t = l[a];
+      l[a] = l[b];
+      l[b] = t;
+#pragma unroll
+      for (int m = 0; m < 3; m++) {
+        const double w = v[a][m];
+        v[a][m] = v[b][m];
+        v[b][m] = w;
+      }
+    }
+  }
+  const double x0 = v[0][1] * v[1][2] - v[0][2] * v[1][1], x1 = v[0][2] * v[1][0] - v[0][0] * v[1][2],
+               x2 = v[0][0] * v[1][1] - v[0][1] * v[1][0];
+  if (x0 * v[2][0] + x1 * v[2][1] + x2 * v[2][2] < 0.0)
+#pragma unroll
+    for (int m = 0; m < 3; m++) v[2][m] = -v[2][m];
+  scales = make_float3((float)sqrt(fmax(l[0], 0.0)), (float)sqrt(fmax(l[1], 0.0)), (float)sqrt(fmax(l[2], 0.0)));
+  // basisToQuat (math.tsl.ts:117-150) of the columns x, y, z
+  const double* x = v[0];
+  const double* y = v[1];
+  const double* z = v[2];
+  double q[4];
+  if (z[2] <= 0.0) {
+    const double dif10 = y[1] - x[0], omm22 = 1.0 - z[2];
+    if (dif10 <= 0.0) {
+      const double four = omm22 - dif10, inv = 0.5 / sqrt(fmax(four, 1e-30));
+      q[0] = four * inv, q[1] = (x[1] + y[0]) * inv, q[2] = (x[2] + z[0]) * inv, q[3] = (y[2] - z[1]) * inv;
+    } else {
+      const double four = omm22 + dif10, inv = 0.5 / sqrt(fmax(four, 1e-30));
+      q[0] = (x[1] + y[0]) * inv, q[1] = four * inv, q[2] = (y[2] + z[1]) * inv, q[3] = (z[0] - x[2]) * inv;
+    }
+  } else {
+    const double sum10 = y[1] + x[0], opm22 = 1.0 + z[2];
+    if (sum10 <= 0.0) {
+      const double four = opm22 - sum10, inv = 0.5 / sqrt(fmax(four, 1e-30));
+      q[0] = (x[2] + z[0]) * inv, q[1] = (y[2] + z[1]) * inv, q[2] = four * inv, q[3] = (x[1] - y[0]) * inv;
+    } else {
+      const double four = opm22 + sum10, inv = 0.5 / sqrt(fmax(four, 1e-30));
+      q[0] = (y[2] - z[1]) * inv, q[1] = (z[0] - x[2]) * inv, q[2] = (x[1] - y[0]) * inv, q[3] = four * inv;
+    }
+  }
+  const double n = 1.0 / sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  const double sg = q[3] < 0.0 ? -n : n;
+  quat = make_float4((float)(q[0] * sg), (float)(q[1] * sg), (float)(q[2] * sg), (float)(q[3] * sg));
+}
+
 // quadOffScreen: every corner mu +- a0 +- a1 in front of the camera and
 // all four beyond the same side of the frustum
 __device__ __forceinline__ bool quad_off_screen(const float* P, float3 mu, float3 a0, float3 a1) {
@@ -417,11 +539,14 @@ __device__ __forceinline__ bool raygs_eye(const EyeU& E, float3 c, float4 q, flo
 
 // ---- the kernel ----------------------------------------------------------------
 
+// NU: the object transform is not a rotation times a uniform scale (the
+// head-space splat is fromCov of H Σ H^T); a compile-time choice, so the
+// common kernels carry none of its registers.
 // One splat per thread: prepareSplat, the head basis with MIP inflation
 // (projections.tsl.ts:172-180, projection.tsl.ts:169-197), per eye the
 // bound (cullDepth only), the words and the key (SplatProjection.ts:131-151;
 // a splat-form sortGuard, projection.tsl.ts:145-155).
-template <int METHOD, int LAYOUT, int SHC, int DEG, int EYES>
+template <int METHOD, int LAYOUT, int SHC, int DEG, int EYES, bool NU>
 __global__ __launch_bounds__(256) void k_project_ray(TableP t, RayU r, RayOut o) {
   const ProjU& u = r.u;
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
@@ -433,10 +558,14 @@ __global__ __launch_bounds__(256) void k_project_ray(TableP t, RayU r, RayOut o)
   float3 coef[15];
   float band[3];
   load_splat<LAYOUT, SHC, DEG>(t, rs, u.enc, i, deg, sp, coef, band);
-  float op, std_dev;
+  if constexpr (LAYOUT == LAYOUT_COV) {
+    // the splat form of a covSplats splat (GpuCovSplatReader.loadSplat, splatAccess.ts:103-105)
+    from_cov(sp.cv, sp.s, sp.q);
+  }
+  float op, std_dev, inflate;
   float3 s;
   bool invisible;
-  prepare<false>(sp, u, op, s, std_dev, invisible);
+  prepare<false>(sp, u, op, s, std_dev, invisible, inflate);
   const float3 rgb = sh_colour(sp, u.h, deg, coef, SHC == SHC_PACKED ? band : nullptr);
 
   // head-space centre (double) and its distance, as the ewa kernel takes them
@@ -455,13 +584,39 @@ __global__ __launch_bounds__(256) void k_project_ray(TableP t, RayU r, RayOut o)
   // viewBasis in the head frame: the head-space rotation and scales, flat
   // splats kept a sliver thick
   const float4 qn = quat_normalize(sp.q);
-  const float4 hq = quat_normalize(quat_mul(make_float4(r.hq[0], r.hq[1], r.hq[2], r.hq[3]), qn));
-  float3 S = make_float3(s.x * r.oscale, s.y * r.oscale, s.z * r.oscale);
+  float4 hq;
+  float3 S;
+  if constexpr (NU) {
+    // transformSplat's general case (splatTransform.tsl.ts:154-165): fromCov
+    // of H Σ H^T, Σ the object-space covariance, then grown by the LoD inflation
+    float R[9];
+    quat_matrix(qn, R);
+    double M[9];  // H R diag(s)
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const double sk = k == 0 ? sp.s.x : (k == 1 ? sp.s.y : sp.s.z);
+        M[3 * a + k] = (u.H[3 * a] * R[k] + u.H[3 * a + 1] * R[3 + k] + u.H[3 * a + 2] * R[6 + k]) * sk;
+      }
+    double c6[6];
+    const int ij[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+#pragma unroll
+    for (int e = 0; e < 6; e++)
+      c6[e] = M[3 * ij[e][0]] * M[3 * ij[e][1]] + M[3 * ij[e][0] + 1] * M[3 * ij[e][1] + 1] +
+              M[3 * ij[e][0] + 2] * M[3 * ij[e][1] + 2];
+    float3 sh;
+    from_cov(c6, sh, hq);
+    S = make_float3(sh.x * inflate, sh.y * inflate, sh.z * inflate);
+  } else {
+    hq = quat_normalize(quat_mul(make_float4(r.hq[0], r.hq[1], r.hq[2], r.hq[3]), qn));
+    S = make_float3(s.x * r.oscale, s.y * r.oscale, s.z * r.oscale);
+  }
+  const float3 s_guard = S;
   {
     const float floor_s = fmaxf(S.x, fmaxf(S.y, S.z)) * 1e-4f;
     S = make_float3(fmaxf(S.x, floor_s), fmaxf(S.y, floor_s), fmaxf(S.z, floor_s));
   }
-  const float3 s_guard = make_float3(s.x * r.oscale, s.y * r.oscale, s.z * r.oscale);
 
   // mipInflate with the head view (eye 0's lens)
   float oph = op;
@@ -503,7 +658,8 @@ __global__ __launch_bounds__(256) void k_project_ray(TableP t, RayU r, RayOut o)
     const bool cull_depth = z >= 0.0f || C.clip[2] < u.ndc_z_min * C.clip[3] || C.clip[2] >= C.clip[3];
     if (ok && !cull_depth) {
       const float3 c = make_float3((float)C.ce[0], (float)C.ce[1], z);
-      const float4 eq = quat_normalize(quat_mul(make_float4(r.eq[e][0], r.eq[e][1], r.eq[e][2], r.eq[e][3]), qn));
+      const float4 eq = NU ? quat_normalize(quat_mul(make_float4(r.ehq[e][0], r.ehq[e][1], r.ehq[e][2], r.ehq[e][3]), hq))
+                           : quat_normalize(quat_mul(make_float4(r.eq[e][0], r.eq[e][1], r.eq[e][2], r.eq[e][3]), qn));
       if constexpr (METHOD == METHOD_RAYGS) {
         if (raygs_eye(E, c, eq, S, std_dev, w + HEAD + EW * e)) visible = true;
       } else {

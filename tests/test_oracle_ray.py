@@ -274,3 +274,63 @@ def test_raygs_culls():
     o = O.project_ray("raygs", [0.3, 0.2, -4.0], [0.2] * 3, [0, 0, 0, 1], [0.5] * 3, 0.9, view, params())
     assert o.visible and (o.words[3] & 0xFF) == max(1, int(math.floor(o.head_opacity * 128 + 0.5)))
     assert o.words[3] & 0x00FFFF00 == 0
+
+
+def from_cov(c6):
+    s, q = np.zeros(3), np.zeros(4)
+    O.lib().orc_from_cov(O.dptr(np.ascontiguousarray(c6, dtype=np.float64)), O.dptr(s), O.dptr(q))
+    return s, q
+
+
+def test_from_cov_single_rotation_known_answer():
+    # symEigens (math.tsl.ts:46-113) on xx = yy = 2, xy = 1, zz = 1/2: one
+    # rotation in the (0, 1) plane, phi = atan2(2, 0) / 2 = pi / 4, leaves
+    # diag(1, 3, 1/2) with rows (1, -1, 0) / sqrt2, (1, 1, 0) / sqrt2, (0, 0, 1);
+    # sorted descending: values 3, 1, 1/2, v0 = (1, 1, 0) / sqrt2, v1 = (1, -1, 0) / sqrt2,
+    # and v0 x v1 = (0, 0, -1) flips v2 to (0, 0, -1). basisToQuat (z.z <= 0,
+    # y.y - x.x <= 0): four = 2 + 2a (a = 1 / sqrt2), q = (1 + a, a, 0, 0) / |.|:
+    # the half turn about (cos 22.5, sin 22.5, 0), which takes x to v0 and z to -z.
+    s, q = from_cov([2.0, 2.0, 0.5, 1.0, 0.0, 0.0])
+    a = math.sqrt(0.5)
+    assert s == pytest.approx([math.sqrt(3.0), 1.0, math.sqrt(0.5)], rel=1e-15)
+    assert q == pytest.approx(np.array([1 + a, a, 0.0, 0.0]) / math.sqrt(2 + math.sqrt(2)), abs=1e-15)
+    R = Rotation.from_quat(q).as_matrix()
+    assert R[:, 0] == pytest.approx([a, a, 0], abs=1e-15) and R[:, 2] == pytest.approx([0, 0, -1], abs=1e-15)
+
+
+def test_from_cov_round_trip():
+    # check-core.ts:429-435's round trip covFromScaleQuat(fromCov(c)), to the
+    # GPU solver's stopping rule: it drops off-diagonal entries within 1e-6
+    # of the trace (math.tsl.ts:49; the CPU twin of check-core stops at 1e-9)
+    rng = np.random.default_rng(5)
+    cov, back = np.zeros(6), np.zeros(6)
+    for _ in range(2000):
+        scales = np.exp(rng.random(3) * 9 - 6)
+        quat = rng.random(4) * 2 - 1
+        quat /= np.linalg.norm(quat)
+        O.lib().orc_cov_from_scale_quat(O.dptr(scales), O.dptr(quat), O.dptr(cov))
+        s, q = from_cov(cov)
+        assert list(s) == sorted(s, reverse=True) and q[3] >= 0
+        O.lib().orc_cov_from_scale_quat(O.dptr(np.ascontiguousarray(s)), O.dptr(q), O.dptr(back))
+        assert np.max(np.abs(back - cov)) <= 1e-6 * np.sum(cov[:3])
+        # the scales are the square roots of the eigenvalues
+        S = np.array([[cov[0], cov[3], cov[4]], [cov[3], cov[1], cov[5]], [cov[4], cov[5], cov[2]]])
+        ev = np.sort(np.linalg.eigvalsh(S))[::-1]
+        assert s ** 2 == pytest.approx(ev, abs=1e-6 * np.sum(cov[:3]))
+
+
+@pytest.mark.parametrize("method", ["3dgut", "htgs", "raygs"])
+def test_non_uniform_transform_through_the_covariance(method):
+    # transformSplat's general case (splatTransform.tsl.ts:154-165): a sphere of
+    # radius s under diag(2, 1, 1/2) has the head covariance diag(4, 1, 1/4) s^2,
+    # whose fromCov needs no rotation: scales (2s, s, s/2), identity. The words
+    # equal those of that ellipsoid under the pure translation.
+    s, t = 0.05, [0.4, -0.2, -6.0]
+    rig = cam.mono(np.eye(4), FOV, W, H, near=0.05, far=100)
+    M = cam.translation(t)
+    M[:3, :3] = np.diag([2.0, 1.0, 0.5])
+    a = O.project_ray(method, [0, 0, 0], [s] * 3, [0, 0, 0, 1], [0.5] * 3, 0.9, O.view_from_rig(rig, M), params())
+    b = O.project_ray(method, [0, 0, 0], [2 * s, s, s / 2], [0, 0, 0, 1], [0.5] * 3, 0.9,
+                      O.view_from_rig(rig, cam.translation(t)), params())
+    assert a.visible and list(a.words) == list(b.words)
+    assert list(a.head_scales) == pytest.approx(list(b.head_scales), rel=1e-15)
