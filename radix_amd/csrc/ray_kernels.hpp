@@ -71,6 +71,9 @@ __device__ __forceinline__ bool bound_from_cov(float a, float b, float d, float 
   const float sc1 = fminf(u.max_pr, std_dev * sqrtf(fmaxf(eig1, 0.0f)));
   const float sc2 = fminf(u.max_pr, std_dev * sqrtf(eig2));
   const bool ok = sc1 >= u.min_pr || sc2 >= u.min_pr;
+  bd.angle = 0.0f;
+  bd.s1 = sc1;
+  bd.s2 = sc2;
   if (ok) {
     float vx, vy;
     if (fabsf(a - eig1) >= fabsf(d - eig1)) {
@@ -90,18 +93,15 @@ __device__ __forceinline__ bool bound_from_cov(float a, float b, float d, float 
     }
     float ang = atan2f(vy, vx);
     bd.angle = ang < 0.0f ? ang + 3.14159265358979323846f : ang;
-    bd.s1 = sc1;
-    bd.s2 = sc2;
   }
   return ok;
 }
 
 // screenBound (projection.tsl.ts:243-248)
 template <typename U>
-__device__ __forceinline__ void screen_bound(const U& u, RayBound& bd) {
-  bd.nx = bd.ny = 0.0f;
-  bd.angle = 0.0f;
-  bd.s1 = bd.s2 = fminf(u.max_pr, sqrtf(u.size[0] * u.size[0] + u.size[1] * u.size[1]) * 0.5f);
+__device__ __forceinline__ RayBound screen_bound(const U& u) {
+  const float s = fminf(u.max_pr, sqrtf(u.size[0] * u.size[0] + u.size[1] * u.size[1]) * 0.5f);
+  return RayBound{0.0f, 0.0f, 0.0f, s, s};
 }
 
 // clipBoundToScreen (projection.tsl.ts:250-275)
@@ -167,7 +167,7 @@ __device__ __forceinline__ bool gut_bound(const float* P, const float* size, flo
     py[i] = cl.y / cl.w * hy;
   }
   if (!front) {
-    screen_bound(u, bd);
+    bd = screen_bound(u);
     return true;
   }
   float mx = 0.0f, my = 0.0f;
@@ -185,9 +185,11 @@ __device__ __forceinline__ bool gut_bound(const float* P, const float* size, flo
     b += dx * dy * (1.0f / 6.0f);
     d += dy * dy * (1.0f / 6.0f);
   }
-  const bool ok = bound_from_cov(a, b, d, std_dev, u, bd);
-  bd.nx = mx / hx;
-  bd.ny = my / hy;
+  RayBound r;
+  r.nx = mx / hx;
+  r.ny = my / hy;
+  const bool ok = bound_from_cov(a, b, d, std_dev, u, r);
+  bd = r;
   return ok;
 }
 
@@ -235,12 +237,14 @@ __device__ __forceinline__ bool htgs_bound(const float* P, const float* size, fl
     valid = a > 0.0f && d > 0.0f && a * d - b * b > 0.0f;
   }
   if (!valid) {
-    screen_bound(u, bd);
+    bd = screen_bound(u);
     return true;
   }
-  const bool ok = bound_from_cov(a, b, d, 1.0f, u, bd);
-  bd.nx = cx;
-  bd.ny = cy;
+  RayBound r;
+  r.nx = cx;
+  r.ny = cy;
+  const bool ok = bound_from_cov(a, b, d, 1.0f, u, r);
+  bd = r;
   return ok;
 }
 
@@ -663,7 +667,7 @@ __global__ __launch_bounds__(256) void k_project_ray(TableP t, RayU r, RayOut o)
       if constexpr (METHOD == METHOD_RAYGS) {
         if (raygs_eye(E, c, eq, S, std_dev, w + HEAD + EW * e)) visible = true;
       } else {
-        RayBound bd;
+        RayBound bd = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         bool bok;
         if constexpr (METHOD == METHOD_3DGUT) {
           bok = gut_bound(E.Pf, u.size, c, eq, S, std_dev, u, bd);
