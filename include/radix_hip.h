@@ -399,10 +399,31 @@ const uint32_t* rdx_batch_keys(const rdx_batch* batch);
  * out_indices_dev + off_v, its count at out_counts_dev[v] and its draw
  * arguments at out_draw_args_dev + 5 v (either may be NULL). flags
  * RDX_BATCH_GRAPH: capture the sequence once into a HIP graph (per output
- * set) and replay it. */
+ * set and form) and replay it. RDX_BATCH_PERSISTENT: the whole batch in one
+ * launch of a persistent kernel (the projection and every radix pass of
+ * every segment, with grid-wide barriers between the phases), available
+ * when the batch's tables are f32 tables of one SH degree and it was
+ * created under the 8-bit, 12-key, 512-thread tuning with 32-bit keys
+ * (RDX_E_UNSUPPORTED otherwise); the same output as the launch sequence.
+ * Runs of one batch must not overlap on the device. */
 #define RDX_BATCH_GRAPH 1
+#define RDX_BATCH_PERSISTENT 2
 int rdx_batch_run(rdx_batch* batch, void* stream, uint32_t* out_indices_dev, uint32_t* out_counts_dev,
                   uint32_t* out_draw_args_dev, int flags);
+/* The persistent form's grid (blocks resident at once, a multiple of 8),
+ * or 0 when the batch has no persistent form. */
+int rdx_batch_persistent_grid(const rdx_batch* batch);
+/* Waits for the batch's last run; RDX_E_HIP (and clears the flag) if a
+ * persistent run's barrier wait ran past its limit of about 1.3 s (a grid
+ * whose blocks were not all resident at once: that run's output is not
+ * valid). */
+int rdx_batch_status(rdx_batch* batch);
+/* Times `reps` graph replays of the batch in each form it has (the launch
+ * sequence, and the persistent launch when available) on `stream` and
+ * reports the faster in *persistent_out (1: the persistent launch) with
+ * both times (ms_persistent -1 without that form). Synchronous. */
+int rdx_batch_choose_form(rdx_batch* batch, void* stream, int reps, int* persistent_out, double* ms_graph,
+                          double* ms_persistent);
 /* The autotuner of a batch (no reference counterpart: the reference fixes
  * 8-bit digits and its keys per thread, WgpuRadixSort.ts:13-15, 80-95):
  * for every supported tuning (rdx_sort_tuning_candidate) it creates the

@@ -127,6 +127,7 @@ class DrawSettings(C.Structure):
 
 
 BATCH_GRAPH = 1
+BATCH_PERSISTENT = 2
 TUNE_CACHED = 1
 STATIC_SORT_MAX_KEYS = 1048576
 BATCH_MAX_VIEWS = 4096
@@ -197,6 +198,10 @@ _SIGNATURES = [
     ("rdx_batch_segment_offset", C.c_uint64, [C.c_void_p, C.c_int]),
     ("rdx_batch_keys", C.c_void_p, [C.c_void_p]),
     ("rdx_batch_run", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    ("rdx_batch_persistent_grid", C.c_int, [C.c_void_p]),
+    ("rdx_batch_status", C.c_int, [C.c_void_p]),
+    ("rdx_batch_choose_form", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("rdx_raster_create", C.c_int, [C.c_void_p, C.POINTER(RasterSettings), C.POINTER(C.c_void_p)]),
     ("rdx_raster_destroy", C.c_int, [C.c_void_p]),
     ("rdx_raster_forward", C.c_int,

@@ -76,6 +76,7 @@ class Batch:
         self.indices = torch.zeros(max(self.total, 1), dtype=torch.int32, device=dev)
         self.counts = torch.zeros(len(self.views), dtype=torch.int32, device=dev)
         self.args = torch.zeros(5 * len(self.views), dtype=torch.int32, device=dev)
+        self.persistent = False
 
     def segment(self, v: int) -> Tuple[int, int]:
         """(offset, splat count) of view v's segment."""
@@ -86,10 +87,36 @@ class Batch:
         N.check(N.lib().rdx_batch_set_cameras(self._h, cams, len(rigs)))
         self.views = [(t, r) for (t, _), r in zip(self.views, rigs)]
 
-    def run(self, graph: bool = True, stream=None) -> None:
+    def run(self, graph: bool = True, stream=None, persistent: Optional[bool] = None) -> None:
+        """persistent: True runs the whole batch as one launch of the
+        persistent kernel (f32 tables of one SH degree, tuning (8, 12, 512));
+        None takes self.persistent (False unless choose_form() picked it)."""
+        if persistent is None:
+            persistent = self.persistent
+        flags = (N.BATCH_GRAPH if graph else 0) | (N.BATCH_PERSISTENT if persistent else 0)
         N.check(N.lib().rdx_batch_run(self._h, _stream(stream), C.c_void_p(self.indices.data_ptr()),
-                                      C.c_void_p(self.counts.data_ptr()), C.c_void_p(self.args.data_ptr()),
-                                      N.BATCH_GRAPH if graph else 0))
+                                      C.c_void_p(self.counts.data_ptr()), C.c_void_p(self.args.data_ptr()), flags))
+
+    @property
+    def persistent_grid(self) -> int:
+        """Blocks of the persistent launch (0: this batch has no persistent form)."""
+        return int(N.lib().rdx_batch_persistent_grid(self._h))
+
+    def status(self) -> None:
+        """Waits for the last run; raises if a persistent run's barrier wait
+        ran out (its output is then not valid)."""
+        N.check(N.lib().rdx_batch_status(self._h))
+
+    def choose_form(self, reps: int = 5, stream=None) -> Tuple[bool, float, Optional[float]]:
+        """Times graph replays of the launch sequence and of the persistent
+        launch (when this batch has one) and keeps the faster in
+        self.persistent: (persistent, ms of the sequence, ms persistent or None)."""
+        p = C.c_int()
+        mg = C.c_double()
+        mp = C.c_double()
+        N.check(N.lib().rdx_batch_choose_form(self._h, _stream(stream), reps, C.byref(p), C.byref(mg), C.byref(mp)))
+        self.persistent = bool(p.value)
+        return self.persistent, mg.value, (mp.value if mp.value >= 0 else None)
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -242,13 +242,10 @@ int small_segments(SortScratch& sc, hipStream_t s, const SortPlan& p, const uint
   return e == hipSuccess ? RDX_OK : hip_fail(e, "segment table");
 }
 
-int run_sort(Context* ctx, SortScratch& sc, hipStream_t s, const SortPlan& p, const SegLayout& L,
-             const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* final_keys, uint32_t* final_vals, uint32_t n_cap,
-             const uint32_t* n_dev, bool drop, uint32_t invalid, bool rule, uint32_t* count_out,
-             uint32_t* draw_args) {
+PassArgs pass_args(SortScratch& sc, const SortPlan& p, const SegLayout& L, int pass, const uint32_t* kin,
+                   const uint32_t* vin, uint32_t* final_keys, uint32_t* final_vals, uint32_t n_cap, const uint32_t* n_dev,
+                   bool drop, uint32_t invalid, bool rule, uint32_t* count_out, uint32_t* draw_args) {
   const int bins = 1 << p.rb;
-  const uint32_t* kin = keys_in;
-  const uint32_t* vin = vals_in;
   // the ranged sort's words (ranged_words): tile and segment ranges, the
   // gated pass's done tickets, tile rows and digit totals
   const size_t nseg = std::max<uint32_t>(L.segments, 1);
@@ -257,67 +254,79 @@ int run_sort(Context* ctx, SortScratch& sc, hipStream_t s, const SortPlan& p, co
   uint32_t* done = seg_mm + 2 * nseg;
   uint32_t* cnt_g = done + nseg;
   uint32_t* tot_g = cnt_g + (size_t)GATED_BINS * L.tiles;
-  for (int pass = 0; pass < p.passes; pass++) {
-    const bool last = pass == p.passes - 1;
-    PassArgs a;
-    memset(&a, 0, sizeof a);
-    a.keys_in = kin;
-    a.vals_in = vin;
-    a.keys_out = last ? final_keys : (uint32_t*)sc.keys[pass & 1].ptr;
-    a.vals_out = last ? final_vals : (uint32_t*)sc.vals[pass & 1].ptr;
-    a.n_dev = n_dev;
-    a.n = n_cap;
-    a.cnt = (uint32_t*)sc.cnt.ptr;
-    a.tot = (uint32_t*)sc.tot.ptr;
-    a.nvalid = (uint32_t*)sc.nvalid.ptr;
-    a.tiles = L.tiles;
-    a.rows = bins + (rule && pass == 0 ? 1 : 0);
-    a.cstride = count_stride(a.rows);
-    a.shift = pass * p.rb;
-    a.key_mask = key_invalid(p.bits);
-    a.invalid = invalid;
-    a.drop = pass == 0 && drop;
-    a.compacted = pass > 0 && drop;
-    a.bits = p.bits;
-    a.count_out = last ? count_out : nullptr;
-    a.draw_args = last ? draw_args : nullptr;
-    a.seg_of_tile = L.seg_of_tile;
-    a.seg = L.seg;
-    a.dst_cap = n_cap;
-    a.tile_keys = (uint32_t)p.tile;
-    // the top digit (skewed for distance keys) peels its common digits by ballots
-    a.peel = pass == p.passes - 1 || (p.ranged && pass == 2);
-    if (p.ranged) {
-      // pass 0 -> scratch 0, 1 -> scratch 1, 2 -> the output (scratch 0 for
-      // segments whose keys reach beyond three passes), 3 (those segments
-      // only): scratch 0 -> the output. The kept count is final after pass 0.
-      a.ranged = pass == 0 ? 1 : 2;
-      a.tile_mm = tile_mm;
-      a.seg_mm = seg_mm;
-      a.gate = pass == 3;
-      a.alt = pass == 2;
-      a.zero_g = pass == 1;
-      a.cnt_g = cnt_g;
-      a.tot_g = tot_g;
-      a.done = done;
-      if (pass < 2) {
-        a.keys_out = (uint32_t*)sc.keys[pass].ptr;
-        a.vals_out = (uint32_t*)sc.vals[pass].ptr;
-      } else {
-        a.keys_out = final_keys;
-        a.vals_out = final_vals;
-      }
-      if (pass == 2) {
-        a.keys_alt = (uint32_t*)sc.keys[0].ptr;
-        a.vals_alt = (uint32_t*)sc.vals[0].ptr;
-      }
-      if (pass == 3) {
-        a.keys_in = (const uint32_t*)sc.keys[0].ptr;
-        a.vals_in = (const uint32_t*)sc.vals[0].ptr;
-      }
-      a.count_out = pass == 0 ? count_out : nullptr;
-      a.draw_args = pass == 0 ? draw_args : nullptr;
+  const bool last = pass == p.passes - 1;
+  PassArgs a;
+  memset(&a, 0, sizeof a);
+  a.keys_in = kin;
+  a.vals_in = vin;
+  a.keys_out = last ? final_keys : (uint32_t*)sc.keys[pass & 1].ptr;
+  a.vals_out = last ? final_vals : (uint32_t*)sc.vals[pass & 1].ptr;
+  a.n_dev = n_dev;
+  a.n = n_cap;
+  a.cnt = (uint32_t*)sc.cnt.ptr;
+  a.tot = (uint32_t*)sc.tot.ptr;
+  a.nvalid = (uint32_t*)sc.nvalid.ptr;
+  a.tiles = L.tiles;
+  a.rows = bins + (rule && pass == 0 ? 1 : 0);
+  a.cstride = count_stride(a.rows);
+  a.shift = pass * p.rb;
+  a.key_mask = key_invalid(p.bits);
+  a.invalid = invalid;
+  a.drop = pass == 0 && drop;
+  a.compacted = pass > 0 && drop;
+  a.bits = p.bits;
+  a.count_out = last ? count_out : nullptr;
+  a.draw_args = last ? draw_args : nullptr;
+  a.seg_of_tile = L.seg_of_tile;
+  a.seg = L.seg;
+  a.dst_cap = n_cap;
+  a.tile_keys = (uint32_t)p.tile;
+  // the top digit (skewed for distance keys) peels its common digits by ballots
+  a.peel = pass == p.passes - 1 || (p.ranged && pass == 2);
+  if (p.ranged) {
+    // pass 0 -> scratch 0, 1 -> scratch 1, 2 -> the output (scratch 0 for
+    // segments whose keys reach beyond three passes), 3 (those segments
+    // only): scratch 0 -> the output. The kept count is final after pass 0.
+    a.ranged = pass == 0 ? 1 : 2;
+    a.tile_mm = tile_mm;
+    a.seg_mm = seg_mm;
+    a.gate = pass == 3;
+    a.alt = pass == 2;
+    a.zero_g = pass == 1;
+    a.cnt_g = cnt_g;
+    a.tot_g = tot_g;
+    a.done = done;
+    if (pass < 2) {
+      a.keys_out = (uint32_t*)sc.keys[pass].ptr;
+      a.vals_out = (uint32_t*)sc.vals[pass].ptr;
+    } else {
+      a.keys_out = final_keys;
+      a.vals_out = final_vals;
     }
+    if (pass == 2) {
+      a.keys_alt = (uint32_t*)sc.keys[0].ptr;
+      a.vals_alt = (uint32_t*)sc.vals[0].ptr;
+    }
+    if (pass == 3) {
+      a.keys_in = (const uint32_t*)sc.keys[0].ptr;
+      a.vals_in = (const uint32_t*)sc.vals[0].ptr;
+    }
+    a.count_out = pass == 0 ? count_out : nullptr;
+    a.draw_args = pass == 0 ? draw_args : nullptr;
+  }
+  return a;
+}
+
+int run_sort(Context* ctx, SortScratch& sc, hipStream_t s, const SortPlan& p, const SegLayout& L,
+             const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* final_keys, uint32_t* final_vals, uint32_t n_cap,
+             const uint32_t* n_dev, bool drop, uint32_t invalid, bool rule, uint32_t* count_out,
+             uint32_t* draw_args) {
+  const int bins = 1 << p.rb;
+  const uint32_t* kin = keys_in;
+  const uint32_t* vin = vals_in;
+  for (int pass = 0; pass < p.passes; pass++) {
+    const PassArgs a = pass_args(sc, p, L, pass, kin, vin, final_keys, final_vals, n_cap, n_dev, drop, invalid, rule,
+                                 count_out, draw_args);
     char name[32];
     // the gated pass is a scatter alone: the third pass counted and scanned its digits
     if (!a.gate) {

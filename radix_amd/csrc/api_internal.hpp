@@ -138,6 +138,12 @@ uint32_t tiles_of(const SortPlan& p, uint32_t n);
 constexpr int SMALL_SEGMENTS = 8;
 int small_segments(SortScratch& sc, hipStream_t s, const SortPlan& p, const uint32_t* off, const uint32_t* n, int count,
                    SegLayout* out);
+// The arguments of digit pass `pass` of the sort run_sort runs: kin / vin
+// are the pass's input (the sort's input for pass 0, the previous pass's
+// keys_out / vals_out after it).
+PassArgs pass_args(SortScratch& sc, const SortPlan& p, const SegLayout& L, int pass, const uint32_t* kin,
+                   const uint32_t* vin, uint32_t* final_keys, uint32_t* final_vals, uint32_t n_cap, const uint32_t* n_dev,
+                   bool drop, uint32_t invalid, bool rule, uint32_t* count_out, uint32_t* draw_args);
 // The P digit passes: pass 0 reads (keys_in, vals_in or the index), the last
 // writes (final_keys, final_vals); between them the scratch pairs alternate.
 // rule: count the host sort's hidden keys into tot[s][MAX_BINS].

@@ -22,7 +22,8 @@ namespace rdx {
 struct BatchKind {
   int layout, shc, deg;  // deg: the f32 tables' degree, -1 for word tables
   std::vector<BatchGroup> groups;
-  uint32_t blocks = 0;
+  uint32_t blocks = 0;   // 256-splat blocks of the projection launch
+  uint32_t chunks = 0;   // PERSIST_BLOCK-splat chunks of the persistent launch
   Buffer d_groups;
 };
 
@@ -49,6 +50,12 @@ struct Batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::tuple<uint32_t*, uint32_t*, uint32_t*> graph_outputs{nullptr, nullptr, nullptr};
+  bool graph_persistent = false;
+  // the persistent form (one kind, the ranged sort plan): its grid and its
+  // barrier words [arrivals, finished blocks, error]
+  bool persist_ok = false;
+  uint32_t persist_grid = 0;
+  Buffer d_sync;
   hipEvent_t last_run = nullptr;
   bool ran = false;
 
@@ -67,6 +74,7 @@ struct Batch {
     d_deg.release();
     d_enc.release();
     d_seg.release();
+    d_sync.release();
     keys_own.release();
     scratch.release();
   }
@@ -127,18 +135,77 @@ static bool launch_batch_kind(hipStream_t s, const BatchKind& k, const BatchDesc
   return true;
 }
 
+template <int LAYOUT, int SHC, int DEG>
+static const void* persist_fn() {
+  return reinterpret_cast<const void*>(&k_batch_persistent<LAYOUT, SHC, DEG>);
+}
+
+// The persistent kernel of a kind: f32 tables only. The word layouts'
+// projection (their codecs and SH bundles held across the views) needs more
+// than the 128 VGPRs a thread has at two 512-thread blocks per CU; it would
+// spill to scratch, so those tables take the launch sequence.
+static const void* persistent_kernel(const BatchKind& k) {
+  if (k.layout == LAYOUT_F32 && k.shc == SHC_F32) {
+    switch (k.deg) {
+      case 0: return persist_fn<LAYOUT_F32, SHC_F32, 0>();
+      case 1: return persist_fn<LAYOUT_F32, SHC_F32, 1>();
+      case 2: return persist_fn<LAYOUT_F32, SHC_F32, 2>();
+      case 3: return persist_fn<LAYOUT_F32, SHC_F32, 3>();
+    }
+    return nullptr;
+  }
+  if (k.layout == LAYOUT_F32 && k.shc == SHC_F32_WIDE)
+    return k.deg == 2 ? persist_fn<LAYOUT_F32, SHC_F32_WIDE, 2>() : persist_fn<LAYOUT_F32, SHC_F32_WIDE, 3>();
+  return nullptr;
+}
+
+static BatchDesc batch_desc(const Batch* b, const BatchKind& k) {
+  BatchDesc d;
+  d.views = (const BatchView*)b->d_views.ptr;
+  d.groups = (const BatchGroup*)k.d_groups.ptr;
+  d.tables = (const TableP*)b->d_tables.ptr;
+  d.table_sh_degree = (const int*)b->d_deg.ptr;
+  d.table_enc = (const float*)b->d_enc.ptr;
+  d.num_groups = (int)k.groups.size();
+  d.num_segments = (uint32_t)b->views.size();
+  return d;
+}
+
+// The whole batch in one launch of the persistent kernel.
+static int run_batch_persistent(Batch* b, hipStream_t s, uint32_t* out_indices, uint32_t* out_counts,
+                                uint32_t* out_args) {
+  const BatchKind& k = b->kinds[0];
+  PersistArgs P;
+  memset(&P, 0, sizeof P);
+  const uint32_t* kin = b->keys;
+  const uint32_t* vin = nullptr;
+  for (int pass = 0; pass < 4; pass++) {
+    P.pass[pass] = pass_args(b->scratch, b->plan, b->layout, pass, kin, vin, nullptr, out_indices, b->total, nullptr,
+                             true, key_invalid(b->params.sort_bits), false, out_counts, out_args);
+    kin = P.pass[pass].keys_out;
+    vin = P.pass[pass].vals_out;
+  }
+  P.items = k.chunks;
+  P.blocks = b->persist_grid;
+  P.bar = (uint32_t*)b->d_sync.ptr;
+  P.err = P.bar + 2;
+  BatchDesc d = batch_desc(b, k);
+  BatchU u = b->u;
+  uint32_t* keys = b->keys;
+  void* args[] = {&d, &u, &keys, &P};
+  Stage st(b->ctx, s, "batch_persistent");
+  hipError_t e = hipLaunchKernel(persistent_kernel(k), dim3(b->persist_grid), dim3(PERSIST_BLOCK), args, 0, s);
+  return e == hipSuccess ? RDX_OK : hip_fail(e, "batch persistent launch");
+}
+
 // The whole batch on one stream: projections, then the segmented sort.
-static int run_batch(Batch* b, hipStream_t s, uint32_t* out_indices, uint32_t* out_counts, uint32_t* out_args) {
+static int run_batch(Batch* b, hipStream_t s, uint32_t* out_indices, uint32_t* out_counts, uint32_t* out_args,
+                     bool persistent) {
+  if (persistent) return run_batch_persistent(b, s, out_indices, out_counts, out_args);
   {
     Stage st(b->ctx, s, "batch_project");
     for (auto& k : b->kinds) {
-      BatchDesc d;
-      d.views = (const BatchView*)b->d_views.ptr;
-      d.groups = (const BatchGroup*)k.d_groups.ptr;
-      d.tables = (const TableP*)b->d_tables.ptr;
-      d.table_sh_degree = (const int*)b->d_deg.ptr;
-      d.table_enc = (const float*)b->d_enc.ptr;
-      d.num_groups = (int)k.groups.size();
+      const BatchDesc d = batch_desc(b, k);
       if (!launch_batch_kind(s, k, d, b->u, b->keys))
         return fail(RDX_E_INVALID, "batch: no kernel for layout %d", k.layout);
     }
@@ -234,6 +301,7 @@ int rdx_batch_create(rdx_context* ctx, const rdx_splat_table* tables, int num_ta
       g.first_view = (uint32_t)b->host_views.size();
       g.num_views = 0;
       g.block0 = k.blocks;
+      g.chunk0 = k.chunks;
       while (i < vs.size() && views[vs[i]].table == t && g.num_views < (uint32_t)BATCH_MAX_VIEWS) {
         b->slot_of_view[vs[i]] = (int)b->host_views.size();
         b->host_views.push_back(BatchView());
@@ -241,6 +309,7 @@ int rdx_batch_create(rdx_context* ctx, const rdx_splat_table* tables, int num_ta
         i++;
       }
       k.blocks += std::max<uint32_t>(1u, (tables[t].count + 255u) / 256u);
+      k.chunks += std::max<uint32_t>(1u, (tables[t].count + PERSIST_BLOCK - 1u) / PERSIST_BLOCK);
       k.groups.push_back(g);
     }
     b->kinds.push_back(std::move(k));
@@ -301,6 +370,25 @@ int rdx_batch_create(rdx_context* ctx, const rdx_splat_table* tables, int num_ta
   }
   hipError_t e = hipEventCreateWithFlags(&b->last_run, hipEventDisableTiming);
   if (e != hipSuccess) return bail(hip_fail(e, "batch event"));
+  // The persistent form: one kind of table and the ranged sort plan; a grid
+  // of the blocks resident at once, a multiple of the 8 XCDs.
+  if (b->kinds.size() == 1 && b->plan.ranged && b->plan.block == PERSIST_BLOCK && b->plan.kpt == PERSIST_KPT &&
+      persistent_kernel(b->kinds[0])) {
+    int dev = 0, cus = 0, per_cu = 0;
+    e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_kernel(b->kinds[0]), PERSIST_BLOCK, 0);
+    if (e != hipSuccess) return bail(hip_fail(e, "batch persistent occupancy"));
+    const uint32_t grid = (uint32_t)(cus * per_cu) / XCDS * XCDS;
+    if (grid >= XCDS) {
+      if ((rc = b->d_sync.ensure(4 * sizeof(uint32_t)))) return bail(rc);
+      e = hipMemset(b->d_sync.ptr, 0, 4 * sizeof(uint32_t));
+      if (e != hipSuccess) return bail(hip_fail(e, "batch persistent barrier"));
+      b->persist_grid = grid;
+      b->persist_ok = true;
+    }
+  }
   *out = reinterpret_cast<rdx_batch*>(b);
   return RDX_OK;
 }
@@ -357,9 +445,14 @@ int rdx_batch_run(rdx_batch* batch, void* stream, uint32_t* out_indices, uint32_
   Batch* b = reinterpret_cast<Batch*>(batch);
   hipStream_t s = (hipStream_t)stream;
   hipError_t e;
+  const bool persistent = (flags & RDX_BATCH_PERSISTENT) != 0;
+  if (persistent && !b->persist_ok)
+    return fail(RDX_E_UNSUPPORTED,
+                "rdx_batch_run: the persistent form needs f32 tables of one SH degree and the 8-bit, 12-key, 512-thread "
+                "tuning at 32-bit keys");
   if (flags & RDX_BATCH_GRAPH) {
     const auto outs = std::make_tuple(out_indices, out_counts, out_args);
-    if (!b->exec || b->graph_outputs != outs) {
+    if (!b->exec || b->graph_outputs != outs || b->graph_persistent != persistent) {
       if (b->exec) (void)hipGraphExecDestroy(b->exec);
       if (b->graph) (void)hipGraphDestroy(b->graph);
       b->exec = nullptr;
@@ -376,7 +469,7 @@ int rdx_batch_run(rdx_batch* batch, void* stream, uint32_t* out_indices, uint32_
         b->ctx->profiling = prof;
         return hip_fail(e, "batch capture");
       }
-      const int rc = run_batch(b, b->capture_stream, out_indices, out_counts, out_args);
+      const int rc = run_batch(b, b->capture_stream, out_indices, out_counts, out_args, persistent);
       e = hipStreamEndCapture(b->capture_stream, &b->graph);
       b->ctx->profiling = prof;
       if (rc) return rc;
@@ -384,16 +477,89 @@ int rdx_batch_run(rdx_batch* batch, void* stream, uint32_t* out_indices, uint32_
       e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
       if (e != hipSuccess) return hip_fail(e, "batch graph instantiate");
       b->graph_outputs = outs;
+      b->graph_persistent = persistent;
     }
     Stage st(b->ctx, s, "batch_graph");
     e = hipGraphLaunch(b->exec, s);
     if (e != hipSuccess) return hip_fail(e, "batch graph launch");
   } else {
-    if (int rc = run_batch(b, s, out_indices, out_counts, out_args)) return rc;
+    if (int rc = run_batch(b, s, out_indices, out_counts, out_args, persistent)) return rc;
   }
   e = hipEventRecord(b->last_run, s);
   if (e != hipSuccess) return hip_fail(e, "rdx_batch_run");
   b->ran = true;
+  return RDX_OK;
+}
+
+int rdx_batch_persistent_grid(const rdx_batch* batch) {
+  if (!batch) return 0;
+  const Batch* b = reinterpret_cast<const Batch*>(batch);
+  return b->persist_ok ? (int)b->persist_grid : 0;
+}
+
+int rdx_batch_status(rdx_batch* batch) {
+  if (!batch) return fail(RDX_E_INVALID, "rdx_batch_status: NULL batch");
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  if (b->ran) {
+    hipError_t e = hipEventSynchronize(b->last_run);
+    if (e != hipSuccess) return hip_fail(e, "rdx_batch_status");
+  }
+  if (!b->persist_ok) return RDX_OK;
+  uint32_t err = 0;
+  hipError_t e = hipMemcpy(&err, (uint32_t*)b->d_sync.ptr + 2, sizeof err, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "rdx_batch_status");
+  if (!err) return RDX_OK;
+  const uint32_t zero = 0;
+  e = hipMemcpy((uint32_t*)b->d_sync.ptr + 2, &zero, sizeof zero, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "rdx_batch_status");
+  return fail(RDX_E_HIP, "batch: a persistent run's grid barrier waited past its limit (blocks not all resident); "
+                         "its output is not valid");
+}
+
+int rdx_batch_choose_form(rdx_batch* batch, void* stream, int reps, int* persistent_out, double* ms_graph,
+                          double* ms_persistent) {
+  if (!batch) return fail(RDX_E_INVALID, "rdx_batch_choose_form: NULL batch");
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  reps = std::max(1, std::min(reps, 100));
+  hipStream_t s = (hipStream_t)stream;
+  Buffer idx, counts, args;
+  int rc = RDX_OK;
+  if ((rc = idx.ensure((size_t)std::max<uint32_t>(b->total, 1) * 4)) || (rc = counts.ensure(b->views.size() * 4)) ||
+      (rc = args.ensure(b->views.size() * 20))) {
+    idx.release();
+    counts.release();
+    args.release();
+    return rc;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  double ms[2] = {1e30, 1e30};
+  for (int form = 0; form < (b->persist_ok ? 2 : 1) && rc == RDX_OK && e == hipSuccess; form++) {
+    const int flags = RDX_BATCH_GRAPH | (form ? RDX_BATCH_PERSISTENT : 0);
+    uint32_t* ip = (uint32_t*)idx.ptr;
+    uint32_t* cp = (uint32_t*)counts.ptr;
+    uint32_t* ap = (uint32_t*)args.ptr;
+    rc = rdx_batch_run(batch, stream, ip, cp, ap, flags);  // capture, and one replay to warm
+    if (rc == RDX_OK) e = hipEventRecord(e0, s);
+    for (int r = 0; r < reps && rc == RDX_OK && e == hipSuccess; r++) rc = rdx_batch_run(batch, stream, ip, cp, ap, flags);
+    if (rc == RDX_OK && e == hipSuccess) e = hipEventRecord(e1, s);
+    if (rc == RDX_OK && e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.0f;
+    if (rc == RDX_OK && e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (rc == RDX_OK && e == hipSuccess) ms[form] = t / reps;
+    if (rc == RDX_OK && e == hipSuccess && form == 1) rc = rdx_batch_status(batch);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  idx.release();
+  counts.release();
+  args.release();
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_fail(e, "rdx_batch_choose_form");
+  if (persistent_out) *persistent_out = ms[1] < ms[0] ? 1 : 0;
+  if (ms_graph) *ms_graph = ms[0];
+  if (ms_persistent) *ms_persistent = b->persist_ok ? ms[1] : -1.0;
   return RDX_OK;
 }
 

@@ -44,9 +44,9 @@ __device__ __forceinline__ uint32_t wave_read_lane(uint32_t x, int src) {
 // `scratch` holds BLOCK/64 words of LDS. Returns the exclusive sum and the
 // block total in *total. Contains two barriers.
 template <int BLOCK>
-__device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t x, uint32_t* scratch, uint32_t* total) {
+__device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t x, uint32_t* scratch, uint32_t* total,
+                                                        int tid = (int)threadIdx.x) {
   constexpr int WAVES = BLOCK / RDX_WAVE;
-  const int tid = threadIdx.x;
   const int w = tid / RDX_WAVE;
   uint32_t inc = wave_inclusive_sum(x);
   if ((tid & 63) == 63) scratch[w] = inc;

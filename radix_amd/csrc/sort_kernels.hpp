@@ -237,23 +237,37 @@ __device__ __forceinline__ void gated_scan(const PassArgs& a, const SegInfo& g, 
 // (by lane), so a round whose keys share a digit (the top digit of
 // clustered distances) contends SUBS ways less on one address; the rows
 // (padded by one word: distinct banks) are summed at the end.
+template <int RB, int BLOCK>
+struct CountLds {
+  static constexpr int BINS = 1 << RB;
+  static constexpr int WAVES = BLOCK / RDX_WAVE;
+  static constexpr int SUBS = BINS >= 512 ? 2 : 4;
+  static constexpr int ROW = BINS + 1;
+  uint32_t rows[WAVES * SUBS * ROW];
+  uint32_t rule[WAVES];
+  uint32_t mm[2];
+};
+
+// One tile's counts by a block of BLOCK threads (the count kernel's block,
+// or a persistent block taking tiles in turn; L is free on entry and may be
+// reused when it returns).
 template <int RB, int BLOCK, int KPT>
-__global__ __launch_bounds__(BLOCK) void k_count(PassArgs a) {
-  constexpr int BINS = 1 << RB;
+__device__ __forceinline__ void count_tile(const PassArgs& a, uint32_t tile, CountLds<RB, BLOCK>& L) {
+  using CL = CountLds<RB, BLOCK>;
+  constexpr int BINS = CL::BINS;
   constexpr uint32_t DMASK = BINS - 1;
-  constexpr int WAVES = BLOCK / RDX_WAVE;
+  constexpr int WAVES = CL::WAVES;
   constexpr int WTILE = RDX_WAVE * KPT;
   constexpr int TILE = BLOCK * KPT;
-  constexpr int SUBS = BINS >= 512 ? 2 : 4;
-  constexpr int ROW = BINS + 1;
-  __shared__ uint32_t s_rows[WAVES * SUBS * ROW];
-  __shared__ uint32_t s_rule[WAVES];
-  __shared__ uint32_t s_mm[2];
+  constexpr int SUBS = CL::SUBS;
+  constexpr int ROW = CL::ROW;
+  uint32_t* s_rows = L.rows;
+  uint32_t* s_rule = L.rule;
+  uint32_t* s_mm = L.mm;
 
   const int tid = threadIdx.x;
   const int w = tid / RDX_WAVE;
   const int lane = tid & (RDX_WAVE - 1);
-  const uint32_t tile = blockIdx.x;
   for (int d = tid; d < WAVES * SUBS * ROW; d += BLOCK) s_rows[d] = 0u;
   if (tid < WAVES) s_rule[tid] = 0u;
   if (tid == 0) {
@@ -324,6 +338,12 @@ __global__ __launch_bounds__(BLOCK) void k_count(PassArgs a) {
   }
 }
 
+template <int RB, int BLOCK, int KPT>
+__global__ __launch_bounds__(BLOCK) void k_count(PassArgs a) {
+  __shared__ CountLds<RB, BLOCK> L;
+  count_tile<RB, BLOCK, KPT>(a, blockIdx.x, L);
+}
+
 // ---- scan ----------------------------------------------------------------------
 
 // Block (s, g): for the CGROUP digit columns [g * CGROUP, ...) of segment
@@ -333,18 +353,35 @@ __global__ __launch_bounds__(BLOCK) void k_count(PassArgs a) {
 // (c, d) loads column d of the c-th run of ITEMS tiles (the 16 threads
 // of a run read one 64-byte line per tile); wave d scans column d's 64 run
 // sums (one per lane) through LDS; the thread writes its run's prefixes.
-template <int ITEMS>
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan(PassArgs a, int bins) {
-  static_assert(SCAN_CHUNKS == RDX_WAVE && SCAN_BLOCK / RDX_WAVE == CGROUP, "a wave per column, a lane per run");
-  __shared__ uint32_t s_sum[SCAN_CHUNKS][CGROUP];
-  __shared__ uint32_t s_tot[CGROUP];
-  const uint32_t groups = (uint32_t)(a.rows + CGROUP - 1) / CGROUP;
-  const uint32_t per_seg = groups + (a.ranged == 1 ? 1u : 0u);  // + the range reduction
-  const uint32_t s = blockIdx.x / per_seg;
-  const int g = (int)(blockIdx.x % per_seg);
-  const int d = threadIdx.x % CGROUP;
-  const int c = threadIdx.x / CGROUP;
-  const int r = g * CGROUP + d;
+// A scan block of NT threads takes NT / 64 columns: a wave per column, a
+// lane per run of tiles (the scan kernel: 1024 threads, CGROUP columns).
+template <int NT>
+struct ScanLds {
+  static constexpr int CG = NT / RDX_WAVE;
+  uint32_t sum[SCAN_CHUNKS][CG];
+  uint32_t tot[CG];
+};
+
+// Scan items per segment: the column groups, and the range reduction in a
+// ranged sort's first pass.
+template <int NT>
+__host__ __device__ __forceinline__ uint32_t scan_items_per_seg(int rows, int ranged) {
+  return (uint32_t)(rows + ScanLds<NT>::CG - 1) / ScanLds<NT>::CG + (ranged == 1 ? 1u : 0u);
+}
+
+template <int ITEMS, int NT>
+__device__ __forceinline__ void scan_item(const PassArgs& a, int bins, uint32_t item, ScanLds<NT>& L) {
+  constexpr int CG = ScanLds<NT>::CG;
+  static_assert(NT / CG == SCAN_CHUNKS && NT / RDX_WAVE == CG, "a wave per column, a lane per run");
+  auto& s_sum = L.sum;
+  auto& s_tot = L.tot;
+  const uint32_t groups = (uint32_t)(a.rows + CG - 1) / CG;
+  const uint32_t per_seg = scan_items_per_seg<NT>(a.rows, a.ranged);
+  const uint32_t s = item / per_seg;
+  const int g = (int)(item % per_seg);
+  const int d = threadIdx.x % CG;
+  const int c = threadIdx.x / CG;
+  const int r = g * CG + d;
   uint32_t tile0 = 0, ntiles = a.tiles;
   if (a.seg) {
     tile0 = a.seg[4 * s];
@@ -358,7 +395,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(PassArgs a, int bins) {
   if (g == (int)groups) {
     // the segment's smallest and largest kept key
     uint32_t mn = 0xffffffffu, mx = 0u;
-    for (uint32_t t = threadIdx.x; t < ntiles; t += SCAN_BLOCK) {
+    for (uint32_t t = threadIdx.x; t < ntiles; t += NT) {
       mn = min(mn, a.tile_mm[2 * (size_t)(tile0 + t)]);
       mx = max(mx, a.tile_mm[2 * (size_t)(tile0 + t) + 1]);
     }
@@ -372,7 +409,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(PassArgs a, int bins) {
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      for (int v = 1; v < SCAN_BLOCK / RDX_WAVE; v++) {
+      for (int v = 1; v < NT / RDX_WAVE; v++) {
         mn = min(mn, s_sum[v][0]);
         mx = max(mx, s_sum[v][1]);
       }
@@ -419,6 +456,12 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(PassArgs a, int bins) {
   if (live && c == 0) a.tot[(size_t)s * TOT_STRIDE + (r < bins ? r : MAX_BINS)] = carry;
 }
 
+template <int ITEMS>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan(PassArgs a, int bins) {
+  __shared__ ScanLds<SCAN_BLOCK> L;
+  scan_item<ITEMS, SCAN_BLOCK>(a, bins, blockIdx.x, L);
+}
+
 // ---- scatter --------------------------------------------------------------------
 
 // One digit pass's write-out. Tile t covers input keys [t * TILE, (t + 1) *
@@ -427,8 +470,25 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(PassArgs a, int bins) {
 // VALS: the pass reads values (a.vals_in); otherwise a key's value is its
 // input index. A compile-time choice: a branch around the value loads would
 // hide the count of loads in flight from the ranking's waits.
+template <int RB, int BLOCK, int KPT>
+struct ScatterLds {
+  static constexpr int BINS = 1 << RB;
+  static constexpr int WAVES = BLOCK / RDX_WAVE;
+  static constexpr int TILE = BLOCK * KPT;
+  uint16_t wave[WAVES][BINS];   // per-wave digit counts, then exclusive offsets (< TILE)
+  uint32_t local[BINS];         // start of each digit within the tile
+  uint32_t out[BINS];           // global start - local start, for the write-out
+  uint64_t stage[TILE];         // ranking: lane masks; then the staged (key, value) pairs
+  uint32_t scan[WAVES];
+  uint32_t last;
+};
+
+// One tile's scatter by a block of BLOCK threads; `bid` is the block's
+// place in the pass (the kernel's blockIdx.x, or a persistent block's turn;
+// workgroup b runs on XCD b % 8 either way) and tid the thread's index in
+// the block. L is free on entry and may be reused when it returns.
 template <int RB, int BLOCK, int KPT, bool VALS>
-__global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
+__device__ __forceinline__ void scatter_tile(const PassArgs& a, uint32_t bid, ScatterLds<RB, BLOCK, KPT>& L, int tid) {
   constexpr int BINS = 1 << RB;
   constexpr uint32_t DMASK = BINS - 1;
   constexpr int WAVES = BLOCK / RDX_WAVE;
@@ -439,14 +499,13 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
 
   static_assert(WAVES * BINS * 8 <= TILE * 8, "lane masks fit in the staging area");
   static_assert(TILE < 65536, "16-bit wave counts");
-  __shared__ uint16_t s_wave[WAVES][BINS];  // per-wave digit counts, then exclusive offsets (< TILE)
-  __shared__ uint32_t s_local[BINS];        // start of each digit within the tile
-  __shared__ uint32_t s_out[BINS];          // global start - local start, for the write-out
-  __shared__ uint64_t s_stage[TILE];        // ranking: lane masks; then the staged (key, value) pairs
-  __shared__ uint32_t s_scan[WAVES];
+  auto& s_wave = L.wave;
+  uint32_t* s_local = L.local;
+  uint32_t* s_out = L.out;
+  uint64_t* s_stage = L.stage;
+  uint32_t* s_scan = L.scan;
   uint64_t (*s_match)[BINS] = reinterpret_cast<uint64_t (*)[BINS]>(s_stage);  // per-wave lane masks of a round
 
-  const int tid = threadIdx.x;
   const int w = tid / RDX_WAVE;
   const int lane = tid & (RDX_WAVE - 1);
   // One segment: the tiles that hold keys are mapped over all XCDs. Its count
@@ -461,9 +520,9 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
   if (!a.seg && !a.seg_of_tile) {
     const uint32_t n0 = seg_count(a, seg_info(a, 0u));
     const uint32_t used = max(1u, (uint32_t)(((uint64_t)n0 + TILE - 1) / TILE));
-    tile = blockIdx.x < used ? xcd_tile(blockIdx.x, used) : blockIdx.x;
+    tile = bid < used ? xcd_tile(bid, used) : bid;
   } else {
-    tile = xcd_tile(blockIdx.x, a.tiles);
+    tile = xcd_tile(bid, a.tiles);
   }
   const SegInfo g = seg_info(a, a.seg_of_tile ? a.seg_of_tile[tile] : 0u);
   if (a.gate && !seg_wide(a, g.s)) return;  // (segments of a batch, each on its own)
@@ -531,7 +590,7 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
   // Digit starts of the segment: exclusive scan of the digit totals; with
   // the tile's prefix from the scanned counts, each digit's global start.
   uint32_t kept;
-  const uint32_t ex = block_exclusive_sum<BLOCK>(c_tot, s_scan, &kept);
+  const uint32_t ex = block_exclusive_sum<BLOCK>(c_tot, s_scan, &kept, tid);
   if (tid < BINS) s_out[tid] = g.off + ex + c_pre;
   const uint32_t sorted_count = a.drop || a.compacted ? kept : n;
   if (local_tile == 0 && tid == 0) {
@@ -617,7 +676,7 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
     const int d = b0 + tid;
     const uint32_t c = d < BINS ? s_local[d] : 0u;
     uint32_t sum;
-    const uint32_t ex = block_exclusive_sum<BLOCK>(c, s_scan, &sum);
+    const uint32_t ex = block_exclusive_sum<BLOCK>(c, s_scan, &sum, tid);
     if (d < BINS) {
       s_local[d] = tile_kept + ex;
       s_out[d] -= tile_kept + ex;
@@ -669,14 +728,19 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
         atomicAdd(&a.cnt_g[(size_t)t * GATED_BINS + d], 1u);
       }
     }
-    __shared__ uint32_t s_last;
     const uint32_t used = (uint32_t)(((uint64_t)n + TILE - 1) / TILE);  // the blocks that reach here
     wait_stores();
     __syncthreads();  // (also: every read of s_stage is done)
-    if (tid == 0) s_last = atomicAdd(&a.done[g.s], 1u) == used - 1u ? 1u : 0u;
+    if (tid == 0) L.last = atomicAdd(&a.done[g.s], 1u) == used - 1u ? 1u : 0u;
     __syncthreads();
-    if (s_last) gated_scan<BLOCK>(a, g, used, reinterpret_cast<uint32_t*>(s_stage));
+    if (L.last) gated_scan<BLOCK>(a, g, used, reinterpret_cast<uint32_t*>(s_stage));
   }
+}
+
+template <int RB, int BLOCK, int KPT, bool VALS>
+__global__ __launch_bounds__(BLOCK) void k_scatter(PassArgs a) {
+  __shared__ ScatterLds<RB, BLOCK, KPT> L;
+  scatter_tile<RB, BLOCK, KPT, VALS>(a, blockIdx.x, L, threadIdx.x);
 }
 
 // Keys of device distances (GpuSortKeys.compute with the CPU's rule for
