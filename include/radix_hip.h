@@ -413,6 +413,13 @@ int rdx_batch_run(rdx_batch* batch, void* stream, uint32_t* out_indices_dev, uin
 /* The persistent form's grid (blocks resident at once, a multiple of 8),
  * or 0 when the batch has no persistent form. */
 int rdx_batch_persistent_grid(const rdx_batch* batch);
+/* Phase times of the batch's last persistent run made while its context
+ * profiled (rdx_context_profile), from per-block s_memrealtime stamps:
+ * phase_ms[k] from the first block starting phase k to the last finishing
+ * it, release_ms[k] from then until the last block left the barrier after
+ * it. Phases: the projection, count / scan / scatter of passes 0-2, the
+ * gated scatter (11). Returns the count written (0: no such run). */
+int rdx_batch_persistent_phases(rdx_batch* batch, double* phase_ms, double* release_ms, int max_phases);
 /* Waits for the batch's last run; RDX_E_HIP (and clears the flag) if a
  * persistent run's barrier wait ran past its limit of about 1.3 s (a grid
  * whose blocks were not all resident at once: that run's output is not

@@ -200,6 +200,8 @@ _SIGNATURES = [
     ("rdx_batch_run", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     ("rdx_batch_persistent_grid", C.c_int, [C.c_void_p]),
     ("rdx_batch_status", C.c_int, [C.c_void_p]),
+    ("rdx_batch_persistent_phases", C.c_int,
+     [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]),
     ("rdx_batch_choose_form", C.c_int,
      [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("rdx_raster_create", C.c_int, [C.c_void_p, C.POINTER(RasterSettings), C.POINTER(C.c_void_p)]),

@@ -102,6 +102,18 @@ class Batch:
         """Blocks of the persistent launch (0: this batch has no persistent form)."""
         return int(N.lib().rdx_batch_persistent_grid(self._h))
 
+    PHASES = ["project"] + [f"{k}{p}" for p in range(3) for k in ("count", "scan", "scatter")] + ["scatter3"]
+
+    def persistent_phases(self) -> dict:
+        """{phase: (ms, barrier release ms)} of the last persistent run made
+        while the batch's context profiled ({} if none)."""
+        ph = (C.c_double * 16)()
+        rel = (C.c_double * 16)()
+        n = N.lib().rdx_batch_persistent_phases(self._h, ph, rel, 16)
+        if n < 0:
+            N.check(n)
+        return {self.PHASES[k]: (ph[k], rel[k]) for k in range(n)}
+
     def status(self) -> None:
         """Waits for the last run; raises if a persistent run's barrier wait
         ran out (its output is then not valid)."""

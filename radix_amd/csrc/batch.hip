@@ -56,6 +56,8 @@ struct Batch {
   bool persist_ok = false;
   uint32_t persist_grid = 0;
   Buffer d_sync;
+  Buffer d_stamps;  // the persistent run's phase stamps while the context profiles
+  bool stamped = false;
   hipEvent_t last_run = nullptr;
   bool ran = false;
 
@@ -75,6 +77,7 @@ struct Batch {
     d_enc.release();
     d_seg.release();
     d_sync.release();
+    d_stamps.release();
     keys_own.release();
     scratch.release();
   }
@@ -136,26 +139,27 @@ static bool launch_batch_kind(hipStream_t s, const BatchKind& k, const BatchDesc
 }
 
 template <int LAYOUT, int SHC, int DEG>
-static const void* persist_fn() {
-  return reinterpret_cast<const void*>(&k_batch_persistent<LAYOUT, SHC, DEG>);
+static const void* persist_fn(bool stamps) {
+  return stamps ? reinterpret_cast<const void*>(&k_batch_persistent<LAYOUT, SHC, DEG, true>)
+                : reinterpret_cast<const void*>(&k_batch_persistent<LAYOUT, SHC, DEG, false>);
 }
 
 // The persistent kernel of a kind: f32 tables only. The word layouts'
 // projection (their codecs and SH bundles held across the views) needs more
 // than the 128 VGPRs a thread has at two 512-thread blocks per CU; it would
 // spill to scratch, so those tables take the launch sequence.
-static const void* persistent_kernel(const BatchKind& k) {
+static const void* persistent_kernel(const BatchKind& k, bool stamps = false) {
   if (k.layout == LAYOUT_F32 && k.shc == SHC_F32) {
     switch (k.deg) {
-      case 0: return persist_fn<LAYOUT_F32, SHC_F32, 0>();
-      case 1: return persist_fn<LAYOUT_F32, SHC_F32, 1>();
-      case 2: return persist_fn<LAYOUT_F32, SHC_F32, 2>();
-      case 3: return persist_fn<LAYOUT_F32, SHC_F32, 3>();
+      case 0: return persist_fn<LAYOUT_F32, SHC_F32, 0>(stamps);
+      case 1: return persist_fn<LAYOUT_F32, SHC_F32, 1>(stamps);
+      case 2: return persist_fn<LAYOUT_F32, SHC_F32, 2>(stamps);
+      case 3: return persist_fn<LAYOUT_F32, SHC_F32, 3>(stamps);
     }
     return nullptr;
   }
   if (k.layout == LAYOUT_F32 && k.shc == SHC_F32_WIDE)
-    return k.deg == 2 ? persist_fn<LAYOUT_F32, SHC_F32_WIDE, 2>() : persist_fn<LAYOUT_F32, SHC_F32_WIDE, 3>();
+    return k.deg == 2 ? persist_fn<LAYOUT_F32, SHC_F32_WIDE, 2>(stamps) : persist_fn<LAYOUT_F32, SHC_F32_WIDE, 3>(stamps);
   return nullptr;
 }
 
@@ -189,12 +193,21 @@ static int run_batch_persistent(Batch* b, hipStream_t s, uint32_t* out_indices, 
   P.blocks = b->persist_grid;
   P.bar = (uint32_t*)b->d_sync.ptr;
   P.err = P.bar + 2;
+  P.stamps = nullptr;
+  b->stamped = false;
+  if (b->ctx->profiling) {
+    const size_t words = (size_t)(PERSIST_PHASES + 1) * 2 * b->persist_grid;
+    if (int rc = b->d_stamps.ensure(words * sizeof(uint64_t))) return rc;
+    P.stamps = (uint64_t*)b->d_stamps.ptr;
+    b->stamped = true;
+  }
   BatchDesc d = batch_desc(b, k);
   BatchU u = b->u;
   uint32_t* keys = b->keys;
   void* args[] = {&d, &u, &keys, &P};
   Stage st(b->ctx, s, "batch_persistent");
-  hipError_t e = hipLaunchKernel(persistent_kernel(k), dim3(b->persist_grid), dim3(PERSIST_BLOCK), args, 0, s);
+  hipError_t e = hipLaunchKernel(persistent_kernel(k, P.stamps != nullptr), dim3(b->persist_grid), dim3(PERSIST_BLOCK),
+                                 args, 0, s);
   return e == hipSuccess ? RDX_OK : hip_fail(e, "batch persistent launch");
 }
 
@@ -495,6 +508,32 @@ int rdx_batch_persistent_grid(const rdx_batch* batch) {
   if (!batch) return 0;
   const Batch* b = reinterpret_cast<const Batch*>(batch);
   return b->persist_ok ? (int)b->persist_grid : 0;
+}
+
+int rdx_batch_persistent_phases(rdx_batch* batch, double* phase_ms, double* release_ms, int max_phases) {
+  if (!batch || !phase_ms || !release_ms) return fail(RDX_E_INVALID, "rdx_batch_persistent_phases: NULL argument");
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  if (!b->stamped) return 0;
+  hipError_t e = hipEventSynchronize(b->last_run);
+  if (e != hipSuccess) return hip_fail(e, "rdx_batch_persistent_phases");
+  const uint32_t G = b->persist_grid;
+  std::vector<uint64_t> st((size_t)(PERSIST_PHASES + 1) * 2 * G);
+  e = hipMemcpy(st.data(), b->d_stamps.ptr, st.size() * sizeof(uint64_t), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "rdx_batch_persistent_phases");
+  auto at = [&](int k, int side, uint32_t blk) { return st[((size_t)k * 2 + side) * G + blk]; };
+  const int n = std::min(max_phases, PERSIST_PHASES);
+  // s_memrealtime counts at 100 MHz
+  for (int k = 0; k < n; k++) {
+    uint64_t first_dep = ~0ull, last_arr = 0, last_dep_next = 0;
+    for (uint32_t blk = 0; blk < G; blk++) {
+      first_dep = std::min(first_dep, at(k, 1, blk));
+      last_arr = std::max(last_arr, at(k + 1, 0, blk));
+      if (k + 1 < PERSIST_PHASES) last_dep_next = std::max(last_dep_next, at(k + 1, 1, blk));
+    }
+    phase_ms[k] = (double)(last_arr - first_dep) * 1e-5;
+    release_ms[k] = k + 1 < PERSIST_PHASES ? (double)(last_dep_next - last_arr) * 1e-5 : 0.0;
+  }
+  return n;
 }
 
 int rdx_batch_status(rdx_batch* batch) {

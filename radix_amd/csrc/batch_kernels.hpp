@@ -143,13 +143,25 @@ constexpr int PERSIST_BLOCK = 512;
 constexpr int PERSIST_KPT = 12;
 constexpr uint64_t PERSIST_WAIT_TICKS = 1ull << 27;
 
+constexpr int PERSIST_PHASES = 11;  // the projection, 3 x (count, scan, scatter), the gated scatter
+
 struct PersistArgs {
   PassArgs pass[4];     // the ranged sort's passes (the fourth gated)
   uint32_t items;       // projection chunks of the kind's groups
   uint32_t blocks;      // the grid
   uint32_t* bar;        // [2] barrier arrivals and finished blocks, zero between launches
   uint32_t* err;        // set by a block whose wait ran out
+  // Profiling (NULL: off): per block, the s_memrealtime of its start and of
+  // its arrival at and departure from each barrier; [PERSIST_PHASES + 1][2][blocks]
+  // (phase k starts at departure k and ends at arrival k + 1).
+  uint64_t* stamps;
 };
+
+template <bool STAMPS>
+__device__ __forceinline__ void persist_stamp(const PersistArgs& P, int k, int side) {
+  if (STAMPS && threadIdx.x == 0)
+    P.stamps[((size_t)k * 2 + side) * P.blocks + blockIdx.x] = __builtin_amdgcn_s_memrealtime();
+}
 
 // threadIdx.x through an opaque move, taken once per work item: the values
 // a tile's scatter derives from the thread index (its keys' and outputs'
@@ -161,9 +173,11 @@ __device__ __forceinline__ int item_tid() {
   return t;
 }
 
+template <bool STAMPS>
 __device__ __forceinline__ void grid_sync(const PersistArgs& P, uint32_t& epoch) {
   __syncthreads();
   epoch++;
+  persist_stamp<STAMPS>(P, (int)epoch, 0);
   if (threadIdx.x == 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __hip_atomic_fetch_add(P.bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -179,10 +193,13 @@ __device__ __forceinline__ void grid_sync(const PersistArgs& P, uint32_t& epoch)
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
+  persist_stamp<STAMPS>(P, (int)epoch, 1);
   __syncthreads();
 }
 
-template <int LAYOUT, int SHC, int DEG>
+// STAMPS: the profiling instantiation (P.stamps set), so that the stamps'
+// registers never touch the production kernel's allocation.
+template <int LAYOUT, int SHC, int DEG, bool STAMPS>
 __global__ __launch_bounds__(PERSIST_BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_batch_persistent(BatchDesc D, BatchU u, uint32_t* keys, PersistArgs P) {
   constexpr int RB = RANGED_RB, BLOCK = PERSIST_BLOCK, KPT = PERSIST_KPT;
   __shared__ union {
@@ -192,6 +209,7 @@ __global__ __launch_bounds__(PERSIST_BLOCK) __attribute__((amdgpu_waves_per_eu(4
   } L;
   const uint32_t G = P.blocks;
   uint32_t epoch = 0;
+  persist_stamp<STAMPS>(P, 0, 1);
 
   // projection: chunk c of the kind belongs to the last group whose first chunk is <= c
   for (uint32_t c = blockIdx.x; c < P.items; c += G) {
@@ -201,7 +219,7 @@ __global__ __launch_bounds__(PERSIST_BLOCK) __attribute__((amdgpu_waves_per_eu(4
     const BatchGroup Gr = D.groups[__builtin_amdgcn_readfirstlane(g)];
     project_batch_splat<LAYOUT, SHC, DEG>(D, u, keys, Gr, (c - Gr.chunk0) * BLOCK + threadIdx.x);
   }
-  grid_sync(P, epoch);
+  grid_sync<STAMPS>(P, epoch);
 
   const int bins = 1 << RB;
 #pragma unroll 1
@@ -211,13 +229,13 @@ __global__ __launch_bounds__(PERSIST_BLOCK) __attribute__((amdgpu_waves_per_eu(4
       count_tile<RB, BLOCK, KPT>(a, t, L.count);
       __syncthreads();
     }
-    grid_sync(P, epoch);
+    grid_sync<STAMPS>(P, epoch);
     const uint32_t items = D.num_segments * scan_items_per_seg<BLOCK>(a.rows, a.ranged);
     for (uint32_t it = blockIdx.x; it < items; it += G) {
       scan_item<8, BLOCK>(a, bins, it, L.scan);
       __syncthreads();
     }
-    grid_sync(P, epoch);
+    grid_sync<STAMPS>(P, epoch);
     for (uint32_t t = blockIdx.x; t < a.tiles; t += G) {
       if (pass == 0)
         scatter_tile<RB, BLOCK, KPT, false>(a, t, L.scatter, item_tid());
@@ -225,13 +243,15 @@ __global__ __launch_bounds__(PERSIST_BLOCK) __attribute__((amdgpu_waves_per_eu(4
         scatter_tile<RB, BLOCK, KPT, true>(a, t, L.scatter, item_tid());
       __syncthreads();
     }
-    grid_sync(P, epoch);
+    grid_sync<STAMPS>(P, epoch);
   }
   // the gated fourth pass (its blocks return at once for segments within three passes)
   for (uint32_t t = blockIdx.x; t < P.pass[3].tiles; t += G) {
     scatter_tile<RB, BLOCK, KPT, true>(P.pass[3], t, L.scatter, item_tid());
     __syncthreads();
   }
+  __syncthreads();
+  persist_stamp<STAMPS>(P, PERSIST_PHASES, 0);
   // the last block out resets the barrier words
   if (threadIdx.x == 0) {
     const uint32_t out = __hip_atomic_fetch_add(P.bar + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
