@@ -489,6 +489,8 @@ def main() -> int:
     ap.add_argument("--tuning", default=None,
                     help="the headline step's sort tuning as digit_bits,keys_per_thread,block_threads (default: the library's)")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and stereo workloads")
+    ap.add_argument("--only", choices=["batched", "stereo", "differentiable"], default=None,
+                    help="run one extra workload alone and print its object (for profiling its kernels)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 512 MiB five times (the reference kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -512,6 +514,17 @@ def main() -> int:
 
     from radix_amd import DeviceTable, Params, Projector, workloads as W
     from radix_amd.sort import Context
+
+    if args.only:
+        steps = max(3, args.steps // 5)
+        if args.only == "batched":
+            res = bench_batched(steps, with_cpu=False)
+        elif args.only == "stereo":
+            res = bench_stereo(steps)
+        else:
+            res = bench_backward(steps)
+        print(json.dumps({args.only: res}), flush=True)
+        return 0
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)

@@ -402,9 +402,10 @@ const uint32_t* rdx_batch_keys(const rdx_batch* batch);
  * set and form) and replay it. RDX_BATCH_PERSISTENT: the whole batch in one
  * launch of a persistent kernel (the projection and every radix pass of
  * every segment, with grid-wide barriers between the phases), available
- * when the batch's tables are f32 tables of one SH degree and it was
- * created under the 8-bit, 12-key, 512-thread tuning with 32-bit keys
- * (RDX_E_UNSUPPORTED otherwise); the same output as the launch sequence.
+ * when the batch's tables are of one kind (layout, SH codec and degree)
+ * and it was created under the 8-bit, 12-key, 512-thread tuning with
+ * 32-bit keys (RDX_E_UNSUPPORTED otherwise); the same output as the launch
+ * sequence.
  * Runs of one batch must not overlap on the device. */
 #define RDX_BATCH_GRAPH 1
 #define RDX_BATCH_PERSISTENT 2

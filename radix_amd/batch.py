@@ -89,7 +89,7 @@ class Batch:
 
     def run(self, graph: bool = True, stream=None, persistent: Optional[bool] = None) -> None:
         """persistent: True runs the whole batch as one launch of the
-        persistent kernel (f32 tables of one SH degree, tuning (8, 12, 512));
+        persistent kernel (tables of one kind, tuning (8, 12, 512), 32-bit keys);
         None takes self.persistent (False unless choose_form() picked it)."""
         if persistent is None:
             persistent = self.persistent

@@ -144,10 +144,7 @@ static const void* persist_fn(bool stamps) {
                 : reinterpret_cast<const void*>(&k_batch_persistent<LAYOUT, SHC, DEG, false>);
 }
 
-// The persistent kernel of a kind: f32 tables only. The word layouts'
-// projection (their codecs and SH bundles held across the views) needs more
-// than the 128 VGPRs a thread has at two 512-thread blocks per CU; it would
-// spill to scratch, so those tables take the launch sequence.
+// The persistent kernel of a kind.
 static const void* persistent_kernel(const BatchKind& k, bool stamps = false) {
   if (k.layout == LAYOUT_F32 && k.shc == SHC_F32) {
     switch (k.deg) {
@@ -160,7 +157,12 @@ static const void* persistent_kernel(const BatchKind& k, bool stamps = false) {
   }
   if (k.layout == LAYOUT_F32 && k.shc == SHC_F32_WIDE)
     return k.deg == 2 ? persist_fn<LAYOUT_F32, SHC_F32_WIDE, 2>(stamps) : persist_fn<LAYOUT_F32, SHC_F32_WIDE, 3>(stamps);
-  return nullptr;
+  if (k.layout == LAYOUT_EXT)
+    return k.shc == SHC_PACKED ? persist_fn<LAYOUT_EXT, SHC_PACKED, -1>(stamps) : persist_fn<LAYOUT_EXT, SHC_EXT, -1>(stamps);
+  if (k.layout == LAYOUT_COV)
+    return k.shc == SHC_PACKED ? persist_fn<LAYOUT_COV, SHC_PACKED, -1>(stamps) : persist_fn<LAYOUT_COV, SHC_EXT, -1>(stamps);
+  return k.shc == SHC_PACKED ? persist_fn<LAYOUT_PACKED, SHC_PACKED, -1>(stamps)
+                             : persist_fn<LAYOUT_PACKED, SHC_EXT, -1>(stamps);
 }
 
 static BatchDesc batch_desc(const Batch* b, const BatchKind& k) {
@@ -461,8 +463,8 @@ int rdx_batch_run(rdx_batch* batch, void* stream, uint32_t* out_indices, uint32_
   const bool persistent = (flags & RDX_BATCH_PERSISTENT) != 0;
   if (persistent && !b->persist_ok)
     return fail(RDX_E_UNSUPPORTED,
-                "rdx_batch_run: the persistent form needs f32 tables of one SH degree and the 8-bit, 12-key, 512-thread "
-                "tuning at 32-bit keys");
+                "rdx_batch_run: the persistent form needs tables of one kind (layout, SH codec and degree) and the 8-bit, "
+                "12-key, 512-thread tuning at 32-bit keys");
   if (flags & RDX_BATCH_GRAPH) {
     const auto outs = std::make_tuple(out_indices, out_counts, out_args);
     if (!b->exec || b->graph_outputs != outs || b->graph_persistent != persistent) {

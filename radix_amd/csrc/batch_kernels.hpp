@@ -47,9 +47,14 @@ struct BatchDesc {
 
 // Splat i of group G's table, projected for each of the group's views: its
 // ewa words and one key per view (view v's at its segment offset + i).
-template <int LAYOUT, int SHC, int DEG>
+// The views' colours come first, each SH-evaluated from its own origin and
+// kept encoded in LDS (s_rgb[v * BLOCK + thread]), so the SH coefficients
+// are dead before the views' projections: the thread then holds no more
+// than the plain projection does (the colour of a view that culls the
+// splat is computed and not used). i >= the table's count: nothing.
+template <int LAYOUT, int SHC, int DEG, int BLOCK>
 __device__ __forceinline__ void project_batch_splat(const BatchDesc& D, const BatchU& u, uint32_t* keys,
-                                                    const BatchGroup& G, uint32_t i) {
+                                                    const BatchGroup& G, uint32_t i, uint32_t* s_rgb) {
   const TableP& t = D.tables[G.table];
   if (i >= t.count) return;
   const int deg = DEG >= 0 ? DEG : D.table_sh_degree[G.table];
@@ -61,15 +66,25 @@ __device__ __forceinline__ void project_batch_splat(const BatchDesc& D, const Ba
   if constexpr (LAYOUT == LAYOUT_F32) rs = f32_rsrc<SHC>(t, deg);
 
   Splat sp;
-  float3 coef[15];
-  float band[3];
-  load_splat<LAYOUT, SHC, DEG>(t, rs, enc, i, deg, sp, coef, band);
   float op, std_dev;
   float3 s;
   bool invisible;
-  prepare<LAYOUT == LAYOUT_COV>(sp, u, op, s, std_dev, invisible);
+  {
+    float3 coef[15];
+    float band[3];
+    load_splat<LAYOUT, SHC, DEG>(t, rs, enc, i, deg, sp, coef, band);
+    prepare<LAYOUT == LAYOUT_COV>(sp, u, op, s, std_dev, invisible);
+    if (!invisible) {
+      for (uint32_t v = 0; v < nv; v++) {
+        const float3 rgb = sh_colour(sp, D.views[G.first_view + v].h, deg, coef, SHC == SHC_PACKED ? band : nullptr);
+        s_rgb[v * BLOCK + threadIdx.x] = encode_ext_rgb(rgb.x, rgb.y, rgb.z);
+      }
+    }
+  }
   float R[9];
-  bool have_r = false;
+  if constexpr (LAYOUT != LAYOUT_COV) {
+    if (!invisible) quat_matrix(sp.q, R);
+  }
   for (uint32_t v = 0; v < nv; v++) {
     const BatchView& V = D.views[G.first_view + v];
     uint4 words = make_uint4(0u, 0u, 0u, 0u);
@@ -77,18 +92,14 @@ __device__ __forceinline__ void project_batch_splat(const BatchDesc& D, const Ba
     EyeCentre C = eye_centre(V.eye, sp.c, u.ndc_z_min, u.clip_xy);
     C.pass = C.pass && !invisible;
     if (C.pass) {
-      const float3 rgb = sh_colour(sp, V.h, deg, coef, SHC == SHC_PACKED ? band : nullptr);
+      const uint32_t rgb_word = s_rgb[v * BLOCK + threadIdx.x];
       EwaResult r;
       if constexpr (LAYOUT == LAYOUT_COV) {
-        r = ewa_eye_cov(V.eye, C, true, sp.cv, op, std_dev, encode_ext_rgb(rgb.x, rgb.y, rgb.z), u);
+        r = ewa_eye_cov(V.eye, C, true, sp.cv, op, std_dev, rgb_word, u);
       } else {
-        if (!have_r) {
-          quat_matrix(sp.q, R);
-          have_r = true;
-        }
         float Q[9];
         eye_axes(V.eye, R, Q);
-        r = ewa_eye(V.eye, C, true, R, Q, s, op, std_dev, encode_ext_rgb(rgb.x, rgb.y, rgb.z), u);
+        r = ewa_eye(V.eye, C, true, R, Q, s, op, std_dev, rgb_word, u);
       }
       if (r.ok) {
         words = r.words;
@@ -107,6 +118,7 @@ template <int LAYOUT, int SHC, int DEG>
 __global__ __launch_bounds__(256) void k_project_batch(BatchDesc D, BatchU u, uint32_t* keys) {
   constexpr int BLOCK = 256;
   __shared__ uint32_t s_group;
+  __shared__ uint32_t s_rgb[BATCH_MAX_VIEWS * BLOCK];
 
   // The block's group: the last group whose first block is <= blockIdx.x.
   if (threadIdx.x == 0) {
@@ -119,7 +131,7 @@ __global__ __launch_bounds__(256) void k_project_batch(BatchDesc D, BatchU u, ui
   // uniform across the block: the group's and its views' fields load through
   // the scalar cache into SGPRs instead of once per lane
   const BatchGroup G = D.groups[__builtin_amdgcn_readfirstlane(s_group)];
-  project_batch_splat<LAYOUT, SHC, DEG>(D, u, keys, G, (blockIdx.x - G.block0) * BLOCK + threadIdx.x);
+  project_batch_splat<LAYOUT, SHC, DEG, BLOCK>(D, u, keys, G, (blockIdx.x - G.block0) * BLOCK + threadIdx.x, s_rgb);
 }
 
 // ---- the persistent form: projection and segmented sort in one launch ----------
@@ -206,6 +218,7 @@ __global__ __launch_bounds__(PERSIST_BLOCK) __attribute__((amdgpu_waves_per_eu(4
     CountLds<RB, BLOCK> count;
     ScanLds<BLOCK> scan;
     ScatterLds<RB, BLOCK, KPT> scatter;
+    uint32_t rgb[BATCH_MAX_VIEWS * BLOCK];  // the projection's view colours
   } L;
   const uint32_t G = P.blocks;
   uint32_t epoch = 0;
@@ -217,7 +230,7 @@ __global__ __launch_bounds__(PERSIST_BLOCK) __attribute__((amdgpu_waves_per_eu(4
     for (int k = 1; k < D.num_groups; k++)
       if (D.groups[k].chunk0 <= c) g = (uint32_t)k;
     const BatchGroup Gr = D.groups[__builtin_amdgcn_readfirstlane(g)];
-    project_batch_splat<LAYOUT, SHC, DEG>(D, u, keys, Gr, (c - Gr.chunk0) * BLOCK + threadIdx.x);
+    project_batch_splat<LAYOUT, SHC, DEG, BLOCK>(D, u, keys, Gr, (c - Gr.chunk0) * BLOCK + threadIdx.x, L.rgb);
   }
   grid_sync<STAMPS>(P, epoch);
 

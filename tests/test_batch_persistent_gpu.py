@@ -142,8 +142,31 @@ def test_persistent_tiny_and_empty_segments(cuda):
     check_views(b, [t1, t2], prm)
 
 
+def test_persistent_word_tables(cuda):
+    # packed and covSplats tables (their codecs in the persistent projection)
+    from tests.test_project_gpu import cov_table
+
+    spec = W.stereo_16m_spec(seed=48, count=9000)
+    words, chans = W.pack(W.generate(spec), 3)
+    packed = DeviceTable.from_packed(words, spec.count, (0.0, 1.0, -10.0, 2.0), chans, 3)
+    ca, cb, cch = cov_table(7000, 49, 3, lod=True)
+    cov = DeviceTable.from_cov(ca, cb, 7000, cch, sh_codec=3, sh_degree=3)
+    prm = Params(sort_bits=32)
+    for tables, views in (([packed], [(0, W.orbit_camera(5 + v, 6.0 + v, 320, 240)) for v in range(3)]),
+                          ([cov], [(0, W.orbit_camera(15 + v, 6.5, 320, 240)) for v in range(2)])):
+        b = Batch(tables, views, prm, tuning=PERSIST)
+        assert b.persistent_grid > 0
+        b.run(graph=False, persistent=False)
+        seq = outputs(b)
+        clear(b)
+        b.run(graph=True, persistent=True)
+        b.status()
+        assert_same(b, outputs(b), seq, "word table")
+        check_views(b, tables, prm)
+
+
 def test_persistent_form_not_offered(cuda):
-    # other tunings and word tables keep the launch sequence: the flag is refused
+    # other tunings, key widths and mixed kinds keep the launch sequence: the flag is refused
     wl, tables = small_batch(seed=14, collections=2, views=3)
     prm = Params(sort_bits=32)
     b = Batch(tables, list(zip(wl.view_table, wl.cameras)), prm, tuning=(8, 16, 512))
@@ -155,10 +178,10 @@ def test_persistent_form_not_offered(cuda):
     spec = W.stereo_16m_spec(seed=48, count=3000)
     words, chans = W.pack(W.generate(spec), 3)
     packed = DeviceTable.from_packed(words, spec.count, (0.0, 1.0, -10.0, 2.0), chans, 3)
-    bp = Batch([packed], [(0, W.orbit_camera(5, 7.0, 320, 240))], prm, tuning=PERSIST)
-    assert bp.persistent_grid == 0
+    bm = Batch([tables[0], packed], [(0, wl.cameras[0]), (1, W.orbit_camera(5, 7.0, 320, 240))], prm, tuning=PERSIST)
+    assert bm.persistent_grid == 0
     with pytest.raises(RadixError, match="persistent form"):
-        bp.run(graph=False, persistent=True)
+        bm.run(graph=False, persistent=True)
 
 
 def test_batched_workload_persistent_at_its_own_shape(cuda):
