@@ -11,7 +11,9 @@ passes (compaction in the first), writing the visible indices back to front.
 from __future__ import annotations
 
 import argparse
+import contextlib
 import ctypes as C
+import gc
 import json
 import os
 import statistics
@@ -175,6 +177,20 @@ def oracle_projection_rate(planes, rig, count: int) -> dict:
             "sample": f"{count:,} splats of the same table, FP64 projection + SH3 + key, 1 thread"}
 
 
+@contextlib.contextmanager
+def no_gc():
+    """Python's cyclic garbage collector held off inside a timed loop (a full
+    collection, tens of ms with torch loaded, would otherwise land between
+    two launches at a point fixed by the process's allocation count and
+    idle the GPU); collected before and after."""
+    gc.collect()
+    gc.disable()
+    try:
+        yield
+    finally:
+        gc.enable()
+
+
 def time_events(fn, steps: int, warmup: int = 2) -> float:
     """ms per call of fn() on the current stream, HIP events around `steps` calls."""
     import torch
@@ -183,11 +199,12 @@ def time_events(fn, steps: int, warmup: int = 2) -> float:
         fn()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(steps):
-        fn()
-    e1.record()
-    torch.cuda.synchronize()
+    with no_gc():
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
     return e0.elapsed_time(e1) / steps
 
 
@@ -554,13 +571,14 @@ def main() -> int:
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     log(f"timing {args.steps} steps")
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        step()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    with no_gc():
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(args.steps):
+            step()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
     ms = e0.elapsed_time(e1) / args.steps
     visible = int(cnt.cpu().item())
 
@@ -568,9 +586,10 @@ def main() -> int:
     N_lib = __import__("radix_amd._native", fromlist=["lib"]).lib()
     N_lib.rdx_context_profile(ctx.handle, 1)
     stage_times(ctx)
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
+    with no_gc():
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
     stages = stage_times(ctx)
     N_lib.rdx_context_profile(ctx.handle, 0)
     proj_ms = stages.get("project", {}).get("ms_avg", float("nan"))
