@@ -152,9 +152,11 @@ export class SplatBatch {
   readonly total: number;
   readonly offsets: number[];
   constructor(tables: SplatTable[], views: { table: number; camera: SplatCameraViews; words?: DeviceBuffer }[],
-    params: SplatProjectionParams, options?: { keys?: DeviceBuffer } & OnStream);
+    params: SplatProjectionParams, options?: { keys?: DeviceBuffer; tuning?: [number, number, number] } & OnStream);
   setCameras(cameras: SplatCameraViews[]): void;
-  run(out: { indices: DeviceBuffer; counts?: DeviceBuffer; drawArgs?: DeviceBuffer; graph?: boolean } & OnStream): void;
+  run(out: { indices: DeviceBuffer; counts?: DeviceBuffer; drawArgs?: DeviceBuffer; graph?: boolean;
+    persistent?: boolean } & OnStream): void;
+  status(): void;
   dispose(): void;
 }
 

@@ -279,12 +279,14 @@ class SplatDrawer {
 // keys and indices are at offsets[v] (SplatCollection's offsets,
 // SplatCollection.ts:54-65).
 class SplatBatch {
-  constructor(tables, views, params, { keys, stream } = {}) {
+  // tuning: [digitBits, keysPerThread, blockThreads] for this batch's sort
+  // (the persistent form needs [8, 12, 512]); default: the context's.
+  constructor(tables, views, params, { keys, stream, tuning } = {}) {
     this.tables = tables;
     this.views = views;
     this.handle = native.batchCreate(tables.map(tableDesc), views.map((v) => ({
       table: v.table, camera: v.camera, words: handleOf(v.words),
-    })), params || {}, handleOf(keys), streamOf({ stream }));
+    })), params || {}, handleOf(keys), streamOf({ stream }), tuning);
     const info = native.batchInfo(this.handle);
     this.total = info.total;
     this.offsets = info.offsets;
@@ -296,8 +298,17 @@ class SplatBatch {
 
   // indices: total uint32; counts: a uint32 per view; drawArgs: 5 per view.
   // graph: capture the sequence into a HIP graph once and replay it.
-  run({ indices, counts, drawArgs, graph = true, stream }) {
-    native.batchRun(this.handle, handleOf(indices), handleOf(counts), handleOf(drawArgs), graph, streamOf({ stream }));
+  // persistent: the whole batch as one launch of the persistent kernel
+  // (tables of one kind, created with tuning [8, 12, 512], 32-bit keys).
+  run({ indices, counts, drawArgs, graph = true, persistent = false, stream }) {
+    native.batchRun(this.handle, handleOf(indices), handleOf(counts), handleOf(drawArgs), graph, persistent,
+      streamOf({ stream }));
+  }
+
+  // Waits for the last run; throws if a persistent run's barrier wait ran out
+  // (that run's output is then not valid).
+  status() {
+    native.batchStatus(this.handle);
   }
 
   dispose() {

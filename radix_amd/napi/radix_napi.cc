@@ -974,9 +974,11 @@ void batch_finalize(napi_env, void* data, void*) {
   delete b;
 }
 
-// batchCreate(tables[], views[{table, camera, words?}], params, keys?, stream?)
+// batchCreate(tables[], views[{table, camera, words?}], params, keys?, stream?, tuning?)
+// tuning: [digitBits, keysPerThread, blockThreads] for this batch's sort (the
+// persistent form needs [8, 12, 512]); the context's own tuning is restored.
 napi_value BatchCreate(napi_env env, napi_callback_info info) {
-  ARGS(5);
+  ARGS(6);
   Call c(env, "SplatBatch", argv[4]);
   const uint32_t nt = length_of(env, argv[0]), nv = length_of(env, argv[1]);
   if (c.ok && (nt < 1 || nv < 1 || nv > RDX_BATCH_MAX_VIEWS))
@@ -1003,8 +1005,20 @@ napi_value BatchCreate(napi_env env, napi_callback_info info) {
   read_params(env, argv[2], &p);
   uint32_t* keys = (uint32_t*)c.buf(argv[3], "keys", 4 * total, true);
   if (!c.ok) return nullptr;
+  rdx_sort_tuning saved, want;
+  const bool tuned = !absent(env, argv[5]);
+  if (tuned) {
+    if (length_of(env, argv[5]) != 3) return throw_error(env, "SplatBatch: tuning is [digitBits, keysPerThread, blockThreads]");
+    want.digit_bits = (int)num(env, element(env, argv[5], 0), 0.0);
+    want.keys_per_thread = (int)num(env, element(env, argv[5], 1), 0.0);
+    want.block_threads = (int)num(env, element(env, argv[5], 2), 0.0);
+    if (!c.result(rdx_context_get_tuning(c.ctx, &saved)) || !c.result(rdx_context_set_tuning(c.ctx, &want)))
+      return nullptr;
+  }
   BatchH* b = new BatchH();
-  if (!c.ok || !c.result(rdx_batch_create(c.ctx, tables.data(), (int)nt, views.data(), (int)nv, &p, keys, &b->h))) {
+  const bool made = c.result(rdx_batch_create(c.ctx, tables.data(), (int)nt, views.data(), (int)nv, &p, keys, &b->h));
+  if (tuned) (void)rdx_context_set_tuning(c.ctx, &saved);
+  if (!made) {
     delete b;
     return nullptr;
   }
@@ -1043,10 +1057,10 @@ napi_value BatchSetCameras(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
-// batchRun(batch, indices, counts, drawArgs, graph, stream?)
+// batchRun(batch, indices, counts, drawArgs, graph, persistent, stream?)
 napi_value BatchRun(napi_env env, napi_callback_info info) {
-  ARGS(6);
-  Call c(env, "SplatBatch.run", argv[5]);
+  ARGS(7);
+  Call c(env, "SplatBatch.run", argv[6]);
   BatchH* b = external<BatchH>(env, argv[0], T_BATCH);
   if (!b || !b->h) return throw_error(env, "SplatBatch.run: not a batch");
   uint32_t* idx = (uint32_t*)c.buf(argv[1], "indices", 4 * std::max<uint64_t>(b->total, 1));
@@ -1056,7 +1070,22 @@ napi_value BatchRun(napi_env env, napi_callback_info info) {
   napi_valuetype t;
   if (!absent(env, argv[4]) && napi_typeof(env, argv[4], &t) == napi_ok && t == napi_boolean)
     napi_get_value_bool(env, argv[4], &graph);
-  if (c.ok) c.result(rdx_batch_run(b->h, c.stream, idx, cnt, args, graph ? RDX_BATCH_GRAPH : 0));
+  bool persistent = false;
+  if (!absent(env, argv[5]) && napi_typeof(env, argv[5], &t) == napi_ok && t == napi_boolean)
+    napi_get_value_bool(env, argv[5], &persistent);
+  if (c.ok)
+    c.result(rdx_batch_run(b->h, c.stream, idx, cnt, args,
+                           (graph ? RDX_BATCH_GRAPH : 0) | (persistent ? RDX_BATCH_PERSISTENT : 0)));
+  return nullptr;
+}
+
+// batchStatus(batch): waits for the last run; throws if a persistent run's barrier wait ran out
+napi_value BatchStatus(napi_env env, napi_callback_info info) {
+  ARGS(1);
+  Call c(env, "SplatBatch.status", nullptr);
+  BatchH* b = external<BatchH>(env, argv[0], T_BATCH);
+  if (!b || !b->h) return throw_error(env, "SplatBatch.status: not a batch");
+  if (c.ok) c.result(rdx_batch_status(b->h));
   return nullptr;
 }
 
@@ -1117,6 +1146,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"batchInfo", BatchInfo},
       {"batchSetCameras", BatchSetCameras},
       {"batchRun", BatchRun},
+      {"batchStatus", BatchStatus},
       {"batchDestroy", BatchDestroy},
   };
   for (const auto& f : fns) {

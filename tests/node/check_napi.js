@@ -420,6 +420,7 @@ async function gpuChecksBatch() {
     const drawArgs = new DeviceBuffer(20 * views.length);
     for (const graph of [true, false, true]) {
       batch.run({ indices, counts, drawArgs, graph });
+      if (graph) batch.status();
       const k = keys.read(0, total);
       const idx = indices.read(0, total);
       const c = counts.read(0, views.length);
@@ -434,6 +435,50 @@ async function gpuChecksBatch() {
         if (m) return `view ${v} (graph ${graph}): ${m}`;
       }
     }
+    batch.dispose();
+    [keys, indices, counts, drawArgs].concat(tables.map((t) => t.planes)).forEach((b) => b.dispose());
+    return undefined;
+  });
+
+  await check('SplatBatch: the persistent launch equals the launch sequence (tuning [8, 12, 512], graph and eager)', () => {
+    const sizes = [7000, 4000];
+    const tables = sizes.map((n, k) => ({ layout: 'f32', count: n, planes: DeviceBuffer.fromArray(randomPlanes(50 + k, n)) }));
+    const views = [[0, 0.1], [1, -0.2], [1, 0.4], [0, -0.3]].map(([table, x]) => ({ table, camera: cameraAt(x) }));
+    const total = views.reduce((s, v) => s + sizes[v.table], 0);
+    const keys = new DeviceBuffer(4 * total);
+    const batch = new SplatBatch(tables, views, { sortBits: 32 }, { keys, tuning: [8, 12, 512] });
+    const indices = new DeviceBuffer(4 * total);
+    const counts = new DeviceBuffer(4 * views.length);
+    const drawArgs = new DeviceBuffer(20 * views.length);
+    batch.run({ indices, counts, drawArgs, graph: false });
+    const refIdx = indices.read(0, total);
+    const refCounts = counts.read(0, views.length);
+    const refArgs = drawArgs.read(0, 5 * views.length);
+    for (const graph of [false, true, true]) {
+      indices.fill(0);
+      counts.fill(0);
+      batch.run({ indices, counts, drawArgs, graph, persistent: true });
+      batch.status();
+      const idx = indices.read(0, total);
+      const c = counts.read(0, views.length);
+      const args = drawArgs.read(0, 5 * views.length);
+      for (let v = 0; v < views.length; v += 1) {
+        if (c[v] !== refCounts[v]) return `view ${v} (graph ${graph}): count ${c[v]} vs ${refCounts[v]}`;
+        const m = firstMismatch(idx.subarray(batch.offsets[v], batch.offsets[v] + c[v]),
+          refIdx.subarray(batch.offsets[v], batch.offsets[v] + c[v]), c[v]);
+        if (m) return `view ${v} (graph ${graph}): ${m}`;
+        for (let k = 0; k < 5; k += 1) if (args[5 * v + k] !== refArgs[5 * v + k]) return `view ${v}: draw arguments`;
+      }
+    }
+    const wrong = new SplatBatch(tables, views, { sortBits: 32 }, { keys, tuning: [8, 16, 512] });
+    let threw = false;
+    try {
+      wrong.run({ indices, counts, drawArgs, persistent: true });
+    } catch (e) {
+      threw = /persistent form/.test(String(e.message));
+    }
+    wrong.dispose();
+    if (!threw) return 'the persistent form ran under a tuning without it';
     batch.dispose();
     [keys, indices, counts, drawArgs].concat(tables.map((t) => t.planes)).forEach((b) => b.dispose());
     return undefined;
