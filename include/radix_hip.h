@@ -520,6 +520,23 @@ int rdx_draw_destroy(rdx_draw* draw);
 int rdx_draw_splats(rdx_draw* draw, void* stream, const uint32_t* words_dev, uint32_t n, const uint32_t* order_dev,
                     const uint32_t* count_dev, uint32_t capacity, uint32_t width, uint32_t height,
                     const rdx_params* params, float* rgba_dev);
+/* The draw of the ray-evaluated methods' words, one eye (splatDraw.ts:98-181
+ * with the methods' vertex and density stages: 3dgut and htgs,
+ * projections.tsl.ts:310-330 and rayVertex / rayDensity,
+ * projection.tsl.ts:412-442; raygs, projections.tsl.ts:432-466): the words
+ * rdx_project_ray wrote (channels w0, w1 and, with two eyes, w2; n splats)
+ * with the camera they were projected for, `eye` of its eyes, whose
+ * SplatCamera fromHead (view * head^-1), projection and render size place
+ * them. 3dgut and htgs: the quad is the eye's screen bound (its corner
+ * shrunk to cut / bound for 3dgut when the bound lies on screen), each pixel
+ * evaluating the Gaussian along its ray in the splat's whitened frame,
+ * z^2 = |localCross|^2 / |localDir|^2; raygs: the facing quad of the eye's
+ * frame, z^2 = |z|^2 c^2 / (c^2 + |z|^2) at the ray's hit. Cutoff, alpha,
+ * blending, order, count and rgba as rdx_draw_splats. method:
+ * RDX_METHOD_3DGUT, RDX_METHOD_HTGS or RDX_METHOD_RAYGS. */
+int rdx_draw_ray(rdx_draw* draw, void* stream, int method, uint32_t* const* channels_dev, uint32_t n,
+                 const uint32_t* order_dev, const uint32_t* count_dev, uint32_t capacity, const rdx_camera* camera,
+                 int eye, const rdx_params* params, float* rgba_dev);
 /* The last draw's drawn positions, (splat, tile) entries and entry
  * capacity; overflowed is 0 (a count past the limit fails the draw).
  * Synchronizes. */

@@ -250,6 +250,13 @@ void orc_sh_basis_grad(const double* dir, double (*grad)[3]);
  * rgba (4 planes of width x height doubles, row 0 at the top). draw.c */
 void orc_draw(const uint32_t* words, uint32_t n, const uint32_t* order, uint32_t count, int width, int height,
               double max_std_dev, double min_alpha, double falloff, int linear, double* rgba);
+/* The draw of the 3dgut (1), htgs (2) and raygs (3) words of one eye
+ * (splatDraw.ts:98-181, projections.tsl.ts:310-330, 432-466,
+ * projection.tsl.ts:396-442), FP64; ch2 NULL with one eye. */
+void orc_draw_ray(int method, const uint32_t* ch0, const uint32_t* ch1, const uint32_t* ch2, uint32_t n, int eye,
+                  const uint32_t* order, uint32_t count, const double* from_head, const double* from_head_quat,
+                  const double* proj, int ortho, int width, int height, double max_std_dev, double min_alpha,
+                  double falloff, int linear, double* rgba);
 
 /* ---- The 3dgut and htgs projection words (ray.c) ------------------------- */
 /* SplatProjection.project / write for "3dgut" (method 1), "htgs" (2) and "raygs" (3)

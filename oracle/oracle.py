@@ -146,6 +146,8 @@ def lib() -> C.CDLL:
             "orc_sh_basis_grad": (None, [_dp, _dp]),
             "orc_draw": (None, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_int, C.c_int, _d, _d, _d, C.c_int,
                                 _dp]),
+            "orc_draw_ray": (None, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p,
+                                    C.c_uint32, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, _d, _d, _d, C.c_int, _dp]),
             "orc_project_ray": (None, [C.c_int, C.POINTER(OrcRaySplat), C.POINTER(OrcView), C.POINTER(OrcParams),
                                        C.POINTER(OrcRayOut)]),
             "orc_project_ray_table_f32": (None, [C.c_int, C.POINTER(OrcTableF32), C.POINTER(OrcView),
@@ -436,4 +438,57 @@ def draw(words, order, count: int, width: int, height: int, max_std_dev: float =
     out = np.zeros((4, height, width), dtype=np.float64)
     lib().orc_draw(w.ctypes.data, w.shape[0], o.ctypes.data if o.size else None, count, width, height,
                    max_std_dev, min_alpha, falloff, 1 if linear else 0, dptr(out))
+    return out
+
+
+def eye_from_head(rig, eye: int):
+    """The eye's head -> eye matrix (row-major 4x4) and its rotation as a
+    quaternion (x, y, z, w): SplatCamera.setViews' fromHead and the
+    quaternion of its decomposition (src/render/splatCamera.ts:119-138)."""
+    m = np.asarray(rig.eyes[eye].view, dtype=np.float64) @ np.linalg.inv(np.asarray(rig.head, dtype=np.float64))
+    r = m[:3, :3].copy()
+    sx = np.linalg.norm(r[:, 0]) * (-1.0 if np.linalg.det(r) < 0 else 1.0)
+    r[:, 0] /= sx
+    r[:, 1] /= np.linalg.norm(r[:, 1])
+    r[:, 2] /= np.linalg.norm(r[:, 2])
+    tr = r[0, 0] + r[1, 1] + r[2, 2]
+    if tr > 0:
+        k = 0.5 / np.sqrt(tr + 1.0)
+        q = [(r[2, 1] - r[1, 2]) * k, (r[0, 2] - r[2, 0]) * k, (r[1, 0] - r[0, 1]) * k, 0.25 / k]
+    elif r[0, 0] > r[1, 1] and r[0, 0] > r[2, 2]:
+        k = 2.0 * np.sqrt(1.0 + r[0, 0] - r[1, 1] - r[2, 2])
+        q = [0.25 * k, (r[0, 1] + r[1, 0]) / k, (r[0, 2] + r[2, 0]) / k, (r[2, 1] - r[1, 2]) / k]
+    elif r[1, 1] > r[2, 2]:
+        k = 2.0 * np.sqrt(1.0 + r[1, 1] - r[0, 0] - r[2, 2])
+        q = [(r[0, 1] + r[1, 0]) / k, 0.25 * k, (r[1, 2] + r[2, 1]) / k, (r[0, 2] - r[2, 0]) / k]
+    else:
+        k = 2.0 * np.sqrt(1.0 + r[2, 2] - r[0, 0] - r[1, 1])
+        q = [(r[0, 2] + r[2, 0]) / k, (r[1, 2] + r[2, 1]) / k, 0.25 * k, (r[1, 0] - r[0, 1]) / k]
+    return m, np.asarray(q, dtype=np.float64)
+
+
+def ray_channels(words: np.ndarray):
+    """A ray method's words (n, head + eyes x eye words) as the projected
+    bundle's channels w0, w1 (n, 4) and w2 (n, 2) (SplatProjection.ts:46-53)."""
+    w = np.zeros((words.shape[0], 10), dtype=np.uint32)
+    w[:, :words.shape[1]] = words
+    return np.ascontiguousarray(w[:, :4]), np.ascontiguousarray(w[:, 4:8]), np.ascontiguousarray(w[:, 8:10])
+
+
+def draw_ray(method: str, words, order, count: int, rig, eye: int = 0, max_std_dev: float = float(np.sqrt(8.0)),
+             min_alpha: float = 0.5 / 255.0, falloff: float = 1.0, linear: bool = False) -> np.ndarray:
+    """The draw of a ray method's words (n, head + eyes x eye words) for one
+    eye of `rig`, splats order[:count] back to front. Returns (4, height,
+    width) float64: premultiplied rgb and alpha, row 0 at the top."""
+    c0, c1, c2 = ray_channels(np.asarray(words, dtype=np.uint32))
+    o = np.ascontiguousarray(order, dtype=np.uint32)
+    count = int(min(count, o.size))
+    m, q = eye_from_head(rig, eye)
+    fh = np.ascontiguousarray(m.T.reshape(-1))
+    pr = np.ascontiguousarray(np.asarray(rig.eyes[eye].projection, dtype=np.float64).T.reshape(-1))
+    out = np.zeros((4, rig.height, rig.width), dtype=np.float64)
+    lib().orc_draw_ray(METHODS[method], c0.ctypes.data, c1.ctypes.data, c2.ctypes.data if len(rig.eyes) == 2 else None,
+                       c0.shape[0], eye, o.ctypes.data if o.size else None, count, dptr(fh), dptr(q), dptr(pr),
+                       1 if rig.eyes[eye].ortho else 0, rig.width, rig.height, max_std_dev, min_alpha, falloff,
+                       1 if linear else 0, dptr(out))
     return out

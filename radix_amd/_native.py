@@ -199,6 +199,9 @@ _SIGNATURES = [
     ("rdx_batch_keys", C.c_void_p, [C.c_void_p]),
     ("rdx_batch_run", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     ("rdx_batch_persistent_grid", C.c_int, [C.c_void_p]),
+    ("rdx_draw_ray", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
+      C.POINTER(Camera), C.c_int, C.POINTER(Params), C.c_void_p]),
     ("rdx_batch_status", C.c_int, [C.c_void_p]),
     ("rdx_batch_persistent_phases", C.c_int,
      [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]),

@@ -11,54 +11,6 @@
 
 namespace rdx {
 
-// Column-major 4x4 (three.js Matrix4.elements): element (r, c) at m[4c + r].
-static inline double& cm(double* m, int r, int c) { return m[4 * c + r]; }
-static inline double cm(const double* m, int r, int c) { return m[4 * c + r]; }
-
-static void mul(const double* a, const double* b, double* out) {
-  double r[16];
-  for (int i = 0; i < 4; i++)
-    for (int j = 0; j < 4; j++) {
-      double s = 0.0;
-      for (int k = 0; k < 4; k++) s += cm(a, i, k) * cm(b, k, j);
-      cm(r, i, j) = s;
-    }
-  memcpy(out, r, sizeof r);
-}
-
-// Inverse by Gauss-Jordan elimination with partial pivoting.
-static bool invert(const double* m, double* out) {
-  double a[4][8];
-  for (int r = 0; r < 4; r++)
-    for (int c = 0; c < 4; c++) {
-      a[r][c] = cm(m, r, c);
-      a[r][4 + c] = r == c ? 1.0 : 0.0;
-    }
-  for (int c = 0; c < 4; c++) {
-    int piv = c;
-    for (int r = c + 1; r < 4; r++)
-      if (fabs(a[r][c]) > fabs(a[piv][c])) piv = r;
-    if (a[piv][c] == 0.0) return false;
-    if (piv != c)
-      for (int k = 0; k < 8; k++) std::swap(a[c][k], a[piv][k]);
-    const double inv = 1.0 / a[c][c];
-    for (int k = 0; k < 8; k++) a[c][k] *= inv;
-    for (int r = 0; r < 4; r++) {
-      if (r == c) continue;
-      const double f = a[r][c];
-      if (f == 0.0) continue;
-      for (int k = 0; k < 8; k++) a[r][k] -= f * a[c][k];
-    }
-  }
-  for (int r = 0; r < 4; r++)
-    for (int c = 0; c < 4; c++) cm(out, r, c) = a[r][4 + c];
-  return true;
-}
-
-static double det3(const double* L) {
-  return L[0] * (L[4] * L[8] - L[5] * L[7]) - L[1] * (L[3] * L[8] - L[5] * L[6]) + L[2] * (L[3] * L[7] - L[4] * L[6]);
-}
-
 // Solve L x = b for a row-major 3x3 (Cramer).
 static void solve3(const double* L, const double* b, double* x) {
   const double d = det3(L);

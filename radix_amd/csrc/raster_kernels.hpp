@@ -420,6 +420,53 @@ __device__ __forceinline__ void load_entry(const uint32_t* emit_idx, const uint3
   p2 = pre[3 * (size_t)sp + 2];
 }
 
+// Records of the draw's ray-evaluated methods (draw_ray_kernels.hpp): 8
+// float4 per draw position, the first three laid out as the ewa record's
+// (centre, opacity, cut^2; ..., ex; colour, ey), the rest the method's.
+constexpr int REC_EWA = 0, REC_BOUND = 1, REC_RAYGS = 2;
+constexpr int RAY_REC = 8;
+
+__device__ __forceinline__ void load_entry_ray(const uint32_t* emit_idx, const uint32_t* emit_splat, const float4* pre,
+                                               uint32_t e, float4* r) {
+  const uint32_t sp = emit_splat[emit_idx[e]];
+#pragma unroll
+  for (int j = 0; j < RAY_REC; j++) r[j] = pre[RAY_REC * (size_t)sp + j];
+}
+
+// A ray method's density at the pixel (dx, dy) from the record's centre, or
+// +inf where the pixel is outside the method's quad (so it never blends).
+//  REC_BOUND (3dgut, htgs): the quad is the screen bound, |u| <= qx and
+//  |v| <= qy along its axes; localCross and localDir are affine in the
+//  pixel, z^2 = |localCross|^2 / |localDir|^2 (rayDensity,
+//  projection.tsl.ts:437-442), 1e30 for a null direction.
+//  REC_RAYGS: z = (U, V) / W, affine numerators and denominator of the
+//  pixel ray's hit on the facing quad; inside where W > 0 and |z| <= zmax;
+//  z^2 = |z|^2 c^2 / (c^2 + |z|^2) (projections.tsl.ts:462-466).
+template <int REC>
+__device__ __forceinline__ float ray_z2(float4 r1, float4 r3, float4 r4, float4 r5, float4 r6, float4 r7, float dx,
+                                        float dy) {
+  if constexpr (REC == REC_BOUND) {
+    const float u = dx * r1.x + dy * r1.y, v = dy * r1.x - dx * r1.y;
+    const bool in = fabsf(u) <= r1.z && fabsf(v) <= r3.x;
+    const float lcx = r3.y + dx * r4.x + dy * r4.w, lcy = r3.z + dx * r4.y + dy * r5.x,
+                lcz = r3.w + dx * r4.z + dy * r5.y;
+    const float ldx = r5.z + dx * r6.y + dy * r7.x, ldy = r5.w + dx * r6.z + dy * r7.y,
+                ldz = r6.x + dx * r6.w + dy * r7.z;
+    const float dd = ldx * ldx + ldy * ldy + ldz * ldz;
+    const float z2 = dd <= 0.0f ? 1e30f : (lcx * lcx + lcy * lcy + lcz * lcz) / dd;
+    return in ? z2 : __builtin_inff();
+  } else {
+    const float U = r3.x + dx * r3.y + dy * r3.z, V = r3.w + dx * r4.x + dy * r4.y, W = r4.z + dx * r4.w + dy * r5.x;
+    const float lim = r1.x * W;
+    const bool in = W > 0.0f && fabsf(U) <= lim && fabsf(V) <= lim;
+    const float iw = 1.0f / W;
+    const float zx = U * iw, zy = V * iw;
+    const float zz = zx * zx + zy * zy;
+    const float c2 = r1.y;
+    return in ? zz * c2 / (c2 + zz) : __builtin_inff();
+  }
+}
+
 // The draw splits a tile's entries into chunks of DRAW_CHUNK: one block
 // composites each (a few tiles of a clustered frame hold tens of thousands
 // of entries) into a partial (colour, transmittance), and k_draw_combine
@@ -432,12 +479,17 @@ struct DrawItems {
   float4* partial;             // per item and pixel slot: (C, T) of a tile of several chunks
 };
 
-template <bool DRAW>
+// REC: the record kind (REC_EWA for the compositor and the ewa draw; the
+// ray methods' records are drawn only, DRAW true).
+template <bool DRAW, int REC = REC_EWA>
 __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render(const uint2* range, const uint32_t* emit_idx,
                                                                    const uint32_t* emit_splat, const float4* pre, DiffU u,
                                                                    float* image, float* alpha_out, float* t_final,
                                                                    uint32_t* n_contrib, DrawItems di) {
+  static_assert(DRAW || REC == REC_EWA, "the ray records are drawn only");
+  constexpr int RX = REC == REC_EWA ? 1 : DIFF_BLOCK;  // the ray records' other five float4
   __shared__ float4 s_p0[DIFF_BLOCK], s_p1[DIFF_BLOCK], s_p2[DIFF_BLOCK];
+  __shared__ float4 s_p3[RX], s_p4[RX], s_p5[RX], s_p6[RX], s_p7[RX];
   __shared__ uint32_t s_ov[DIFF_BLOCK];
   uint32_t tile = blockIdx.x;
   uint2 rg;
@@ -458,16 +510,31 @@ __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render(const uint2* range, 
   float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
   uint32_t last = 0;
   bool done = !inside;
-  float4 n0 = make_float4(0.f, 0.f, 0.f, 0.f), n1 = n0, n2 = n0;
-  if (rg.x + threadIdx.x < rg.y) load_entry(emit_idx, emit_splat, pre, rg.x + threadIdx.x, n0, n1, n2);
+  float4 nr[REC == REC_EWA ? 3 : RAY_REC];
+#pragma unroll
+  for (int j = 0; j < (REC == REC_EWA ? 3 : RAY_REC); j++) nr[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+  auto load = [&](uint32_t e) {
+    if constexpr (REC == REC_EWA)
+      load_entry(emit_idx, emit_splat, pre, e, nr[0], nr[1], nr[2]);
+    else
+      load_entry_ray(emit_idx, emit_splat, pre, e, nr);
+  };
+  if (rg.x + threadIdx.x < rg.y) load(rg.x + threadIdx.x);
   for (uint32_t b = rg.x; b < rg.y; b += DIFF_BLOCK) {
     if (__syncthreads_count(!done) == 0) break;
     const uint32_t e = b + threadIdx.x;
-    s_p0[threadIdx.x] = n0;
-    s_p1[threadIdx.x] = n1;
-    s_p2[threadIdx.x] = n2;
-    s_ov[threadIdx.x] = e < rg.y ? quad_mask(q, n0, n1, n2) : 0u;
-    if (e + DIFF_BLOCK < rg.y) load_entry(emit_idx, emit_splat, pre, e + DIFF_BLOCK, n0, n1, n2);
+    s_p0[threadIdx.x] = nr[0];
+    s_p1[threadIdx.x] = nr[1];
+    s_p2[threadIdx.x] = nr[2];
+    if constexpr (REC != REC_EWA) {
+      s_p3[threadIdx.x] = nr[3];
+      s_p4[threadIdx.x] = nr[4];
+      s_p5[threadIdx.x] = nr[5];
+      s_p6[threadIdx.x] = nr[6];
+      s_p7[threadIdx.x] = nr[7];
+    }
+    s_ov[threadIdx.x] = e < rg.y ? quad_mask(q, nr[0], nr[1], nr[2]) : 0u;
+    if (e + DIFF_BLOCK < rg.y) load(e + DIFF_BLOCK);
     __syncthreads();
     const uint32_t cnt = min((uint32_t)DIFF_BLOCK, rg.y - b);
     for (uint32_t c0 = 0; c0 < cnt; c0 += RDX_WAVE) {
@@ -478,7 +545,11 @@ __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render(const uint2* range, 
         m &= m - 1;
         const float4 p0 = s_p0[k], p1 = s_p1[k];
         const float dx = q.px - p0.x, dy = q.py - p0.y;
-        const float z2 = p1.x * dx * dx + p1.y * dx * dy + p1.z * dy * dy;
+        float z2;
+        if constexpr (REC == REC_EWA)
+          z2 = p1.x * dx * dx + p1.y * dx * dy + p1.z * dy * dy;
+        else
+          z2 = ray_z2<REC>(p1, s_p3[k], s_p4[k], s_p5[k], s_p6[k], s_p7[k], dx, dy);
         const float a = DRAW ? draw_alpha(p0.z, gauss(z2), u.falloff) : p0.z * gauss(z2);
         const bool hit = !done && z2 <= p0.w && a >= cut_alpha;
         const float Tn = T * (1.0f - a);

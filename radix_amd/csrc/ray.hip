@@ -13,43 +13,6 @@
 
 namespace rdx {
 
-// The quaternion (x, y, z, w) of a row-major rotation (Shepperd: the
-// largest of the four squared components first).
-static void quat_of(const double* R, float* q) {
-  const double tr = R[0] + R[4] + R[8];
-  double x, y, z, w;
-  if (tr > 0.0) {
-    const double s = 0.5 / std::sqrt(tr + 1.0);
-    w = 0.25 / s;
-    x = (R[7] - R[5]) * s;
-    y = (R[2] - R[6]) * s;
-    z = (R[3] - R[1]) * s;
-  } else if (R[0] > R[4] && R[0] > R[8]) {
-    const double s = 2.0 * std::sqrt(1.0 + R[0] - R[4] - R[8]);
-    w = (R[7] - R[5]) / s;
-    x = 0.25 * s;
-    y = (R[1] + R[3]) / s;
-    z = (R[2] + R[6]) / s;
-  } else if (R[4] > R[8]) {
-    const double s = 2.0 * std::sqrt(1.0 + R[4] - R[0] - R[8]);
-    w = (R[2] - R[6]) / s;
-    x = (R[1] + R[3]) / s;
-    y = 0.25 * s;
-    z = (R[5] + R[7]) / s;
-  } else {
-    const double s = 2.0 * std::sqrt(1.0 + R[8] - R[0] - R[4]);
-    w = (R[3] - R[1]) / s;
-    x = (R[2] + R[6]) / s;
-    y = (R[5] + R[7]) / s;
-    z = 0.25 * s;
-  }
-  const double n = std::sqrt(x * x + y * y + z * z + w * w);
-  q[0] = (float)(x / n);
-  q[1] = (float)(y / n);
-  q[2] = (float)(z / n);
-  q[3] = (float)(w / n);
-}
-
 static double det3r(const double* M) {
   return M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6]) + M[2] * (M[3] * M[7] - M[4] * M[6]);
 }
@@ -98,10 +61,10 @@ int rdx_project_ray(rdx_context* ctx, void* stream, const rdx_splat_table* table
   if (s > 0.0) {
     r.oscale = (float)s;
     for (int k = 0; k < 9; k++) R[k] = r.u.H[k] / s;
-    quat_of(R, r.hq);
+    quat_of_rotation(R, r.hq);
     for (int e = 0; e < camera->eyes; e++) {
       for (int k = 0; k < 9; k++) R[k] = r.u.eye[e].L[k] / s;
-      quat_of(R, r.eq[e]);
+      quat_of_rotation(R, r.eq[e]);
     }
   } else {
     const double d = det3r(r.u.H);
@@ -119,7 +82,7 @@ int rdx_project_ray(rdx_context* ctx, void* stream, const rdx_splat_table* table
       const double* L = r.u.eye[e].L;
       for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) R[3 * a + b] = L[3 * a] * inv[b] + L[3 * a + 1] * inv[3 + b] + L[3 * a + 2] * inv[6 + b];
-      quat_of(R, r.ehq[e]);
+      quat_of_rotation(R, r.ehq[e]);
     }
   }
   const uint32_t n = table->count;

@@ -14,6 +14,10 @@ from . import _native as N
 from .sort import on_stream
 
 
+# RDX_METHOD_3DGUT, RDX_METHOD_HTGS, RDX_METHOD_RAYGS
+RAY_METHODS = {"3dgut": 1, "htgs": 2, "raygs": 3}
+
+
 class Drawer:
     """rdx_draw_*: composite sorted splats into an RGBA image on the device."""
 
@@ -54,6 +58,42 @@ class Drawer:
         N.check(N.lib().rdx_draw_splats(self._h, _stream(stream), C.c_void_p(words.data_ptr()), n,
                                         C.c_void_p(order.data_ptr()), C.c_void_p(count.data_ptr()), order.numel(),
                                         width, height, C.byref(params.desc()), C.c_void_p(out.data_ptr())))
+        return out
+
+    @on_stream
+    def draw_ray(self, method: str, channels, order, count, rig, params, eye: int = 0, out=None, stream=None):
+        """The draw of a ray method's words ("3dgut", "htgs" or "raygs"):
+        channels as Projector.project_ray returns them (w0, w1 of (n, 4) and
+        with two eyes w2 of (n, 2) int32 device tensors), projected for
+        `rig`, whose eye `eye` is drawn at rig's render size; order and count
+        as in draw(). Returns (4, height, width) float32, premultiplied, row
+        0 at the top."""
+        import torch
+
+        from .sort import _stream
+
+        if method not in RAY_METHODS:
+            raise ValueError(f"draw_ray: method {method!r} is not one of {sorted(RAY_METHODS)}")
+        m = RAY_METHODS[method]
+        eyes = len(rig.eyes)
+        need = 3 if eyes == 2 else 2
+        if len(channels) < need:
+            raise ValueError(f"draw_ray: {need} word channels for {eyes} eye(s)")
+        for name, t in [("order", order), ("count", count)] + [(f"w{k}", c) for k, c in enumerate(channels[:need])]:
+            if t.dtype != torch.int32 or not t.is_cuda or not t.is_contiguous():
+                raise TypeError(f"{name} must be a contiguous int32 device tensor")
+        n = channels[0].numel() // 4
+        if channels[1].numel() < 4 * n or (eyes == 2 and channels[2].numel() < 2 * n):
+            raise ValueError("draw_ray: word channels shorter than w0")
+        width, height = int(rig.width), int(rig.height)
+        if out is None:
+            out = torch.empty((4, height, width), dtype=torch.float32, device=channels[0].device)
+        elif out.dtype != torch.float32 or out.numel() < 4 * width * height or not out.is_contiguous():
+            raise TypeError("out must be a contiguous float32 tensor of 4 x height x width")
+        cp = (C.c_void_p * 3)(*[c.data_ptr() for c in channels[:need]] + [None] * (3 - need))
+        N.check(N.lib().rdx_draw_ray(self._h, _stream(stream), m, cp, n, C.c_void_p(order.data_ptr()),
+                                     C.c_void_p(count.data_ptr()), order.numel(), C.byref(rig.desc()), eye,
+                                     C.byref(params.desc()), C.c_void_p(out.data_ptr())))
         return out
 
     def stats(self, stream=None) -> dict:
