@@ -410,15 +410,21 @@ def bench_loaders(steps: int, n: int = 4194304) -> dict:
 def bench_ray(steps: int, proj, table, rig) -> dict:
     """The ray methods' projection words over the plain_4m table: the f32
     SH3 splat in (236 B), a key and 8 words out (36 B; raygs writes 7 of them
-    and a zero)."""
+    and a zero). Then each method's frame sorted (rdx_sort_splats) and drawn
+    (rdx_draw_ray) at 1920x1080."""
     import torch
 
     from radix_amd import Params
     from radix_amd import _native as N
+    from radix_amd.draw import Drawer
+    from radix_amd.sort import SplatSorter
 
     import ctypes as C
 
     prm = Params(sort_bits=32)
+    sorter = SplatSorter(table.count, context=proj.context)
+    drawer = Drawer(context=proj.context)
+    image = torch.empty((4, int(rig.height), int(rig.width)), dtype=torch.float32, device="cuda")
     n = table.count
     chans = [torch.empty((n, 4), dtype=torch.int32, device="cuda") for _ in range(2)]
     keys = torch.empty(n, dtype=torch.int32, device="cuda")
@@ -438,7 +444,16 @@ def bench_ray(steps: int, proj, table, rig) -> dict:
         N.lib().rdx_context_profile(proj.context.handle, 0)
         out[name] = {"ms": round(ms, 4), "kernel_ms": round(k_ms, 4), "Gsplats_per_s": round(n / ms / 1e6, 3),
                      "hbm_frac": round(272 * n / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-    out["workload"] = "3dgut, htgs and raygs words of the plain_4m table (4,194,304 f32 SH3 splats, 1920x1080), 32-bit keys"
+        sorter.sort(keys, n, 32)
+        order = sorter.values
+        draw = lambda: drawer.draw_ray(name, chans, order, sorter.count, rig, prm, out=image)  # noqa: E731
+        draw()
+        draw_ms = time_events(draw, steps)
+        st = drawer.stats()
+        out[name].update({"draw_ms": round(draw_ms, 4), "drawn": st["drawn"], "draw_entries": st["entries"]})
+    drawer.close()
+    out["workload"] = ("3dgut, htgs and raygs words of the plain_4m table (4,194,304 f32 SH3 splats, 1920x1080), "
+                       "32-bit keys; draw_ms: the sorted frame drawn (every fragment, premultiplied over)")
     out["bytes_per_splat"] = 272
     return out
 

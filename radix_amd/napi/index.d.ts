@@ -143,6 +143,9 @@ export class SplatDrawer {
   constructor(settings?: { tStop?: number; linearOutput?: boolean; maxEntriesPerSplat?: number } & OnStream);
   draw(words: DeviceBuffer, n: number, order: DeviceBuffer, count: DeviceBuffer, capacity: number, width: number,
     height: number, params: SplatProjectionParams, rgba: DeviceBuffer, options?: OnStream): void;
+  drawRay(method: '3dgut' | 'htgs' | 'raygs', channels: DeviceBuffer[], n: number, order: DeviceBuffer,
+    count: DeviceBuffer, capacity: number, camera: SplatCameraViews, params: SplatProjectionParams, rgba: DeviceBuffer,
+    options?: { eye?: number } & OnStream): void;
   stats(options?: OnStream): { drawn: number; entries: number; capacity: number };
   dispose(): void;
 }

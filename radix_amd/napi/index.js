@@ -75,6 +75,12 @@ class GpuStream {
 
 const streamOf = (options) => (options && options.stream ? options.stream.handle : undefined);
 const handleOf = (b) => (b && b.handle !== undefined ? b.handle : b === undefined ? null : b);
+// The ray-evaluated methods' codes (RDX_METHOD_3DGUT, _HTGS, _RAYGS)
+const rayMethod = (method) => {
+  const code = { '3dgut': 1, htgs: 2, raygs: 3 }[method];
+  if (code === undefined) throw new Error(`SplatProjection: unknown method ${method}`);
+  return code;
+};
 
 // Device memory (the storage of a WgpuBuffer, src/wgpu/WgpuBuffer.ts)
 class DeviceBuffer {
@@ -205,9 +211,7 @@ class SplatProjector {
   // The words of "3dgut", "htgs" or "raygs" into the projected bundle's
   // channels w0, w1 (and w2 with two eyes) (SplatProjection.ts:46-53, 141-151)
   projectRay(table, camera, params, method, { channels, keys, at = 0, stream }) {
-    const code = { '3dgut': 1, htgs: 2, raygs: 3 }[method];
-    if (code === undefined) throw new Error(`SplatProjection: unknown method ${method}`);
-    native.projectRay(tableDesc(table), camera, params || {}, code, channels.map(handleOf), handleOf(keys), at,
+    native.projectRay(tableDesc(table), camera, params || {}, rayMethod(method), channels.map(handleOf), handleOf(keys), at,
       streamOf({ stream }));
   }
 }
@@ -261,6 +265,14 @@ class SplatDrawer {
   draw(words, n, order, count, capacity, width, height, params, rgba, options) {
     native.drawSplats(this.handle, handleOf(words), n, handleOf(order), handleOf(count), capacity, width, height,
       params || {}, handleOf(rgba), streamOf(options));
+  }
+
+  // One eye of the "3dgut", "htgs" or "raygs" words projectRay wrote
+  // (channels w0, w1 and, with two eyes, w2) with the camera they were
+  // projected for, at its render size
+  drawRay(method, channels, n, order, count, capacity, camera, params, rgba, { eye = 0, stream } = {}) {
+    native.drawRay(this.handle, rayMethod(method), channels.map(handleOf), n, handleOf(order), handleOf(count),
+      capacity, camera, eye, params || {}, handleOf(rgba), streamOf({ stream }));
   }
 
   stats(options) {

@@ -939,6 +939,35 @@ napi_value DrawSplats(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// drawRay(draw, method, channels[], n, order, count, capacity, camera, eye, params, rgba, stream?): one eye of
+// the 3dgut (1), htgs (2) or raygs (3) words at the camera's render size
+napi_value DrawRay(napi_env env, napi_callback_info info) {
+  ARGS(12);
+  Call c(env, "SplatDrawer.drawRay", argv[11]);
+  DrawH* d = external<DrawH>(env, argv[0], T_DRAW);
+  if (!d || !d->h) return throw_error(env, "SplatDrawer.drawRay: not a drawer");
+  rdx_camera cam;
+  rdx_params p;
+  read_camera(c, argv[7], &cam);
+  read_params(env, argv[9], &p);
+  const uint32_t n = count_arg(c, argv[3], "n");
+  const uint32_t cap = count_arg(c, argv[6], "capacity");
+  const double w = cam.size[0], h = cam.size[1];
+  if (c.ok && !(w >= 1 && h >= 1 && w <= 16384 && h <= 16384))
+    c.fail("SplatDrawer.drawRay: render size %gx%g (whole pixels, 1..16384)", w, h);
+  uint32_t* channels[3] = {nullptr, nullptr, nullptr};
+  const char* names[3] = {"channels[0]", "channels[1]", "channels[2]"};
+  for (int k = 0; c.ok && k < (cam.eyes == 2 ? 3 : 2); k++)
+    channels[k] = (uint32_t*)c.buf(element(env, argv[2], (uint32_t)k), names[k], (k == 2 ? 8ull : 16ull) * n, n == 0);
+  const uint32_t* order = (const uint32_t*)c.buf(argv[4], "order", 4ull * cap, cap == 0);
+  const uint32_t* cnt = (const uint32_t*)c.buf(argv[5], "count", 4);
+  float* rgba = c.ok ? (float*)c.buf(argv[10], "rgba", 16ull * (uint64_t)w * (uint64_t)h) : nullptr;
+  if (c.ok)
+    c.result(rdx_draw_ray(d->h, c.stream, (int)num(env, argv[1], 0), channels, n, order, cnt, cap, &cam,
+                          (int)num(env, argv[8], 0), &p, rgba));
+  return nullptr;
+}
+
 // drawStats(draw, stream?) -> {drawn, entries, capacity}
 napi_value DrawStats(napi_env env, napi_callback_info info) {
   ARGS(2);
@@ -1140,6 +1169,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"rasterDestroy", RasterDestroy},
       {"drawCreate", DrawCreate},
       {"drawSplats", DrawSplats},
+      {"drawRay", DrawRay},
       {"drawStats", DrawStats},
       {"drawDestroy", DrawDestroy},
       {"batchCreate", BatchCreate},

@@ -97,7 +97,7 @@ async function cpuBoundsChecks() {
 
 // Bounds checks against real device buffers: each call throws, the device untouched
 async function gpuBoundsChecks() {
-  await check('bounds: short device buffers throw (sorts, projection, loaders, compositor, draw)', () => {
+  await check('bounds: short device buffers throw (sorts, projection, loaders, compositor, draws)', () => {
     const small = new DeviceBuffer(40);
     const big = new DeviceBuffer(4096);
     const planes = new DeviceBuffer(14 * 4 * 8);
@@ -112,7 +112,9 @@ async function gpuBoundsChecks() {
       || throwsWith(() => untransposeBytes(big, 100, 16, big, 3000), 'out holds 4096 bytes, 4600 needed')
       || throwsWith(() => native.spzDecode(big.handle, 100, 10, 3, 0, 12, small.handle, 10), 'planes holds 40 bytes, 560 needed')
       || throwsWith(() => rast.forward(small, 100, camera, {}, { image: big }), 'params holds 40 bytes, 23600 needed')
-      || throwsWith(() => drawer.draw(big, 10, big, big, 10, 16, 16, {}, small), 'rgba holds 40 bytes, 4096 needed');
+      || throwsWith(() => drawer.draw(big, 10, big, big, 10, 16, 16, {}, small), 'rgba holds 40 bytes, 4096 needed')
+      || throwsWith(() => drawer.drawRay('htgs', [big, small], 10, big, big, 10, camera, {}, big), 'channels[1] holds 40 bytes, 160 needed')
+      || throwsWith(() => drawer.drawRay('htgs', [big, big], 10, big, big, 10, camera, {}, small), 'rgba holds 40 bytes, 4096 needed');
     [small, big, planes].forEach((b) => b.dispose());
     rast.dispose();
     drawer.dispose();
@@ -539,6 +541,24 @@ async function gpuFixture(dir) {
     const stats = drawer.stats();
     drawer.dispose();
     [words, order, count, rgba].forEach((b) => b.dispose());
+    return stats.drawn === spec.count ? undefined : `drawn ${stats.drawn} of ${spec.count}`;
+  });
+  await check('SplatDrawer.drawRay: the fixture raygs frame, second eye of a stereo camera', () => {
+    const spec = JSON.parse(fs.readFileSync(path.join(dir, 'ray.json'), 'utf8'));
+    const channels = [0, 1, 2].map((k) => DeviceBuffer.fromArray(readU32(path.join(dir, `ray_w${k}.u32`))));
+    const order = DeviceBuffer.fromArray(readU32(path.join(dir, 'ray_order.u32')));
+    const count = DeviceBuffer.fromArray(new Uint32Array([spec.count]));
+    const [width, height] = spec.camera.size;
+    const rgba = new DeviceBuffer(16 * width * height);
+    const drawer = new SplatDrawer();
+    drawer.drawRay(spec.method, channels, spec.n, order, count, spec.count, spec.camera, {}, rgba, { eye: spec.eye });
+    fs.writeFileSync(path.join(dir, 'ray_rgba.f32'), Buffer.from(rgba.read(0, 4 * width * height, Float32Array).buffer));
+    const stats = drawer.stats();
+    const unknown = throwsWith(() => drawer.drawRay('ewa', channels, spec.n, order, count, spec.count, spec.camera, {},
+      rgba), 'unknown method');
+    drawer.dispose();
+    [...channels, order, count, rgba].forEach((b) => b.dispose());
+    if (unknown) return unknown;
     return stats.drawn === spec.count ? undefined : `drawn ${stats.drawn} of ${spec.count}`;
   });
   await check('SplatProjector: projectSort of the fixture covSplats table (packedSh bundles)', () => {

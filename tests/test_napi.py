@@ -33,7 +33,7 @@ def test_napi_gpu_checks():
 @pytest.mark.gpu
 def test_napi_adjoint_and_draw_match_the_fp64_oracle(tmp_path):
     """The Node binding's compositor (Rasterizer forward + adjoint, on a
-    GpuStream) and draw (SplatDrawer) on fixtures written here: the adjoint
+    GpuStream), draw and ray draw (SplatDrawer) on fixtures written here: the adjoint
     of the 400-splat scene's L2 loss within 1e-3 of the FP64 host gradient
     per parameter kind, the image and the drawn frame to the compositor's
     and draw's parity bar."""
@@ -79,6 +79,17 @@ def test_napi_adjoint_and_draw_match_the_fp64_oracle(tmp_path):
     ccam = {"head": list(cam.cm(crig.head)), "size": [640, 480], "ndcZMin": 0,
             "eyes": [{"view": list(cam.cm(crig.eyes[0].view)), "projection": list(cam.cm(crig.eyes[0].projection))}]}
     (tmp_path / "cov.json").write_text(json.dumps({"n": nc, "camera": ccam}))
+    # the ray draw: a stereo raygs frame (channels w0, w1, w2) projected and sorted by the oracle, eye 1
+    rrig = cam.stereo(cam.orbit_world(cam.random_direction(47), 4.0), 0.064, 70.0, 256, 192)
+    rproj = O.project_ray_f32("raygs", planes[:, :8000].copy(), 0, O.view_from_rig(rrig), O.params(sort_bits=32))
+    rorder, rvisible = O.sort(rproj["keys"], 32)
+    for k, ch in enumerate(O.ray_channels(rproj["words"])):
+        ch.tofile(tmp_path / f"ray_w{k}.u32")
+    np.ascontiguousarray(rorder[:rvisible], dtype=np.uint32).tofile(tmp_path / "ray_order.u32")
+    rcam = {"head": list(cam.cm(rrig.head)), "size": [256, 192], "ndcZMin": 0,
+            "eyes": [{"view": list(cam.cm(e.view)), "projection": list(cam.cm(e.projection))} for e in rrig.eyes]}
+    (tmp_path / "ray.json").write_text(json.dumps({"n": 8000, "count": int(rvisible), "method": "raygs", "eye": 1,
+                                                   "camera": rcam}))
     node = shutil.which("node")
     if node is None:
         pytest.skip("node is not installed")
@@ -102,6 +113,11 @@ def test_napi_adjoint_and_draw_match_the_fp64_oracle(tmp_path):
     rgba = np.fromfile(tmp_path / "rgba.f32", dtype=np.float32).reshape(4, 240, 320).astype(np.float64)
     dref = O.draw(words, ordering[:visible], int(visible), 320, 240)
     dd = np.abs(rgba - dref)
+    assert np.mean(dd.max(axis=0) > 1e-4) < 0.005 and np.median(dd) < 1e-6
+    rgba = np.fromfile(tmp_path / "ray_rgba.f32", dtype=np.float32).reshape(4, 192, 256).astype(np.float64)
+    rref = O.draw_ray("raygs", rproj["words"], rorder[:rvisible], int(rvisible), rrig, eye=1)
+    assert rref[3].max() > 0.3
+    dd = np.abs(rgba - rref)
     assert np.mean(dd.max(axis=0) > 1e-4) < 0.005 and np.median(dd) < 1e-6
     # the covSplats table through the binding equals the Python host's call of the same kernels,
     # and its order is SortKeys.sort of its keys
