@@ -167,7 +167,7 @@ static int draw_run(Draw* d, hipStream_t s, const DiffU& u, const uint32_t* coun
                        (float4*)d->partial.ptr};
     hipLaunchKernelGGL((k_diff_render<true, REC>), dim3(d->max_items), dim3(DIFF_BLOCK), 0, s, (const uint2*)d->range.ptr,
                        (const uint32_t*)d->emit_idx.ptr, (const uint32_t*)d->emit_splat.ptr, (const float4*)d->pre.ptr, u,
-                       rgba_dev, rgba_dev + 3 * plane, nullptr, nullptr, di);
+                       rgba_dev, rgba_dev + 3 * plane, nullptr, nullptr, di, DiffChunks{});
     hipLaunchKernelGGL(k_draw_combine, dim3(tiles), dim3(DIFF_BLOCK), 0, s, (const uint2*)d->range.ptr,
                        (const uint32_t*)d->item_start.ptr, (const float4*)d->partial.ptr, u, rgba_dev,
                        rgba_dev + 3 * plane);

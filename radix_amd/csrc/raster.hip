@@ -20,7 +20,10 @@ struct Raster {
   SortScratch sort_splats, sort_entries;
   Buffer pre, keys, touch, rect, order, counts, offsets, bsum, tile_key, emit_idx, emit_splat, range, t_final,
       n_contrib, gentry, gpix;
-  uint32_t cap = 0;  // entries
+  // the forward's chunk records and the adjoint's (tile, chunk) items
+  Buffer chunk_rec, tile_reach, item_tile, item_chunk, items;
+  uint32_t cap = 0;   // entries
+  uint32_t rows = 0;  // chunk record rows (and the adjoint's largest item count) of the last forward
   // the last forward
   DiffU u;
   const float* params = nullptr;
@@ -29,8 +32,9 @@ struct Raster {
   void release() {
     sort_splats.release();
     sort_entries.release();
-    Buffer* b[] = {&pre, &keys, &touch, &rect, &order, &counts, &offsets, &bsum, &tile_key, &emit_idx, &emit_splat,
-                   &range, &t_final, &n_contrib, &gentry, &gpix};
+    Buffer* b[] = {&pre,   &keys,      &touch,     &rect,       &order,     &counts,     &offsets,
+                   &bsum,  &tile_key,  &emit_idx,  &emit_splat, &range,     &t_final,    &n_contrib,
+                   &gentry, &gpix,     &chunk_rec, &tile_reach, &item_tile, &item_chunk, &items};
     for (Buffer* x : b) x->release();
   }
 };
@@ -104,7 +108,8 @@ static int reserve(Raster* r, uint32_t n, const DiffU& u) {
       (rc = r->bsum.ensure((nn / DSCAN_TILE + 2) * 4)))
     return rc;
   if ((rc = r->range.ensure(tiles * sizeof(uint2))) || (rc = r->t_final.ensure(px * 4)) || (rc = r->n_contrib.ensure(px * 4)) ||
-      (rc = r->gpix.ensure(nn * DIFF_PIX_GRADS * 4)))
+      (rc = r->gpix.ensure(nn * DIFF_PIX_GRADS * 4)) || (rc = r->tile_reach.ensure(tiles * 4)) ||
+      (rc = r->items.ensure(4)))
     return rc;
   if ((rc = r->sort_splats.reserve(n, 1, tiles_of(p, n)))) return rc;
   return RDX_OK;
@@ -182,6 +187,7 @@ int rdx_raster_forward(rdx_raster* raster, void* stream, const float* params_dev
   if (!raster || !camera || !params || !image_dev || (n && !params_dev))
     return fail(RDX_E_INVALID, "rdx_raster_forward: NULL argument");
   Raster* r = reinterpret_cast<Raster*>(raster);
+  r->have_forward = false;  // a forward that fails leaves nothing to differentiate
   hipStream_t s = (hipStream_t)stream;
   DiffU u;
   if (int rc = make_diff_uniforms(camera, params, r->set.sh_degree, r->set.t_stop, &u)) return rc;
@@ -215,6 +221,15 @@ int rdx_raster_forward(rdx_raster* raster, void* stream, const float* params_dev
   }
   if (int rc = size_entries(r, s, counts + C_ENTRIES, n)) return rc;
   {
+    // chunk records of the forward, work items of the adjoint: one per
+    // (tile, chunk) at most
+    const uint32_t rows = diff_chunk_rows(tiles, r->cap);
+    if (int rc = r->chunk_rec.ensure((size_t)rows * DIFF_BLOCK * sizeof(float4))) return rc;
+    if (int rc = r->item_tile.ensure((size_t)rows * 4)) return rc;
+    if (int rc = r->item_chunk.ensure((size_t)rows * 4)) return rc;
+    r->rows = rows;
+  }
+  {
     Stage st(c, s, "diff_scan");
     hipLaunchKernelGGL(k_diff_scan_final, dim3(blocks), dim3(DSCAN_BLOCK), 0, s, (const uint32_t*)r->order.ptr,
                        (const uint32_t*)r->touch.ptr, counts + C_VISIBLE, (const uint32_t*)r->bsum.ptr,
@@ -245,7 +260,8 @@ int rdx_raster_forward(rdx_raster* raster, void* stream, const float* params_dev
     Stage st(c, s, "diff_render");
     hipLaunchKernelGGL(k_diff_render<false>, dim3(tiles), dim3(DIFF_BLOCK), 0, s, (const uint2*)r->range.ptr,
                        (const uint32_t*)r->emit_idx.ptr, (const uint32_t*)r->emit_splat.ptr, (const float4*)r->pre.ptr, u,
-                       image_dev, alpha_dev, (float*)r->t_final.ptr, (uint32_t*)r->n_contrib.ptr, DrawItems{});
+                       image_dev, alpha_dev, (float*)r->t_final.ptr, (uint32_t*)r->n_contrib.ptr, DrawItems{},
+                       DiffChunks{(float4*)r->chunk_rec.ptr, (uint32_t*)r->tile_reach.ptr});
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "rdx_raster_forward");
@@ -268,10 +284,13 @@ int rdx_raster_backward(rdx_raster* raster, void* stream, const float* grad_imag
   uint32_t* counts = (uint32_t*)r->counts.ptr;
   {
     Stage st(c, s, "diff_render_bwd");
-    hipLaunchKernelGGL(k_diff_render_bwd, dim3(tiles), dim3(DIFF_BLOCK), 0, s, (const uint2*)r->range.ptr,
+    hipLaunchKernelGGL(k_diff_bwd_items, dim3(1), dim3(1024), 0, s, (const uint32_t*)r->tile_reach.ptr, tiles,
+                       (uint32_t*)r->item_tile.ptr, (uint32_t*)r->item_chunk.ptr, (uint32_t*)r->items.ptr);
+    const BwdItems bi{(const uint32_t*)r->items.ptr, (const uint32_t*)r->item_tile.ptr,
+                      (const uint32_t*)r->item_chunk.ptr, (const float4*)r->chunk_rec.ptr};
+    hipLaunchKernelGGL(k_diff_render_bwd, dim3(r->rows), dim3(DIFF_BLOCK), 0, s, (const uint2*)r->range.ptr,
                        (const uint32_t*)r->emit_idx.ptr, (const uint32_t*)r->emit_splat.ptr, (const float4*)r->pre.ptr, u,
-                       (const float*)r->t_final.ptr, (const uint32_t*)r->n_contrib.ptr, grad_image_dev,
-                       (float*)r->gentry.ptr);
+                       (const uint32_t*)r->n_contrib.ptr, grad_image_dev, (float*)r->gentry.ptr, bi);
   }
   if (n) {
     Stage st(c, s, "diff_gather");

@@ -479,13 +479,35 @@ struct DrawItems {
   float4* partial;             // per item and pixel slot: (C, T) of a tile of several chunks
 };
 
+// The compositor's adjoint runs per (tile, chunk of DIFF_CHUNK entries of
+// the tile's list), so a tile that holds thousands of entries is spread over
+// as many blocks (one block per tile left the few densest tiles running
+// three times as long as the average one). The forward, which composites a
+// tile's list in order in one block (its early stop is sequential), records
+// at the end of each chunk, per pixel, the colour that chunk's contributors
+// added and the transmittance after them; a chunk's adjoint starts from
+// those: its pixels' transmittance behind the chunk, and the colour of the
+// later chunks over it. Chunk c of the tile whose list starts at entry x has
+// record row tile + x / DIFF_CHUNK + c (distinct for all (tile, chunk): a
+// tile of e entries has at most max(1, ceil(e / DIFF_CHUNK)) chunks), so the
+// rows need no scan: at most tiles + entries / DIFF_CHUNK + 1.
+constexpr uint32_t DIFF_CHUNK = DIFF_BLOCK;  // the forward's batches of entries
+struct DiffChunks {
+  float4* rec;           // [row][DIFF_BLOCK]: (colour of the chunk's contributors, transmittance after them)
+  uint32_t* tile_reach;  // per tile: its pixels' furthest contributor (the largest n_contrib)
+};
+__host__ __device__ constexpr uint32_t diff_chunk_rows(uint32_t tiles, uint32_t entries) {
+  return tiles + entries / DIFF_CHUNK + 1u;
+}
+
 // REC: the record kind (REC_EWA for the compositor and the ewa draw; the
 // ray methods' records are drawn only, DRAW true).
 template <bool DRAW, int REC = REC_EWA>
 __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render(const uint2* range, const uint32_t* emit_idx,
                                                                    const uint32_t* emit_splat, const float4* pre, DiffU u,
                                                                    float* image, float* alpha_out, float* t_final,
-                                                                   uint32_t* n_contrib, DrawItems di) {
+                                                                   uint32_t* n_contrib, DrawItems di, DiffChunks dc) {
+  static_assert(DIFF_CHUNK == DIFF_BLOCK, "a chunk is one batch of the forward");
   static_assert(DRAW || REC == REC_EWA, "the ray records are drawn only");
   constexpr int RX = REC == REC_EWA ? 1 : DIFF_BLOCK;  // the ray records' other five float4
   __shared__ float4 s_p0[DIFF_BLOCK], s_p1[DIFF_BLOCK], s_p2[DIFF_BLOCK];
@@ -508,6 +530,7 @@ __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render(const uint2* range, 
   const bool inside = q.x < u.width && q.y < u.height;
   const float cut_alpha = u.min_alpha;
   float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
+  float Cc0 = 0.0f, Cc1 = 0.0f, Cc2 = 0.0f;  // this batch's contributors
   uint32_t last = 0;
   bool done = !inside;
   float4 nr[REC == REC_EWA ? 3 : RAY_REC];
@@ -562,15 +585,34 @@ __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render(const uint2* range, 
         if (ballot(use)) {
           const float4 c = s_p2[k];
           const float wgt = use ? a * T : 0.0f;
-          C0 = fmaf(c.x, wgt, C0);
-          C1 = fmaf(c.y, wgt, C1);
-          C2 = fmaf(c.z, wgt, C2);
+          Cc0 = fmaf(c.x, wgt, Cc0);
+          Cc1 = fmaf(c.y, wgt, Cc1);
+          Cc2 = fmaf(c.z, wgt, Cc2);
           T = use ? Tn : T;
           last = use ? b - rg.x + k + 1 : last;
         }
       }
     }
+    if constexpr (!DRAW)
+      dc.rec[((size_t)tile + rg.x / DIFF_CHUNK + (b - rg.x) / DIFF_CHUNK) * DIFF_BLOCK + threadIdx.x] =
+          make_float4(Cc0, Cc1, Cc2, T);
+    C0 += Cc0;
+    C1 += Cc1;
+    C2 += Cc2;
+    Cc0 = Cc1 = Cc2 = 0.0f;
     __syncthreads();
+  }
+  if constexpr (!DRAW) {
+    // the tile's reach: how many of its entries the adjoint walks
+    uint32_t r = last;
+    for (int off = 32; off >= 1; off >>= 1) r = max(r, (uint32_t)__shfl_xor((int)r, off));
+    if (q.lane == 0) s_ov[q.w] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t m = 0;
+      for (int v = 0; v < DIFF_BLOCK / RDX_WAVE; v++) m = max(m, s_ov[v]);
+      dc.tile_reach[tile] = m;
+    }
   }
   if constexpr (DRAW) {
     if (nchunks > 1) {
@@ -609,15 +651,48 @@ __device__ __forceinline__ void row_sums_to_last(float* v) {
     v[j] += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v[j]), 0x118, 0xf, 0xf, true));
 }
 
-// The adjoint of the renderer: per tile, back to front over each pixel's
-// contributors (the pixel layout and the entry culling of k_diff_render);
-// every entry's pixel-side partials summed over the tile's pixels (per row
-// of 16 lanes with DPP, then the 16 rows of the tile in a fixed order: no
-// float atomics, reproducible) into gentry[emission index][9].
+// One block: the adjoint's work items, (tile, chunk) for each tile's chunks
+// up to its reach (at least one: a tile's first item writes the zero
+// partials of the entries no pixel reached), in tile order; their count to
+// *items.
+static __global__ __launch_bounds__(1024) void k_diff_bwd_items(const uint32_t* tile_reach, uint32_t tiles,
+                                                                uint32_t* item_tile, uint32_t* item_chunk, uint32_t* items) {
+  __shared__ uint32_t s_scan[1024 / RDX_WAVE];
+  uint32_t carry = 0;
+  for (uint32_t b0 = 0; b0 < tiles; b0 += 1024) {
+    const uint32_t t = b0 + threadIdx.x;
+    const uint32_t k = t < tiles ? max(1u, (tile_reach[t] + DIFF_CHUNK - 1) / DIFF_CHUNK) : 0u;
+    uint32_t sum;
+    const uint32_t at = carry + block_exclusive_sum<1024>(k, s_scan, &sum);
+    for (uint32_t c = 0; c < k; c++) {
+      item_tile[at + c] = t;
+      item_chunk[at + c] = c;
+    }
+    carry += sum;
+  }
+  if (threadIdx.x == 0) *items = carry;
+}
+
+struct BwdItems {
+  const uint32_t* items;       // [1]: the number of work items
+  const uint32_t* item_tile;   // per item: its tile
+  const uint32_t* item_chunk;  // per item: its chunk of the tile's entries
+  const float4* rec;           // the forward's chunk records (DiffChunks::rec)
+};
+
+// The adjoint of the renderer: per (tile, chunk) item, back to front over
+// each pixel's contributors within the chunk (the pixel layout and the entry
+// culling of k_diff_render), starting from the forward's record of the
+// chunk: the pixel's transmittance after the chunk's contributors, and B,
+// the colour composited behind them (the later chunks' colours over that
+// transmittance); every entry's pixel-side partials summed over the tile's
+// pixels (per row of 16 lanes with DPP, then the 16 rows of the tile in a
+// fixed order: no float atomics, reproducible) into gentry[emission
+// index][9]. Each entry is written by the item of its chunk alone.
 static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render_bwd(const uint2* range, const uint32_t* emit_idx,
                                                                        const uint32_t* emit_splat, const float4* pre, DiffU u,
-                                                                       const float* t_final, const uint32_t* n_contrib,
-                                                                       const float* grad_image, float* gentry) {
+                                                                       const uint32_t* n_contrib, const float* grad_image,
+                                                                       float* gentry, BwdItems bi) {
   constexpr int WAVES = DIFF_BLOCK / RDX_WAVE;
   constexpr int BATCH = DIFF_BWD_BATCH;
   constexpr int SLOTS = WAVES * 4;  // a row of 16 lanes per slot
@@ -629,12 +704,14 @@ static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render_bwd(const uin
   __shared__ uint32_t s_max[WAVES];
   static_assert(GWORDS % 4 == 0, "partials cleared as float4");
   static_assert(BATCH % RDX_WAVE == 0 && BATCH <= DIFF_BLOCK, "whole waves of entries");
-  const uint32_t tile = blockIdx.x;
+  if (blockIdx.x >= *bi.items) return;  // uniform per block
+  const uint32_t tile = bi.item_tile[blockIdx.x], chunk = bi.item_chunk[blockIdx.x];
   const DiffQuad q = diff_quad(tile, u);
   const bool inside = q.x < u.width && q.y < u.height;
   const uint2 rg = range[tile];
+  const uint32_t c0e = chunk * DIFF_CHUNK;  // the chunk's first entry, in the tile's list
+  const uint32_t cs = rg.x + c0e, ce = min(cs + DIFF_CHUNK, rg.y);
   const size_t p = (size_t)q.y * u.width + q.x, plane_px = (size_t)u.width * u.height;
-  float T = inside ? t_final[p] : 1.0f;
   const uint32_t nc = inside ? n_contrib[p] : 0u;
   float dC0 = 0.0f, dC1 = 0.0f, dC2 = 0.0f;
   if (inside) {
@@ -642,7 +719,24 @@ static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render_bwd(const uin
     dC1 = grad_image[plane_px + p];
     dC2 = grad_image[2 * plane_px + p];
   }
-  float B0 = 0.0f, B1 = 0.0f, B2 = 0.0f;
+  // The pixel behind this chunk: a pixel whose contributors reach into or
+  // past it starts from its transmittance after the chunk (>= t_stop > 0)
+  // and B = the later chunks' colours (summed back to front) over it.
+  float T = 1.0f, B0 = 0.0f, B1 = 0.0f, B2 = 0.0f;
+  if (nc > c0e) {
+    const float4* rp = bi.rec + ((size_t)tile + rg.x / DIFF_CHUNK) * DIFF_BLOCK + threadIdx.x;
+    T = rp[(size_t)chunk * DIFF_BLOCK].w;
+    for (uint32_t m = (nc - 1u) / DIFF_CHUNK; m > chunk; m--) {
+      const float4 v = rp[(size_t)m * DIFF_BLOCK];
+      B0 += v.x;
+      B1 += v.y;
+      B2 += v.z;
+    }
+    const float iT = 1.0f / T;
+    B0 *= iT;
+    B1 *= iT;
+    B2 *= iT;
+  }
   // the furthest any pixel of the wave, and of the tile, went
   uint32_t wreach = nc;
   for (int off = 32; off >= 1; off >>= 1) wreach = max(wreach, (uint32_t)__shfl_xor((int)wreach, off));
@@ -651,14 +745,20 @@ static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render_bwd(const uin
   uint32_t reach = 0;
   for (int v = 0; v < WAVES; v++) reach = max(reach, s_max[v]);
   const uint32_t end = rg.x + reach;
-  // entries never reached get zero partials
-  for (uint32_t e = end + threadIdx.x; e < rg.y; e += DIFF_BLOCK)
-    for (int k = 0; k < DIFF_PIX_GRADS; k++) gentry[(size_t)emit_idx[e] * DIFF_PIX_GRADS + k] = 0.0f;
+  // entries no pixel reached get zero partials: this chunk's, and after the
+  // tile's last chunk item everything to the end of its list
+  {
+    const bool last_item = chunk + 1u == max(1u, (reach + DIFF_CHUNK - 1) / DIFF_CHUNK);
+    const uint32_t zhi = last_item ? rg.y : ce;
+    for (uint32_t e = max(cs, end) + threadIdx.x; e < zhi; e += DIFF_BLOCK)
+      for (int k = 0; k < DIFF_PIX_GRADS; k++) gentry[(size_t)emit_idx[e] * DIFF_PIX_GRADS + k] = 0.0f;
+  }
+  const uint32_t top = min(ce, max(end, cs));  // this chunk's entries below the reach: [cs, top)
   float4 n0 = make_float4(0.f, 0.f, 0.f, 0.f), n1 = n0, n2 = n0;
   uint32_t nq = 0;
   {
-    const uint32_t lo0 = end - min((uint32_t)BATCH, end - rg.x);
-    if (threadIdx.x < BATCH && lo0 + threadIdx.x < end) {
+    const uint32_t lo0 = top - min((uint32_t)BATCH, top - cs);
+    if (threadIdx.x < BATCH && lo0 + threadIdx.x < top) {
       nq = emit_idx[lo0 + threadIdx.x];
       const uint32_t sp = emit_splat[nq];
       n0 = pre[3 * (size_t)sp];
@@ -666,9 +766,9 @@ static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render_bwd(const uin
       n2 = pre[3 * (size_t)sp + 2];
     }
   }
-  uint32_t hi = end;
-  while (hi > rg.x) {
-    const uint32_t lo = hi - min((uint32_t)BATCH, hi - rg.x);
+  uint32_t hi = top;
+  while (hi > cs) {
+    const uint32_t lo = hi - min((uint32_t)BATCH, hi - cs);
     const uint32_t cnt = hi - lo;
     __syncthreads();  // the previous batch's write-out is done
     if (threadIdx.x < BATCH) {
@@ -683,8 +783,8 @@ static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render_bwd(const uin
 #pragma unroll
       for (int k = threadIdx.x; k < GWORDS / 4; k += DIFF_BLOCK) g4[k] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    if (lo > rg.x) {
-      const uint32_t lo2 = lo - min((uint32_t)BATCH, lo - rg.x);
+    if (lo > cs) {
+      const uint32_t lo2 = lo - min((uint32_t)BATCH, lo - cs);
       if (threadIdx.x < BATCH && lo2 + threadIdx.x < lo) {
         nq = emit_idx[lo2 + threadIdx.x];
         const uint32_t sp = emit_splat[nq];
