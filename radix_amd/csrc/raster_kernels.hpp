@@ -652,25 +652,33 @@ __device__ __forceinline__ void row_sums_to_last(float* v) {
 }
 
 // One block: the adjoint's work items, (tile, chunk) for each tile's chunks
-// up to its reach (at least one: a tile's first item writes the zero
-// partials of the entries no pixel reached), in tile order; their count to
-// *items.
+// up to its reach (at least one: a tile's last item writes the zero
+// partials of the entries no pixel reached); their count to *items. The
+// full chunks come first and each tile's partial last chunk after them, so
+// the blocks dispatched last are the short ones.
 static __global__ __launch_bounds__(1024) void k_diff_bwd_items(const uint32_t* tile_reach, uint32_t tiles,
                                                                 uint32_t* item_tile, uint32_t* item_chunk, uint32_t* items) {
   __shared__ uint32_t s_scan[1024 / RDX_WAVE];
-  uint32_t carry = 0;
-  for (uint32_t b0 = 0; b0 < tiles; b0 += 1024) {
-    const uint32_t t = b0 + threadIdx.x;
-    const uint32_t k = t < tiles ? max(1u, (tile_reach[t] + DIFF_CHUNK - 1) / DIFF_CHUNK) : 0u;
-    uint32_t sum;
-    const uint32_t at = carry + block_exclusive_sum<1024>(k, s_scan, &sum);
-    for (uint32_t c = 0; c < k; c++) {
-      item_tile[at + c] = t;
-      item_chunk[at + c] = c;
+  uint32_t full = 0;
+  for (int part = 0; part < 2; part++) {
+    uint32_t carry = part ? full : 0u;
+    for (uint32_t b0 = 0; b0 < tiles; b0 += 1024) {
+      const uint32_t t = b0 + threadIdx.x;
+      const uint32_t reach = t < tiles ? tile_reach[t] : 0u;
+      const uint32_t nfull = reach / DIFF_CHUNK;
+      // part 0: the full chunks; part 1: the rest (an empty tile's one item)
+      const uint32_t k = t >= tiles ? 0u : part == 0 ? nfull : (reach % DIFF_CHUNK != 0u || reach == 0u ? 1u : 0u);
+      uint32_t sum;
+      const uint32_t at = carry + block_exclusive_sum<1024>(k, s_scan, &sum);
+      for (uint32_t c = 0; c < k; c++) {
+        item_tile[at + c] = t;
+        item_chunk[at + c] = part == 0 ? c : nfull;
+      }
+      carry += sum;
     }
-    carry += sum;
+    if (part == 0) full = carry;
+    else if (threadIdx.x == 0) *items = carry;
   }
-  if (threadIdx.x == 0) *items = carry;
 }
 
 struct BwdItems {
@@ -883,18 +891,25 @@ static __global__ __launch_bounds__(256) void k_diff_pre_bwd(const float* __rest
                                                              const float* __restrict__ gpix, float* __restrict__ grad) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  // each parameter's gradient is stored as soon as it is known, so no
+  // array of all 59 stays live (157 -> fewer VGPRs: more waves resident)
+  float* gp = grad + i;
+  auto put = [&](int k, float v) { gp[(size_t)k * n] = v; };
   DiffSplat s;
   diff_splat_forward(th, n, i, u, s);
-  float gout[DIFF_PARAMS];
-  for (int k = 0; k < DIFF_PARAMS; k++) gout[k] = 0.0f;
-  if (s.ok) {
+  if (!s.ok) {
+#pragma unroll
+    for (int k = 0; k < DIFF_PARAMS; k++) put(k, 0.0f);
+    return;
+  }
+  {
     const float* G = gpix + (size_t)i * DIFF_PIX_GRADS;
     const float gmu2x = G[0], gmu2y = G[1], gQ0 = G[2], gQ1 = G[3], gQ2 = G[4], gopc = G[5];
     // colour
     const int nk = u.sh_degree >= 3 ? 15 : (u.sh_degree == 2 ? 8 : (u.sh_degree == 1 ? 3 : 0));
     float graw[3];
     for (int c = 0; c < 3; c++) graw[c] = s.col_raw[c] > 0.0f ? G[6 + c] : 0.0f;
-    for (int c = 0; c < 3; c++) gout[11 + c] = graw[c];
+    for (int c = 0; c < 3; c++) put(11 + c, graw[c]);
     float3 Yg[15];
     sh_basis15_grad(s.dir[0], s.dir[1], s.dir[2], Yg);
     float gdir[3] = {0.0f, 0.0f, 0.0f};
@@ -905,12 +920,15 @@ static __global__ __launch_bounds__(256) void k_diff_pre_bwd(const float* __rest
       if (k < nk) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          gout[14 + 3 * k + c] = s.Y[k] * graw[c];
+          put(14 + 3 * k + c, s.Y[k] * graw[c]);
           const float cg = plane(th, n, 14 + 3 * k + c, i) * graw[c];
           gdir[0] = fmaf(cg, Yg[k].x, gdir[0]);
           gdir[1] = fmaf(cg, Yg[k].y, gdir[1]);
           gdir[2] = fmaf(cg, Yg[k].z, gdir[2]);
         }
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; c++) put(14 + 3 * k + c, 0.0f);
       }
     }
     float gmu[3] = {0.0f, 0.0f, 0.0f};
@@ -936,7 +954,7 @@ static __global__ __launch_bounds__(256) void k_diff_pre_bwd(const float* __rest
       gb += gopc * h * (-2.0f * s.b * io + 2.0f * s.b * ip);
       gop = gopc * sqrtf(s.ratio);
     }
-    gout[10] = gop * s.op * (1.0f - s.op);
+    put(10, gop * s.op * (1.0f - s.op));
     // cov2D = T Sigma T^T
     const float G2[4] = {ga, 0.5f * gb, 0.5f * gb, gd};
     float gSig[9], gT[6];
@@ -970,7 +988,7 @@ static __global__ __launch_bounds__(256) void k_diff_pre_bwd(const float* __rest
     const float gcw = -(gmu2x * 0.5f * u.width * (float)s.clip[0] + gmu2y * 0.5f * u.height * (float)s.clip[1]) * (iw * iw);
     for (int k = 0; k < 3; k++) gt[k] += gcx * (float)u.P[k] + gcy * (float)u.P[4 + k] + gcw * (float)u.P[12 + k];
     for (int k = 0; k < 3; k++) gmu[k] += (float)u.Vl[k] * gt[0] + (float)u.Vl[3 + k] * gt[1] + (float)u.Vl[6 + k] * gt[2];
-    for (int k = 0; k < 3; k++) gout[k] = gmu[k];
+    for (int k = 0; k < 3; k++) put(k, gmu[k]);
     // Sigma = M M^T, M = R diag(s)
     float gM[9];
     for (int r = 0; r < 3; r++)
@@ -986,7 +1004,7 @@ static __global__ __launch_bounds__(256) void k_diff_pre_bwd(const float* __rest
         gs += gM[3 * r + c] * s.R[3 * r + c];
         gR[3 * r + c] = gM[3 * r + c] * s.s[c];
       }
-      gout[3 + c] = gs * s.s[c];
+      put(3 + c, gs * s.s[c]);
     }
     const float qx = s.qv[0], qy = s.qv[1], qz = s.qv[2], qw = s.qv[3];
     float gq[4];
@@ -997,10 +1015,11 @@ static __global__ __launch_bounds__(256) void k_diff_pre_bwd(const float* __rest
     if (s.qlen > 0.0f) {
       const float dq = gq[0] * qx + gq[1] * qy + gq[2] * qz + gq[3] * qw;
       const float iq = __builtin_amdgcn_rcpf(s.qlen);
-      for (int k = 0; k < 4; k++) gout[6 + k] = (gq[k] - s.qv[k] * dq) * iq;
+      for (int k = 0; k < 4; k++) put(6 + k, (gq[k] - s.qv[k] * dq) * iq);
+    } else {
+      for (int k = 0; k < 4; k++) put(6 + k, 0.0f);
     }
   }
-  for (int k = 0; k < DIFF_PARAMS; k++) grad[(size_t)k * n + i] = gout[k];
 }
 
 }  // namespace rdx
