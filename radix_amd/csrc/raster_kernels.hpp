@@ -704,11 +704,15 @@ static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render_bwd(const uin
   constexpr int WAVES = DIFF_BLOCK / RDX_WAVE;
   constexpr int BATCH = DIFF_BWD_BATCH;
   constexpr int SLOTS = WAVES * 4;  // a row of 16 lanes per slot
-  constexpr int GWORDS = SLOTS * BATCH * DIFF_PIX_GRADS;
+  // a slot's partials, padded so that the four rows of a wave, which write
+  // together, fall in different LDS banks (BATCH * 9 words is a multiple of
+  // the bank count)
+  constexpr int GSTRIDE = BATCH * DIFF_PIX_GRADS + 4;
+  constexpr int GWORDS = SLOTS * GSTRIDE;
   __shared__ float4 s_p0[BATCH], s_p1[BATCH], s_p2[BATCH];
   __shared__ uint32_t s_q[BATCH];
   __shared__ uint32_t s_ov[BATCH];
-  __shared__ __align__(16) float s_g[SLOTS][BATCH][DIFF_PIX_GRADS];
+  __shared__ __align__(16) float s_g[GWORDS];
   __shared__ uint32_t s_max[WAVES];
   static_assert(GWORDS % 4 == 0, "partials cleared as float4");
   static_assert(BATCH % RDX_WAVE == 0 && BATCH <= DIFF_BLOCK, "whole waves of entries");
@@ -787,7 +791,7 @@ static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render_bwd(const uin
       s_ov[threadIdx.x] = threadIdx.x < cnt ? quad_mask(q, n0, n1, n2) : 0u;
     }
     {
-      float4* g4 = reinterpret_cast<float4*>(&s_g[0][0][0]);
+      float4* g4 = reinterpret_cast<float4*>(s_g);
 #pragma unroll
       for (int k = threadIdx.x; k < GWORDS / 4; k += DIFF_BLOCK) g4[k] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
@@ -844,7 +848,7 @@ static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render_bwd(const uin
         for (int j = 0; j < DIFF_PIX_GRADS; j++) gv[j] = contrib ? gv[j] : 0.0f;
         row_sums_to_last<DIFF_PIX_GRADS>(gv);
         if ((q.lane & 15) == 15) {
-          float* gs = &s_g[q.w * 4 + (q.lane >> 4)][k][0];
+          float* gs = s_g + (q.w * 4 + (q.lane >> 4)) * GSTRIDE + k * DIFF_PIX_GRADS;
 #pragma unroll
           for (int j = 0; j < DIFF_PIX_GRADS; j++) gs[j] = gv[j];
         }
@@ -855,7 +859,7 @@ static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render_bwd(const uin
     for (uint32_t o = threadIdx.x; o < cnt * DIFF_PIX_GRADS; o += DIFF_BLOCK) {
       float v = 0.0f;
 #pragma unroll
-      for (int sl = 0; sl < SLOTS; sl++) v += (&s_g[sl][0][0])[o];
+      for (int sl = 0; sl < SLOTS; sl++) v += s_g[sl * GSTRIDE + o];
       const uint32_t k = o / DIFF_PIX_GRADS, j = o - k * DIFF_PIX_GRADS;
       gentry[(size_t)s_q[k] * DIFF_PIX_GRADS + j] = v;
     }
