@@ -824,28 +824,28 @@ static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render_bwd(const uin
         if (!ballot(contrib)) continue;
         const float4 c = s_p2[k];
         float gv[DIFF_PIX_GRADS];
-        const float Tb = T * __builtin_amdgcn_rcpf(1.0f - a);  // the transmittance in front of this splat
-        const float aT = a * Tb;
+        // Branch-free: a lane that does not composite this splat takes alpha
+        // 0, which leaves its transmittance (divided by 1) and B exactly as
+        // they were and makes every partial below vanish.
+        const float ae = contrib ? a : 0.0f;
+        const float Tb = T * __builtin_amdgcn_rcpf(1.0f - ae);  // the transmittance in front of this splat
+        const float aT = ae * Tb;
         const float galpha = Tb * ((c.x - B0) * dC0 + (c.y - B1) * dC1 + (c.z - B2) * dC2);
-        const float gz2 = -0.5f * a * galpha;
+        const float gz2 = -0.5f * ae * galpha;
         const float q1 = 0.5f * p1.y;
         gv[0] = gz2 * -2.0f * (p1.x * dx + q1 * dy);
         gv[1] = gz2 * -2.0f * (q1 * dx + p1.z * dy);
         gv[2] = gz2 * dx * dx;
         gv[3] = gz2 * 2.0f * dx * dy;
         gv[4] = gz2 * dy * dy;
-        gv[5] = g * galpha;
+        gv[5] = (contrib ? g : 0.0f) * galpha;
         gv[6] = aT * dC0;
         gv[7] = aT * dC1;
         gv[8] = aT * dC2;
-        if (contrib) {
-          T = Tb;
-          B0 = fmaf(c.x, a, (1.0f - a) * B0);
-          B1 = fmaf(c.y, a, (1.0f - a) * B1);
-          B2 = fmaf(c.z, a, (1.0f - a) * B2);
-        }
-#pragma unroll
-        for (int j = 0; j < DIFF_PIX_GRADS; j++) gv[j] = contrib ? gv[j] : 0.0f;
+        T = Tb;
+        B0 = fmaf(c.x, ae, (1.0f - ae) * B0);
+        B1 = fmaf(c.y, ae, (1.0f - ae) * B1);
+        B2 = fmaf(c.z, ae, (1.0f - ae) * B2);
         row_sums_to_last<DIFF_PIX_GRADS>(gv);
         if ((q.lane & 15) == 15) {
           float* gs = s_g + (q.w * 4 + (q.lane >> 4)) * GSTRIDE + k * DIFF_PIX_GRADS;
