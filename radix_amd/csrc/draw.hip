@@ -141,7 +141,7 @@ static int draw_run(Draw* d, hipStream_t s, const DiffU& u, const uint32_t* coun
     hipLaunchKernelGGL(k_diff_emit, dim3(blocks_for(d->capacity)), dim3(256), 0, s, nullptr, counts + D_DRAWN,
                        (const uint32_t*)d->touch.ptr, (const uint4*)d->rect.ptr, (const uint32_t*)d->offsets.ptr,
                        (uint32_t)u.tiles_x, d->cap, (uint32_t*)d->tile_key.ptr, (uint32_t*)d->emit_idx.ptr,
-                       (uint32_t*)d->emit_splat.ptr, counts + D_OVERFLOW);
+                       (uint32_t*)d->emit_splat.ptr, nullptr, counts + D_OVERFLOW);
   }
   // stable sort of the entries by tile: front to back within each tile
   const SortPlan pe = draw_entry_plan();

@@ -19,7 +19,7 @@ struct Raster {
   rdx_raster_settings set;
   SortScratch sort_splats, sort_entries;
   Buffer pre, keys, touch, rect, order, counts, offsets, bsum, tile_key, emit_idx, emit_splat, range, t_final,
-      n_contrib, gentry, gpix;
+      n_contrib, gentry, first_entry;
   // the forward's chunk records and the adjoint's (tile, chunk) items
   Buffer chunk_rec, tile_reach, item_tile, item_chunk, items;
   uint32_t cap = 0;   // entries
@@ -34,7 +34,7 @@ struct Raster {
     sort_entries.release();
     Buffer* b[] = {&pre,   &keys,      &touch,     &rect,       &order,     &counts,     &offsets,
                    &bsum,  &tile_key,  &emit_idx,  &emit_splat, &range,     &t_final,    &n_contrib,
-                   &gentry, &gpix,     &chunk_rec, &tile_reach, &item_tile, &item_chunk, &items};
+                   &gentry, &first_entry, &chunk_rec, &tile_reach, &item_tile, &item_chunk, &items};
     for (Buffer* x : b) x->release();
   }
 };
@@ -108,7 +108,7 @@ static int reserve(Raster* r, uint32_t n, const DiffU& u) {
       (rc = r->bsum.ensure((nn / DSCAN_TILE + 2) * 4)))
     return rc;
   if ((rc = r->range.ensure(tiles * sizeof(uint2))) || (rc = r->t_final.ensure(px * 4)) || (rc = r->n_contrib.ensure(px * 4)) ||
-      (rc = r->gpix.ensure(nn * DIFF_PIX_GRADS * 4)) || (rc = r->tile_reach.ensure(tiles * 4)) ||
+      (rc = r->first_entry.ensure(nn * 4)) || (rc = r->tile_reach.ensure(tiles * 4)) ||
       (rc = r->items.ensure(4)))
     return rc;
   if ((rc = r->sort_splats.reserve(n, 1, tiles_of(p, n)))) return rc;
@@ -240,7 +240,7 @@ int rdx_raster_forward(rdx_raster* raster, void* stream, const float* params_dev
     hipLaunchKernelGGL(k_diff_emit, dim3(grid_for(n)), dim3(256), 0, s, (const uint32_t*)r->order.ptr, counts + C_VISIBLE,
                        (const uint32_t*)r->touch.ptr, (const uint4*)r->rect.ptr, (const uint32_t*)r->offsets.ptr,
                        (uint32_t)u.tiles_x, r->cap, (uint32_t*)r->tile_key.ptr, (uint32_t*)r->emit_idx.ptr,
-                       (uint32_t*)r->emit_splat.ptr, counts + C_OVERFLOW);
+                       (uint32_t*)r->emit_splat.ptr, (uint32_t*)r->first_entry.ptr, counts + C_OVERFLOW);
   }
   // stable sort of the entries by tile: front to back within each tile
   const SortPlan pe = entry_plan();
@@ -293,14 +293,9 @@ int rdx_raster_backward(rdx_raster* raster, void* stream, const float* grad_imag
                        (const uint32_t*)r->n_contrib.ptr, grad_image_dev, (float*)r->gentry.ptr, bi);
   }
   if (n) {
-    Stage st(c, s, "diff_gather");
-    hipLaunchKernelGGL(k_diff_gather, dim3(grid_for(n)), dim3(256), 0, s, (const uint32_t*)r->order.ptr, counts + C_VISIBLE,
-                       (const uint32_t*)r->touch.ptr, (const uint32_t*)r->offsets.ptr, (const float*)r->gentry.ptr,
-                       counts + C_ENTRIES, (float*)r->gpix.ptr, n);
-  }
-  if (n) {
     Stage st(c, s, "diff_pre_bwd");
-    hipLaunchKernelGGL(k_diff_pre_bwd, dim3(grid_for(n)), dim3(256), 0, s, r->params, n, u, (const float*)r->gpix.ptr,
+    hipLaunchKernelGGL(k_diff_pre_bwd, dim3(grid_for(n)), dim3(256), 0, s, r->params, n, u, (const uint32_t*)r->touch.ptr,
+                       (const uint32_t*)r->first_entry.ptr, (const float*)r->gentry.ptr, counts + C_ENTRIES,
                        grad_params_dev);
   }
   hipError_t e = hipGetLastError();

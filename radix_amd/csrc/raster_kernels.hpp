@@ -25,8 +25,8 @@
 // of the entries by tile -> k_diff_ranges -> k_diff_render (a block per
 // tile). Backward: k_diff_render_bwd (a block per tile, back to front from
 // each pixel's last contributor, transmittance recomputed by division from
-// the final one), k_diff_gather (per splat, the sum of its entries'
-// partials) and k_diff_pre_bwd (the chain rule to the 59 parameters).
+// the final one) and k_diff_pre_bwd (per splat, the sum of its entries'
+// partials and the chain rule to the 59 parameters).
 #pragma once
 
 #include "common.hpp"
@@ -312,7 +312,7 @@ static __global__ __launch_bounds__(DSCAN_BLOCK) void k_diff_scan_final(const ui
 static __global__ __launch_bounds__(256) void k_diff_emit(const uint32_t* order, const uint32_t* m_dev, const uint32_t* touch,
                                                           const uint4* rect, const uint32_t* offsets, uint32_t tiles_x,
                                                           uint32_t cap, uint32_t* tile_key, uint32_t* emit_idx,
-                                                          uint32_t* emit_splat, uint32_t* overflow) {
+                                                          uint32_t* emit_splat, uint32_t* first_entry, uint32_t* overflow) {
   const uint32_t m = *m_dev;
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= m) return;
@@ -325,6 +325,7 @@ static __global__ __launch_bounds__(256) void k_diff_emit(const uint32_t* order,
     store_agent(overflow, 1u);
     return;
   }
+  if (first_entry) first_entry[sp] = q;  // the adjoint sums the splat's entries [q, q + cnt)
   for (uint32_t ty = r.y; ty <= r.w; ty++)
     for (uint32_t tx = r.x; tx <= r.z; tx++, q++) {
       tile_key[q] = ty * tiles_x + tx;
@@ -867,32 +868,14 @@ static __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render_bwd(const uin
   }
 }
 
-// Per splat (front-to-back position j): the sum of its entries' partials.
-static __global__ void k_diff_gather(const uint32_t* order, const uint32_t* m_dev, const uint32_t* touch,
-                                     const uint32_t* offsets, const float* gentry, const uint32_t* e_dev, float* gpix,
-                                     uint32_t n) {
-  const uint32_t m = *m_dev;
-  const uint32_t cap = *e_dev;
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  if (j >= m) return;
-  const uint32_t sp = order[m - 1 - j];
-  const uint32_t q0 = offsets[j], cnt = touch[sp];
-  float acc[DIFF_PIX_GRADS];
-  for (int k = 0; k < DIFF_PIX_GRADS; k++) acc[k] = 0.0f;
-  if (q0 + cnt <= cap)
-    for (uint32_t q = q0; q < q0 + cnt; q++)
-      for (int k = 0; k < DIFF_PIX_GRADS; k++) acc[k] += gentry[(size_t)q * DIFF_PIX_GRADS + k];
-  for (int k = 0; k < DIFF_PIX_GRADS; k++) gpix[(size_t)sp * DIFF_PIX_GRADS + k] = acc[k];
-}
-
-static __global__ void k_diff_clear_f32(float* p, size_t count) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0f;
-}
-
-// The chain rule from the pixel-side partials to the 59 parameters.
+// The chain rule from the pixel-side partials to the 59 parameters. A
+// splat's pixel-side partials are the sum of its entries' (emitted
+// consecutively from first_entry, summed in emission order).
 static __global__ __launch_bounds__(256) void k_diff_pre_bwd(const float* __restrict__ th, uint32_t n, DiffU u,
-                                                             const float* __restrict__ gpix, float* __restrict__ grad) {
+                                                             const uint32_t* __restrict__ touch,
+                                                             const uint32_t* __restrict__ first_entry,
+                                                             const float* __restrict__ gentry, const uint32_t* e_dev,
+                                                             float* __restrict__ grad) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   // each parameter's gradient is stored as soon as it is known, so no
@@ -907,7 +890,17 @@ static __global__ __launch_bounds__(256) void k_diff_pre_bwd(const float* __rest
     return;
   }
   {
-    const float* G = gpix + (size_t)i * DIFF_PIX_GRADS;
+    float G[DIFF_PIX_GRADS];
+#pragma unroll
+    for (int k = 0; k < DIFF_PIX_GRADS; k++) G[k] = 0.0f;
+    const uint32_t cnt = touch[i];
+    if (cnt) {
+      const uint32_t q0 = first_entry[i];
+      if (q0 + cnt <= *e_dev)  // always, unless the emission overflowed (an error of the forward)
+        for (uint32_t q = q0; q < q0 + cnt; q++)
+#pragma unroll
+          for (int k = 0; k < DIFF_PIX_GRADS; k++) G[k] += gentry[(size_t)q * DIFF_PIX_GRADS + k];
+    }
     const float gmu2x = G[0], gmu2y = G[1], gQ0 = G[2], gQ1 = G[3], gQ2 = G[4], gopc = G[5];
     // colour
     const int nk = u.sh_degree >= 3 ? 15 : (u.sh_degree == 2 ? 8 : (u.sh_degree == 1 ? 3 : 0));
