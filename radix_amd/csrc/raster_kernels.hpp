@@ -882,6 +882,33 @@ static __global__ __launch_bounds__(256) void k_diff_pre_bwd(const float* __rest
   // array of all 59 stays live (157 -> fewer VGPRs: more waves resident)
   float* gp = grad + i;
   auto put = [&](int k, float v) { gp[(size_t)k * n] = v; };
+  // The splat's pixel-side partials first, while few registers are live:
+  // two entries' loads in flight at a time, added in entry order.
+  float G[DIFF_PIX_GRADS];
+#pragma unroll
+  for (int k = 0; k < DIFF_PIX_GRADS; k++) G[k] = 0.0f;
+  {
+    const uint32_t cnt = touch[i];
+    const uint32_t q0 = cnt ? first_entry[i] : 0u;
+    if (cnt && q0 + cnt <= *e_dev) {  // always, unless the emission overflowed (an error of the forward)
+      const float* ge = gentry + (size_t)q0 * DIFF_PIX_GRADS;
+      uint32_t q = 0;
+      for (; q + 2 <= cnt; q += 2) {
+        float v[2][DIFF_PIX_GRADS];
+#pragma unroll
+        for (int e = 0; e < 2; e++)
+#pragma unroll
+          for (int k = 0; k < DIFF_PIX_GRADS; k++) v[e][k] = ge[(size_t)(q + e) * DIFF_PIX_GRADS + k];
+#pragma unroll
+        for (int e = 0; e < 2; e++)
+#pragma unroll
+          for (int k = 0; k < DIFF_PIX_GRADS; k++) G[k] += v[e][k];
+      }
+      if (q < cnt)
+#pragma unroll
+        for (int k = 0; k < DIFF_PIX_GRADS; k++) G[k] += ge[(size_t)q * DIFF_PIX_GRADS + k];
+    }
+  }
   DiffSplat s;
   diff_splat_forward(th, n, i, u, s);
   if (!s.ok) {
@@ -890,17 +917,6 @@ static __global__ __launch_bounds__(256) void k_diff_pre_bwd(const float* __rest
     return;
   }
   {
-    float G[DIFF_PIX_GRADS];
-#pragma unroll
-    for (int k = 0; k < DIFF_PIX_GRADS; k++) G[k] = 0.0f;
-    const uint32_t cnt = touch[i];
-    if (cnt) {
-      const uint32_t q0 = first_entry[i];
-      if (q0 + cnt <= *e_dev)  // always, unless the emission overflowed (an error of the forward)
-        for (uint32_t q = q0; q < q0 + cnt; q++)
-#pragma unroll
-          for (int k = 0; k < DIFF_PIX_GRADS; k++) G[k] += gentry[(size_t)q * DIFF_PIX_GRADS + k];
-    }
     const float gmu2x = G[0], gmu2y = G[1], gQ0 = G[2], gQ1 = G[3], gQ2 = G[4], gopc = G[5];
     // colour
     const int nk = u.sh_degree >= 3 ? 15 : (u.sh_degree == 2 ? 8 : (u.sh_degree == 1 ? 3 : 0));
