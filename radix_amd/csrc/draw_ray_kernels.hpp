@@ -51,7 +51,7 @@ __device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a
 __device__ __forceinline__ float3 div3(float3 a, float3 s) { return make_float3(a.x / s.x, a.y / s.y, a.z / s.z); }
 
 // Draw position i (back to front) of m: splat order[i]'s words of the eye,
-// as the record the renderer reads (8 float4, RAY_REC), its touched tiles
+// as the record the renderer reads (7 float4, RAY_REC), its touched tiles
 // and rectangle. Positions past n, and splats this eye culls (a zero second
 // eye word: look.visible), are not drawn.
 template <int METHOD>
@@ -172,11 +172,12 @@ __global__ __launch_bounds__(256) void k_draw_ray_pre(const uint4* __restrict__ 
       LCx = cross3(lox, ld);
       LCy = cross3(loy, ld);
     }
-    r[3] = make_float4(qy, LC0.x, LC0.y, LC0.z);
-    r[4] = make_float4(LCx.x, LCx.y, LCx.z, LCy.x);
-    r[5] = make_float4(LCy.y, LCy.z, LD0.x, LD0.y);
-    r[6] = make_float4(LD0.z, LDx.x, LDx.y, LDx.z);
-    r[7] = make_float4(LDy.x, LDy.y, LDy.z, 0.0f);
+    // |L0 + dx Lx + dy Ly|^2 = |L0|^2 + 2 dx L0.Lx + 2 dy L0.Ly + dx^2 |Lx|^2
+    // + 2 dx dy Lx.Ly + dy^2 |Ly|^2 for localCross (N) and localDir (D)
+    r[3] = make_float4(qy, dot3(LC0, LC0), 2.0f * dot3(LC0, LCx), 2.0f * dot3(LC0, LCy));
+    r[4] = make_float4(dot3(LCx, LCx), 2.0f * dot3(LCx, LCy), dot3(LCy, LCy), dot3(LD0, LD0));
+    r[5] = make_float4(2.0f * dot3(LD0, LDx), 2.0f * dot3(LD0, LDy), dot3(LDx, LDx), 2.0f * dot3(LDx, LDy));
+    r[6] = make_float4(dot3(LDy, LDy), 0.0f, 0.0f, 0.0f);
   } else {
     // the facing quad (raygs vertex, projections.tsl.ts:442-460): frame, c^2, axis lengths
     const float4 q = quat_oct_decode(ew[0], 10, 12);

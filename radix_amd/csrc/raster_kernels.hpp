@@ -421,11 +421,11 @@ __device__ __forceinline__ void load_entry(const uint32_t* emit_idx, const uint3
   p2 = pre[3 * (size_t)sp + 2];
 }
 
-// Records of the draw's ray-evaluated methods (draw_ray_kernels.hpp): 8
+// Records of the draw's ray-evaluated methods (draw_ray_kernels.hpp): 7
 // float4 per draw position, the first three laid out as the ewa record's
 // (centre, opacity, cut^2; ..., ex; colour, ey), the rest the method's.
 constexpr int REC_EWA = 0, REC_BOUND = 1, REC_RAYGS = 2;
-constexpr int RAY_REC = 8;
+constexpr int RAY_REC = 7;
 
 __device__ __forceinline__ void load_entry_ray(const uint32_t* emit_idx, const uint32_t* emit_splat, const float4* pre,
                                                uint32_t e, float4* r) {
@@ -438,33 +438,31 @@ __device__ __forceinline__ void load_entry_ray(const uint32_t* emit_idx, const u
 // +inf where the pixel is outside the method's quad (so it never blends).
 //  REC_BOUND (3dgut, htgs): the quad is the screen bound, |u| <= qx and
 //  |v| <= qy along its axes; localCross and localDir are affine in the
-//  pixel, z^2 = |localCross|^2 / |localDir|^2 (rayDensity,
-//  projection.tsl.ts:437-442), 1e30 for a null direction.
+//  pixel, so their squared lengths are quadratic forms in (dx, dy), held as
+//  6 coefficients each: z^2 = |localCross|^2 / |localDir|^2 (rayDensity,
+//  projection.tsl.ts:437-442), 1e30 for a null direction. (The record's
+//  centre is the projected splat centre, where localCross is about 0, so
+//  the expanded squares lose nothing to cancellation.)
 //  REC_RAYGS: z = (U, V) / W, affine numerators and denominator of the
 //  pixel ray's hit on the facing quad; inside where W > 0 and |z| <= zmax;
-//  z^2 = |z|^2 c^2 / (c^2 + |z|^2) (projections.tsl.ts:462-466).
+//  z^2 = |z|^2 c^2 / (c^2 + |z|^2) = (U^2 + V^2) c^2 / (c^2 W^2 + U^2 + V^2)
+//  (projections.tsl.ts:462-466).
 template <int REC>
-__device__ __forceinline__ float ray_z2(float4 r1, float4 r3, float4 r4, float4 r5, float4 r6, float4 r7, float dx,
-                                        float dy) {
+__device__ __forceinline__ float ray_z2(float4 r1, float4 r3, float4 r4, float4 r5, float4 r6, float dx, float dy) {
   if constexpr (REC == REC_BOUND) {
     const float u = dx * r1.x + dy * r1.y, v = dy * r1.x - dx * r1.y;
     const bool in = fabsf(u) <= r1.z && fabsf(v) <= r3.x;
-    const float lcx = r3.y + dx * r4.x + dy * r4.w, lcy = r3.z + dx * r4.y + dy * r5.x,
-                lcz = r3.w + dx * r4.z + dy * r5.y;
-    const float ldx = r5.z + dx * r6.y + dy * r7.x, ldy = r5.w + dx * r6.z + dy * r7.y,
-                ldz = r6.x + dx * r6.w + dy * r7.z;
-    const float dd = ldx * ldx + ldy * ldy + ldz * ldz;
-    const float z2 = dd <= 0.0f ? 1e30f : (lcx * lcx + lcy * lcy + lcz * lcz) / dd;
+    const float xx = dx * dx, xy = dx * dy, yy = dy * dy;
+    const float num = r3.y + dx * r3.z + dy * r3.w + xx * r4.x + xy * r4.y + yy * r4.z;
+    const float den = r4.w + dx * r5.x + dy * r5.y + xx * r5.z + xy * r5.w + yy * r6.x;
+    const float z2 = den <= 0.0f ? 1e30f : fmaxf(num, 0.0f) * __builtin_amdgcn_rcpf(den);
     return in ? z2 : __builtin_inff();
   } else {
     const float U = r3.x + dx * r3.y + dy * r3.z, V = r3.w + dx * r4.x + dy * r4.y, W = r4.z + dx * r4.w + dy * r5.x;
     const float lim = r1.x * W;
     const bool in = W > 0.0f && fabsf(U) <= lim && fabsf(V) <= lim;
-    const float iw = 1.0f / W;
-    const float zx = U * iw, zy = V * iw;
-    const float zz = zx * zx + zy * zy;
-    const float c2 = r1.y;
-    return in ? zz * c2 / (c2 + zz) : __builtin_inff();
+    const float c2 = r1.y, ss = U * U + V * V;
+    return in ? ss * c2 * __builtin_amdgcn_rcpf(fmaf(c2 * W, W, ss)) : __builtin_inff();
   }
 }
 
@@ -510,9 +508,10 @@ __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render(const uint2* range, 
                                                                    uint32_t* n_contrib, DrawItems di, DiffChunks dc) {
   static_assert(DIFF_CHUNK == DIFF_BLOCK, "a chunk is one batch of the forward");
   static_assert(DRAW || REC == REC_EWA, "the ray records are drawn only");
-  constexpr int RX = REC == REC_EWA ? 1 : DIFF_BLOCK;  // the ray records' other five float4
+  constexpr int RX = REC == REC_EWA ? 1 : DIFF_BLOCK;  // the ray records' other four float4
   __shared__ float4 s_p0[DIFF_BLOCK], s_p1[DIFF_BLOCK], s_p2[DIFF_BLOCK];
-  __shared__ float4 s_p3[RX], s_p4[RX], s_p5[RX], s_p6[RX], s_p7[RX];
+  __shared__ float4 s_p3[RX], s_p4[RX], s_p5[RX], s_p6[RX];
+  static_assert(REC == REC_EWA || RAY_REC == 7, "the ray records' float4 staged here");
   __shared__ uint32_t s_ov[DIFF_BLOCK];
   uint32_t tile = blockIdx.x;
   uint2 rg;
@@ -555,7 +554,6 @@ __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render(const uint2* range, 
       s_p4[threadIdx.x] = nr[4];
       s_p5[threadIdx.x] = nr[5];
       s_p6[threadIdx.x] = nr[6];
-      s_p7[threadIdx.x] = nr[7];
     }
     s_ov[threadIdx.x] = e < rg.y ? quad_mask(q, nr[0], nr[1], nr[2]) : 0u;
     if (e + DIFF_BLOCK < rg.y) load(e + DIFF_BLOCK);
@@ -573,7 +571,7 @@ __global__ __launch_bounds__(DIFF_BLOCK) void k_diff_render(const uint2* range, 
         if constexpr (REC == REC_EWA)
           z2 = p1.x * dx * dx + p1.y * dx * dy + p1.z * dy * dy;
         else
-          z2 = ray_z2<REC>(p1, s_p3[k], s_p4[k], s_p5[k], s_p6[k], s_p7[k], dx, dy);
+          z2 = ray_z2<REC>(p1, s_p3[k], s_p4[k], s_p5[k], s_p6[k], dx, dy);
         const float a = DRAW ? draw_alpha(p0.z, gauss(z2), u.falloff) : p0.z * gauss(z2);
         const bool hit = !done && z2 <= p0.w && a >= cut_alpha;
         const float Tn = T * (1.0f - a);
