@@ -295,7 +295,7 @@ int rdx_raster_backward(rdx_raster* raster, void* stream, const float* grad_imag
   if (n) {
     Stage st(c, s, "diff_pre_bwd");
     hipLaunchKernelGGL(k_diff_pre_bwd, dim3(grid_for(n)), dim3(256), 0, s, r->params, n, u, (const uint32_t*)r->touch.ptr,
-                       (const uint32_t*)r->first_entry.ptr, (const float*)r->gentry.ptr, counts + C_ENTRIES,
+                       (const uint32_t*)r->first_entry.ptr, (const float*)r->gentry.ptr, counts + C_ENTRIES, r->cap,
                        grad_params_dev);
   }
   hipError_t e = hipGetLastError();

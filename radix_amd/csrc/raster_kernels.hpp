@@ -873,7 +873,7 @@ static __global__ __launch_bounds__(256) void k_diff_pre_bwd(const float* __rest
                                                              const uint32_t* __restrict__ touch,
                                                              const uint32_t* __restrict__ first_entry,
                                                              const float* __restrict__ gentry, const uint32_t* e_dev,
-                                                             float* __restrict__ grad) {
+                                                             uint32_t cap, float* __restrict__ grad) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   // each parameter's gradient is stored as soon as it is known, so no
@@ -888,7 +888,9 @@ static __global__ __launch_bounds__(256) void k_diff_pre_bwd(const float* __rest
   {
     const uint32_t cnt = touch[i];
     const uint32_t q0 = cnt ? first_entry[i] : 0u;
-    if (cnt && q0 + cnt <= *e_dev) {  // always, unless the emission overflowed (an error of the forward)
+    // always, unless the emission overflowed (an error of the forward): the
+    // entries written, within the buffer
+    if (cnt && q0 + cnt <= min(*e_dev, cap)) {
       const float* ge = gentry + (size_t)q0 * DIFF_PIX_GRADS;
       uint32_t q = 0;
       for (; q + 2 <= cnt; q += 2) {

@@ -34,11 +34,14 @@ def on_stream(fn):
     import functools
     import inspect
 
-    sig = inspect.signature(fn)
+    # the position of `stream` among the parameters, found once (binding the
+    # signature on every call cost about 10 us of host time per launch)
+    names = list(inspect.signature(fn).parameters)
+    at = names.index("stream")
 
     @functools.wraps(fn)
     def run(*args, **kwargs):
-        stream = sig.bind(*args, **kwargs).arguments.get("stream")
+        stream = kwargs.get("stream", args[at] if len(args) > at else None)
         if stream is None:
             return fn(*args, **kwargs)
         import torch
@@ -149,10 +152,17 @@ def default_context() -> Context:
 
 
 def _stream(stream=None):
+    """The HIP stream to launch on: `stream`, else torch's current stream
+    (read as its raw handle where torch exposes it, without building a
+    Stream object)."""
+    if stream is not None:
+        return C.c_void_p(stream.cuda_stream)
     import torch
 
-    s = stream if stream is not None else torch.cuda.current_stream()
-    return C.c_void_p(s.cuda_stream)
+    raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+    if raw is not None:
+        return C.c_void_p(raw(torch._C._cuda_getDevice()))
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
 def _ptr(t) -> C.c_void_p:
