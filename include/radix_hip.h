@@ -477,8 +477,11 @@ int rdx_raster_destroy(rdx_raster* raster);
 int rdx_raster_forward(rdx_raster* raster, void* stream, const float* params_dev, uint32_t n, const rdx_camera* camera,
                        const rdx_params* params, float* image_dev, float* alpha_dev);
 /* The adjoint of the last forward: dL/dimage (3 planes) -> dL/dparams (59
- * planes of n, overwritten). Transmittance is recomputed from each pixel's
- * final one, back to front over its contributors. */
+ * planes of n, overwritten). Transmittance is recomputed back to front over
+ * each pixel's contributors, by division, from the value the forward kept
+ * at the end of each 256-entry chunk of its tile's list (one per chunk, not
+ * per contributor); a forward that failed leaves nothing to differentiate
+ * (RDX_E_INVALID). */
 int rdx_raster_backward(rdx_raster* raster, void* stream, const float* grad_image_dev, float* grad_params_dev);
 /* The last forward's drawn splats, (splat, tile) entries and entry capacity;
  * overflowed is 0 (a count past the limit fails the forward). Synchronizes. */
