@@ -119,6 +119,10 @@ def test_entry_limit_raises_and_no_limit_sizes_to_the_frame(cuda):
     with pytest.raises(N.RadixError, match=r"entries exceed the limit of 100") as err:
         rast.forward(D.params_tensor(theta), scene_rig(), Params())
     assert err.value.code == N.RDX_E_CAPACITY
+    # the failed forward left nothing to differentiate
+    with pytest.raises(N.RadixError, match=r"no forward to differentiate") as err:
+        rast.backward(torch.zeros((3, 64, 64), dtype=torch.float32, device="cuda"))
+    assert err.value.code == N.RDX_E_INVALID
     free = D.Rasterizer(sh_degree=3)
     img, alpha = free.forward(D.params_tensor(theta), scene_rig(), Params())
     torch.cuda.synchronize()
