@@ -76,9 +76,9 @@ class GpuStream {
 const streamOf = (options) => (options && options.stream ? options.stream.handle : undefined);
 const handleOf = (b) => (b && b.handle !== undefined ? b.handle : b === undefined ? null : b);
 // The ray-evaluated methods' codes (RDX_METHOD_3DGUT, _HTGS, _RAYGS)
-const rayMethod = (method) => {
+const rayMethod = (method, who = 'SplatProjection') => {
   const code = { '3dgut': 1, htgs: 2, raygs: 3 }[method];
-  if (code === undefined) throw new Error(`SplatProjection: unknown method ${method}`);
+  if (code === undefined) throw new Error(`${who}: unknown method ${method}`);
   return code;
 };
 
@@ -271,7 +271,7 @@ class SplatDrawer {
   // (channels w0, w1 and, with two eyes, w2) with the camera they were
   // projected for, at its render size
   drawRay(method, channels, n, order, count, capacity, camera, params, rgba, { eye = 0, stream } = {}) {
-    native.drawRay(this.handle, rayMethod(method), channels.map(handleOf), n, handleOf(order), handleOf(count),
+    native.drawRay(this.handle, rayMethod(method, 'SplatDrawer.drawRay'), channels.map(handleOf), n, handleOf(order), handleOf(count),
       capacity, camera, eye, params || {}, handleOf(rgba), streamOf({ stream }));
   }
 
