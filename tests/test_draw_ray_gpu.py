@@ -2,13 +2,15 @@
 FP64 oracle (orc_draw_ray, pinned by tests/test_oracle_draw_ray.py) on the same
 words and order: each method's words of a projected cloud, mono and stereo
 (either eye), an orthographic camera, LoD opacities above one, linear output,
-out-of-range indices and a count past the order."""
+out-of-range indices and a count past the order; a zero count, and an eye
+the camera does not have."""
 import numpy as np
 import pytest
 import torch
 
 from oracle import oracle as O
 from radix_amd import DeviceTable, Params, Projector
+from radix_amd import _native as N
 from radix_amd import camera as cam
 from radix_amd import workloads as W
 from radix_amd.draw import Drawer
@@ -105,3 +107,24 @@ def test_skipped_indices_and_clamped_count(cuda):
     order[::5] = 3000 + np.arange(order[::5].size, dtype=np.uint32)
     img, ref = run("htgs", table, rig, prm, order=order, count=len(order) + 100)
     compare(img, ref)
+
+
+def test_zero_count_draws_a_transparent_frame(cuda):
+    table = cloud(66, 2000)
+    rig = W.orbit_camera(66, 4.0, 96, 64)
+    prm = Params(sort_bits=32)
+    img, ref = run("raygs", table, rig, prm, order=np.arange(2000, dtype=np.uint32), count=0)
+    assert not img.any() and not ref.any()
+
+
+def test_an_eye_the_camera_lacks_is_refused(cuda):
+    table = cloud(67, 1000)
+    rig = W.orbit_camera(67, 4.0, 96, 64)  # one eye
+    prm = Params(sort_bits=32)
+    out = Projector().project_ray(table, rig, prm, "htgs")
+    order, count = dev_i32(np.arange(1000, dtype=np.uint32)), dev_i32(np.array([1000]))
+    with pytest.raises(N.RadixError, match=r"eye 1 of a camera of 1 eyes") as err:
+        Drawer().draw_ray("htgs", out["channels"], order, count, rig, prm, eye=1)
+    assert err.value.code == N.RDX_E_INVALID
+    with pytest.raises(ValueError, match=r"is not one of"):
+        Drawer().draw_ray("ewa", out["channels"], order, count, rig, prm)
